@@ -1,0 +1,229 @@
+"""ctypes binding of the C ABI in include/gkc_hip.h (libgkc_hip.so).
+
+Host-side mirror of the reference's two counting calls:
+  Counter.partition  ~ GenSuperkmerGPU + SKMStore::add_skms (gkc_cuda.hpp:58-61,
+                       superkmers.hpp:167-186), bins kept on the device
+  Counter.count      ~ kmc_counting_GPU over every partition (kmer_counting.hpp:9-12)
+  Counter.superkmer_arrays  the per-sub-batch arrays GenSuperkmerGPU hands to its
+                       callback (T_h_data, types.h:32-46)
+Errors raise GkcError with the library's message (the reference prints and
+exits, gkc_cuda.hpp:4-9).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+DEVICE_INPUT = 1
+BATCHED = 2
+PRESIZED = 4
+
+_u8p = C.POINTER(C.c_uint8)
+_u32p = C.POINTER(C.c_uint32)
+_i32p = C.POINTER(C.c_int32)
+_u64p = C.POINTER(C.c_uint64)
+_f32p = C.POINTER(C.c_float)
+
+
+class Params(C.Structure):
+    _fields_ = [("k", C.c_int32), ("p", C.c_int32), ("hpc", C.c_int32), ("n_parts", C.c_uint32),
+                ("filter_mode", C.c_int32), ("part_mode", C.c_int32), ("min_count", C.c_uint32),
+                ("max_count", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("ms_partition", C.c_double), ("ms_count", C.c_double), ("ms_extract", C.c_double),
+                ("ms_sort", C.c_double), ("ms_runlength", C.c_double), ("ms_sort_pass_avg", C.c_double),
+                ("n_kmers", C.c_uint64), ("n_superkmers", C.c_uint64), ("n_words", C.c_uint64),
+                ("n_distinct", C.c_uint64), ("n_entries", C.c_uint64), ("n_pass", C.c_uint32),
+                ("fallbacks", C.c_uint32), ("host_syncs", C.c_uint32), ("pad", C.c_uint32)]
+
+    def asdict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
+
+
+# exported symbols and their signatures (every symbol include/gkc_hip.h declares)
+SIGNATURES = {
+    "gkc_last_error": (C.c_char_p, []),
+    "gkc_version": (C.c_int, []),
+    "gkc_create": (C.c_void_p, [C.c_int, C.POINTER(Params)]),
+    "gkc_destroy": (None, [C.c_void_p]),
+    "gkc_stream": (C.c_void_p, [C.c_void_p]),
+    "gkc_synchronize": (C.c_int, [C.c_void_p]),
+    "gkc_superkmer_arrays": (C.c_int, [C.c_void_p, C.c_void_p, _u64p, C.c_int64, _u32p, _i32p, C.c_void_p, _i32p, _i32p]),
+    "gkc_partition": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_uint32]),
+    "gkc_reset": (C.c_int, [C.c_void_p]),
+    "gkc_count": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "gkc_table_info": (C.c_int, [C.c_void_p, _u64p, _u64p, _u64p]),
+    "gkc_table_copy": (C.c_int64, [C.c_void_p, C.c_int32, _u64p, _u64p, _u32p, C.c_uint64]),
+    "gkc_table_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(_u32p), _u64p]),
+    "gkc_count_packed": (C.c_int, [C.c_void_p, _u32p, C.c_uint64, _u32p, C.c_uint64, _u64p, _u64p, _u32p, C.c_uint64,
+                                   _u64p, _u64p]),
+    "gkc_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "gkc_sketch": (C.c_int, [C.c_void_p, _f32p]),
+    "gkc_set_table_caps": (C.c_int, [C.c_void_p, _u32p]),
+    "gkc_part_kmers": (C.c_int, [C.c_void_p, _u64p]),
+}
+
+_bound = None
+
+
+def lib() -> C.CDLL:
+    global _bound
+    if _bound is None:
+        L = _lib.hip()
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _bound = L
+    return _bound
+
+
+class GkcError(RuntimeError):
+    pass
+
+
+def _check(rc: int) -> int:
+    if rc < 0:
+        raise GkcError(f"gkc error {rc}: {lib().gkc_last_error().decode()}")
+    return rc
+
+
+def _p(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+class Counter:
+    """One counting context on one GPU."""
+
+    def __init__(self, k: int, p: int, n_parts: int, hpc: bool = False, filter_mode: int = 1, part_mode: int = 0,
+                 min_count: int = 1, max_count: int = 0, device: int = 0):
+        self.k, self.p, self.n_parts = k, p, n_parts
+        self.prm = Params(k, p, int(hpc), n_parts, filter_mode, part_mode, min_count, max_count)
+        h = lib().gkc_create(device, C.byref(self.prm))
+        if not h:
+            raise GkcError(lib().gkc_last_error().decode())
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().gkc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- phase 1
+    def partition(self, reads, offs, n: int | None = None, flags: int = 0) -> None:
+        """reads/offs: host numpy arrays, or device addresses (int) with DEVICE_INPUT."""
+        if flags & DEVICE_INPUT:
+            _check(lib().gkc_partition(self.h, C.c_void_p(reads), C.c_void_p(offs), n, flags))
+        else:
+            reads = np.ascontiguousarray(reads, np.uint8)
+            offs = np.ascontiguousarray(offs, np.uint64)
+            _check(lib().gkc_partition(self.h, reads.ctypes.data, offs.ctypes.data, len(offs) - 1, flags))
+
+    def reset(self) -> None:
+        _check(lib().gkc_reset(self.h))
+
+    # -- phase 2
+    def count(self, flags: int = 0) -> None:
+        _check(lib().gkc_count(self.h, flags))
+
+    def table_info(self):
+        P = self.n_parts
+        po = np.zeros(P + 1, np.uint64)
+        pd = np.zeros(P, np.uint64)
+        pk = np.zeros(P, np.uint64)
+        _check(lib().gkc_table_info(self.h, _p(po, _u64p), _p(pd, _u64p), _p(pk, _u64p)))
+        return po, pd, pk
+
+    def table(self, part: int = -1):
+        po, pd, pk = self.table_info()
+        n = int(po[-1]) if part < 0 else int(po[part + 1] - po[part])
+        lo = np.zeros(max(n, 1), np.uint64)
+        hi = np.zeros(max(n, 1), np.uint64) if self.k > 32 else None
+        cnt = np.zeros(max(n, 1), np.uint32)
+        got = _check(lib().gkc_table_copy(self.h, part, _p(lo, _u64p), _p(hi, _u64p) if hi is not None else None,
+                                          _p(cnt, _u32p), max(n, 1)))
+        return lo[:got], (None if hi is None else hi[:got]), cnt[:got], po, pd, pk
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(lib().gkc_get_stats(self.h, C.byref(s)))
+        return s.asdict()
+
+    def sketch(self) -> np.ndarray:
+        out = np.zeros((self.n_parts, 256), np.float32)
+        _check(lib().gkc_sketch(self.h, _p(out, _f32p)))
+        return out
+
+    def set_table_caps(self, caps) -> None:
+        caps = np.ascontiguousarray(caps, np.uint32)
+        _check(lib().gkc_set_table_caps(self.h, _p(caps, _u32p)))
+
+    def part_kmers(self) -> np.ndarray:
+        out = np.zeros(self.n_parts, np.uint64)
+        _check(lib().gkc_part_kmers(self.h, _p(out, _u64p)))
+        return out
+
+    def synchronize(self) -> None:
+        _check(lib().gkc_synchronize(self.h))
+
+    # -- drop-in #1 arrays
+    def superkmer_arrays(self, reads, offs) -> dict:
+        reads = np.ascontiguousarray(reads, np.uint8)
+        offs = np.ascontiguousarray(offs, np.uint64)
+        n = len(offs) - 1
+        B = int(offs[-1] - offs[0])
+        mm = np.zeros(max(B, 1), np.uint32)
+        skm = np.zeros(max(B, 1), np.int32)
+        ro = np.zeros(max(B, 1), np.uint8)
+        rl = np.zeros(max(n, 1), np.int32)
+        hp = np.zeros(max(B, 1), np.int32)
+        _check(lib().gkc_superkmer_arrays(self.h, reads.ctypes.data, _p(offs, _u64p), n, _p(mm, _u32p),
+                                          _p(skm, _i32p), ro.ctypes.data, _p(rl, _i32p), _p(hp, _i32p)))
+        return {"minimizers": mm[:B], "superkmer_offs": skm[:B], "reads": ro[:B], "read_len": rl[:n],
+                "hpc_orig_pos": hp[:B]}
+
+    # -- drop-in #2 on packed host superkmers
+    def count_packed(self, words: np.ndarray, lens: np.ndarray, cap: int):
+        words = np.ascontiguousarray(words, np.uint32)
+        lens = np.ascontiguousarray(lens, np.uint32)
+        lo = np.zeros(max(cap, 1), np.uint64)
+        hi = np.zeros(max(cap, 1), np.uint64) if self.k > 32 else None
+        cnt = np.zeros(max(cap, 1), np.uint32)
+        ne = C.c_uint64(0)
+        nd = C.c_uint64(0)
+        _check(lib().gkc_count_packed(self.h, _p(words, _u32p), len(words), _p(lens, _u32p), len(lens),
+                                      _p(lo, _u64p), _p(hi, _u64p) if hi is not None else None, _p(cnt, _u32p),
+                                      max(cap, 1), C.byref(ne), C.byref(nd)))
+        n = ne.value
+        return lo[:n], (None if hi is None else hi[:n]), cnt[:n], nd.value
+
+
+def pack_superkmers(seqs: list[bytes]) -> tuple[np.ndarray, np.ndarray]:
+    """2-bit MSB-first words (16 bases each), every superkmer word-aligned."""
+    code = np.zeros(256, np.uint32)
+    for ch, v in zip(b"ACGTacgt", [0, 1, 2, 3, 0, 1, 2, 3]):
+        code[ch] = v
+    words, lens = [], []
+    for s in seqs:
+        a = code[np.frombuffer(s, np.uint8)]
+        pad = (-len(a)) % 16
+        a = np.concatenate([a, np.zeros(pad, np.uint32)]).reshape(-1, 16)
+        sh = (30 - 2 * np.arange(16)).astype(np.uint32)
+        words.append((a << sh).sum(axis=1, dtype=np.uint64).astype(np.uint32))
+        lens.append(len(s))
+    w = np.concatenate(words) if words else np.zeros(0, np.uint32)
+    return w, np.array(lens, np.uint32)

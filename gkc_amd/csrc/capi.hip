@@ -1,0 +1,775 @@
+// C ABI and host-side driver of the k-mer counting path (gkc_hip.h).
+//
+// A context owns a HIP stream, the device partition bins (2-bit packed
+// superkmers + headers per partition, the device counterpart of the
+// reference's SKMStore, superkmers.hpp:138-187), the key buffers of the
+// radix sort and the (k-mer, count) table.
+//
+// Two forms of the same kernels:
+//   plain    count-then-allocate with host-synchronised sizes: phase 1 runs a
+//            count pass, then the scatter; phase 2 counts one partition at a
+//            time (extract -> passes -> run-length count -> sync -> emit).
+//   batched  (GKC_BATCHED) every partition in one launch per stage from a
+//            device work queue (persistent workgroups claim tiles from
+//            counters), the launch sequence captured once as a hipGraph and
+//            replayed with sizes read from device memory; with GKC_PRESIZED the
+//            table is sized from predicted distinct counts and the count pass
+//            is skipped, an on-device overflow flag triggering the plain
+//            count-then-allocate emission as fallback.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+#include <cmath>
+
+#include "../../include/gkc_hip.h"
+#include "kernels.hpp"
+
+using namespace gkc;
+
+static thread_local std::string g_err;
+
+struct HipError {
+    hipError_t e; std::string where;
+};
+#define HIPX(call)                                                                             \
+    do {                                                                                       \
+        hipError_t _e = (call);                                                                \
+        if (_e != hipSuccess) throw HipError{_e, std::string(#call) + " @" + std::to_string(__LINE__)}; \
+    } while (0)
+struct ArgError { std::string msg; int code; };
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    // grow (contents not kept)
+    void ensure(size_t n) {
+        if (n <= bytes) return;
+        if (p) HIPX(hipFree(p));
+        p = nullptr; bytes = 0;
+        size_t nb = std::max<size_t>(n, 256);
+        hipError_t e = hipMalloc(&p, nb);
+        if (e != hipSuccess) { (void)hipGetLastError(); throw ArgError{"device allocation of " + std::to_string(nb) + " bytes failed", GKC_ENOMEM}; }
+        bytes = nb;
+    }
+    template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace
+
+struct gkc_ctx {
+    std::mutex mu;
+    int device = 0;
+    gkc_params prm{};
+    int keyw = 8;                 // key bytes
+    int npass = 8;
+    int n_cu = 256;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[8]{};
+    hipEvent_t evp[2 * MAX_PASS]{};   // per-pass timing
+
+    // bins
+    DevBuf words, hdrs;
+    std::vector<uint64_t> h_word_base, h_hdr_base;
+    std::vector<uint32_t> h_word_cap, h_hdr_cap, h_word_cur, h_hdr_cur, h_kmer_cur;
+    DevBuf d_word_base, d_hdr_base, d_word_cap, d_hdr_cap, d_word_cur, d_hdr_cur, d_kmer_cur, d_overflow;
+    DevBuf d_tmp_hdr, d_tmp_word, d_tmp_kmer;    // per-batch counts of the count pass
+    DevBuf d_sketch;
+    uint64_t sketch_windows = 0;
+
+    // batch staging
+    DevBuf d_reads, d_offs, d_lens, d_hpc_reads, d_hpc_pos, d_chunk, d_ctr, d_mm;
+
+    // counting
+    DevBuf keysA, keysB, lookback, d_plan, d_segk, d_segh, d_keystart, d_itemstart, d_tilestart, d_rlstart, d_hist;
+    DevBuf d_tstart, d_tcap, d_nent, d_ndist, d_toverflow;
+    DevBuf tkeys, tcnts;
+    std::vector<uint64_t> part_off, part_distinct, part_kmers;
+    std::vector<uint32_t> presize_caps;
+    bool counted = false;
+    gkc_stats stats{};
+};
+
+static int fail(gkc_ctx* c, int code, const std::string& m) {
+    g_err = m;
+    (void)c;
+    return code;
+}
+
+template <typename F>
+static int guarded(gkc_ctx* c, F&& f) {
+    if (!c) { g_err = "null context"; return GKC_EINVAL; }
+    std::lock_guard<std::mutex> lk(c->mu);
+    try {
+        HIPX(hipSetDevice(c->device));
+        return f();
+    } catch (const HipError& e) {
+        (void)hipGetLastError();
+        return fail(c, GKC_EHIP, std::string("HIP error ") + hipGetErrorString(e.e) + " in " + e.where);
+    } catch (const ArgError& e) {
+        return fail(c, e.code, e.msg);
+    }
+}
+
+static int occupancy_blocks(const void* fn, int nt, int n_cu) {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, nt, 0) != hipSuccess || b < 1) { (void)hipGetLastError(); b = 1; }
+    return b * n_cu;
+}
+
+// ---------------------------------------------------------------- bins
+
+static void bins_init(gkc_ctx* c) {
+    const uint32_t P = c->prm.n_parts;
+    c->h_word_base.assign(P, 0); c->h_hdr_base.assign(P, 0);
+    c->h_word_cap.assign(P, 0); c->h_hdr_cap.assign(P, 0);
+    c->h_word_cur.assign(P, 0); c->h_hdr_cur.assign(P, 0); c->h_kmer_cur.assign(P, 0);
+    c->d_word_base.ensure(P * 8); c->d_hdr_base.ensure(P * 8);
+    c->d_word_cap.ensure(P * 4); c->d_hdr_cap.ensure(P * 4);
+    c->d_word_cur.ensure(P * 4); c->d_hdr_cur.ensure(P * 4); c->d_kmer_cur.ensure(P * 4);
+    c->d_tmp_hdr.ensure(P * 4); c->d_tmp_word.ensure(P * 4); c->d_tmp_kmer.ensure(P * 4);
+    c->d_overflow.ensure(16);
+    c->d_sketch.ensure((size_t)P * 256 * 4);
+    HIPX(hipMemsetAsync(c->d_word_cur.p, 0, P * 4, c->st));
+    HIPX(hipMemsetAsync(c->d_hdr_cur.p, 0, P * 4, c->st));
+    HIPX(hipMemsetAsync(c->d_kmer_cur.p, 0, P * 4, c->st));
+    HIPX(hipMemsetAsync(c->d_word_cap.p, 0, P * 4, c->st));
+    HIPX(hipMemsetAsync(c->d_hdr_cap.p, 0, P * 4, c->st));
+    HIPX(hipMemsetAsync(c->d_sketch.p, 0, (size_t)P * 256 * 4, c->st));
+    HIPX(hipMemsetAsync(c->d_overflow.p, 0, 16, c->st));
+    c->sketch_windows = 0;
+    c->counted = false;
+}
+
+// Make room for `need_w` / `need_h` more words / headers per partition
+// (cursor-relative).  Regrows all regions with slack and copies contents.
+static void bins_reserve(gkc_ctx* c, const std::vector<uint32_t>& need_w, const std::vector<uint32_t>& need_h) {
+    const uint32_t P = c->prm.n_parts;
+    bool grow = false;
+    for (uint32_t q = 0; q < P; q++)
+        if ((uint64_t)c->h_word_cur[q] + need_w[q] > c->h_word_cap[q] || (uint64_t)c->h_hdr_cur[q] + need_h[q] > c->h_hdr_cap[q]) grow = true;
+    if (!grow) return;
+    std::vector<uint64_t> nwb(P), nhb(P);
+    std::vector<uint32_t> nwc(P), nhc(P);
+    uint64_t tw = 0, th = 0;
+    for (uint32_t q = 0; q < P; q++) {
+        uint64_t ww = (uint64_t)c->h_word_cur[q] + need_w[q], hh = (uint64_t)c->h_hdr_cur[q] + need_h[q];
+        if (ww > c->h_word_cap[q]) ww = ww + ww / 4 + 64; else ww = c->h_word_cap[q];
+        if (hh > c->h_hdr_cap[q]) hh = hh + hh / 4 + 16; else hh = c->h_hdr_cap[q];
+        if (ww >= (1ull << 32) || hh >= (1ull << 32)) throw ArgError{"partition bin exceeds 2^32 words", GKC_EINVAL};
+        nwb[q] = tw; nhb[q] = th; nwc[q] = (uint32_t)ww; nhc[q] = (uint32_t)hh;
+        tw += ww; th += hh;
+    }
+    DevBuf nw, nh;
+    nw.ensure((tw + 8) * 4);      // +8 words: extraction reads up to 4 words past a k-mer
+    nh.ensure(th * sizeof(SkmHdr) + 16);
+    for (uint32_t q = 0; q < P; q++) {
+        if (c->h_word_cur[q]) HIPX(hipMemcpyAsync(nw.as<uint32_t>() + nwb[q], c->words.as<uint32_t>() + c->h_word_base[q],
+                                                  (size_t)c->h_word_cur[q] * 4, hipMemcpyDeviceToDevice, c->st));
+        if (c->h_hdr_cur[q]) HIPX(hipMemcpyAsync(nh.as<SkmHdr>() + nhb[q], c->hdrs.as<SkmHdr>() + c->h_hdr_base[q],
+                                                 (size_t)c->h_hdr_cur[q] * sizeof(SkmHdr), hipMemcpyDeviceToDevice, c->st));
+    }
+    HIPX(hipMemsetAsync(static_cast<char*>(nw.p) + tw * 4, 0, 32, c->st));
+    HIPX(hipStreamSynchronize(c->st));
+    std::swap(c->words.p, nw.p); std::swap(c->words.bytes, nw.bytes);
+    std::swap(c->hdrs.p, nh.p); std::swap(c->hdrs.bytes, nh.bytes);
+    c->h_word_base = nwb; c->h_hdr_base = nhb; c->h_word_cap = nwc; c->h_hdr_cap = nhc;
+    HIPX(hipMemcpyAsync(c->d_word_base.p, nwb.data(), P * 8, hipMemcpyHostToDevice, c->st));
+    HIPX(hipMemcpyAsync(c->d_hdr_base.p, nhb.data(), P * 8, hipMemcpyHostToDevice, c->st));
+    HIPX(hipMemcpyAsync(c->d_word_cap.p, nwc.data(), P * 4, hipMemcpyHostToDevice, c->st));
+    HIPX(hipMemcpyAsync(c->d_hdr_cap.p, nhc.data(), P * 4, hipMemcpyHostToDevice, c->st));
+    HIPX(hipStreamSynchronize(c->st));
+}
+
+static BinSet binset(gkc_ctx* c, bool counting) {
+    BinSet b;
+    b.words = c->words.as<uint32_t>(); b.hdrs = c->hdrs.as<SkmHdr>();
+    b.word_base = c->d_word_base.as<uint64_t>(); b.hdr_base = c->d_hdr_base.as<uint64_t>();
+    b.word_cap = c->d_word_cap.as<uint32_t>(); b.hdr_cap = c->d_hdr_cap.as<uint32_t>();
+    if (counting) {
+        b.word_cur = c->d_tmp_word.as<uint32_t>(); b.hdr_cur = c->d_tmp_hdr.as<uint32_t>(); b.kmer_cur = c->d_tmp_kmer.as<uint32_t>();
+    } else {
+        b.word_cur = c->d_word_cur.as<uint32_t>(); b.hdr_cur = c->d_hdr_cur.as<uint32_t>(); b.kmer_cur = c->d_kmer_cur.as<uint32_t>();
+    }
+    b.overflow = c->d_overflow.as<uint32_t>();
+    return b;
+}
+
+// Stage a batch on the device; returns the number of bases. Fills d_chunk.
+static uint64_t stage_batch(gkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n, uint32_t flags,
+                            ChunkDesc& cd) {
+    uint64_t o0, on;
+    const uint8_t* dreads;
+    const uint64_t* doffs;
+    if (flags & GKC_DEVICE_INPUT) {
+        HIPX(hipMemcpy(&o0, offs, 8, hipMemcpyDeviceToHost));
+        HIPX(hipMemcpy(&on, offs + n, 8, hipMemcpyDeviceToHost));
+        if (reinterpret_cast<uintptr_t>(reads) & 15) throw ArgError{"device reads must be 16-byte aligned", GKC_EINVAL};
+        dreads = reinterpret_cast<const uint8_t*>(reads);
+        doffs = offs;
+    } else {
+        o0 = offs[0]; on = offs[n];
+        const uint64_t B = on - o0;
+        c->d_reads.ensure(B + 64);
+        c->d_offs.ensure((n + 1) * 8);
+        if (B) HIPX(hipMemcpyAsync(c->d_reads.p, reads, B, hipMemcpyHostToDevice, c->st));
+        HIPX(hipMemcpyAsync(c->d_offs.p, offs, (n + 1) * 8, hipMemcpyHostToDevice, c->st));
+        dreads = c->d_reads.as<uint8_t>();
+        doffs = c->d_offs.as<uint64_t>();
+    }
+    const uint64_t B = on - o0;
+    if (B >= (1ull << 31)) throw ArgError{"a batch must hold fewer than 2^31 bases", GKC_EINVAL};
+    cd.reads = dreads; cd.offs = doffs; cd.lens = nullptr; cd.base = o0;
+    cd.n_reads = (uint32_t)n; cd.n_bases = (uint32_t)B; cd.n_tiles = cdiv(B, P1_T); cd.pad = 0;
+    if (c->prm.hpc && n > 0) {
+        c->d_hpc_reads.ensure(B + 64);
+        c->d_lens.ensure((n + 1) * 4);
+        c->d_hpc_pos.ensure((B + 1) * 4);
+        int blocks = std::min<int64_t>((n + 3) / 4, 65535);
+        hipLaunchKernelGGL(k_hpc, dim3(blocks), dim3(256), 0, c->st, dreads, doffs, (uint32_t)n,
+                           c->d_hpc_reads.as<uint8_t>(), c->d_lens.as<int32_t>(), c->d_hpc_pos.as<int32_t>());
+        HIPX(hipGetLastError());
+        cd.reads = c->d_hpc_reads.as<uint8_t>();
+        cd.lens = c->d_lens.as<int32_t>();
+    }
+    c->d_chunk.ensure(sizeof(ChunkDesc));
+    hipLaunchKernelGGL(k_set_chunk, dim3(1), dim3(1), 0, c->st, c->d_chunk.as<ChunkDesc>(), cd);
+    HIPX(hipGetLastError());
+    return B;
+}
+
+static void launch_superkmers(gkc_ctx* c, const ChunkDesc& cd, P1Params prm, BinSet bs) {
+    c->d_ctr.ensure(64);
+    HIPX(hipMemsetAsync(c->d_ctr.p, 0, 4, c->st));
+    if (cd.n_tiles == 0) return;
+    const bool wide = c->prm.p > 16;
+    const void* fn = wide ? (const void*)k_superkmers<uint64_t> : (const void*)k_superkmers<uint32_t>;
+    int grid = std::min<int>(cd.n_tiles, occupancy_blocks(fn, P1_NT, c->n_cu));
+    if (wide)
+        hipLaunchKernelGGL(k_superkmers<uint64_t>, dim3(grid), dim3(P1_NT), 0, c->st, c->d_chunk.as<ChunkDesc>(), prm, bs, c->d_ctr.as<uint32_t>());
+    else
+        hipLaunchKernelGGL(k_superkmers<uint32_t>, dim3(grid), dim3(P1_NT), 0, c->st, c->d_chunk.as<ChunkDesc>(), prm, bs, c->d_ctr.as<uint32_t>());
+    HIPX(hipGetLastError());
+}
+
+static P1Params p1params(gkc_ctx* c, int scatter) {
+    P1Params p;
+    p.k = c->prm.k; p.p = c->prm.p; p.fmode = c->prm.filter_mode; p.pmode = c->prm.part_mode;
+    p.P = c->prm.n_parts; p.scatter = scatter; p.sketch = nullptr; p.mm_out = nullptr;
+    return p;
+}
+
+// ---------------------------------------------------------------- counting
+
+template <typename K>
+static void count_impl(gkc_ctx* c, uint32_t flags);
+
+extern "C" {
+
+const char* gkc_last_error(void) { return g_err.c_str(); }
+int gkc_version(void) { return 10000; }
+
+gkc_ctx* gkc_create(int device, const gkc_params* prm) {
+    if (!prm) { g_err = "null params"; return nullptr; }
+    const gkc_params& q = *prm;
+    if (q.k < 1 || q.k > 64 || q.p < 3 || q.p > 32 || q.p > q.k || q.n_parts < 1 || q.n_parts > 1024 ||
+        (q.filter_mode != 0 && q.filter_mode != 1) || (q.part_mode != 0 && q.part_mode != 1)) {
+        g_err = "invalid parameters (need 1<=k<=64, 3<=p<=min(k,32), 1<=n_parts<=1024, modes 0/1)";
+        return nullptr;
+    }
+    gkc_ctx* c = new gkc_ctx();
+    c->device = device;
+    c->prm = q;
+    if (c->prm.min_count == 0) c->prm.min_count = 1;
+    c->keyw = q.k <= 32 ? 8 : 16;
+    c->npass = (2 * q.k + 7) / 8;
+    try {
+        HIPX(hipSetDevice(device));
+        hipDeviceProp_t pr;
+        HIPX(hipGetDeviceProperties(&pr, device));
+        c->n_cu = pr.multiProcessorCount;
+        HIPX(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+        for (auto& e : c->ev) HIPX(hipEventCreate(&e));
+        for (auto& e : c->evp) HIPX(hipEventCreate(&e));
+        bins_init(c);
+        HIPX(hipStreamSynchronize(c->st));
+    } catch (const HipError& e) {
+        (void)hipGetLastError();
+        g_err = std::string("HIP error ") + hipGetErrorString(e.e) + " in " + e.where;
+        delete c;
+        return nullptr;
+    } catch (const ArgError& e) {
+        g_err = e.msg;
+        delete c;
+        return nullptr;
+    }
+    return c;
+}
+
+void gkc_destroy(gkc_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->evp) if (e) (void)hipEventDestroy(e);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+void* gkc_stream(gkc_ctx* c) { return c ? (void*)c->st : nullptr; }
+
+int gkc_synchronize(gkc_ctx* c) {
+    return guarded(c, [&] { HIPX(hipStreamSynchronize(c->st)); return 0; });
+}
+
+int gkc_reset(gkc_ctx* c) {
+    return guarded(c, [&] {
+        bins_init(c);
+        HIPX(hipStreamSynchronize(c->st));
+        return 0;
+    });
+}
+
+int gkc_superkmer_arrays(gkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n, uint32_t* minimizers,
+                         int32_t* skm_offs, char* reads_out, int32_t* read_len, int32_t* hpc_orig_pos) {
+    return guarded(c, [&] {
+        if (n < 0 || !offs || (n > 0 && !reads)) throw ArgError{"bad batch", GKC_EINVAL};
+        if (c->prm.p > 16) throw ArgError{"the reference's u32 minimizer arrays need p <= 16", GKC_EINVAL};
+        if (n == 0) return 0;
+        ChunkDesc cd;
+        const uint64_t B = stage_batch(c, reads, offs, n, 0, cd);
+        c->d_mm.ensure((B + 1) * 4);
+        HIPX(hipMemsetAsync(c->d_mm.p, 0, (B + 1) * 4, c->st));
+        P1Params prm = p1params(c, 0);
+        prm.mm_out = c->d_mm.as<uint32_t>();
+        BinSet bs = binset(c, true);
+        HIPX(hipMemsetAsync(c->d_tmp_hdr.p, 0, c->prm.n_parts * 4, c->st));
+        HIPX(hipMemsetAsync(c->d_tmp_word.p, 0, c->prm.n_parts * 4, c->st));
+        HIPX(hipMemsetAsync(c->d_tmp_kmer.p, 0, c->prm.n_parts * 4, c->st));
+        launch_superkmers(c, cd, prm, bs);
+        // superkmer lists (reuse d_hpc_pos-sized scratch: i32 per base)
+        DevBuf d_skm;
+        d_skm.ensure((B + 1) * 4);
+        HIPX(hipMemsetAsync(d_skm.p, 0, (B + 1) * 4, c->st));
+        int blocks = std::min<int64_t>((n + 3) / 4, 65535);
+        hipLaunchKernelGGL(k_skm_lists, dim3(blocks), dim3(256), 0, c->st, cd.offs, cd.lens, (uint32_t)n, c->prm.k,
+                           c->d_mm.as<uint32_t>(), d_skm.as<int32_t>());
+        HIPX(hipGetLastError());
+        if (minimizers) HIPX(hipMemcpyAsync(minimizers, c->d_mm.p, B * 4, hipMemcpyDeviceToHost, c->st));
+        if (skm_offs) HIPX(hipMemcpyAsync(skm_offs, d_skm.p, B * 4, hipMemcpyDeviceToHost, c->st));
+        if (reads_out) HIPX(hipMemcpyAsync(reads_out, cd.reads, B, hipMemcpyDeviceToHost, c->st));
+        if (c->prm.hpc) {
+            if (read_len) HIPX(hipMemcpyAsync(read_len, c->d_lens.p, n * 4, hipMemcpyDeviceToHost, c->st));
+            if (hpc_orig_pos) HIPX(hipMemcpyAsync(hpc_orig_pos, c->d_hpc_pos.p, B * 4, hipMemcpyDeviceToHost, c->st));
+        }
+        HIPX(hipStreamSynchronize(c->st));
+        return 0;
+    });
+}
+
+int gkc_partition(gkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n, uint32_t flags) {
+    return guarded(c, [&] {
+        if (n < 0 || !offs || (n > 0 && !reads)) throw ArgError{"bad batch", GKC_EINVAL};
+        if (n == 0) return 0;
+        const uint32_t P = c->prm.n_parts;
+        ChunkDesc cd;
+        HIPX(hipEventRecord(c->ev[0], c->st));
+        stage_batch(c, reads, offs, n, flags, cd);
+        // count pass: per-partition sizes of this batch
+        HIPX(hipMemsetAsync(c->d_tmp_hdr.p, 0, P * 4, c->st));
+        HIPX(hipMemsetAsync(c->d_tmp_word.p, 0, P * 4, c->st));
+        HIPX(hipMemsetAsync(c->d_tmp_kmer.p, 0, P * 4, c->st));
+        P1Params pc = p1params(c, 0);
+        pc.sketch = c->d_sketch.as<uint32_t>();
+        launch_superkmers(c, cd, pc, binset(c, true));
+        std::vector<uint32_t> nh(P), nw(P), nk(P);
+        HIPX(hipMemcpyAsync(nh.data(), c->d_tmp_hdr.p, P * 4, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipMemcpyAsync(nw.data(), c->d_tmp_word.p, P * 4, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipMemcpyAsync(nk.data(), c->d_tmp_kmer.p, P * 4, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipStreamSynchronize(c->st));
+        for (uint32_t q = 0; q < P; q++)
+            if ((uint64_t)c->h_kmer_cur[q] + nk[q] >= (1ull << 30)) throw ArgError{"partition exceeds 2^30 k-mers; use more partitions", GKC_EINVAL};
+        bins_reserve(c, nw, nh);
+        // scatter pass
+        launch_superkmers(c, cd, p1params(c, 1), binset(c, false));
+        HIPX(hipEventRecord(c->ev[1], c->st));
+        HIPX(hipStreamSynchronize(c->st));
+        uint32_t ovf = 0;
+        HIPX(hipMemcpy(&ovf, c->d_overflow.p, 4, hipMemcpyDeviceToHost));
+        if (ovf) throw ArgError{"internal: bin overflow after exact reservation", GKC_ESTATE};
+        for (uint32_t q = 0; q < P; q++) { c->h_hdr_cur[q] += nh[q]; c->h_word_cur[q] += nw[q]; c->h_kmer_cur[q] += nk[q]; }
+        float ms = 0;
+        HIPX(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        c->stats.ms_partition = ms;
+        c->counted = false;
+        return 0;
+    });
+}
+
+int gkc_count(gkc_ctx* c, uint32_t flags) {
+    return guarded(c, [&] {
+        if (c->keyw == 8) count_impl<uint64_t>(c, flags);
+        else count_impl<u128k>(c, flags);
+        c->counted = true;
+        return 0;
+    });
+}
+
+int gkc_table_info(gkc_ctx* c, uint64_t* part_off, uint64_t* part_distinct, uint64_t* part_kmers) {
+    return guarded(c, [&] {
+        if (!c->counted) throw ArgError{"gkc_count has not run", GKC_ESTATE};
+        const uint32_t P = c->prm.n_parts;
+        if (part_off) std::memcpy(part_off, c->part_off.data(), (P + 1) * 8);
+        if (part_distinct) std::memcpy(part_distinct, c->part_distinct.data(), P * 8);
+        if (part_kmers) std::memcpy(part_kmers, c->part_kmers.data(), P * 8);
+        return 0;
+    });
+}
+
+int64_t gkc_table_copy(gkc_ctx* c, int32_t part, uint64_t* key_lo, uint64_t* key_hi, uint32_t* cnt, uint64_t cap) {
+    int64_t ret = 0;
+    int rc = guarded(c, [&] {
+        if (!c->counted) throw ArgError{"gkc_count has not run", GKC_ESTATE};
+        const uint32_t P = c->prm.n_parts;
+        if (part >= (int32_t)P) throw ArgError{"partition out of range", GKC_EINVAL};
+        const uint64_t a = part < 0 ? 0 : c->part_off[part], b = part < 0 ? c->part_off[P] : c->part_off[part + 1];
+        const uint64_t n = b - a;
+        if (n > cap) throw ArgError{"table copy buffer too small", GKC_ECAP};
+        if (n) {
+            if (c->keyw == 8) {
+                HIPX(hipMemcpyAsync(key_lo, c->tkeys.as<uint64_t>() + a, n * 8, hipMemcpyDeviceToHost, c->st));
+            } else {
+                std::vector<u128k> tmp(n);
+                HIPX(hipMemcpyAsync(tmp.data(), c->tkeys.as<u128k>() + a, n * 16, hipMemcpyDeviceToHost, c->st));
+                HIPX(hipStreamSynchronize(c->st));
+                for (uint64_t i = 0; i < n; i++) { key_lo[i] = tmp[i].lo; if (key_hi) key_hi[i] = tmp[i].hi; }
+            }
+            if (cnt) HIPX(hipMemcpyAsync(cnt, c->tcnts.as<uint32_t>() + a, n * 4, hipMemcpyDeviceToHost, c->st));
+            HIPX(hipStreamSynchronize(c->st));
+        }
+        ret = (int64_t)n;
+        return 0;
+    });
+    return rc < 0 ? rc : ret;
+}
+
+int gkc_table_device(gkc_ctx* c, void** keys, uint32_t** counts, uint64_t* n_entries) {
+    return guarded(c, [&] {
+        if (!c->counted) throw ArgError{"gkc_count has not run", GKC_ESTATE};
+        if (keys) *keys = c->tkeys.p;
+        if (counts) *counts = c->tcnts.as<uint32_t>();
+        if (n_entries) *n_entries = c->part_off[c->prm.n_parts];
+        return 0;
+    });
+}
+
+int gkc_get_stats(gkc_ctx* c, gkc_stats* out) {
+    return guarded(c, [&] { if (out) *out = c->stats; return 0; });
+}
+
+int gkc_sketch(gkc_ctx* c, float* out) {
+    return guarded(c, [&] {
+        const uint32_t P = c->prm.n_parts;
+        std::vector<uint32_t> h((size_t)P * 256);
+        HIPX(hipMemcpy(h.data(), c->d_sketch.p, h.size() * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < h.size(); i++) out[i] = std::log1p((float)h[i]);
+        return 0;
+    });
+}
+
+int gkc_set_table_caps(gkc_ctx* c, const uint32_t* caps) {
+    return guarded(c, [&] {
+        c->presize_caps.assign(caps, caps + c->prm.n_parts);
+        return 0;
+    });
+}
+
+int gkc_part_kmers(gkc_ctx* c, uint64_t* out) {
+    return guarded(c, [&] {
+        for (uint32_t q = 0; q < c->prm.n_parts; q++) out[q] = c->h_kmer_cur[q];
+        return 0;
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- counting impl
+
+template <typename K>
+static void ensure_count_buffers(gkc_ctx* c, uint64_t max_keys, uint32_t n_segs) {
+    const size_t kw = sizeof(K);
+    c->keysA.ensure((max_keys + 1) * kw);
+    c->keysB.ensure((max_keys + 1) * kw);
+    const uint64_t tiles = max_keys / (SORT_NT * (kw == 8 ? 16 : 8)) + n_segs + 1;
+    c->lookback.ensure(tiles * RADIX * 4);
+    c->d_plan.ensure(sizeof(Plan));
+    c->d_keystart.ensure((n_segs + 1) * 8);
+    c->d_itemstart.ensure((n_segs + 1) * 4);
+    c->d_tilestart.ensure((n_segs + 1) * 4);
+    c->d_rlstart.ensure((n_segs + 1) * 4);
+    c->d_hist.ensure((size_t)n_segs * MAX_PASS * RADIX * 4);
+    c->d_tstart.ensure(n_segs * 8 + 8);
+    c->d_tcap.ensure(n_segs * 4 + 4);
+    c->d_nent.ensure(n_segs * 4 + 4);
+    c->d_ndist.ensure(n_segs * 4 + 4);
+    c->d_toverflow.ensure(16);
+}
+
+static void zero(gkc_ctx* c, void* p, uint64_t n32) {
+    if (!n32) return;
+    int blocks = (int)std::min<uint64_t>((n32 + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_zero32, dim3(blocks), dim3(256), 0, c->st, static_cast<uint32_t*>(p), n32);
+    HIPX(hipGetLastError());
+}
+
+// Run extraction + sort for a plan whose per-segment arrays are set; the
+// sorted keys end in keysA or keysB (returned).  Host knows n_keys_total and
+// the sort tile count upper bound.
+template <typename K>
+static K* sort_segments(gkc_ctx* c, SegArrays sa, const uint64_t* hdr_base, const uint64_t* word_base,
+                        uint32_t n_segs, uint64_t n_keys, uint64_t n_hdrs, float* ms_ex, float* ms_sort, float* ms_pass_sum) {
+    Plan* plan = c->d_plan.as<Plan>();
+    const int TILE = SORT_NT * (sizeof(K) == 8 ? 16 : 8);
+    hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(64), 0, c->st, plan, n_segs, (uint32_t)c->npass);
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, c->st, plan, sa, TILE);
+    HIPX(hipGetLastError());
+    zero(c, sa.hist, (uint64_t)n_segs * MAX_PASS * RADIX);
+    HIPX(hipEventRecord(c->ev[2], c->st));
+    const uint64_t items = n_hdrs / EX_HDRS + n_segs;
+    int g_ex = (int)std::min<uint64_t>(items, occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu));
+    hipLaunchKernelGGL(k_extract<K>, dim3(std::max(g_ex, 1)), dim3(SORT_NT), 0, c->st, plan, sa, c->hdrs.as<SkmHdr>(), hdr_base,
+                       c->words.as<uint32_t>(), word_base, c->keysA.as<K>(), c->prm.k);
+    HIPX(hipGetLastError());
+    hipLaunchKernelGGL(k_hist_scan, dim3(std::max<uint32_t>(1, (n_segs * MAX_PASS + 3) / 4)), dim3(256), 0, c->st, plan, sa.hist);
+    HIPX(hipGetLastError());
+    HIPX(hipEventRecord(c->ev[3], c->st));
+    const uint64_t tiles = n_keys / TILE + n_segs;
+    int g_s = (int)std::min<uint64_t>(tiles, occupancy_blocks((const void*)k_onesweep<K>, SORT_NT, c->n_cu));
+    K* src = c->keysA.as<K>();
+    K* dst = c->keysB.as<K>();
+    for (int ps = 0; ps < c->npass; ps++) {
+        zero(c, c->lookback.p, tiles * RADIX);
+        HIPX(hipEventRecord(c->evp[2 * ps], c->st));
+        hipLaunchKernelGGL(k_onesweep<K>, dim3(std::max(g_s, 1)), dim3(SORT_NT), 0, c->st, plan, sa, src, dst, c->lookback.as<uint32_t>(), ps);
+        HIPX(hipGetLastError());
+        HIPX(hipEventRecord(c->evp[2 * ps + 1], c->st));
+        std::swap(src, dst);
+    }
+    HIPX(hipEventRecord(c->ev[6], c->st));
+    HIPX(hipEventSynchronize(c->ev[6]));
+    float pass_sum = 0;
+    for (int ps = 0; ps < c->npass; ps++) {
+        float m = 0;
+        HIPX(hipEventElapsedTime(&m, c->evp[2 * ps], c->evp[2 * ps + 1]));
+        pass_sum += m;
+    }
+    float a = 0, b = 0;
+    HIPX(hipEventElapsedTime(&a, c->ev[2], c->ev[3]));
+    HIPX(hipEventElapsedTime(&b, c->ev[3], c->ev[6]));
+    if (ms_ex) *ms_ex += a;
+    if (ms_sort) *ms_sort += b;
+    if (ms_pass_sum) *ms_pass_sum += pass_sum;
+    return src;
+}
+
+template <typename K>
+static void table_reserve(gkc_ctx* c, uint64_t n) {
+    c->tkeys.ensure((n + 1) * sizeof(K));
+    c->tcnts.ensure((n + 1) * 4);
+}
+
+// Count one segment whose keys are sorted in `sorted`: run-length count pass,
+// host sync, table growth, emit pass at table offset `out_off`.
+template <typename K>
+static void table_grow_keep(gkc_ctx* c, uint64_t used, uint64_t need) {
+    if (need + 1 <= c->tkeys.bytes / sizeof(K) && need + 1 <= c->tcnts.bytes / 4) return;
+    DevBuf nk, nc;
+    uint64_t ncap = need * 3 / 2 + 1024;
+    nk.ensure((ncap + 1) * sizeof(K)); nc.ensure((ncap + 1) * 4);
+    if (used) {
+        HIPX(hipMemcpyAsync(nk.p, c->tkeys.p, used * sizeof(K), hipMemcpyDeviceToDevice, c->st));
+        HIPX(hipMemcpyAsync(nc.p, c->tcnts.p, used * 4, hipMemcpyDeviceToDevice, c->st));
+    }
+    HIPX(hipStreamSynchronize(c->st));
+    std::swap(c->tkeys.p, nk.p); std::swap(c->tkeys.bytes, nk.bytes);
+    std::swap(c->tcnts.p, nc.p); std::swap(c->tcnts.bytes, nc.bytes);
+}
+
+template <typename K>
+static void runlength_one(gkc_ctx* c, SegArrays sa, const K* sorted, uint64_t nkeys, uint64_t out_off,
+                          uint32_t* ne_out, uint32_t* nd_out, float* ms_rl) {
+    HIPX(hipEventRecord(c->ev[2], c->st));
+    TableOut to;
+    to.keys = nullptr; to.counts = nullptr;
+    to.table_start = c->d_tstart.as<uint64_t>(); to.table_cap = c->d_tcap.as<uint32_t>();
+    to.n_entries = c->d_nent.as<uint32_t>(); to.n_distinct = c->d_ndist.as<uint32_t>();
+    to.overflow = c->d_toverflow.as<uint32_t>();
+    to.min_cnt = c->prm.min_count; to.max_cnt = c->prm.max_count; to.emit = 0;
+    const uint64_t rl_tiles = nkeys / RL_TILE + 1;
+    int g_r = (int)std::min<uint64_t>(rl_tiles, occupancy_blocks((const void*)k_runlength<K>, SORT_NT, c->n_cu));
+    Plan* plan = c->d_plan.as<Plan>();
+    zero(c, c->d_nent.p, 1); zero(c, c->d_ndist.p, 1); zero(c, c->d_toverflow.p, 1);
+    zero(c, c->lookback.p, rl_tiles);
+    zero(c, &plan->ctr[1 + MAX_PASS], 1);
+    hipLaunchKernelGGL(k_runlength<K>, dim3(g_r), dim3(SORT_NT), 0, c->st, plan, sa, sorted, to, c->lookback.as<uint32_t>());
+    HIPX(hipGetLastError());
+    uint32_t ne = 0, nd = 0;
+    HIPX(hipMemcpyAsync(&ne, c->d_nent.p, 4, hipMemcpyDeviceToHost, c->st));
+    HIPX(hipMemcpyAsync(&nd, c->d_ndist.p, 4, hipMemcpyDeviceToHost, c->st));
+    HIPX(hipStreamSynchronize(c->st));
+    c->stats.host_syncs++;
+    table_grow_keep<K>(c, out_off, out_off + ne);
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, c->st, c->d_tstart.as<uint64_t>(), (uint64_t)out_off);
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, c->st, c->d_tcap.as<uint32_t>(), ne);
+    to.emit = 1; to.keys = c->tkeys.p; to.counts = c->tcnts.as<uint32_t>();
+    zero(c, c->d_nent.p, 1); zero(c, c->d_ndist.p, 1);
+    zero(c, c->lookback.p, rl_tiles);
+    zero(c, &plan->ctr[1 + MAX_PASS], 1);
+    hipLaunchKernelGGL(k_runlength<K>, dim3(g_r), dim3(SORT_NT), 0, c->st, plan, sa, sorted, to, c->lookback.as<uint32_t>());
+    HIPX(hipGetLastError());
+    HIPX(hipEventRecord(c->ev[3], c->st));
+    HIPX(hipEventSynchronize(c->ev[3]));
+    float m = 0;
+    HIPX(hipEventElapsedTime(&m, c->ev[2], c->ev[3]));
+    if (ms_rl) *ms_rl += m;
+    *ne_out = ne; *nd_out = nd;
+}
+
+static SegArrays seg_arrays_single(gkc_ctx* c, const uint32_t* seg_keys, const uint32_t* seg_hdrs) {
+    SegArrays sa;
+    sa.seg_keys = seg_keys; sa.seg_hdrs = seg_hdrs;
+    sa.key_start = c->d_keystart.as<uint64_t>();
+    sa.item_start = c->d_itemstart.as<uint32_t>();
+    sa.sort_tile_start = c->d_tilestart.as<uint32_t>();
+    sa.rl_tile_start = c->d_rlstart.as<uint32_t>();
+    sa.hist = c->d_hist.as<uint32_t>();
+    return sa;
+}
+
+template <typename K>
+static void count_impl(gkc_ctx* c, uint32_t flags) {
+    const uint32_t P = c->prm.n_parts;
+    (void)flags;
+    c->part_off.assign(P + 1, 0);
+    c->part_distinct.assign(P, 0);
+    c->part_kmers.assign(P, 0);
+    uint64_t max_keys = 0, tot_keys = 0, tot_hdrs = 0, tot_words = 0;
+    for (uint32_t q = 0; q < P; q++) {
+        c->part_kmers[q] = c->h_kmer_cur[q];
+        max_keys = std::max<uint64_t>(max_keys, c->h_kmer_cur[q]);
+        tot_keys += c->h_kmer_cur[q];
+        tot_hdrs += c->h_hdr_cur[q];
+        tot_words += c->h_word_cur[q];
+    }
+    c->stats = gkc_stats{};
+    c->stats.n_kmers = tot_keys; c->stats.n_superkmers = tot_hdrs; c->stats.n_words = tot_words;
+    c->stats.n_pass = c->npass;
+    HIPX(hipEventRecord(c->ev[0], c->st));
+    // ---- plain: one partition at a time, count-then-allocate ----
+    ensure_count_buffers<K>(c, max_keys, 1);
+    table_grow_keep<K>(c, 0, std::max<uint64_t>(tot_keys / 4, 1024));
+    float ms_ex = 0, ms_sort = 0, ms_pass = 0, ms_rl = 0;
+    uint64_t out_off = 0;
+    for (uint32_t q = 0; q < P; q++) {
+        c->part_off[q] = out_off;
+        if (c->h_kmer_cur[q] == 0) continue;
+        SegArrays sa = seg_arrays_single(c, c->d_kmer_cur.as<uint32_t>() + q, c->d_hdr_cur.as<uint32_t>() + q);
+        K* sorted = sort_segments<K>(c, sa, c->d_hdr_base.as<uint64_t>() + q, c->d_word_base.as<uint64_t>() + q, 1,
+                                     c->h_kmer_cur[q], c->h_hdr_cur[q], &ms_ex, &ms_sort, &ms_pass);
+        uint32_t ne = 0, nd = 0;
+        runlength_one<K>(c, sa, sorted, c->h_kmer_cur[q], out_off, &ne, &nd, &ms_rl);
+        out_off += ne;
+        c->part_distinct[q] = nd;
+    }
+    c->part_off[P] = out_off;
+    HIPX(hipEventRecord(c->ev[1], c->st));
+    HIPX(hipEventSynchronize(c->ev[1]));
+    float total = 0;
+    HIPX(hipEventElapsedTime(&total, c->ev[0], c->ev[1]));
+    c->stats.ms_count = total;
+    c->stats.ms_extract = ms_ex; c->stats.ms_sort = ms_sort; c->stats.ms_runlength = ms_rl;
+    uint64_t launched = 0;
+    for (uint32_t q = 0; q < P; q++) if (c->h_kmer_cur[q]) launched++;
+    c->stats.ms_sort_pass_avg = ms_pass / std::max<uint64_t>(1, (uint64_t)c->npass * launched);
+    uint64_t nd = 0;
+    for (uint32_t q = 0; q < P; q++) nd += c->part_distinct[q];
+    c->stats.n_distinct = nd; c->stats.n_entries = out_off;
+}
+
+extern "C" int gkc_count_packed(gkc_ctx* c, const uint32_t* words, uint64_t n_words, const uint32_t* lens, uint64_t n_skm,
+                                uint64_t* key_lo, uint64_t* key_hi, uint32_t* cnt, uint64_t cap, uint64_t* n_entries,
+                                uint64_t* n_distinct) {
+    return guarded(c, [&]() -> int {
+        const int k = c->prm.k;
+        std::vector<SkmHdr> hd;
+        hd.reserve(n_skm);
+        uint64_t wo = 0, ko = 0;
+        for (uint64_t i = 0; i < n_skm; i++) {
+            const uint32_t L = lens[i];
+            if (L >= (uint32_t)k) {
+                SkmHdr h{(uint32_t)wo, (uint32_t)ko, L - k + 1, 0};
+                hd.push_back(h);
+                ko += L - k + 1;
+            }
+            wo += (L + 15) / 16;
+        }
+        if (wo > n_words) throw ArgError{"superkmer lengths exceed the word buffer", GKC_EINVAL};
+        if (ko >= (1ull << 30)) throw ArgError{"partition exceeds 2^30 k-mers", GKC_EINVAL};
+        *n_entries = 0; *n_distinct = 0;
+        if (ko == 0) return 0;
+        DevBuf dw, dh, dmeta;
+        dw.ensure((n_words + 8) * 4);
+        dh.ensure(hd.size() * sizeof(SkmHdr));
+        dmeta.ensure(32);
+        HIPX(hipMemsetAsync(dw.p, 0, (n_words + 8) * 4, c->st));
+        HIPX(hipMemcpyAsync(dw.p, words, n_words * 4, hipMemcpyHostToDevice, c->st));
+        HIPX(hipMemcpyAsync(dh.p, hd.data(), hd.size() * sizeof(SkmHdr), hipMemcpyHostToDevice, c->st));
+        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, c->st, dmeta.as<uint32_t>(), (uint32_t)ko);
+        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, c->st, dmeta.as<uint32_t>() + 1, (uint32_t)hd.size());
+        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, c->st, reinterpret_cast<uint64_t*>(static_cast<char*>(dmeta.p) + 16), (uint64_t)0);
+        const uint32_t* segk = dmeta.as<uint32_t>();
+        const uint32_t* segh = dmeta.as<uint32_t>() + 1;
+        const uint64_t* base0 = reinterpret_cast<const uint64_t*>(static_cast<char*>(dmeta.p) + 16);
+        // point the bins at the temporary storage for this call
+        std::swap(c->words.p, dw.p); std::swap(c->words.bytes, dw.bytes);
+        std::swap(c->hdrs.p, dh.p); std::swap(c->hdrs.bytes, dh.bytes);
+        struct Restore { gkc_ctx* c; DevBuf* dw; DevBuf* dh;
+            ~Restore() { std::swap(c->words.p, dw->p); std::swap(c->words.bytes, dw->bytes);
+                         std::swap(c->hdrs.p, dh->p); std::swap(c->hdrs.bytes, dh->bytes); } } restore{c, &dw, &dh};
+        uint32_t ne = 0, nd = 0;
+        auto run = [&](auto tag) {
+            using K = decltype(tag);
+            ensure_count_buffers<K>(c, ko, 1);
+            SegArrays sa = seg_arrays_single(c, segk, segh);
+            K* sorted = sort_segments<K>(c, sa, base0, base0, 1, ko, hd.size(), nullptr, nullptr, nullptr);
+            table_grow_keep<K>(c, 0, 1024);
+            runlength_one<K>(c, sa, sorted, ko, 0, &ne, &nd, nullptr);
+            if (ne > cap) throw ArgError{"output buffer too small", GKC_ECAP};
+            if (ne) {
+                if (sizeof(K) == 8) {
+                    HIPX(hipMemcpyAsync(key_lo, c->tkeys.p, (size_t)ne * 8, hipMemcpyDeviceToHost, c->st));
+                } else {
+                    std::vector<u128k> tmp(ne);
+                    HIPX(hipMemcpyAsync(tmp.data(), c->tkeys.p, (size_t)ne * 16, hipMemcpyDeviceToHost, c->st));
+                    HIPX(hipStreamSynchronize(c->st));
+                    for (uint32_t i = 0; i < ne; i++) { key_lo[i] = tmp[i].lo; if (key_hi) key_hi[i] = tmp[i].hi; }
+                }
+                HIPX(hipMemcpyAsync(cnt, c->tcnts.p, (size_t)ne * 4, hipMemcpyDeviceToHost, c->st));
+                HIPX(hipStreamSynchronize(c->st));
+            }
+        };
+        if (c->keyw == 8) run(uint64_t{}); else run(u128k{});
+        c->counted = false;     // the context table now holds this call's entries only
+        *n_entries = ne; *n_distinct = nd;
+        return 0;
+    });
+}

@@ -1,0 +1,118 @@
+// Types and kernel declarations shared by the translation units of the
+// gfx950 k-mer counting library.
+#pragma once
+#include "common.hpp"
+
+namespace gkc {
+
+// ---- phase 1 (partition.hip) ----
+constexpr int P1_NT = 256;              // threads per block (4 waves)
+constexpr int P1_V = 8;                 // window starts per thread
+constexpr int P1_T = P1_NT * P1_V;      // window starts per tile
+constexpr int MAXK = 64;
+constexpr uint32_t MAX_PARTS = 1024;
+
+struct BinSet {
+    uint32_t* words;            // 2-bit packed superkmers, 16 bases / word, MSB first
+    SkmHdr* hdrs;
+    const uint64_t* word_base;  // [P] start of each partition's word region
+    const uint64_t* hdr_base;   // [P]
+    const uint32_t* word_cap;   // [P]
+    const uint32_t* hdr_cap;    // [P]
+    uint32_t* word_cur;         // [P] cursors (persist across chunks)
+    uint32_t* hdr_cur;          // [P]
+    uint32_t* kmer_cur;         // [P] k-mers appended so far
+    uint32_t* overflow;         // [1] set when any capacity is exceeded
+};
+
+struct P1Params {
+    int k, p, fmode, pmode;
+    uint32_t P;
+    int scatter;                // 0: count only, 1: write bins
+    uint32_t* sketch;           // [P][256] sampled 4-mer counts, or null
+    uint32_t* mm_out;           // per-window minimizer (compat), or null
+};
+
+template <typename MM>
+__global__ void k_superkmers(const ChunkDesc* __restrict__ cdp, P1Params prm, BinSet bins, uint32_t* __restrict__ tile_ctr);
+__global__ void k_hpc(const uint8_t* __restrict__ reads, const uint64_t* __restrict__ offs, uint32_t n_reads,
+                      uint8_t* __restrict__ out, int32_t* __restrict__ lens, int32_t* __restrict__ orig_pos);
+__global__ void k_skm_lists(const uint64_t* __restrict__ offs, const int32_t* __restrict__ lens, uint32_t n_reads, int k,
+                            const uint32_t* __restrict__ mm, int32_t* __restrict__ skm);
+
+// ---- phase 2 (sort.hip) ----
+constexpr int RADIX_BITS = 8;
+constexpr int RADIX = 1 << RADIX_BITS;
+constexpr int MAX_PASS = 16;
+constexpr int SORT_NT = 256;
+constexpr int RL_ITEMS = 16;
+constexpr int RL_TILE = SORT_NT * RL_ITEMS;
+constexpr int EX_HDRS = 256;
+
+// Device-resident plan of one counting launch (all sizes live here so the
+// launch sequence can be captured once in a hipGraph and replayed).
+struct Plan {
+    uint32_t n_segs;
+    uint32_t n_pass;
+    uint32_t tot_items;         // extraction work items (EX_HDRS headers each)
+    uint32_t tot_sort_tiles;
+    uint32_t tot_rl_tiles;
+    uint32_t pad;
+    uint64_t tot_keys;
+    uint32_t ctr[48];           // work counters: [0] extract, [1+pass] sort passes, [1+MAX_PASS] run-length
+};
+
+struct SegArrays {
+    const uint32_t* seg_keys;      // [S] keys per segment
+    const uint32_t* seg_hdrs;      // [S] superkmer headers per segment
+    uint64_t* key_start;           // [S+1]
+    uint32_t* item_start;          // [S+1]
+    uint32_t* sort_tile_start;     // [S+1]
+    uint32_t* rl_tile_start;       // [S+1]
+    uint32_t* hist;                // [S][MAX_PASS][RADIX]
+};
+
+struct TableOut {
+    void* keys;                    // K[], entry i of segment s at table_start[s] + i
+    uint32_t* counts;
+    const uint64_t* table_start;   // [S]
+    const uint32_t* table_cap;     // [S]
+    uint32_t* n_entries;           // [S] entries kept (may exceed cap => overflow)
+    uint32_t* n_distinct;          // [S] distinct keys before the count filter
+    uint32_t* overflow;            // [1]
+    uint32_t min_cnt, max_cnt;     // max 0: unbounded
+    int emit;                      // 0: count only
+};
+
+template <typename K> struct KeyTraits;
+template <> struct KeyTraits<uint64_t> {
+    static constexpr int ITEMS = 16;               // keys per thread in a sort tile
+    __device__ static inline uint32_t digit(uint64_t x, int sh) { return (uint32_t)(x >> sh) & (RADIX - 1); }
+};
+template <> struct KeyTraits<u128k> {
+    static constexpr int ITEMS = 8;
+    __device__ static inline uint32_t digit(const u128k& x, int sh) {
+        return sh < 64 ? (uint32_t)(x.lo >> sh) & (RADIX - 1) : (uint32_t)(x.hi >> (sh - 64)) & (RADIX - 1);
+    }
+};
+template <typename K> constexpr int sort_tile() { return SORT_NT * KeyTraits<K>::ITEMS; }
+
+__global__ void k_plan(Plan* plan, SegArrays sa, int sort_tile_keys);
+__global__ void k_zero32(uint32_t* p, uint64_t n);
+__global__ void k_set_u64(uint64_t* p, uint64_t v);
+__global__ void k_set_u32(uint32_t* p, uint32_t v);
+__global__ void k_plan_init(Plan* plan, uint32_t n_segs, uint32_t n_pass);
+__global__ void k_set_chunk(ChunkDesc* dst, ChunkDesc v);
+template <typename K>
+__global__ void k_extract(Plan* __restrict__ plan, SegArrays sa, const SkmHdr* __restrict__ hdrs,
+                          const uint64_t* __restrict__ hdr_base, const uint32_t* __restrict__ words,
+                          const uint64_t* __restrict__ word_base, K* __restrict__ keys, int k);
+__global__ void k_hist_scan(Plan* __restrict__ plan, uint32_t* __restrict__ hist);
+template <typename K>
+__global__ void k_onesweep(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ in, K* __restrict__ out,
+                           uint32_t* __restrict__ lookback, int pass);
+template <typename K>
+__global__ void k_runlength(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys, TableOut to,
+                            uint32_t* __restrict__ lookback);
+
+}  // namespace gkc

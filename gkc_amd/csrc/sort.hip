@@ -1,0 +1,458 @@
+// Phase 2 of the k-mer counting path on gfx950: k-mer extraction from the
+// 2-bit superkmer bins (with the onesweep digit histograms of every pass
+// fused in), a segmented onesweep LSD radix sort (LDS ranking by wave64
+// ballots, decoupled look-back across workgroups), and a fused run-length
+// count + threshold pass that writes the (k-mer, count) table.
+//
+// Reference semantics followed (file:line in the reference tree):
+//   forward k-mers of each superkmer, rolling         kmer_counting.cu:95-113
+//   canonical = min(x, revcomp(x))                    kmer_counting.cu:33-44
+//   ascending sort                                    kmer_counting.cu:134
+//   run-length count, clamp to 0xFFFFFFFF, [min,max]  kmer_counting.cu:148-191
+// A segment is one partition: every kernel here works on all segments of a
+// plan at once (the plain path passes a plan with a single segment).
+#include "kernels.hpp"
+
+namespace gkc {
+
+// Segment owning global work index t (largest s with start[s] <= t).
+__device__ __forceinline__ uint32_t find_seg(const uint32_t* start, uint32_t n_segs, uint32_t t) {
+    uint32_t lo = 0, hi = n_segs;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (start[mid] <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Build the segment layout from per-segment key and header counts (one block).
+__global__ __launch_bounds__(1024) void k_plan(Plan* plan, SegArrays sa, int sort_tile_keys) {
+    __shared__ uint64_t s_k[1024];
+    __shared__ uint32_t s_i[1024], s_t[1024], s_r[1024];
+    const uint32_t S = plan->n_segs;
+    uint64_t accK = 0; uint32_t accI = 0, accT = 0, accR = 0;
+    for (uint32_t base = 0; base < S; base += 1024) {
+        const uint32_t s = base + threadIdx.x;
+        uint64_t nk = 0; uint32_t ni = 0, nt = 0, nr = 0;
+        if (s < S) {
+            nk = sa.seg_keys[s];
+            ni = (sa.seg_hdrs[s] + EX_HDRS - 1) / EX_HDRS;
+            nt = (uint32_t)((nk + sort_tile_keys - 1) / sort_tile_keys);
+            nr = (uint32_t)((nk + RL_TILE - 1) / RL_TILE);
+        }
+        s_k[threadIdx.x] = nk; s_i[threadIdx.x] = ni; s_t[threadIdx.x] = nt; s_r[threadIdx.x] = nr;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            uint64_t a = threadIdx.x >= off ? s_k[threadIdx.x - off] : 0;
+            uint32_t b = threadIdx.x >= off ? s_i[threadIdx.x - off] : 0;
+            uint32_t c = threadIdx.x >= off ? s_t[threadIdx.x - off] : 0;
+            uint32_t d = threadIdx.x >= off ? s_r[threadIdx.x - off] : 0;
+            __syncthreads();
+            s_k[threadIdx.x] += a; s_i[threadIdx.x] += b; s_t[threadIdx.x] += c; s_r[threadIdx.x] += d;
+            __syncthreads();
+        }
+        if (s < S) {
+            sa.key_start[s] = accK + s_k[threadIdx.x] - nk;
+            sa.item_start[s] = accI + s_i[threadIdx.x] - ni;
+            sa.sort_tile_start[s] = accT + s_t[threadIdx.x] - nt;
+            sa.rl_tile_start[s] = accR + s_r[threadIdx.x] - nr;
+        }
+        accK += s_k[1023]; accI += s_i[1023]; accT += s_t[1023]; accR += s_r[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        sa.key_start[S] = accK; sa.item_start[S] = accI;
+        sa.sort_tile_start[S] = accT; sa.rl_tile_start[S] = accR;
+        plan->tot_keys = accK; plan->tot_items = accI;
+        plan->tot_sort_tiles = accT; plan->tot_rl_tiles = accR;
+    }
+}
+
+// Small parameter stores done by kernels (kernel arguments are captured at
+// launch, so no host buffer has to outlive an asynchronous copy).
+__global__ void k_set_u64(uint64_t* p, uint64_t v) { *p = v; }
+__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+__global__ void k_plan_init(Plan* plan, uint32_t n_segs, uint32_t n_pass) {
+    const int t = threadIdx.x;
+    if (t == 0) { plan->n_segs = n_segs; plan->n_pass = n_pass; plan->tot_items = 0; plan->tot_sort_tiles = 0;
+                  plan->tot_rl_tiles = 0; plan->pad = 0; plan->tot_keys = 0; }
+    if (t < 48) plan->ctr[t] = 0;
+}
+__global__ void k_set_chunk(ChunkDesc* dst, ChunkDesc v) { *dst = v; }
+
+// Zero the work counters of a plan and the histograms (fill kernel, so the
+// sequence stays capturable in a graph).
+__global__ void k_zero32(uint32_t* p, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// Extract the canonical k-mers of each superkmer into the segment's key
+// range and accumulate the digit histograms of all passes.
+// Bases are 2-bit, 16 per u32 word, MSB first.
+template <typename K>
+__device__ __forceinline__ K kmer_at(const uint32_t* wds, uint32_t o, int k);
+
+template <>
+__device__ __forceinline__ uint64_t kmer_at<uint64_t>(const uint32_t* wds, uint32_t o, int k) {
+    const uint32_t wi = o >> 4, sh = (o & 15) * 2;
+    const uint64_t a = ((uint64_t)wds[wi] << 32) | wds[wi + 1];
+    uint64_t x = sh ? (a << sh) | ((uint64_t)wds[wi + 2] >> (32 - sh)) : a;
+    return x >> (64 - 2 * k);
+}
+template <>
+__device__ __forceinline__ u128k kmer_at<u128k>(const uint32_t* wds, uint32_t o, int k) {
+    const uint32_t wi = o >> 4, sh = (o & 15) * 2;
+    uint64_t h = ((uint64_t)wds[wi] << 32) | wds[wi + 1];
+    uint64_t l = ((uint64_t)wds[wi + 2] << 32) | wds[wi + 3];
+    if (sh) {
+        const uint64_t n = (uint64_t)wds[wi + 4] << 32;
+        h = (h << sh) | (l >> (64 - sh));
+        l = (l << sh) | (n >> (64 - sh));
+    }
+    // top 2k bits of h:l -> value
+    const int s = 128 - 2 * k;
+    u128k r;
+    if (s == 0) { r.hi = h; r.lo = l; }
+    else if (s < 64) { r.lo = (l >> s) | (h << (64 - s)); r.hi = h >> s; }
+    else { r.lo = h >> (s - 64); r.hi = 0; }
+    return r;
+}
+__device__ __forceinline__ uint64_t canon_of(uint64_t x, int k) { return canonical64(x, k); }
+__device__ __forceinline__ u128k canon_of(u128k x, int k) {
+    u128k r = revcomp128(x, k);
+    return less128(r, x) ? r : x;
+}
+
+template <typename K>
+__global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, SegArrays sa, const SkmHdr* __restrict__ hdrs,
+                                                     const uint64_t* __restrict__ hdr_base,
+                                                     const uint32_t* __restrict__ words,
+                                                     const uint64_t* __restrict__ word_base, K* __restrict__ keys, int k) {
+    __shared__ uint32_t s_hist[MAX_PASS * RADIX];
+    __shared__ uint32_t s_item;
+    const int npass = plan->n_pass;
+    const uint32_t tot = plan->tot_items, S = plan->n_segs;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(&plan->ctr[0], 1u);
+        for (int i = threadIdx.x; i < npass * RADIX; i += SORT_NT) s_hist[i] = 0;
+        __syncthreads();
+        const uint32_t item = s_item;
+        if (item >= tot) break;
+        const uint32_t seg = find_seg(sa.item_start, S, item);
+        const uint32_t h = (item - sa.item_start[seg]) * EX_HDRS + threadIdx.x;
+        if (h < sa.seg_hdrs[seg]) {
+            const SkmHdr hd = hdrs[hdr_base[seg] + h];
+            const uint32_t* wds = words + word_base[seg] + hd.word_off;
+            K* out = keys + sa.key_start[seg] + hd.kmer_off;
+            for (uint32_t j = 0; j < hd.n_kmers; j++) {
+                const K c = canon_of(kmer_at<K>(wds, j, k), k);
+                out[j] = c;
+                for (int ps = 0; ps < npass; ps++)
+                    atomicAdd(&s_hist[ps * RADIX + KeyTraits<K>::digit(c, ps * RADIX_BITS)], 1u);
+            }
+        }
+        __syncthreads();
+        uint32_t* gh = sa.hist + (size_t)seg * MAX_PASS * RADIX;
+        for (int i = threadIdx.x; i < npass * RADIX; i += SORT_NT)
+            if (s_hist[i]) atomicAdd(&gh[i], s_hist[i]);
+        __syncthreads();
+    }
+}
+
+// In-place exclusive scan of every (segment, pass) histogram; one wave each.
+__global__ __launch_bounds__(256) void k_hist_scan(Plan* __restrict__ plan, uint32_t* __restrict__ hist) {
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / GKC_WAVE;
+    const uint32_t lane = __lane_id();
+    const uint32_t nw = gridDim.x * blockDim.x / GKC_WAVE;
+    const uint32_t total = plan->n_segs * MAX_PASS;
+    for (uint32_t w = wave; w < total; w += nw) {
+        if ((w % MAX_PASS) >= plan->n_pass) continue;
+        uint32_t* hp = hist + (size_t)w * RADIX;
+        uint32_t v[RADIX / GKC_WAVE];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int i = 0; i < RADIX / GKC_WAVE; i++) { v[i] = hp[lane * (RADIX / GKC_WAVE) + i]; sum += v[i]; }
+        uint32_t inc = sum;
+#pragma unroll
+        for (int off = 1; off < GKC_WAVE; off <<= 1) {
+            uint32_t y = __shfl_up(inc, off);
+            if ((int)lane >= off) inc += y;
+        }
+        uint32_t run = inc - sum;
+#pragma unroll
+        for (int i = 0; i < RADIX / GKC_WAVE; i++) { hp[lane * (RADIX / GKC_WAVE) + i] = run; run += v[i]; }
+    }
+}
+
+// One onesweep pass over all segments.  Tile = SORT_NT * ITEMS keys of one
+// segment; tiles are claimed in order from a counter so every tile's
+// predecessors are already running (forward progress of the look-back).
+template <typename K>
+__global__ __launch_bounds__(SORT_NT) void k_onesweep(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ in,
+                                                      K* __restrict__ out, uint32_t* __restrict__ lookback, int pass) {
+    constexpr int ITEMS = KeyTraits<K>::ITEMS;
+    constexpr int TILE = SORT_NT * ITEMS;
+    constexpr int NWAVE = SORT_NT / GKC_WAVE;
+    __shared__ K s_keys[TILE];
+    __shared__ uint32_t s_whist[NWAVE][RADIX];
+    __shared__ uint32_t s_dstart[RADIX];
+    __shared__ uint64_t s_gbase[RADIX];
+    __shared__ uint32_t s_tile;
+    const int sh = pass * RADIX_BITS;
+    const uint32_t tot = plan->tot_sort_tiles, S = plan->n_segs;
+    const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(&plan->ctr[1 + pass], 1u);
+        for (int i = threadIdx.x; i < NWAVE * RADIX; i += SORT_NT) (&s_whist[0][0])[i] = 0;
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        if (tile >= tot) break;
+        const uint32_t seg = find_seg(sa.sort_tile_start, S, tile);
+        const uint32_t first_tile = sa.sort_tile_start[seg];
+        const uint32_t ti = tile - first_tile;
+        const uint64_t kbase = sa.key_start[seg];
+        const uint32_t nseg = sa.seg_keys[seg];
+        const uint32_t t0 = ti * TILE;
+        const uint32_t tn = min((uint32_t)TILE, nseg - t0);
+
+        // load: wave wv owns tile keys [wv*64*ITEMS, ...), item i of lane l at +i*64+l
+        K key[ITEMS];
+        uint32_t rank[ITEMS];
+        uint32_t dig[ITEMS];
+        const uint32_t wbase = wv * GKC_WAVE * ITEMS;
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            const uint32_t j = wbase + i * GKC_WAVE + lane;
+            if (j < tn) key[i] = in[kbase + t0 + j];
+            dig[i] = (j < tn) ? KeyTraits<K>::digit(key[i], sh) : RADIX;
+        }
+        // rank within the wave in (item, lane) order: match lanes of equal digit
+        // with RADIX_BITS ballots, count the lower lanes, chain per-wave counters
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            const uint32_t d = dig[i];
+            uint64_t peers = __ballot(d < RADIX);
+#pragma unroll
+            for (int b = 0; b < RADIX_BITS; b++) {
+                const uint64_t bal = __ballot((d >> b) & 1u);
+                peers &= ((d >> b) & 1u) ? bal : ~bal;
+            }
+            if (d < RADIX) {
+                const uint32_t before = s_whist[wv][d];
+                const uint64_t lower = peers & lanemask_lt();
+                rank[i] = before + (uint32_t)__popcll(lower);
+                if (lower == 0) s_whist[wv][d] = before + (uint32_t)__popcll(peers);
+            }
+        }
+        __syncthreads();
+        // per digit: wave prefix, tile total, publish aggregate
+        const uint32_t d = threadIdx.x;     // SORT_NT == RADIX
+        uint32_t total = 0;
+#pragma unroll
+        for (int w = 0; w < NWAVE; w++) { const uint32_t c = s_whist[w][d]; s_whist[w][d] = total; total += c; }
+        uint32_t* lb = lookback + (size_t)tile * RADIX;
+        if (ti == 0) st_agent(&lb[d], LB_INC | total);
+        else st_agent(&lb[d], LB_AGG | total);
+        // tile-local digit starts (block exclusive scan over digits)
+        s_dstart[d] = total;
+        __syncthreads();
+        {
+            // Hillis-Steele over 256 entries
+            for (int off = 1; off < RADIX; off <<= 1) {
+                const uint32_t x = d >= (uint32_t)off ? s_dstart[d - off] : 0;
+                __syncthreads();
+                s_dstart[d] += x;
+                __syncthreads();
+            }
+        }
+        const uint32_t dstart = s_dstart[d] - total;
+        // decoupled look-back across the segment's earlier tiles
+        uint32_t excl = 0;
+        if (ti > 0) {
+            int64_t j = (int64_t)tile - 1;
+            for (;;) {
+                uint32_t v = ld_agent(&lookback[(size_t)j * RADIX + d]);
+                while ((v & ~LB_VAL) == 0) { __builtin_amdgcn_s_sleep(1); v = ld_agent(&lookback[(size_t)j * RADIX + d]); }
+                excl += v & LB_VAL;
+                if (v & LB_INC) break;
+                j--;
+            }
+            st_agent(&lb[d], LB_INC | (excl + total));
+        }
+        const uint32_t* hs = sa.hist + ((size_t)seg * MAX_PASS + pass) * RADIX;
+        s_gbase[d] = kbase + hs[d] + excl - dstart;
+        __syncthreads();
+        s_dstart[d] = dstart;
+        __syncthreads();
+        // scatter into LDS in digit order, then write runs out
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            const uint32_t dd = dig[i];
+            if (dd < RADIX) s_keys[s_dstart[dd] + s_whist[wv][dd] + rank[i]] = key[i];
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < tn; j += SORT_NT) {
+            const K x = s_keys[j];
+            out[s_gbase[KeyTraits<K>::digit(x, sh)] + j] = x;
+        }
+        __syncthreads();
+    }
+}
+
+
+// Run-length count + clamp + threshold on sorted segments, compacted with a
+// decoupled look-back (one value per tile).  Thread t of a tile holds keys
+// [t0 + 16t, t0 + 16t + 16); the tile's last run is followed past the tile.
+template <typename K>
+__global__ __launch_bounds__(SORT_NT) void k_runlength(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys,
+                                                       TableOut to, uint32_t* __restrict__ lookback) {
+    __shared__ uint32_t s_first[SORT_NT];       // first head index (tile-local) of each thread, or RL_TILE
+    __shared__ uint32_t s_scan[SORT_NT];
+    __shared__ uint32_t s_tile, s_end, s_nd;
+    __shared__ uint64_t s_excl;
+    const uint32_t tot = plan->tot_rl_tiles, S = plan->n_segs;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(&plan->ctr[1 + MAX_PASS], 1u);
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        if (tile >= tot) break;
+        const uint32_t seg = find_seg(sa.rl_tile_start, S, tile);
+        const uint32_t ti = tile - sa.rl_tile_start[seg];
+        const K* kp = keys + sa.key_start[seg];
+        const uint32_t nseg = sa.seg_keys[seg];
+        const uint32_t t0 = ti * RL_TILE;
+        const uint32_t tn = min((uint32_t)RL_TILE, nseg - t0);
+        const uint32_t my0 = threadIdx.x * RL_ITEMS;
+        K x[RL_ITEMS];
+        uint32_t headmask = 0;
+#pragma unroll
+        for (int i = 0; i < RL_ITEMS; i++) {
+            const uint32_t j = my0 + i;
+            if (j < tn) x[i] = kp[t0 + j];
+        }
+        K prevk;
+        const bool has_prev = (t0 + my0) > 0 && my0 < tn;
+        if (has_prev) prevk = kp[t0 + my0 - 1];
+#pragma unroll
+        for (int i = 0; i < RL_ITEMS; i++) {
+            const uint32_t j = my0 + i;
+            if (j < tn) {
+                const bool head = (i == 0) ? (!has_prev || !(x[0] == prevk)) : !(x[i] == x[i - 1]);
+                if (head) headmask |= 1u << i;
+            }
+        }
+        // first head of each thread -> next-head lookup across threads
+        s_first[threadIdx.x] = headmask ? my0 + __builtin_ctz(headmask) : RL_TILE;
+        if (threadIdx.x == 0) { s_end = tn; s_nd = 0; }
+        __syncthreads();
+        // suffix minimum over threads of first-head positions
+        for (int off = 1; off < SORT_NT; off <<= 1) {
+            const uint32_t y = (threadIdx.x + off < SORT_NT) ? s_first[threadIdx.x + off] : RL_TILE;
+            __syncthreads();
+            s_first[threadIdx.x] = min(s_first[threadIdx.x], y);
+            __syncthreads();
+        }
+        // end of the tile's last run: follow it past the tile (block-wide)
+        {
+            const uint32_t last = tn - 1;
+            const K lastk = (threadIdx.x == (last / RL_ITEMS)) ? x[last % RL_ITEMS] : x[0];
+            __shared__ K s_lastk;
+            if (threadIdx.x == last / RL_ITEMS) s_lastk = lastk;
+            __syncthreads();
+            const K lk = s_lastk;
+            uint32_t e = t0 + tn;
+            while (e < nseg) {
+                const uint32_t j = e + threadIdx.x;
+                const bool diff = j >= nseg || !(kp[j] == lk);
+                const uint64_t m = __ballot(diff);
+                __shared__ uint32_t s_hit;
+                if (threadIdx.x == 0) s_hit = 0xFFFFFFFFu;
+                __syncthreads();
+                if (m && __lane_id() == (uint32_t)__builtin_ctzll(m)) atomicMin(&s_hit, j);
+                __syncthreads();
+                const uint32_t hit = s_hit;
+                __syncthreads();
+                if (hit != 0xFFFFFFFFu) { e = hit; break; }
+                e += SORT_NT;
+            }
+            if (threadIdx.x == 0) s_end = min(e, nseg) - t0;   // tile-local end of last run
+            __syncthreads();
+        }
+        // counts and filter
+        uint32_t keep = 0, nheads = 0;
+        uint32_t cnt[RL_ITEMS];
+        {
+            const uint32_t nxt_thread = (threadIdx.x + 1 < SORT_NT) ? s_first[threadIdx.x + 1] : RL_TILE;
+            uint32_t next = min(nxt_thread, s_end);
+#pragma unroll
+            for (int i = RL_ITEMS - 1; i >= 0; i--) {
+                cnt[i] = 0;
+                if (headmask & (1u << i)) {
+                    const uint32_t pos = my0 + i;
+                    const uint32_t run = next - pos;
+                    cnt[i] = run;
+                    next = pos;
+                    nheads++;
+                    if (run >= to.min_cnt && (to.max_cnt == 0 || run <= to.max_cnt)) keep |= 1u << i;
+                }
+            }
+        }
+        const uint32_t nk = __popc(keep);
+        s_scan[threadIdx.x] = nk;
+        atomicAdd(&s_nd, nheads);
+        __syncthreads();
+        for (int off = 1; off < SORT_NT; off <<= 1) {
+            const uint32_t y = threadIdx.x >= (uint32_t)off ? s_scan[threadIdx.x - off] : 0;
+            __syncthreads();
+            s_scan[threadIdx.x] += y;
+            __syncthreads();
+        }
+        const uint32_t tile_total = s_scan[SORT_NT - 1];
+        // decoupled look-back for the tile's output offset within the segment
+        if (threadIdx.x == 0) {
+            uint32_t* lb = lookback + tile;
+            uint32_t excl = 0;
+            if (ti == 0) st_agent(lb, LB_INC | tile_total);
+            else {
+                st_agent(lb, LB_AGG | tile_total);
+                int64_t j = (int64_t)tile - 1;
+                for (;;) {
+                    uint32_t v = ld_agent(&lookback[j]);
+                    while ((v & ~LB_VAL) == 0) { __builtin_amdgcn_s_sleep(1); v = ld_agent(&lookback[j]); }
+                    excl += v & LB_VAL;
+                    if (v & LB_INC) break;
+                    j--;
+                }
+                st_agent(lb, LB_INC | (excl + tile_total));
+            }
+            s_excl = excl;
+            atomicAdd(&to.n_distinct[seg], s_nd);
+            atomicAdd(&to.n_entries[seg], tile_total);
+        }
+        __syncthreads();
+        if (to.emit && keep) {
+            const uint32_t cap = to.table_cap[seg];
+            uint32_t idx = (uint32_t)s_excl + s_scan[threadIdx.x] - nk;
+            K* tk = reinterpret_cast<K*>(to.keys) + to.table_start[seg];
+            uint32_t* tc = to.counts + to.table_start[seg];
+#pragma unroll
+            for (int i = 0; i < RL_ITEMS; i++) {
+                if (keep & (1u << i)) {
+                    if (idx < cap) { tk[idx] = x[i]; tc[idx] = cnt[i]; }
+                    else atomicOr(to.overflow, 1u);
+                    idx++;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template __global__ void k_extract<uint64_t>(Plan*, SegArrays, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, uint64_t*, int);
+template __global__ void k_extract<u128k>(Plan*, SegArrays, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, u128k*, int);
+template __global__ void k_onesweep<uint64_t>(Plan*, SegArrays, const uint64_t*, uint64_t*, uint32_t*, int);
+template __global__ void k_onesweep<u128k>(Plan*, SegArrays, const u128k*, u128k*, uint32_t*, int);
+template __global__ void k_runlength<uint64_t>(Plan*, SegArrays, const uint64_t*, TableOut, uint32_t*);
+template __global__ void k_runlength<u128k>(Plan*, SegArrays, const u128k*, TableOut, uint32_t*);
+
+}  // namespace gkc

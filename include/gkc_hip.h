@@ -1,0 +1,143 @@
+/*
+ * gkc_hip.h -- C ABI of the MI355X (gfx950) k-mer counting library.
+ *
+ * Plain C types only (pointers, sizes, PODs); no HIP or torch types appear
+ * in any signature.  Every function returns 0 (or a non-negative count) on
+ * success and a negative GKC_E* code on failure; gkc_last_error() gives the
+ * message of the calling thread's last failure.
+ *
+ * The entry points replace the two calls of the reference's counting path:
+ *   GenSuperkmerGPU   reference gkc_cuda.hpp:58-61 (impl. gkc_cuda.cu:376-541)
+ *   kmc_counting_GPU  reference kmer_counting.hpp:9-12 (impl. kmer_counting.cu:59-199)
+ * The C++ drop-in wrappers with the reference's exact signatures are in
+ * include/gkc/gkc_cuda.hpp and include/gkc/kmer_counting.hpp.
+ */
+#ifndef GKC_HIP_H
+#define GKC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GKC_OK = 0,
+    GKC_EINVAL = -1,     /* bad argument */
+    GKC_EHIP = -2,       /* HIP runtime error */
+    GKC_ENOMEM = -3,     /* device allocation failed */
+    GKC_ESTATE = -4,     /* call out of sequence */
+    GKC_ECAP = -5        /* output buffer too small */
+};
+
+/* Flags of gkc_partition / gkc_count. */
+#define GKC_DEVICE_INPUT   1u   /* reads/offs are device pointers (already in HBM) */
+#define GKC_BATCHED        2u   /* persistent kernels + device work queue + hipGraph replay */
+#define GKC_PRESIZED       4u   /* gkc_count: size tables from gkc_set_table_caps / the learned
+                                   predictor instead of a count pass (overflow falls back) */
+#define GKC_SYNC_ONLY      8u   /* internal: no host sync beyond what the mode needs */
+
+/* Counting parameters (the reference's PAR fields K_kmer, P_minimizer, HPC,
+ * SKM_partitions, kmer_min_freq, kmer_max_freq; utilities.hpp:104-123). */
+typedef struct gkc_params {
+    int32_t k;             /* 1..64; k <= 32 uses 64-bit keys, else 128-bit (hi, lo) */
+    int32_t p;             /* minimizer length, 3..min(k, 32) */
+    int32_t hpc;           /* homopolymer compression of reads */
+    uint32_t n_parts;      /* superkmer partitions, 1..1024 */
+    int32_t filter_mode;   /* 1: new_filter2 (reference default), 0: accept every p-mer */
+    int32_t part_mode;     /* 0: minimizer % P (reference), 1: hashed minimizer % P */
+    uint32_t min_count;    /* keep (k-mer, count) with min_count <= count ... */
+    uint32_t max_count;    /* ... <= max_count; 0 = unbounded */
+} gkc_params;
+
+typedef struct gkc_ctx gkc_ctx;
+
+/* Per-stage device times of the last calls (milliseconds, HIP events). */
+typedef struct gkc_stats {
+    double ms_partition;       /* last gkc_partition (all its kernels) */
+    double ms_count;           /* last gkc_count */
+    double ms_extract, ms_sort, ms_runlength;   /* inside the last gkc_count */
+    double ms_sort_pass_avg;   /* average duration of one onesweep pass launch */
+    uint64_t n_kmers;          /* k-mer occurrences in the bins */
+    uint64_t n_superkmers;
+    uint64_t n_words;          /* 2-bit words in the bins */
+    uint64_t n_distinct;       /* distinct k-mers (before the count filter) */
+    uint64_t n_entries;        /* table entries kept */
+    uint32_t n_pass;           /* radix passes run per key */
+    uint32_t fallbacks;        /* presize overflows that fell back to count-then-allocate */
+    uint32_t host_syncs;       /* host synchronisations inside the last gkc_count */
+    uint32_t pad;
+} gkc_stats;
+
+const char *gkc_last_error(void);
+int gkc_version(void);
+
+/* Create a context on `device` (one per host thread; calls on one context
+ * are serialised by the library). NULL on failure (see gkc_last_error). */
+gkc_ctx *gkc_create(int device, const gkc_params *prm);
+void gkc_destroy(gkc_ctx *ctx);
+/* The context's HIP stream (hipStream_t) as an opaque pointer. */
+void *gkc_stream(gkc_ctx *ctx);
+int gkc_synchronize(gkc_ctx *ctx);
+
+/* ---- drop-in #1: GenSuperkmerGPU (gkc_cuda.hpp:58-61) ----
+ * For one sub-batch of n reads (host CSR: reads at reads[0..offs[n]-offs[0]),
+ * absolute offsets offs[0..n]) fill the host arrays the reference hands to its
+ * callback in T_h_data (types.h:32-46), indexed by batch-local base position:
+ *   minimizers[b]   u32 window minimizer (valid at [0, len-k] of each read)
+ *   skm_offs[b]     i32 superkmer offset lists [0, i1, .., len-k+1, 0] per read
+ *   reads_out[b]    the (HPC-compressed, when hpc) bases           (may be NULL)
+ *   read_len[r]     HPC read lengths                 (HPC only, may be NULL)
+ *   hpc_orig_pos[b] original in-read index of each HPC base (HPC only, may be NULL)
+ * Requires p <= 16 (the reference's u32 minimizer). */
+int gkc_superkmer_arrays(gkc_ctx *ctx, const char *reads, const uint64_t *offs, int64_t n,
+                         uint32_t *minimizers, int32_t *skm_offs, char *reads_out,
+                         int32_t *read_len, int32_t *hpc_orig_pos);
+
+/* ---- device-resident path (what GenSuperkmerGPU + SKMStore::add_skms do,
+ * superkmers.hpp:167-186, without leaving the GPU) ----
+ * Encode, minimizer, superkmer and scatter one batch of reads into the
+ * context's partition bins (appending across calls).  With
+ * GKC_DEVICE_INPUT the pointers are device memory. */
+int gkc_partition(gkc_ctx *ctx, const char *reads, const uint64_t *offs, int64_t n, uint32_t flags);
+/* Clear the bins and the table. */
+int gkc_reset(gkc_ctx *ctx);
+
+/* ---- drop-in #2: kmc_counting_GPU (kmer_counting.hpp:9-12) ----
+ * Count every partition in the bins: extract canonical k-mers, onesweep
+ * radix sort, run-length count with the [min_count, max_count] filter.
+ * The table stays in device memory, partition by partition, keys ascending. */
+int gkc_count(gkc_ctx *ctx, uint32_t flags);
+/* Table layout after gkc_count: part_off[P+1] entry offsets, part_distinct[P]
+ * distinct k-mers before the filter, part_kmers[P] k-mer occurrences. */
+int gkc_table_info(gkc_ctx *ctx, uint64_t *part_off, uint64_t *part_distinct, uint64_t *part_kmers);
+/* Copy the table (or partition `part`, or all when part < 0) to host memory:
+ * key_lo[i] (k <= 32: the key), key_hi[i] (k > 32 only, else may be NULL), cnt[i]. */
+int64_t gkc_table_copy(gkc_ctx *ctx, int32_t part, uint64_t *key_lo, uint64_t *key_hi, uint32_t *cnt,
+                       uint64_t cap);
+/* Device pointers of the table (valid until the next gkc_count/gkc_reset). */
+int gkc_table_device(gkc_ctx *ctx, void **keys, uint32_t **counts, uint64_t *n_entries);
+
+/* Count one partition given as packed superkmers in host memory (the
+ * SKMStore of the drop-in): n_skm superkmers, lens[i] bases each, bases 2-bit
+ * MSB-first 16 per u32 word, each superkmer starting on a word boundary in
+ * `words`.  Writes up to `cap` entries, sets *n_entries / *n_distinct. */
+int gkc_count_packed(gkc_ctx *ctx, const uint32_t *words, uint64_t n_words, const uint32_t *lens,
+                     uint64_t n_skm, uint64_t *key_lo, uint64_t *key_hi, uint32_t *cnt, uint64_t cap,
+                     uint64_t *n_entries, uint64_t *n_distinct);
+
+int gkc_get_stats(gkc_ctx *ctx, gkc_stats *out);
+
+/* Per-partition 4-mer sketch taken during gkc_partition: float [P][256]
+ * holding log1p(count) of the 4-mer at every 16th window start of each read,
+ * by the window's partition. */
+int gkc_sketch(gkc_ctx *ctx, float *out);
+/* Table capacities used by gkc_count(GKC_PRESIZED): caps[P] entries. */
+int gkc_set_table_caps(gkc_ctx *ctx, const uint32_t *caps);
+/* Per-partition k-mer totals of the bins (known exactly after gkc_partition). */
+int gkc_part_kmers(gkc_ctx *ctx, uint64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
