@@ -20,6 +20,7 @@ from . import _lib
 DEVICE_INPUT = 1
 BATCHED = 2
 PRESIZED = 4
+EMIT_WINDOWS = 32
 
 _u8p = C.POINTER(C.c_uint8)
 _u32p = C.POINTER(C.c_uint32)
@@ -54,15 +55,33 @@ SIGNATURES = {
     "gkc_stream": (C.c_void_p, [C.c_void_p]),
     "gkc_synchronize": (C.c_int, [C.c_void_p]),
     "gkc_superkmer_arrays": (C.c_int, [C.c_void_p, C.c_void_p, _u64p, C.c_int64, _u32p, _i32p, C.c_void_p, _i32p, _i32p]),
+    "gkc_window_minimizers": (C.c_int, [C.c_void_p, C.c_void_p, _u64p, C.c_int64, _u32p, _u8p, C.POINTER(C.c_int8)]),
+    "gkc_windows_device": (C.c_int, [C.c_void_p, C.POINTER(_u32p), C.POINTER(_u8p), C.POINTER(C.POINTER(C.c_int8)), _u64p]),
     "gkc_partition": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_uint32]),
     "gkc_reset": (C.c_int, [C.c_void_p]),
     "gkc_count": (C.c_int, [C.c_void_p, C.c_uint32]),
     "gkc_table_info": (C.c_int, [C.c_void_p, _u64p, _u64p, _u64p]),
     "gkc_table_copy": (C.c_int64, [C.c_void_p, C.c_int32, _u64p, _u64p, _u32p, C.c_uint64]),
+    "gkc_table_starts": (C.c_int, [C.c_void_p, _u64p]),
     "gkc_table_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(_u32p), _u64p]),
     "gkc_count_packed": (C.c_int, [C.c_void_p, _u32p, C.c_uint64, _u32p, C.c_uint64, _u64p, _u64p, _u32p, C.c_uint64,
                                    _u64p, _u64p]),
     "gkc_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "gkc_host_alloc": (C.c_void_p, [C.c_uint64]),
+    "gkc_host_free": (None, [C.c_void_p]),
+    "gkc_device_select": (C.c_int, [C.c_int]),
+    "gkc_mlp_n_params": (C.c_uint64, []),
+    "gkc_mlp_last_error": (C.c_char_p, []),
+    "gkc_mlp_create": (C.c_void_p, [C.c_int, C.c_uint64]),
+    "gkc_mlp_destroy": (None, [C.c_void_p]),
+    "gkc_mlp_get_params": (C.c_int, [C.c_void_p, _f32p]),
+    "gkc_mlp_set_params": (C.c_int, [C.c_void_p, _f32p]),
+    "gkc_mlp_forward": (C.c_int, [C.c_void_p, _f32p, C.c_int64, _f32p]),
+    "gkc_mlp_forward_bf16": (C.c_int, [C.c_void_p, _f32p, C.c_int64, _f32p]),
+    "gkc_mlp_loss_grad": (C.c_int, [C.c_void_p, _f32p, _f32p, C.c_int64, _f32p, _f32p]),
+    "gkc_mlp_train": (C.c_int, [C.c_void_p, _f32p, _f32p, C.c_int64, C.c_int, C.c_int, C.c_float, C.c_float,
+                                C.c_uint64, _f32p]),
+    "gkc_bf16_matmul32": (C.c_int, [C.c_int, _f32p, _f32p, C.c_int, C.c_int, _f32p]),
     "gkc_sketch": (C.c_int, [C.c_void_p, _f32p]),
     "gkc_set_table_caps": (C.c_int, [C.c_void_p, _u32p]),
     "gkc_part_kmers": (C.c_int, [C.c_void_p, _u64p]),
@@ -179,6 +198,17 @@ class Counter:
 
     def synchronize(self) -> None:
         _check(lib().gkc_synchronize(self.h))
+
+    def window_minimizers(self, reads, offs) -> dict:
+        reads = np.ascontiguousarray(reads, np.uint8)
+        offs = np.ascontiguousarray(offs, np.uint64)
+        B = int(offs[-1] - offs[0])
+        mm = np.zeros(max(B, 1), np.uint32)
+        pos = np.zeros(max(B, 1), np.uint8)
+        st = np.zeros(max(B, 1), np.int8)
+        _check(lib().gkc_window_minimizers(self.h, reads.ctypes.data, _p(offs, _u64p), len(offs) - 1, _p(mm, _u32p),
+                                           _p(pos, _u8p), st.ctypes.data_as(C.POINTER(C.c_int8))))
+        return {"minimizers": mm[:B], "pos": pos[:B], "strand": st[:B]}
 
     # -- drop-in #1 arrays
     def superkmer_arrays(self, reads, offs) -> dict:

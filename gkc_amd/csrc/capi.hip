@@ -88,12 +88,20 @@ struct gkc_ctx {
 
     // batch staging
     DevBuf d_reads, d_offs, d_lens, d_hpc_reads, d_hpc_pos, d_chunk, d_ctr, d_mm;
+    DevBuf d_win_mm, d_win_pos, d_win_str;     // per-window records of the last partition call
+    uint64_t win_n = 0;
+    uint32_t win_lead = 0;
 
     // counting
     DevBuf keysA, keysB, lookback, d_plan, d_segk, d_segh, d_keystart, d_itemstart, d_tilestart, d_rlstart, d_hist;
     DevBuf d_tstart, d_tcap, d_nent, d_ndist, d_toverflow;
     DevBuf tkeys, tcnts;
-    std::vector<uint64_t> part_off, part_distinct, part_kmers;
+    std::vector<uint64_t> part_off, part_distinct, part_kmers, part_start;   // part_start: physical table offsets
+    std::vector<uint32_t> h_caps, h_nent;
+    std::vector<uint64_t> h_tstart;
+    // captured launch sequence of the batched form (re-captured when buffers move)
+    hipGraphExec_t g_sort = nullptr, g_emit = nullptr, g_cnt = nullptr;
+    std::vector<const void*> g_key;
     std::vector<uint32_t> presize_caps;
     bool counted = false;
     gkc_stats stats{};
@@ -120,9 +128,20 @@ static int guarded(gkc_ctx* c, F&& f) {
     }
 }
 
+#include <map>
+static std::mutex g_occ_mu;
+static std::map<const void*, int> g_occ;
+// Resident workgroups per CU for a kernel (cached, so no runtime query runs
+// while a stream is being captured) times the CU count.
 static int occupancy_blocks(const void* fn, int nt, int n_cu) {
+    std::lock_guard<std::mutex> lk(g_occ_mu);
+    auto it = g_occ.find(fn);
     int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, nt, 0) != hipSuccess || b < 1) { (void)hipGetLastError(); b = 1; }
+    if (it != g_occ.end()) b = it->second;
+    else {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, nt, 0) != hipSuccess || b < 1) { (void)hipGetLastError(); b = 1; }
+        g_occ[fn] = b;
+    }
     return b * n_cu;
 }
 
@@ -210,12 +229,15 @@ static uint64_t stage_batch(gkc_ctx* c, const char* reads, const uint64_t* offs,
     uint64_t o0, on;
     const uint8_t* dreads;
     const uint64_t* doffs;
+    uint32_t lead = 0;
     if (flags & GKC_DEVICE_INPUT) {
         HIPX(hipMemcpy(&o0, offs, 8, hipMemcpyDeviceToHost));
         HIPX(hipMemcpy(&on, offs + n, 8, hipMemcpyDeviceToHost));
-        if (reinterpret_cast<uintptr_t>(reads) & 15) throw ArgError{"device reads must be 16-byte aligned", GKC_EINVAL};
-        dreads = reinterpret_cast<const uint8_t*>(reads);
+        // align the base pointer down to 16 bytes; positions [0, lead) belong to no read
+        lead = (uint32_t)(reinterpret_cast<uintptr_t>(reads) & 15);
+        dreads = reinterpret_cast<const uint8_t*>(reads) - lead;
         doffs = offs;
+        if (c->prm.hpc && lead) throw ArgError{"HPC needs 16-byte aligned device reads", GKC_EINVAL};
     } else {
         o0 = offs[0]; on = offs[n];
         const uint64_t B = on - o0;
@@ -227,9 +249,9 @@ static uint64_t stage_batch(gkc_ctx* c, const char* reads, const uint64_t* offs,
         doffs = c->d_offs.as<uint64_t>();
     }
     const uint64_t B = on - o0;
-    if (B >= (1ull << 31)) throw ArgError{"a batch must hold fewer than 2^31 bases", GKC_EINVAL};
-    cd.reads = dreads; cd.offs = doffs; cd.lens = nullptr; cd.base = o0;
-    cd.n_reads = (uint32_t)n; cd.n_bases = (uint32_t)B; cd.n_tiles = cdiv(B, P1_T); cd.pad = 0;
+    if (B + lead >= (1ull << 31)) throw ArgError{"a batch must hold fewer than 2^31 bases", GKC_EINVAL};
+    cd.reads = dreads; cd.offs = doffs; cd.lens = nullptr; cd.base = o0 - lead;
+    cd.n_reads = (uint32_t)n; cd.n_bases = (uint32_t)(B + lead); cd.n_tiles = cdiv(B + lead, P1_T); cd.lead = lead;
     if (c->prm.hpc && n > 0) {
         c->d_hpc_reads.ensure(B + 64);
         c->d_lens.ensure((n + 1) * 4);
@@ -251,20 +273,27 @@ static void launch_superkmers(gkc_ctx* c, const ChunkDesc& cd, P1Params prm, Bin
     c->d_ctr.ensure(64);
     HIPX(hipMemsetAsync(c->d_ctr.p, 0, 4, c->st));
     if (cd.n_tiles == 0) return;
+    const bool win = prm.win_mm != nullptr;
     const bool wide = c->prm.p > 16;
-    const void* fn = wide ? (const void*)k_superkmers<uint64_t> : (const void*)k_superkmers<uint32_t>;
+    const void* fn = win ? (const void*)k_superkmers<uint64_t, true>
+                         : wide ? (const void*)k_superkmers<uint64_t, false> : (const void*)k_superkmers<uint32_t, false>;
     int grid = std::min<int>(cd.n_tiles, occupancy_blocks(fn, P1_NT, c->n_cu));
-    if (wide)
-        hipLaunchKernelGGL(k_superkmers<uint64_t>, dim3(grid), dim3(P1_NT), 0, c->st, c->d_chunk.as<ChunkDesc>(), prm, bs, c->d_ctr.as<uint32_t>());
+    if (win)
+        hipLaunchKernelGGL((k_superkmers<uint64_t, true>), dim3(grid), dim3(P1_NT), 0, c->st, c->d_chunk.as<ChunkDesc>(), prm, bs, c->d_ctr.as<uint32_t>());
+    else if (wide)
+        hipLaunchKernelGGL((k_superkmers<uint64_t, false>), dim3(grid), dim3(P1_NT), 0, c->st, c->d_chunk.as<ChunkDesc>(), prm, bs, c->d_ctr.as<uint32_t>());
     else
-        hipLaunchKernelGGL(k_superkmers<uint32_t>, dim3(grid), dim3(P1_NT), 0, c->st, c->d_chunk.as<ChunkDesc>(), prm, bs, c->d_ctr.as<uint32_t>());
+        hipLaunchKernelGGL((k_superkmers<uint32_t, false>), dim3(grid), dim3(P1_NT), 0, c->st, c->d_chunk.as<ChunkDesc>(), prm, bs, c->d_ctr.as<uint32_t>());
     HIPX(hipGetLastError());
 }
+
+static void zero(gkc_ctx* c, void* p, uint64_t n32);
 
 static P1Params p1params(gkc_ctx* c, int scatter) {
     P1Params p;
     p.k = c->prm.k; p.p = c->prm.p; p.fmode = c->prm.filter_mode; p.pmode = c->prm.part_mode;
     p.P = c->prm.n_parts; p.scatter = scatter; p.sketch = nullptr; p.mm_out = nullptr;
+    p.win_mm = nullptr; p.win_pos = nullptr; p.win_strand = nullptr;
     return p;
 }
 
@@ -272,10 +301,34 @@ static P1Params p1params(gkc_ctx* c, int scatter) {
 
 template <typename K>
 static void count_impl(gkc_ctx* c, uint32_t flags);
+template <typename K>
+static void count_batched(gkc_ctx* c, uint32_t flags);
 
 extern "C" {
 
 const char* gkc_last_error(void) { return g_err.c_str(); }
+
+void* gkc_host_alloc(uint64_t bytes) {
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        g_err = "pinned host allocation of " + std::to_string(bytes) + " bytes failed";
+        return nullptr;
+    }
+    return p;
+}
+
+void gkc_host_free(void* p) {
+    if (p) (void)hipHostFree(p);
+}
+
+int gkc_device_select(int device) {
+    if (hipSetDevice(device) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        g_err = std::string("cannot select device ") + std::to_string(device) + ": " + hipGetErrorString(hipGetLastError());
+        return GKC_EHIP;
+    }
+    return 0;
+}
 int gkc_version(void) { return 10000; }
 
 gkc_ctx* gkc_create(int device, const gkc_params* prm) {
@@ -319,6 +372,9 @@ void gkc_destroy(gkc_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->st) (void)hipStreamSynchronize(c->st);
+    if (c->g_sort) (void)hipGraphExecDestroy(c->g_sort);
+    if (c->g_emit) (void)hipGraphExecDestroy(c->g_emit);
+    if (c->g_cnt) (void)hipGraphExecDestroy(c->g_cnt);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : c->evp) if (e) (void)hipEventDestroy(e);
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -333,7 +389,14 @@ int gkc_synchronize(gkc_ctx* c) {
 
 int gkc_reset(gkc_ctx* c) {
     return guarded(c, [&] {
-        bins_init(c);
+        // keep the bin regions and capacities, clear the cursors and the sketch
+        const uint32_t P = c->prm.n_parts;
+        std::fill(c->h_word_cur.begin(), c->h_word_cur.end(), 0u);
+        std::fill(c->h_hdr_cur.begin(), c->h_hdr_cur.end(), 0u);
+        std::fill(c->h_kmer_cur.begin(), c->h_kmer_cur.end(), 0u);
+        zero(c, c->d_word_cur.p, P); zero(c, c->d_hdr_cur.p, P); zero(c, c->d_kmer_cur.p, P);
+        zero(c, c->d_sketch.p, (uint64_t)P * 256); zero(c, c->d_overflow.p, 4);
+        c->counted = false;
         HIPX(hipStreamSynchronize(c->st));
         return 0;
     });
@@ -376,6 +439,41 @@ int gkc_superkmer_arrays(gkc_ctx* c, const char* reads, const uint64_t* offs, in
     });
 }
 
+int gkc_window_minimizers(gkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n, uint32_t* mm, uint8_t* pos,
+                          int8_t* strand) {
+    return guarded(c, [&] {
+        if (n < 0 || !offs || (n > 0 && !reads)) throw ArgError{"bad batch", GKC_EINVAL};
+        if (c->prm.p > 25) throw ArgError{"per-window emission needs p <= 25", GKC_EINVAL};
+        if (n == 0) return 0;
+        ChunkDesc cd;
+        const uint64_t B = stage_batch(c, reads, offs, n, 0, cd);
+        c->d_win_mm.ensure((B + 1) * 4); c->d_win_pos.ensure(B + 1); c->d_win_str.ensure(B + 1);
+        P1Params prm = p1params(c, 0);
+        prm.win_mm = c->d_win_mm.as<uint32_t>(); prm.win_pos = c->d_win_pos.as<uint8_t>();
+        prm.win_strand = c->d_win_str.as<int8_t>();
+        const uint32_t P = c->prm.n_parts;
+        zero(c, c->d_tmp_hdr.p, P); zero(c, c->d_tmp_word.p, P); zero(c, c->d_tmp_kmer.p, P);
+        launch_superkmers(c, cd, prm, binset(c, true));
+        if (mm) HIPX(hipMemcpyAsync(mm, c->d_win_mm.p, B * 4, hipMemcpyDeviceToHost, c->st));
+        if (pos) HIPX(hipMemcpyAsync(pos, c->d_win_pos.p, B, hipMemcpyDeviceToHost, c->st));
+        if (strand) HIPX(hipMemcpyAsync(strand, c->d_win_str.p, B, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipStreamSynchronize(c->st));
+        return 0;
+    });
+}
+
+int gkc_windows_device(gkc_ctx* c, uint32_t** mm, uint8_t** pos, int8_t** strand, uint64_t* n) {
+    return guarded(c, [&] {
+        if (!c->win_n) throw ArgError{"no per-window records (gkc_partition without GKC_EMIT_WINDOWS)", GKC_ESTATE};
+        // records are indexed by batch-local base position (+ the alignment lead)
+        if (mm) *mm = c->d_win_mm.as<uint32_t>() + c->win_lead;
+        if (pos) *pos = c->d_win_pos.as<uint8_t>() + c->win_lead;
+        if (strand) *strand = c->d_win_str.as<int8_t>() + c->win_lead;
+        if (n) *n = c->win_n - c->win_lead;
+        return 0;
+    });
+}
+
 int gkc_partition(gkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n, uint32_t flags) {
     return guarded(c, [&] {
         if (n < 0 || !offs || (n > 0 && !reads)) throw ArgError{"bad batch", GKC_EINVAL};
@@ -399,8 +497,17 @@ int gkc_partition(gkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n
         for (uint32_t q = 0; q < P; q++)
             if ((uint64_t)c->h_kmer_cur[q] + nk[q] >= (1ull << 30)) throw ArgError{"partition exceeds 2^30 k-mers; use more partitions", GKC_EINVAL};
         bins_reserve(c, nw, nh);
-        // scatter pass
-        launch_superkmers(c, cd, p1params(c, 1), binset(c, false));
+        // scatter pass (optionally also emitting the per-window minimizer records)
+        P1Params ps = p1params(c, 1);
+        if (flags & GKC_EMIT_WINDOWS) {
+            if (c->prm.p > 25) throw ArgError{"per-window emission needs p <= 25", GKC_EINVAL};
+            const uint64_t B = cd.n_bases;
+            c->d_win_mm.ensure((B + 1) * 4); c->d_win_pos.ensure(B + 1); c->d_win_str.ensure(B + 1);
+            ps.win_mm = c->d_win_mm.as<uint32_t>(); ps.win_pos = c->d_win_pos.as<uint8_t>();
+            ps.win_strand = c->d_win_str.as<int8_t>();
+            c->win_n = B; c->win_lead = cd.lead;
+        }
+        launch_superkmers(c, cd, ps, binset(c, false));
         HIPX(hipEventRecord(c->ev[1], c->st));
         HIPX(hipStreamSynchronize(c->st));
         uint32_t ovf = 0;
@@ -417,8 +524,13 @@ int gkc_partition(gkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n
 
 int gkc_count(gkc_ctx* c, uint32_t flags) {
     return guarded(c, [&] {
-        if (c->keyw == 8) count_impl<uint64_t>(c, flags);
-        else count_impl<u128k>(c, flags);
+        if (flags & GKC_BATCHED) {
+            if (c->keyw == 8) count_batched<uint64_t>(c, flags);
+            else count_batched<u128k>(c, flags);
+        } else {
+            if (c->keyw == 8) count_impl<uint64_t>(c, flags);
+            else count_impl<u128k>(c, flags);
+        }
         c->counted = true;
         return 0;
     });
@@ -441,25 +553,39 @@ int64_t gkc_table_copy(gkc_ctx* c, int32_t part, uint64_t* key_lo, uint64_t* key
         if (!c->counted) throw ArgError{"gkc_count has not run", GKC_ESTATE};
         const uint32_t P = c->prm.n_parts;
         if (part >= (int32_t)P) throw ArgError{"partition out of range", GKC_EINVAL};
-        const uint64_t a = part < 0 ? 0 : c->part_off[part], b = part < 0 ? c->part_off[P] : c->part_off[part + 1];
-        const uint64_t n = b - a;
+        const uint32_t q0 = part < 0 ? 0 : part, q1 = part < 0 ? P : part + 1;
+        const uint64_t a = c->part_off[q0], n = c->part_off[q1] - a;
         if (n > cap) throw ArgError{"table copy buffer too small", GKC_ECAP};
-        if (n) {
-            if (c->keyw == 8) {
-                HIPX(hipMemcpyAsync(key_lo, c->tkeys.as<uint64_t>() + a, n * 8, hipMemcpyDeviceToHost, c->st));
-            } else {
-                std::vector<u128k> tmp(n);
-                HIPX(hipMemcpyAsync(tmp.data(), c->tkeys.as<u128k>() + a, n * 16, hipMemcpyDeviceToHost, c->st));
-                HIPX(hipStreamSynchronize(c->st));
-                for (uint64_t i = 0; i < n; i++) { key_lo[i] = tmp[i].lo; if (key_hi) key_hi[i] = tmp[i].hi; }
+        std::vector<u128k> tmp;
+        if (c->keyw == 16) tmp.resize(n);
+        // runs of partitions that are physically contiguous are copied at once
+        uint32_t q = q0;
+        while (q < q1) {
+            uint32_t e = q + 1;
+            while (e < q1 && c->part_start[e] == c->part_start[q] + (c->part_off[e] - c->part_off[q])) e++;
+            const uint64_t m = c->part_off[e] - c->part_off[q], src = c->part_start[q], dst = c->part_off[q] - a;
+            if (m) {
+                if (c->keyw == 8) HIPX(hipMemcpyAsync(key_lo + dst, c->tkeys.as<uint64_t>() + src, m * 8, hipMemcpyDeviceToHost, c->st));
+                else HIPX(hipMemcpyAsync(tmp.data() + dst, c->tkeys.as<u128k>() + src, m * 16, hipMemcpyDeviceToHost, c->st));
+                if (cnt) HIPX(hipMemcpyAsync(cnt + dst, c->tcnts.as<uint32_t>() + src, m * 4, hipMemcpyDeviceToHost, c->st));
             }
-            if (cnt) HIPX(hipMemcpyAsync(cnt, c->tcnts.as<uint32_t>() + a, n * 4, hipMemcpyDeviceToHost, c->st));
-            HIPX(hipStreamSynchronize(c->st));
+            q = e;
         }
+        HIPX(hipStreamSynchronize(c->st));
+        if (c->keyw == 16)
+            for (uint64_t i = 0; i < n; i++) { key_lo[i] = tmp[i].lo; if (key_hi) key_hi[i] = tmp[i].hi; }
         ret = (int64_t)n;
         return 0;
     });
     return rc < 0 ? rc : ret;
+}
+
+int gkc_table_starts(gkc_ctx* c, uint64_t* starts) {
+    return guarded(c, [&] {
+        if (!c->counted) throw ArgError{"gkc_count has not run", GKC_ESTATE};
+        std::memcpy(starts, c->part_start.data(), c->prm.n_parts * 8);
+        return 0;
+    });
 }
 
 int gkc_table_device(gkc_ctx* c, void** keys, uint32_t** counts, uint64_t* n_entries) {
@@ -691,6 +817,7 @@ static void count_impl(gkc_ctx* c, uint32_t flags) {
         c->part_distinct[q] = nd;
     }
     c->part_off[P] = out_off;
+    c->part_start.assign(c->part_off.begin(), c->part_off.end() - 1);
     HIPX(hipEventRecord(c->ev[1], c->st));
     HIPX(hipEventSynchronize(c->ev[1]));
     float total = 0;
@@ -772,4 +899,219 @@ extern "C" int gkc_count_packed(gkc_ctx* c, const uint32_t* words, uint64_t n_wo
         *n_entries = ne; *n_distinct = nd;
         return 0;
     });
+}
+
+// ---------------------------------------------------------------- batched form
+
+// Enqueue extraction + all onesweep passes over every partition at once.
+// Sizes are read by the kernels from the device plan, so the sequence can be
+// captured once and replayed.  With `timed`, per-pass events are recorded.
+template <typename K>
+static K* enqueue_sort_all(gkc_ctx* c, SegArrays sa, uint64_t tiles_cap, bool timed) {
+    const uint32_t P = c->prm.n_parts;
+    Plan* plan = c->d_plan.as<Plan>();
+    const int TILE = sort_tile<K>();
+    hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(64), 0, c->st, plan, P, (uint32_t)c->npass);
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, c->st, plan, sa, TILE);
+    zero(c, sa.hist, (uint64_t)P * MAX_PASS * RADIX);
+    if (timed) HIPX(hipEventRecord(c->ev[2], c->st));
+    const int g_ex = occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu);
+    hipLaunchKernelGGL(k_extract<K>, dim3(g_ex), dim3(SORT_NT), 0, c->st, plan, sa, c->hdrs.as<SkmHdr>(),
+                       c->d_hdr_base.as<uint64_t>(), c->words.as<uint32_t>(), c->d_word_base.as<uint64_t>(),
+                       c->keysA.as<K>(), c->prm.k);
+    hipLaunchKernelGGL(k_hist_scan, dim3((P * MAX_PASS + 3) / 4), dim3(256), 0, c->st, plan, sa.hist);
+    if (timed) HIPX(hipEventRecord(c->ev[3], c->st));
+    const int g_s = occupancy_blocks((const void*)k_onesweep<K>, SORT_NT, c->n_cu);
+    K* src = c->keysA.as<K>();
+    K* dst = c->keysB.as<K>();
+    for (int ps = 0; ps < c->npass; ps++) {
+        zero(c, c->lookback.p, tiles_cap * RADIX);
+        if (timed) HIPX(hipEventRecord(c->evp[2 * ps], c->st));
+        hipLaunchKernelGGL(k_onesweep<K>, dim3(g_s), dim3(SORT_NT), 0, c->st, plan, sa, src, dst, c->lookback.as<uint32_t>(), ps);
+        if (timed) HIPX(hipEventRecord(c->evp[2 * ps + 1], c->st));
+        std::swap(src, dst);
+    }
+    if (timed) HIPX(hipEventRecord(c->ev[6], c->st));
+    HIPX(hipGetLastError());
+    (void)TILE;
+    return src;
+}
+
+template <typename K>
+static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const K* sorted, uint64_t rl_tiles_cap, int emit) {
+    const uint32_t P = c->prm.n_parts;
+    Plan* plan = c->d_plan.as<Plan>();
+    TableOut to;
+    to.keys = emit ? c->tkeys.p : nullptr; to.counts = emit ? c->tcnts.as<uint32_t>() : nullptr;
+    to.table_start = c->d_tstart.as<uint64_t>(); to.table_cap = c->d_tcap.as<uint32_t>();
+    to.n_entries = c->d_nent.as<uint32_t>(); to.n_distinct = c->d_ndist.as<uint32_t>();
+    to.overflow = c->d_toverflow.as<uint32_t>();
+    to.min_cnt = c->prm.min_count; to.max_cnt = c->prm.max_count; to.emit = emit;
+    zero(c, c->d_nent.p, P); zero(c, c->d_ndist.p, P); zero(c, c->d_toverflow.p, 1);
+    zero(c, c->lookback.p, rl_tiles_cap);
+    zero(c, &plan->ctr[1 + MAX_PASS], 1);
+    const int g_r = occupancy_blocks((const void*)k_runlength<K>, SORT_NT, c->n_cu);
+    hipLaunchKernelGGL(k_runlength<K>, dim3(g_r), dim3(SORT_NT), 0, c->st, plan, sa, sorted, to, c->lookback.as<uint32_t>());
+    HIPX(hipGetLastError());
+}
+
+template <typename F>
+static hipGraphExec_t capture(gkc_ctx* c, F&& enqueue) {
+    hipGraph_t g;
+    HIPX(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue();
+    } catch (...) {
+        hipGraph_t dead;
+        (void)hipStreamEndCapture(c->st, &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+        throw;
+    }
+    HIPX(hipStreamEndCapture(c->st, &g));
+    hipGraphExec_t ex;
+    HIPX(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+    HIPX(hipGraphDestroy(g));
+    return ex;
+}
+
+template <typename K>
+static void count_batched(gkc_ctx* c, uint32_t flags) {
+    const uint32_t P = c->prm.n_parts;
+    const bool timed = (flags & GKC_SYNC_ONLY) == 0 && getenv("GKC_NO_GRAPH") != nullptr;
+    c->part_off.assign(P + 1, 0);
+    c->part_distinct.assign(P, 0);
+    c->part_kmers.assign(P, 0);
+    uint64_t tot_keys = 0, tot_hdrs = 0, tot_words = 0;
+    for (uint32_t q = 0; q < P; q++) {
+        c->part_kmers[q] = c->h_kmer_cur[q];
+        tot_keys += c->h_kmer_cur[q]; tot_hdrs += c->h_hdr_cur[q]; tot_words += c->h_word_cur[q];
+    }
+    c->stats = gkc_stats{};
+    c->stats.n_kmers = tot_keys; c->stats.n_superkmers = tot_hdrs; c->stats.n_words = tot_words;
+    c->stats.n_pass = c->npass;
+    ensure_count_buffers<K>(c, tot_keys, P);
+    const uint64_t tiles_cap = tot_keys / sort_tile<K>() + P + 1;
+    const uint64_t rl_tiles_cap = tot_keys / RL_TILE + P + 1;
+    SegArrays sa;
+    sa.seg_keys = c->d_kmer_cur.as<uint32_t>(); sa.seg_hdrs = c->d_hdr_cur.as<uint32_t>();
+    sa.key_start = c->d_keystart.as<uint64_t>(); sa.item_start = c->d_itemstart.as<uint32_t>();
+    sa.sort_tile_start = c->d_tilestart.as<uint32_t>(); sa.rl_tile_start = c->d_rlstart.as<uint32_t>();
+    sa.hist = c->d_hist.as<uint32_t>();
+    const K* sorted = (c->npass % 2 == 0) ? c->keysA.as<K>() : c->keysB.as<K>();
+
+    // table capacities: predicted (presized) or a count pass (count-then-allocate)
+    const bool presized = (flags & GKC_PRESIZED) && c->presize_caps.size() == P;
+    (void)occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu);
+    (void)occupancy_blocks((const void*)k_onesweep<K>, SORT_NT, c->n_cu);
+    (void)occupancy_blocks((const void*)k_runlength<K>, SORT_NT, c->n_cu);
+    HIPX(hipEventRecord(c->ev[0], c->st));
+    std::vector<const void*> key = {c->keysA.p, c->keysB.p, c->lookback.p, c->words.p, c->hdrs.p,
+                                    (const void*)(uintptr_t)tiles_cap, (const void*)(uintptr_t)c->npass};
+    if (timed) {
+        enqueue_sort_all<K>(c, sa, tiles_cap, true);
+    } else {
+        if (!c->g_sort || c->g_key != key) {
+            if (c->g_sort) { HIPX(hipGraphExecDestroy(c->g_sort)); c->g_sort = nullptr; }
+            if (c->g_cnt) { HIPX(hipGraphExecDestroy(c->g_cnt)); c->g_cnt = nullptr; }
+            if (c->g_emit) { HIPX(hipGraphExecDestroy(c->g_emit)); c->g_emit = nullptr; }
+            c->g_sort = capture(c, [&] { enqueue_sort_all<K>(c, sa, tiles_cap, false); });
+            c->g_cnt = capture(c, [&] { enqueue_runlength_all<K>(c, sa, sorted, rl_tiles_cap, 0); });
+            c->g_key = key;
+        }
+        HIPX(hipGraphLaunch(c->g_sort, c->st));
+    }
+    HIPX(hipEventRecord(c->ev[4], c->st));
+    std::vector<uint32_t>& caps = c->h_caps;
+    caps.assign(P, 0);
+    if (presized) {
+        caps = c->presize_caps;
+    } else {
+        if (timed) enqueue_runlength_all<K>(c, sa, sorted, rl_tiles_cap, 0);
+        else HIPX(hipGraphLaunch(c->g_cnt, c->st));
+        HIPX(hipMemcpyAsync(caps.data(), c->d_nent.p, P * 4, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipStreamSynchronize(c->st));
+        c->stats.host_syncs++;
+    }
+    // table placement (physical starts) and capacity
+    c->h_tstart.assign(P, 0);
+    uint64_t tot_cap = 0;
+    for (uint32_t q = 0; q < P; q++) { c->h_tstart[q] = tot_cap; tot_cap += caps[q]; }
+    const void* old_tk = c->tkeys.p;
+    c->tkeys.ensure((tot_cap + 1) * sizeof(K));
+    c->tcnts.ensure((tot_cap + 1) * 4);
+    HIPX(hipMemcpyAsync(c->d_tstart.p, c->h_tstart.data(), P * 8, hipMemcpyHostToDevice, c->st));
+    HIPX(hipMemcpyAsync(c->d_tcap.p, caps.data(), P * 4, hipMemcpyHostToDevice, c->st));
+    if (timed) {
+        enqueue_runlength_all<K>(c, sa, sorted, rl_tiles_cap, 1);
+    } else {
+        if (!c->g_emit || old_tk != c->tkeys.p) {
+            if (c->g_emit) { HIPX(hipGraphExecDestroy(c->g_emit)); c->g_emit = nullptr; }
+            c->g_emit = capture(c, [&] { enqueue_runlength_all<K>(c, sa, sorted, rl_tiles_cap, 1); });
+        }
+        HIPX(hipGraphLaunch(c->g_emit, c->st));
+    }
+    HIPX(hipEventRecord(c->ev[5], c->st));
+    std::vector<uint32_t>& nent = c->h_nent;
+    nent.assign(P, 0);
+    std::vector<uint32_t> ndist(P);
+    uint32_t ovf = 0;
+    HIPX(hipMemcpyAsync(nent.data(), c->d_nent.p, P * 4, hipMemcpyDeviceToHost, c->st));
+    HIPX(hipMemcpyAsync(ndist.data(), c->d_ndist.p, P * 4, hipMemcpyDeviceToHost, c->st));
+    HIPX(hipMemcpyAsync(&ovf, c->d_toverflow.p, 4, hipMemcpyDeviceToHost, c->st));
+    HIPX(hipStreamSynchronize(c->st));
+    c->stats.host_syncs++;
+    if (ovf) {
+        // under-prediction caught on device: count-then-allocate emission with
+        // the exact per-partition entry counts the emit pass just measured
+        c->stats.fallbacks++;
+        caps = nent;
+        tot_cap = 0;
+        for (uint32_t q = 0; q < P; q++) { c->h_tstart[q] = tot_cap; tot_cap += caps[q]; }
+        const void* before = c->tkeys.p;
+        c->tkeys.ensure((tot_cap + 1) * sizeof(K));
+        c->tcnts.ensure((tot_cap + 1) * 4);
+        HIPX(hipMemcpyAsync(c->d_tstart.p, c->h_tstart.data(), P * 8, hipMemcpyHostToDevice, c->st));
+        HIPX(hipMemcpyAsync(c->d_tcap.p, caps.data(), P * 4, hipMemcpyHostToDevice, c->st));
+        if (timed || before != c->tkeys.p) {
+            if (!timed && c->g_emit) { HIPX(hipGraphExecDestroy(c->g_emit)); c->g_emit = nullptr; }
+            enqueue_runlength_all<K>(c, sa, sorted, rl_tiles_cap, 1);
+        } else {
+            HIPX(hipGraphLaunch(c->g_emit, c->st));
+        }
+        HIPX(hipMemcpyAsync(nent.data(), c->d_nent.p, P * 4, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipMemcpyAsync(&ovf, c->d_toverflow.p, 4, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipStreamSynchronize(c->st));
+        c->stats.host_syncs++;
+        if (ovf) throw ArgError{"internal: table overflow after exact sizing", GKC_ESTATE};
+    }
+    HIPX(hipEventRecord(c->ev[1], c->st));
+    HIPX(hipEventSynchronize(c->ev[1]));
+    c->part_start = c->h_tstart;
+    uint64_t acc = 0, nd = 0;
+    for (uint32_t q = 0; q < P; q++) {
+        c->part_off[q] = acc; acc += nent[q];
+        c->part_distinct[q] = ndist[q]; nd += ndist[q];
+    }
+    c->part_off[P] = acc;
+    float total = 0, s1 = 0, s2 = 0;
+    HIPX(hipEventElapsedTime(&total, c->ev[0], c->ev[1]));
+    HIPX(hipEventElapsedTime(&s1, c->ev[0], c->ev[4]));
+    HIPX(hipEventElapsedTime(&s2, c->ev[4], c->ev[5]));
+    c->stats.ms_count = total;
+    c->stats.ms_runlength = s2;
+    if (timed) {
+        float a = 0, b = 0, pass_sum = 0;
+        HIPX(hipEventElapsedTime(&a, c->ev[2], c->ev[3]));
+        HIPX(hipEventElapsedTime(&b, c->ev[3], c->ev[6]));
+        for (int ps = 0; ps < c->npass; ps++) {
+            float m = 0;
+            HIPX(hipEventElapsedTime(&m, c->evp[2 * ps], c->evp[2 * ps + 1]));
+            pass_sum += m;
+        }
+        c->stats.ms_extract = a; c->stats.ms_sort = b;
+        c->stats.ms_sort_pass_avg = pass_sum / c->npass;
+    } else {
+        c->stats.ms_sort = s1;
+    }
+    c->stats.n_distinct = nd; c->stats.n_entries = acc;
 }

@@ -14,11 +14,11 @@ struct ChunkDesc {
     const uint8_t* reads;
     const uint64_t* offs;     // n_reads + 1
     const int32_t* lens;      // optional per-read valid length (HPC), else null
-    uint64_t base;            // offs[0]
+    uint64_t base;            // offs[0] - lead
     uint32_t n_reads;
     uint32_t n_bases;         // offs[n] - offs[0]
     uint32_t n_tiles;         // ceil(n_bases / tile)
-    uint32_t pad;
+    uint32_t lead;            // positions [0, lead) precede read 0 (16-byte alignment of `reads`)
 };
 
 // Superkmer header in a partition bin (16 bytes).

@@ -31,9 +31,12 @@ struct P1Params {
     int scatter;                // 0: count only, 1: write bins
     uint32_t* sketch;           // [P][256] sampled 4-mer counts, or null
     uint32_t* mm_out;           // per-window minimizer (compat), or null
+    uint32_t* win_mm;           // per-window (minimizer, position, strand) output, or null
+    uint8_t* win_pos;           //   offset in the window of the chosen p-mer (255: none passes)
+    int8_t* win_strand;         //   0 forward, 1 reverse complement, -1 tie / none
 };
 
-template <typename MM>
+template <typename MM, bool WIN>
 __global__ void k_superkmers(const ChunkDesc* __restrict__ cdp, P1Params prm, BinSet bins, uint32_t* __restrict__ tile_ctr);
 __global__ void k_hpc(const uint8_t* __restrict__ reads, const uint64_t* __restrict__ offs, uint32_t n_reads,
                       uint8_t* __restrict__ out, int32_t* __restrict__ lens, int32_t* __restrict__ orig_pos);

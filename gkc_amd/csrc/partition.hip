@@ -24,9 +24,15 @@ __device__ __forceinline__ uint32_t find_read(const uint64_t* offs, uint32_t n, 
     return lo;
 }
 
-template <typename MM>
+// With WIN, every candidate carries its position: the window minimum runs
+// over composites (value << 14 | no-candidate << 13 | 8191 - start), so ties
+// go to the later p-mer start (the reference's update order, gkc_cuda.cu:281-284)
+// and the chosen start and strand are recovered per window.
+template <typename MM, bool WIN>
 __global__ __launch_bounds__(P1_NT) void k_superkmers(const ChunkDesc* __restrict__ cdp, P1Params prm,
                                                        BinSet bins, uint32_t* __restrict__ tile_ctr) {
+    constexpr int CSH = WIN ? 14 : 0;
+    __shared__ int8_t s_str[WIN ? P1_T + MAXK : 1];
     __shared__ __align__(16) uint8_t s_bytes[P1_T + MAXK + 32];
     __shared__ MM s_v[P1_T + MAXK];
     __shared__ MM s_last[P1_NT];
@@ -43,6 +49,7 @@ __global__ __launch_bounds__(P1_NT) void k_superkmers(const ChunkDesc* __restric
     const int tid = threadIdx.x;
     const int k = prm.k, p = prm.p, w = k - p + 1;
     const MM M = (MM)((p >= (int)(sizeof(MM) * 4)) ? ~(MM)0 : (((MM)1 << (2 * p)) - 1));
+    const MM MC = WIN ? (MM)((M << CSH) | (MM)0x3FFF) : M;     // largest composite
     const uint32_t P = prm.P;
 
     for (;;) {
@@ -81,9 +88,20 @@ __global__ __launch_bounds__(P1_NT) void k_superkmers(const ChunkDesc* __restric
                 uint32_t c = base_code(s_bytes[j0 + s + p - 1]);
                 f = ((f << 2) | c) & M;
                 r = (r >> 2) | ((MM)(3u - c) << rs);
-                MM a = pmer_pass(f, p, prm.fmode) ? f : M;
-                MM b = pmer_pass(r, p, prm.fmode) ? r : M;
-                s_v[j0 + s] = a < b ? a : b;
+                const bool fp = pmer_pass(f, p, prm.fmode), rp = pmer_pass(r, p, prm.fmode);
+                if constexpr (WIN) {
+                    MM val; int sent = 0; int8_t st;
+                    if (fp && rp) { val = f < r ? f : r; st = f < r ? 0 : (r < f ? 1 : -1); }
+                    else if (fp) { val = f; st = 0; }
+                    else if (rp) { val = r; st = 1; }
+                    else { val = M; sent = 1; st = -1; }
+                    s_v[j0 + s] = (val << CSH) | ((MM)sent << 13) | (MM)(8191 - (j0 + s));
+                    s_str[j0 + s] = st;
+                } else {
+                    MM a = fp ? f : M;
+                    MM b = rp ? r : M;
+                    s_v[j0 + s] = a < b ? a : b;
+                }
             }
         };
         fill_v(tid * P1_V, P1_V);
@@ -107,11 +125,11 @@ __global__ __launch_bounds__(P1_NT) void k_superkmers(const ChunkDesc* __restric
                 // window s = [b+s, b+s+w-1] = left(s) U [b+V-1, b+w-1] U right(s), where
                 // left(s) = vv[s..V-2] (suffix minima) and right(s) = s_v[b+w .. b+w+s-1]
                 MM left[P1_V];
-                left[P1_V - 1] = M;
-                MM suf = M;
+                left[P1_V - 1] = MC;
+                MM suf = MC;
 #pragma unroll
                 for (int s = P1_V - 2; s >= 0; s--) { MM x = vv[s]; suf = x < suf ? x : suf; left[s] = suf; }
-                MM pre = M;
+                MM pre = MC;
 #pragma unroll
                 for (int s = 0; s < P1_V; s++) {
                     MM a = left[s] < mid ? left[s] : mid;
@@ -121,12 +139,12 @@ __global__ __launch_bounds__(P1_NT) void k_superkmers(const ChunkDesc* __restric
             } else {
 #pragma unroll
                 for (int s = 0; s < P1_V; s++) {
-                    MM a = M;
+                    MM a = MC;
                     for (int j = 0; j < w; j++) { MM x = s_v[b + s + j]; a = x < a ? x : a; }
                     mm[s] = a;
                 }
             }
-            s_last[tid] = mm[P1_V - 1];
+            s_last[tid] = mm[P1_V - 1] >> CSH;
         }
         __syncthreads();
 
@@ -138,12 +156,17 @@ __global__ __launch_bounds__(P1_NT) void k_superkmers(const ChunkDesc* __restric
             uint32_t r = 0;
             uint64_t rbeg = 0, rnext = 0, rend = 0;
             if (tid * P1_V < (int)tlen) {
-                r = find_read(cd.offs, cd.n_reads, cd.base + g0);
+                r = find_read(cd.offs, cd.n_reads, cd.base + max(g0, cd.lead));
                 rbeg = cd.offs[r] - cd.base;
                 rnext = cd.offs[r + 1] - cd.base;
                 rend = cd.lens ? rbeg + (uint32_t)cd.lens[r] : rnext;
             }
             MM prev = tid > 0 ? s_last[tid - 1] : (MM)0;
+            MM cmp[P1_V];
+#pragma unroll
+            for (int s = 0; s < P1_V; s++) cmp[s] = mm[s];
+#pragma unroll
+            for (int s = 0; s < P1_V; s++) mm[s] = cmp[s] >> CSH;      // minimizer values
 #pragma unroll
             for (int s = 0; s < P1_V; s++) {
                 const uint32_t g = g0 + s;
@@ -155,13 +178,22 @@ __global__ __launch_bounds__(P1_NT) void k_superkmers(const ChunkDesc* __restric
                         rnext = cd.offs[r + 1] - cd.base;
                         rend = cd.lens ? rbeg + (uint32_t)cd.lens[r] : rnext;
                     }
-                    const bool valid = (uint64_t)g + k <= rend;
+                    const bool valid = g >= rbeg && (uint64_t)g + k <= rend;
                     if (valid) {
                         const bool first = (g == rbeg) || (g == t0);
                         if (first || mm[s] != prev) bmask |= 1u << s;
                         uint64_t we = rend - k + 1 - t0;           // tile-local end of windows
                         wend[s] = (uint32_t)min(we, (uint64_t)tlen);
                         if (prm.mm_out) prm.mm_out[g] = (uint32_t)mm[s];
+                        if constexpr (WIN) {
+                            if (prm.win_mm) {
+                                const bool none = (cmp[s] >> 13) & 1;
+                                const uint32_t js = 8191u - (uint32_t)(cmp[s] & 8191u);   // tile-local p-mer start
+                                prm.win_mm[g] = (uint32_t)mm[s];
+                                prm.win_pos[g] = none ? (uint8_t)255 : (uint8_t)(js - (g - t0));
+                                prm.win_strand[g] = none ? (int8_t)-1 : s_str[js];
+                            }
+                        }
                         if (prm.sketch && ((g - rbeg) & 15u) == 0) {
                             uint32_t q = 0;
                             const int jl = g - t0;
@@ -342,7 +374,8 @@ __global__ __launch_bounds__(256) void k_skm_lists(const uint64_t* __restrict__ 
     }
 }
 
-template __global__ void k_superkmers<uint32_t>(const ChunkDesc*, P1Params, BinSet, uint32_t*);
-template __global__ void k_superkmers<uint64_t>(const ChunkDesc*, P1Params, BinSet, uint32_t*);
+template __global__ void k_superkmers<uint32_t, false>(const ChunkDesc*, P1Params, BinSet, uint32_t*);
+template __global__ void k_superkmers<uint64_t, false>(const ChunkDesc*, P1Params, BinSet, uint32_t*);
+template __global__ void k_superkmers<uint64_t, true>(const ChunkDesc*, P1Params, BinSet, uint32_t*);
 
 }  // namespace gkc

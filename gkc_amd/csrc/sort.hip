@@ -352,32 +352,21 @@ __global__ __launch_bounds__(SORT_NT) void k_runlength(Plan* __restrict__ plan, 
             s_first[threadIdx.x] = min(s_first[threadIdx.x], y);
             __syncthreads();
         }
-        // end of the tile's last run: follow it past the tile (block-wide)
-        {
-            const uint32_t last = tn - 1;
-            const K lastk = (threadIdx.x == (last / RL_ITEMS)) ? x[last % RL_ITEMS] : x[0];
-            __shared__ K s_lastk;
-            if (threadIdx.x == last / RL_ITEMS) s_lastk = lastk;
-            __syncthreads();
-            const K lk = s_lastk;
+        // end of the tile's last run: wave 0 follows it past the tile, 64 keys
+        // per step, until the key changes or the segment ends
+        if (threadIdx.x < GKC_WAVE) {
+            const K lk = kp[t0 + tn - 1];
             uint32_t e = t0 + tn;
-            while (e < nseg) {
-                const uint32_t j = e + threadIdx.x;
+            for (;;) {
+                const uint32_t j = e + __lane_id();
                 const bool diff = j >= nseg || !(kp[j] == lk);
                 const uint64_t m = __ballot(diff);
-                __shared__ uint32_t s_hit;
-                if (threadIdx.x == 0) s_hit = 0xFFFFFFFFu;
-                __syncthreads();
-                if (m && __lane_id() == (uint32_t)__builtin_ctzll(m)) atomicMin(&s_hit, j);
-                __syncthreads();
-                const uint32_t hit = s_hit;
-                __syncthreads();
-                if (hit != 0xFFFFFFFFu) { e = hit; break; }
-                e += SORT_NT;
+                if (m) { e += (uint32_t)__builtin_ctzll(m); break; }
+                e += GKC_WAVE;
             }
-            if (threadIdx.x == 0) s_end = min(e, nseg) - t0;   // tile-local end of last run
-            __syncthreads();
+            if (threadIdx.x == 0) s_end = e - t0;     // tile-local end of the last run
         }
+        __syncthreads();
         // counts and filter
         uint32_t keep = 0, nheads = 0;
         uint32_t cnt[RL_ITEMS];

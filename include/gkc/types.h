@@ -1,0 +1,60 @@
+/*
+ * Scalar types and host records of the counting path, laid out as the
+ * reference declares them (reference types.h:10-53) so code written against
+ * the reference compiles unchanged against the drop-in.
+ */
+#ifndef GKC_DROPIN_TYPES_H
+#define GKC_DROPIN_TYPES_H
+
+#include <cstddef>
+#include <cstdint>
+
+#ifndef _in_
+#define _in_
+#endif
+#ifndef _out_
+#define _out_
+#endif
+
+typedef size_t T_CSR_cap;          /* CSR offsets */
+typedef int T_CSR_count;
+typedef unsigned short T_skm_len;  /* the reference's superkmer length type */
+typedef int T_read_len;
+typedef int T_read_cnt;
+typedef unsigned int T_minimizer;  /* minimizer values, p <= 16 */
+const T_minimizer T_MM_MAX = (T_minimizer)0xFFFFFFFFu;
+typedef unsigned char T_kvalue;
+
+struct ReadPtr {
+    const char *read;
+    T_read_len len;
+};
+
+typedef unsigned long long T_kmer;       /* k <= 32 */
+typedef unsigned long long T_2bitkmer;
+
+/* Host arrays handed to GenSuperkmerGPU's callback once per sub-batch. */
+struct T_h_data {
+    T_read_cnt reads_cnt;
+    char *reads;                  /* the sub-batch's bases (HPC-compressed when HPC) */
+    T_CSR_cap *reads_offs;        /* absolute offsets, reads_cnt + 1 */
+    T_read_len *read_len;         /* HPC read lengths (HPC only) */
+    T_read_len *hpc_orig_pos;     /* original in-read index per HPC base (HPC only) */
+    T_minimizer *minimizers;      /* per batch-local base: window minimizer */
+    T_read_len *superkmer_offs;   /* per read: [0, i1, .., len-k+1, 0] at the read's batch-local start */
+};
+
+typedef unsigned int T_kmer_cnt;
+const T_kmer_cnt MAX_KMER_CNT = (T_kmer_cnt)0xFFFFFFFFu;
+struct T_kmc {
+    T_kmer kmer;
+    T_kmer_cnt cnt;
+};
+/* k in (32, 64]: the reference has no 128-bit record; this one holds the
+ * canonical k-mer as hi:lo. */
+struct T_kmc128 {
+    unsigned long long hi, lo;
+    T_kmer_cnt cnt;
+};
+
+#endif

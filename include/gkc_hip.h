@@ -36,6 +36,7 @@ enum {
 #define GKC_PRESIZED       4u   /* gkc_count: size tables from gkc_set_table_caps / the learned
                                    predictor instead of a count pass (overflow falls back) */
 #define GKC_SYNC_ONLY      8u   /* internal: no host sync beyond what the mode needs */
+#define GKC_EMIT_WINDOWS  32u   /* gkc_partition: also write per-window (minimizer, position, strand) */
 
 /* Counting parameters (the reference's PAR fields K_kmer, P_minimizer, HPC,
  * SKM_partitions, kmer_min_freq, kmer_max_freq; utilities.hpp:104-123). */
@@ -100,6 +101,18 @@ int gkc_superkmer_arrays(gkc_ctx *ctx, const char *reads, const uint64_t *offs, 
  * context's partition bins (appending across calls).  With
  * GKC_DEVICE_INPUT the pointers are device memory. */
 int gkc_partition(gkc_ctx *ctx, const char *reads, const uint64_t *offs, int64_t n, uint32_t flags);
+/* Per-window (minimizer, position, strand) records of one batch (host
+ * arrays, batch-local base index, HPC coordinates when hpc): the reference
+ * declares these fields but never fills them (gkc_cuda.cu:159-160,
+ * CountTask::StoreMinimizerPos gkc_cuda.hpp:50).  pos = offset in the window
+ * of the chosen p-mer (ties: the later start, forward before reverse
+ * complement), 255 when no p-mer passes the filter; strand 0 forward,
+ * 1 reverse complement, -1 when both strands tie or none passes.  p <= 25. */
+int gkc_window_minimizers(gkc_ctx *ctx, const char *reads, const uint64_t *offs, int64_t n,
+                          uint32_t *mm, uint8_t *pos, int8_t *strand);
+/* Device pointers to the records written by the last gkc_partition with
+ * GKC_EMIT_WINDOWS (valid until the next such call). */
+int gkc_windows_device(gkc_ctx *ctx, uint32_t **mm, uint8_t **pos, int8_t **strand, uint64_t *n);
 /* Clear the bins and the table. */
 int gkc_reset(gkc_ctx *ctx);
 
@@ -115,6 +128,9 @@ int gkc_table_info(gkc_ctx *ctx, uint64_t *part_off, uint64_t *part_distinct, ui
  * key_lo[i] (k <= 32: the key), key_hi[i] (k > 32 only, else may be NULL), cnt[i]. */
 int64_t gkc_table_copy(gkc_ctx *ctx, int32_t part, uint64_t *key_lo, uint64_t *key_hi, uint32_t *cnt,
                        uint64_t cap);
+/* Physical start of each partition's entries in the device table (a presized
+ * count leaves unused capacity between partitions; part_off is logical). */
+int gkc_table_starts(gkc_ctx *ctx, uint64_t *starts);
 /* Device pointers of the table (valid until the next gkc_count/gkc_reset). */
 int gkc_table_device(gkc_ctx *ctx, void **keys, uint32_t **counts, uint64_t *n_entries);
 
@@ -128,6 +144,14 @@ int gkc_count_packed(gkc_ctx *ctx, const uint32_t *words, uint64_t n_words, cons
 
 int gkc_get_stats(gkc_ctx *ctx, gkc_stats *out);
 
+/* Pinned host memory for read batches (the reference's PinnedCSR uses
+ * cudaHostAlloc, gkc_cuda.cu:121-139).  NULL on failure. */
+void *gkc_host_alloc(uint64_t bytes);
+void gkc_host_free(void *p);
+/* Make `device` current for the calling thread and wait for it to be idle
+ * (the reference's GPUReset, gkc_cuda.cu:367-373, minus the device reset). */
+int gkc_device_select(int device);
+
 /* Per-partition 4-mer sketch taken during gkc_partition: float [P][256]
  * holding log1p(count) of the 4-mer at every 16th window start of each read,
  * by the window's partition. */
@@ -136,6 +160,30 @@ int gkc_sketch(gkc_ctx *ctx, float *out);
 int gkc_set_table_caps(gkc_ctx *ctx, const uint32_t *caps);
 /* Per-partition k-mer totals of the bins (known exactly after gkc_partition). */
 int gkc_part_kmers(gkc_ctx *ctx, uint64_t *out);
+
+/* ---- learned partition pre-sizing (no reference counterpart: the
+ * reference sizes nothing ahead, kmer_counting.cu:95-191 grows host vectors)
+ * MLP 256 -> 128 -> 128 -> 2 with ReLU, FP32 master weights; flat parameter
+ * order W1[256][128], b1[128], W2[128][128], b2[128], W3[128][2], b3[2]. */
+typedef struct gkc_mlp gkc_mlp;
+uint64_t gkc_mlp_n_params(void);
+const char *gkc_mlp_last_error(void);
+gkc_mlp *gkc_mlp_create(int device, uint64_t seed);
+void gkc_mlp_destroy(gkc_mlp *m);
+int gkc_mlp_get_params(gkc_mlp *m, float *out);
+int gkc_mlp_set_params(gkc_mlp *m, const float *in);
+/* FP32 forward of n rows x[n][256] -> y[n][2] (HIP kernels). */
+int gkc_mlp_forward(gkc_mlp *m, const float *x, int64_t n, float *y);
+/* BF16 matrix-core (MFMA) inference with FP32 accumulation. */
+int gkc_mlp_forward_bf16(gkc_mlp *m, const float *x, int64_t n, float *y);
+/* Mean-squared-error loss against t[n][2] and its gradient (flat, n_params). */
+int gkc_mlp_loss_grad(gkc_mlp *m, const float *x, const float *t, int64_t n, float *loss, float *grad);
+/* Adam training (HIP forward + backward kernels); losses[epochs] per epoch. */
+int gkc_mlp_train(gkc_mlp *m, const float *x, const float *t, int64_t n, int epochs, int batch, float lr,
+                  float weight_decay, uint64_t seed, float *losses);
+/* C[32][N] = A[32][K] * B[K][N] on the matrix cores (bf16 in, fp32 out), the
+ * tile routine of the inference kernel; K <= 256, K % 16 == 0. */
+int gkc_bf16_matmul32(int device, const float *A, const float *B, int K, int N, float *C);
 
 #ifdef __cplusplus
 }

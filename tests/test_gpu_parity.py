@@ -155,3 +155,74 @@ def test_count_packed_single_partition(gpu):
     assert dict(zip(lo.tolist(), cnt.tolist())) == bf
     assert nd == len(bf)
     assert np.all(np.diff(lo.astype(np.float64)) > 0) or len(lo) < 2
+
+
+@pytest.mark.parametrize("cid,scale,P", [(1, 0.05, 64), (4, 0.002, 64), (3, 0.002, 512)])
+def test_batched_form_matches_oracle(gpu, cid, scale, P):
+    w = W.workload(cid, scale=scale)
+    rs = W.generate(w)
+    _say("batched", w.name)
+    gt, st = _run_gpu(rs, w.k, w.p, P, hpc=w.hpc, min_count=w.min_count, batches=2, flags=capi.BATCHED)
+    ot = O.count(rs.reads, rs.offs, w.k, w.p, P, hpc=w.hpc, min_count=w.min_count)
+    assert_tables_equal(gt, ot, P, w.k)
+    assert st["host_syncs"] == 2 and st["fallbacks"] == 0
+
+
+def test_batched_presized_and_fallback(gpu):
+    """Exact caps, generous caps and under-predicted caps (device overflow flag ->
+    count-then-allocate fallback) all give the bit-exact table."""
+    w = W.workload(1, scale=0.05)
+    rs = W.generate(w)
+    ot = O.count(rs.reads, rs.offs, 31, 11, 64)
+    exact = np.diff(ot.part_off).astype(np.uint32)
+    for name, caps, fb in [("exact", exact, 0), ("generous", exact * 2 + 5, 0),
+                           ("under", np.maximum(exact // 2, 1), 1)]:
+        _say("presized", name)
+        with capi.Counter(31, 11, 64) as c:
+            c.partition(rs.reads, rs.offs)
+            c.set_table_caps(caps)
+            c.count(capi.BATCHED | capi.PRESIZED)
+            gt = c.table()
+            st = c.stats()
+        assert_tables_equal(gt, ot, 64, 31)
+        assert st["fallbacks"] == fb, name
+        assert st["host_syncs"] == 1 + fb, name
+
+
+def test_batched_graph_replay_stable(gpu):
+    """The captured graph replayed several times (eager work in between) gives
+    the same table every time."""
+    w = W.workload(1, scale=0.03)
+    rs = W.generate(w)
+    ot = O.count(rs.reads, rs.offs, 31, 11, 64)
+    with capi.Counter(31, 11, 64) as c:
+        for rep in range(4):
+            _say("replay", rep)
+            c.reset()
+            c.partition(rs.reads, rs.offs)
+            c.count(capi.BATCHED)
+            assert_tables_equal(c.table(), ot, 64, 31)
+
+
+@pytest.mark.parametrize("cid,scale,k,p,hpc", [(1, 0.01, 31, 11, False), (3, 0.001, 25, 15, True),
+                                               (4, 0.0005, 63, 19, False), (1, 0.01, 20, 20, False)])
+def test_window_records(gpu, cid, scale, k, p, hpc):
+    """(minimizer, position, strand) per window against the oracle's statement
+    of the reference's update order."""
+    rs = W.generate(W.workload(cid, scale=scale))
+    _say("windows", cid, k, p, hpc)
+    with capi.Counter(k, p, 64, hpc=hpc) as c:
+        out = c.window_minimizers(rs.reads, rs.offs)
+        arr = c.superkmer_arrays(rs.reads, rs.offs) if p <= 16 else None
+    src, lens = rs.reads, None
+    if hpc:
+        src, lens, _ = O.hpc(rs.reads, rs.offs)
+        np.testing.assert_array_equal(arr["read_len"], lens)
+    mm, pos, strand = O.window_minimizers(src, rs.offs, k, p, 1, lens=lens)
+    L = lens if hpc else np.diff(rs.offs).astype(np.int64)
+    valid = np.zeros(rs.bases, bool)
+    for r in range(rs.n):
+        valid[int(rs.offs[r]):int(rs.offs[r]) + max(0, int(L[r]) - k + 1)] = True
+    np.testing.assert_array_equal(out["minimizers"][valid], mm[valid].astype(np.uint32))
+    np.testing.assert_array_equal(out["pos"][valid], pos[valid])
+    np.testing.assert_array_equal(out["strand"][valid], strand[valid])
