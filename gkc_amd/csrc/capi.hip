@@ -443,7 +443,7 @@ int gkc_window_minimizers(gkc_ctx* c, const char* reads, const uint64_t* offs, i
                           int8_t* strand) {
     return guarded(c, [&] {
         if (n < 0 || !offs || (n > 0 && !reads)) throw ArgError{"bad batch", GKC_EINVAL};
-        if (c->prm.p > 25) throw ArgError{"per-window emission needs p <= 25", GKC_EINVAL};
+        if (c->prm.p > 16) throw ArgError{"per-window records hold u32 minimizers: p <= 16", GKC_EINVAL};
         if (n == 0) return 0;
         ChunkDesc cd;
         const uint64_t B = stage_batch(c, reads, offs, n, 0, cd);
@@ -500,7 +500,7 @@ int gkc_partition(gkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n
         // scatter pass (optionally also emitting the per-window minimizer records)
         P1Params ps = p1params(c, 1);
         if (flags & GKC_EMIT_WINDOWS) {
-            if (c->prm.p > 25) throw ArgError{"per-window emission needs p <= 25", GKC_EINVAL};
+            if (c->prm.p > 16) throw ArgError{"per-window records hold u32 minimizers: p <= 16", GKC_EINVAL};
             const uint64_t B = cd.n_bases;
             c->d_win_mm.ensure((B + 1) * 4); c->d_win_pos.ensure(B + 1); c->d_win_str.ensure(B + 1);
             ps.win_mm = c->d_win_mm.as<uint32_t>(); ps.win_pos = c->d_win_pos.as<uint8_t>();

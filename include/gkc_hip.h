@@ -107,7 +107,7 @@ int gkc_partition(gkc_ctx *ctx, const char *reads, const uint64_t *offs, int64_t
  * CountTask::StoreMinimizerPos gkc_cuda.hpp:50).  pos = offset in the window
  * of the chosen p-mer (ties: the later start, forward before reverse
  * complement), 255 when no p-mer passes the filter; strand 0 forward,
- * 1 reverse complement, -1 when both strands tie or none passes.  p <= 25. */
+ * 1 reverse complement, -1 when both strands tie or none passes.  p <= 16. */
 int gkc_window_minimizers(gkc_ctx *ctx, const char *reads, const uint64_t *offs, int64_t n,
                           uint32_t *mm, uint8_t *pos, int8_t *strand);
 /* Device pointers to the records written by the last gkc_partition with

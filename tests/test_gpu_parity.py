@@ -205,7 +205,7 @@ def test_batched_graph_replay_stable(gpu):
 
 
 @pytest.mark.parametrize("cid,scale,k,p,hpc", [(1, 0.01, 31, 11, False), (3, 0.001, 25, 15, True),
-                                               (4, 0.0005, 63, 19, False), (1, 0.01, 20, 20, False)])
+                                               (4, 0.0005, 45, 13, False), (1, 0.01, 16, 16, False)])
 def test_window_records(gpu, cid, scale, k, p, hpc):
     """(minimizer, position, strand) per window against the oracle's statement
     of the reference's update order."""
