@@ -40,10 +40,10 @@ class Stats(C.Structure):
                 ("ms_sort", C.c_double), ("ms_runlength", C.c_double), ("ms_sort_pass_avg", C.c_double),
                 ("n_kmers", C.c_uint64), ("n_superkmers", C.c_uint64), ("n_words", C.c_uint64),
                 ("n_distinct", C.c_uint64), ("n_entries", C.c_uint64), ("n_pass", C.c_uint32),
-                ("fallbacks", C.c_uint32), ("host_syncs", C.c_uint32), ("pad", C.c_uint32)]
+                ("fallbacks", C.c_uint32), ("host_syncs", C.c_uint32), ("partition_fallbacks", C.c_uint32)]
 
     def asdict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 # exported symbols and their signatures (every symbol include/gkc_hip.h declares)

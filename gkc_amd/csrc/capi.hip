@@ -89,6 +89,11 @@ struct gkc_ctx {
     // batch staging
     DevBuf d_reads, d_offs, d_lens, d_hpc_reads, d_hpc_pos, d_chunk, d_ctr, d_mm;
     DevBuf d_win_mm, d_win_pos, d_win_str;     // per-window records of the last partition call
+    DevBuf d_backup;                           // cursors before a presized scatter (3 x P)
+    uint64_t bases_done = 0;                   // bases partitioned since the last reset
+    uint32_t p1_fallbacks = 0;                 // presized scatters redone since the last reset
+    hipGraphExec_t g_p1 = nullptr;             // captured presized scatter of one chunk
+    std::vector<const void*> g_p1_key;
     uint64_t win_n = 0;
     uint32_t win_lead = 0;
 
@@ -144,6 +149,9 @@ static int occupancy_blocks(const void* fn, int nt, int n_cu) {
     }
     return b * n_cu;
 }
+
+static void zero(gkc_ctx* c, void* p, uint64_t n32);
+template <typename F> hipGraphExec_t capture(gkc_ctx* c, F&& enqueue);
 
 // ---------------------------------------------------------------- bins
 
@@ -269,15 +277,16 @@ static uint64_t stage_batch(gkc_ctx* c, const char* reads, const uint64_t* offs,
     return B;
 }
 
-static void launch_superkmers(gkc_ctx* c, const ChunkDesc& cd, P1Params prm, BinSet bs) {
+static void launch_superkmers(gkc_ctx* c, const ChunkDesc& cd, P1Params prm, BinSet bs, bool persistent = false) {
     c->d_ctr.ensure(64);
-    HIPX(hipMemsetAsync(c->d_ctr.p, 0, 4, c->st));
-    if (cd.n_tiles == 0) return;
+    zero(c, c->d_ctr.p, 1);
+    if (cd.n_tiles == 0 && !persistent) return;
     const bool win = prm.win_mm != nullptr;
     const bool wide = c->prm.p > 16;
     const void* fn = win ? (const void*)k_superkmers<uint64_t, true>
                          : wide ? (const void*)k_superkmers<uint64_t, false> : (const void*)k_superkmers<uint32_t, false>;
-    int grid = std::min<int>(cd.n_tiles, occupancy_blocks(fn, P1_NT, c->n_cu));
+    int grid = occupancy_blocks(fn, P1_NT, c->n_cu);
+    if (!persistent) grid = std::min<int>(std::max<uint32_t>(cd.n_tiles, 1), grid);
     if (win)
         hipLaunchKernelGGL((k_superkmers<uint64_t, true>), dim3(grid), dim3(P1_NT), 0, c->st, c->d_chunk.as<ChunkDesc>(), prm, bs, c->d_ctr.as<uint32_t>());
     else if (wide)
@@ -286,8 +295,6 @@ static void launch_superkmers(gkc_ctx* c, const ChunkDesc& cd, P1Params prm, Bin
         hipLaunchKernelGGL((k_superkmers<uint32_t, false>), dim3(grid), dim3(P1_NT), 0, c->st, c->d_chunk.as<ChunkDesc>(), prm, bs, c->d_ctr.as<uint32_t>());
     HIPX(hipGetLastError());
 }
-
-static void zero(gkc_ctx* c, void* p, uint64_t n32);
 
 static P1Params p1params(gkc_ctx* c, int scatter) {
     P1Params p;
@@ -375,6 +382,7 @@ void gkc_destroy(gkc_ctx* c) {
     if (c->g_sort) (void)hipGraphExecDestroy(c->g_sort);
     if (c->g_emit) (void)hipGraphExecDestroy(c->g_emit);
     if (c->g_cnt) (void)hipGraphExecDestroy(c->g_cnt);
+    if (c->g_p1) (void)hipGraphExecDestroy(c->g_p1);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : c->evp) if (e) (void)hipEventDestroy(e);
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -396,6 +404,8 @@ int gkc_reset(gkc_ctx* c) {
         std::fill(c->h_kmer_cur.begin(), c->h_kmer_cur.end(), 0u);
         zero(c, c->d_word_cur.p, P); zero(c, c->d_hdr_cur.p, P); zero(c, c->d_kmer_cur.p, P);
         zero(c, c->d_sketch.p, (uint64_t)P * 256); zero(c, c->d_overflow.p, 4);
+        c->bases_done = 0;
+        c->p1_fallbacks = 0;
         c->counted = false;
         HIPX(hipStreamSynchronize(c->st));
         return 0;
@@ -474,46 +484,122 @@ int gkc_windows_device(gkc_ctx* c, uint32_t** mm, uint8_t** pos, int8_t** strand
     });
 }
 
+// Plain phase 1 of one batch: count pass, host sync, exact reservation, scatter.
+static void partition_plain(gkc_ctx* c, const ChunkDesc& cd, uint32_t flags, bool sketch_done) {
+    const uint32_t P = c->prm.n_parts;
+    zero(c, c->d_tmp_hdr.p, P); zero(c, c->d_tmp_word.p, P); zero(c, c->d_tmp_kmer.p, P);
+    P1Params pc = p1params(c, 0);
+    if (!sketch_done) pc.sketch = c->d_sketch.as<uint32_t>();
+    launch_superkmers(c, cd, pc, binset(c, true));
+    std::vector<uint32_t> nh(P), nw(P), nk(P);
+    HIPX(hipMemcpyAsync(nh.data(), c->d_tmp_hdr.p, P * 4, hipMemcpyDeviceToHost, c->st));
+    HIPX(hipMemcpyAsync(nw.data(), c->d_tmp_word.p, P * 4, hipMemcpyDeviceToHost, c->st));
+    HIPX(hipMemcpyAsync(nk.data(), c->d_tmp_kmer.p, P * 4, hipMemcpyDeviceToHost, c->st));
+    HIPX(hipStreamSynchronize(c->st));
+    for (uint32_t q = 0; q < P; q++)
+        if ((uint64_t)c->h_kmer_cur[q] + nk[q] >= (1ull << 30)) throw ArgError{"partition exceeds 2^30 k-mers; use more partitions", GKC_EINVAL};
+    bins_reserve(c, nw, nh);
+    P1Params ps = p1params(c, 1);
+    if (flags & GKC_EMIT_WINDOWS) {
+        if (c->prm.p > 16) throw ArgError{"per-window records hold u32 minimizers: p <= 16", GKC_EINVAL};
+        const uint64_t B = cd.n_bases;
+        c->d_win_mm.ensure((B + 1) * 4); c->d_win_pos.ensure(B + 1); c->d_win_str.ensure(B + 1);
+        ps.win_mm = c->d_win_mm.as<uint32_t>(); ps.win_pos = c->d_win_pos.as<uint8_t>();
+        ps.win_strand = c->d_win_str.as<int8_t>();
+        c->win_n = B; c->win_lead = cd.lead;
+    }
+    launch_superkmers(c, cd, ps, binset(c, false));
+    uint32_t ovf = 0;
+    HIPX(hipMemcpyAsync(&ovf, c->d_overflow.p, 4, hipMemcpyDeviceToHost, c->st));
+    HIPX(hipStreamSynchronize(c->st));
+    if (ovf) throw ArgError{"internal: bin overflow after exact reservation", GKC_ESTATE};
+    for (uint32_t q = 0; q < P; q++) { c->h_hdr_cur[q] += nh[q]; c->h_word_cur[q] += nw[q]; c->h_kmer_cur[q] += nk[q]; }
+}
+
+__global__ void k_copy_u32(uint32_t* __restrict__ dst, const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
+                           const uint32_t* __restrict__ cc, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        dst[i] = a[i]; dst[n + i] = b[i]; dst[2 * n + i] = cc[i];
+    }
+}
+__global__ void k_restore_u32(const uint32_t* __restrict__ src, uint32_t* __restrict__ a, uint32_t* __restrict__ b,
+                              uint32_t* __restrict__ cc, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        a[i] = src[i]; b[i] = src[n + i]; cc[i] = src[2 * n + i];
+    }
+}
+
+// Batched phase 1 of one batch: bins presized from the per-partition shares of
+// the batches already partitioned, a single scatter pass replayed from a
+// captured graph (the chunk descriptor lives in device memory), one host
+// sync.  Returns false when there is no history yet or a bin overflowed (the
+// cursors are then restored and the caller runs the plain form).
+static bool partition_presized(gkc_ctx* c, const ChunkDesc& cd) {
+    const uint32_t P = c->prm.n_parts;
+    if (c->bases_done == 0) return false;
+    const double f = (double)cd.n_bases / (double)c->bases_done;
+    std::vector<uint32_t> nw(P), nh(P);
+    for (uint32_t q = 0; q < P; q++) {
+        nw[q] = (uint32_t)std::min<double>(std::ceil(c->h_word_cur[q] * f * 1.25) + 256, 4.0e9);
+        nh[q] = (uint32_t)std::min<double>(std::ceil(c->h_hdr_cur[q] * f * 1.25) + 64, 4.0e9);
+    }
+    bins_reserve(c, nw, nh);
+    c->d_backup.ensure((size_t)3 * P * 4);
+    std::vector<const void*> key = {c->words.p, c->hdrs.p, c->d_chunk.p, c->d_backup.p};
+    auto enqueue = [&] {
+        hipLaunchKernelGGL(k_copy_u32, dim3((P + 255) / 256), dim3(256), 0, c->st, c->d_backup.as<uint32_t>(),
+                           c->d_word_cur.as<uint32_t>(), c->d_hdr_cur.as<uint32_t>(), c->d_kmer_cur.as<uint32_t>(), P);
+        zero(c, c->d_overflow.p, 1);
+        P1Params ps = p1params(c, 1);
+        ps.sketch = c->d_sketch.as<uint32_t>();
+        launch_superkmers(c, cd, ps, binset(c, false), true);
+    };
+    // occupancy is queried (and cached) outside the capture
+    (void)occupancy_blocks(c->prm.p > 16 ? (const void*)k_superkmers<uint64_t, false> : (const void*)k_superkmers<uint32_t, false>,
+                           P1_NT, c->n_cu);
+    if (!c->g_p1 || c->g_p1_key != key) {
+        if (c->g_p1) { HIPX(hipGraphExecDestroy(c->g_p1)); c->g_p1 = nullptr; }
+        c->g_p1 = capture(c, enqueue);
+        c->g_p1_key = key;
+    }
+    HIPX(hipGraphLaunch(c->g_p1, c->st));
+    std::vector<uint32_t> cur(3 * (size_t)P);
+    uint32_t ovf = 0;
+    HIPX(hipMemcpyAsync(cur.data(), c->d_word_cur.p, P * 4, hipMemcpyDeviceToHost, c->st));
+    HIPX(hipMemcpyAsync(cur.data() + P, c->d_hdr_cur.p, P * 4, hipMemcpyDeviceToHost, c->st));
+    HIPX(hipMemcpyAsync(cur.data() + 2 * P, c->d_kmer_cur.p, P * 4, hipMemcpyDeviceToHost, c->st));
+    HIPX(hipMemcpyAsync(&ovf, c->d_overflow.p, 4, hipMemcpyDeviceToHost, c->st));
+    HIPX(hipStreamSynchronize(c->st));
+    bool too_big = false;
+    for (uint32_t q = 0; q < P; q++) too_big |= cur[2 * P + q] >= (1u << 30);
+    if (ovf || too_big) {
+        hipLaunchKernelGGL(k_restore_u32, dim3((P + 255) / 256), dim3(256), 0, c->st, c->d_backup.as<uint32_t>(),
+                           c->d_word_cur.as<uint32_t>(), c->d_hdr_cur.as<uint32_t>(), c->d_kmer_cur.as<uint32_t>(), P);
+        zero(c, c->d_overflow.p, 1);
+        HIPX(hipStreamSynchronize(c->st));
+        c->p1_fallbacks++;
+        return false;
+    }
+    for (uint32_t q = 0; q < P; q++) { c->h_word_cur[q] = cur[q]; c->h_hdr_cur[q] = cur[P + q]; c->h_kmer_cur[q] = cur[2 * P + q]; }
+    return true;
+}
+
 int gkc_partition(gkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n, uint32_t flags) {
     return guarded(c, [&] {
         if (n < 0 || !offs || (n > 0 && !reads)) throw ArgError{"bad batch", GKC_EINVAL};
         if (n == 0) return 0;
-        const uint32_t P = c->prm.n_parts;
         ChunkDesc cd;
         HIPX(hipEventRecord(c->ev[0], c->st));
         stage_batch(c, reads, offs, n, flags, cd);
-        // count pass: per-partition sizes of this batch
-        HIPX(hipMemsetAsync(c->d_tmp_hdr.p, 0, P * 4, c->st));
-        HIPX(hipMemsetAsync(c->d_tmp_word.p, 0, P * 4, c->st));
-        HIPX(hipMemsetAsync(c->d_tmp_kmer.p, 0, P * 4, c->st));
-        P1Params pc = p1params(c, 0);
-        pc.sketch = c->d_sketch.as<uint32_t>();
-        launch_superkmers(c, cd, pc, binset(c, true));
-        std::vector<uint32_t> nh(P), nw(P), nk(P);
-        HIPX(hipMemcpyAsync(nh.data(), c->d_tmp_hdr.p, P * 4, hipMemcpyDeviceToHost, c->st));
-        HIPX(hipMemcpyAsync(nw.data(), c->d_tmp_word.p, P * 4, hipMemcpyDeviceToHost, c->st));
-        HIPX(hipMemcpyAsync(nk.data(), c->d_tmp_kmer.p, P * 4, hipMemcpyDeviceToHost, c->st));
-        HIPX(hipStreamSynchronize(c->st));
-        for (uint32_t q = 0; q < P; q++)
-            if ((uint64_t)c->h_kmer_cur[q] + nk[q] >= (1ull << 30)) throw ArgError{"partition exceeds 2^30 k-mers; use more partitions", GKC_EINVAL};
-        bins_reserve(c, nw, nh);
-        // scatter pass (optionally also emitting the per-window minimizer records)
-        P1Params ps = p1params(c, 1);
-        if (flags & GKC_EMIT_WINDOWS) {
-            if (c->prm.p > 16) throw ArgError{"per-window records hold u32 minimizers: p <= 16", GKC_EINVAL};
-            const uint64_t B = cd.n_bases;
-            c->d_win_mm.ensure((B + 1) * 4); c->d_win_pos.ensure(B + 1); c->d_win_str.ensure(B + 1);
-            ps.win_mm = c->d_win_mm.as<uint32_t>(); ps.win_pos = c->d_win_pos.as<uint8_t>();
-            ps.win_strand = c->d_win_str.as<int8_t>();
-            c->win_n = B; c->win_lead = cd.lead;
+        bool done = false, sketched = false;
+        if ((flags & GKC_BATCHED) && !(flags & GKC_EMIT_WINDOWS)) {
+            done = partition_presized(c, cd);
+            sketched = !done && c->bases_done > 0;   // a failed attempt already sketched this chunk
         }
-        launch_superkmers(c, cd, ps, binset(c, false));
+        if (!done) partition_plain(c, cd, flags, sketched);
+        c->bases_done += cd.n_bases - cd.lead;
         HIPX(hipEventRecord(c->ev[1], c->st));
-        HIPX(hipStreamSynchronize(c->st));
-        uint32_t ovf = 0;
-        HIPX(hipMemcpy(&ovf, c->d_overflow.p, 4, hipMemcpyDeviceToHost));
-        if (ovf) throw ArgError{"internal: bin overflow after exact reservation", GKC_ESTATE};
-        for (uint32_t q = 0; q < P; q++) { c->h_hdr_cur[q] += nh[q]; c->h_word_cur[q] += nw[q]; c->h_kmer_cur[q] += nk[q]; }
+        HIPX(hipEventSynchronize(c->ev[1]));
         float ms = 0;
         HIPX(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
         c->stats.ms_partition = ms;
@@ -599,7 +685,11 @@ int gkc_table_device(gkc_ctx* c, void** keys, uint32_t** counts, uint64_t* n_ent
 }
 
 int gkc_get_stats(gkc_ctx* c, gkc_stats* out) {
-    return guarded(c, [&] { if (out) *out = c->stats; return 0; });
+    return guarded(c, [&] {
+        c->stats.partition_fallbacks = c->p1_fallbacks;
+        if (out) *out = c->stats;
+        return 0;
+    });
 }
 
 int gkc_sketch(gkc_ctx* c, float* out) {
@@ -956,7 +1046,7 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const K* sorted, uin
 }
 
 template <typename F>
-static hipGraphExec_t capture(gkc_ctx* c, F&& enqueue) {
+hipGraphExec_t capture(gkc_ctx* c, F&& enqueue) {
     hipGraph_t g;
     HIPX(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
     try {

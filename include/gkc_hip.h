@@ -67,7 +67,7 @@ typedef struct gkc_stats {
     uint32_t n_pass;           /* radix passes run per key */
     uint32_t fallbacks;        /* presize overflows that fell back to count-then-allocate */
     uint32_t host_syncs;       /* host synchronisations inside the last gkc_count */
-    uint32_t pad;
+    uint32_t partition_fallbacks;  /* presized batched scatters redone by count-then-allocate since gkc_reset */
 } gkc_stats;
 
 const char *gkc_last_error(void);
