@@ -45,6 +45,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-reads", type=int, default=1_500_000, help="reads in the CPU baseline sample")
     ap.add_argument("--forms", default="plain,batched")
+    ap.add_argument("--tune", action="store_true", help="run the autotuner on this workload and cache the result")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -56,7 +57,7 @@ def main() -> int:
     import numpy as np
     import torch
 
-    from gkc_amd import capi, learned
+    from gkc_amd import autotune, capi, learned
     from gkc_amd import workloads as W
 
     w = W.workload(args.config, scale=args.scale)
@@ -84,6 +85,16 @@ def main() -> int:
     ctx = capi.Counter(w.k, w.p, w.n_parts, hpc=w.hpc, filter_mode=w.filter_mode, part_mode=w.part_mode,
                        min_count=w.min_count, max_count=w.max_count)
     forms = args.forms.split(",")
+    tkey = autotune.bucket_key(w.k, rs.bases / max(rs.n, 1), N / w.n_parts)
+    if args.tune:
+        run_step(ctx, capi.BATCHED, capi.BATCHED)
+        res = autotune.tune_sort(ctx, log=_log)
+        cache = autotune.load_cache()
+        cache[tkey] = {"sort_items": res["sort_items"], "sort_blocks_per_cu": res["sort_blocks_per_cu"],
+                       "ms_pass": res["ms_pass"], "workload": w.name}
+        autotune.save_cache(cache)
+    tuning = autotune.apply(ctx, tkey)
+    _log(f"tuning {tkey}: {tuning or 'defaults'}")
     results = {}
     predictor = learned.Predictor.load_default(w.n_parts) if "batched" in forms else None
 
@@ -159,6 +170,7 @@ def main() -> int:
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_launch": pass_bytes, "avg_launch_ms": pass_ms},
         "full_size_invariants_ok": full_ok,
+        "tuning": {"key": tkey, **ctx.get_tuning()},
     }
 
     if not args.no_cpu:

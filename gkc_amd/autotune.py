@@ -1,0 +1,97 @@
+"""Autotuner of the counting kernels.
+
+Picks the onesweep tile size (keys per thread) and the number of persistent
+workgroups per CU of the sort passes by timing every candidate on the data
+already in a counter's bins (the batched form, instrumented pass timing), and
+caches the winner keyed on (key width, read-length bucket, partition-size
+bucket) in gkc_amd/tune_cache.json.  Every candidate produces the same table
+(bit-exactness across variants is checked in tests/test_autotune.py); only
+speed differs.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+
+from . import capi
+
+CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_cache.json")
+CANDIDATES_64 = [(i, b) for i in (8, 12, 16, 20, 24) for b in (0, 2)]
+CANDIDATES_128 = [(i, b) for i in (4, 8, 12) for b in (0, 2)]
+
+
+def bucket_key(k: int, mean_read_len: float, keys_per_partition: float) -> str:
+    kw = "u64" if k <= 32 else "u128"
+    rl = int(round(math.log2(max(mean_read_len, 1.0))))
+    ps = int(round(math.log2(max(keys_per_partition, 1.0))))
+    return f"{kw}/rl{rl}/ps{ps}"
+
+
+def load_cache() -> dict:
+    if os.path.exists(CACHE):
+        with open(CACHE) as f:
+            return json.load(f)
+    return {}
+
+
+def save_cache(cache: dict) -> None:
+    with open(CACHE, "w") as f:
+        json.dump(cache, f, indent=1, sort_keys=True)
+
+
+def lookup(key: str) -> dict | None:
+    return load_cache().get(key)
+
+
+def nearest(key: str) -> dict | None:
+    """Exact bucket, else the entry with the same key width and the closest
+    partition-size bucket."""
+    cache = load_cache()
+    if key in cache:
+        return cache[key]
+    kw, _, ps = key.split("/")
+    best, dist = None, None
+    for k2, v in cache.items():
+        kw2, _, ps2 = k2.split("/")
+        if kw2 != kw:
+            continue
+        d = abs(int(ps2[2:]) - int(ps[2:]))
+        if dist is None or d < dist:
+            best, dist = v, d
+    return best
+
+
+def apply(counter: "capi.Counter", key: str) -> dict | None:
+    t = nearest(key)
+    if t:
+        counter.set_tuning(t["sort_items"], t["sort_blocks_per_cu"])
+    return t
+
+
+def tune_sort(counter: "capi.Counter", reps: int = 2, log=None) -> dict:
+    """Time every candidate on the counter's current bins; returns the best
+    {sort_items, sort_blocks_per_cu, ms_pass} and the full sweep."""
+    cands = CANDIDATES_64 if counter.k <= 32 else CANDIDATES_128
+    sweep = []
+    prev = os.environ.get("GKC_NO_GRAPH")
+    os.environ["GKC_NO_GRAPH"] = "1"
+    try:
+        for items, bpc in cands:
+            counter.set_tuning(items, bpc)
+            best = None
+            for _ in range(reps):
+                counter.count(capi.BATCHED)
+                ms = counter.stats()["ms_sort_pass_avg"]
+                best = ms if best is None else min(best, ms)
+            sweep.append({"sort_items": items, "sort_blocks_per_cu": bpc, "ms_pass": best})
+            if log:
+                log(f"tune items={items} blocks/CU={bpc or 'max'}: {best:.3f} ms/pass")
+    finally:
+        if prev is None:
+            os.environ.pop("GKC_NO_GRAPH", None)
+        else:
+            os.environ["GKC_NO_GRAPH"] = prev
+    win = min(sweep, key=lambda r: r["ms_pass"])
+    counter.set_tuning(win["sort_items"], win["sort_blocks_per_cu"])
+    return {**win, "sweep": sweep}

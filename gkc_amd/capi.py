@@ -35,6 +35,11 @@ class Params(C.Structure):
                 ("max_count", C.c_uint32)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("sort_items", C.c_int32), ("sort_blocks_per_cu", C.c_int32), ("reserved0", C.c_int32),
+                ("reserved1", C.c_int32)]
+
+
 class Stats(C.Structure):
     _fields_ = [("ms_partition", C.c_double), ("ms_count", C.c_double), ("ms_extract", C.c_double),
                 ("ms_sort", C.c_double), ("ms_runlength", C.c_double), ("ms_sort_pass_avg", C.c_double),
@@ -84,6 +89,8 @@ SIGNATURES = {
     "gkc_bf16_matmul32": (C.c_int, [C.c_int, _f32p, _f32p, C.c_int, C.c_int, _f32p]),
     "gkc_sketch": (C.c_int, [C.c_void_p, _f32p]),
     "gkc_set_table_caps": (C.c_int, [C.c_void_p, _u32p]),
+    "gkc_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "gkc_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "gkc_part_kmers": (C.c_int, [C.c_void_p, _u64p]),
 }
 
@@ -190,6 +197,15 @@ class Counter:
     def set_table_caps(self, caps) -> None:
         caps = np.ascontiguousarray(caps, np.uint32)
         _check(lib().gkc_set_table_caps(self.h, _p(caps, _u32p)))
+
+    def set_tuning(self, sort_items: int = 0, sort_blocks_per_cu: int = 0) -> None:
+        t = Tuning(sort_items, sort_blocks_per_cu, 0, 0)
+        _check(lib().gkc_set_tuning(self.h, C.byref(t)))
+
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        _check(lib().gkc_get_tuning(self.h, C.byref(t)))
+        return {"sort_items": t.sort_items, "sort_blocks_per_cu": t.sort_blocks_per_cu}
 
     def part_kmers(self) -> np.ndarray:
         out = np.zeros(self.n_parts, np.uint64)

@@ -92,6 +92,7 @@ struct gkc_ctx {
     DevBuf d_backup;                           // cursors before a presized scatter (3 x P)
     uint64_t bases_done = 0;                   // bases partitioned since the last reset
     uint32_t p1_fallbacks = 0;                 // presized scatters redone since the last reset
+    gkc_tuning tune{};                         // kernel variants picked by the autotuner
     hipGraphExec_t g_p1 = nullptr;             // captured presized scatter of one chunk
     std::vector<const void*> g_p1_key;
     uint64_t win_n = 0;
@@ -152,6 +153,7 @@ static int occupancy_blocks(const void* fn, int nt, int n_cu) {
 
 static void zero(gkc_ctx* c, void* p, uint64_t n32);
 template <typename F> hipGraphExec_t capture(gkc_ctx* c, F&& enqueue);
+template <typename K> static int sort_items(gkc_ctx* c);
 
 // ---------------------------------------------------------------- bins
 
@@ -702,6 +704,23 @@ int gkc_sketch(gkc_ctx* c, float* out) {
     });
 }
 
+int gkc_set_tuning(gkc_ctx* c, const gkc_tuning* t) {
+    return guarded(c, [&] {
+        if (!t) throw ArgError{"null tuning", GKC_EINVAL};
+        c->tune = *t;
+        return 0;
+    });
+}
+
+int gkc_get_tuning(gkc_ctx* c, gkc_tuning* t) {
+    return guarded(c, [&] {
+        gkc_tuning r = c->tune;
+        r.sort_items = c->keyw == 8 ? sort_items<uint64_t>(c) : sort_items<u128k>(c);
+        if (t) *t = r;
+        return 0;
+    });
+}
+
 int gkc_set_table_caps(gkc_ctx* c, const uint32_t* caps) {
     return guarded(c, [&] {
         c->presize_caps.assign(caps, caps + c->prm.n_parts);
@@ -725,7 +744,7 @@ static void ensure_count_buffers(gkc_ctx* c, uint64_t max_keys, uint32_t n_segs)
     const size_t kw = sizeof(K);
     c->keysA.ensure((max_keys + 1) * kw);
     c->keysB.ensure((max_keys + 1) * kw);
-    const uint64_t tiles = max_keys / (SORT_NT * (kw == 8 ? 16 : 8)) + n_segs + 1;
+    const uint64_t tiles = max_keys / (SORT_NT * (kw == 8 ? 8 : 4)) + n_segs + 1;   // smallest tile variant
     c->lookback.ensure(tiles * RADIX * 4);
     c->d_plan.ensure(sizeof(Plan));
     c->d_keystart.ensure((n_segs + 1) * 8);
@@ -750,11 +769,50 @@ static void zero(gkc_ctx* c, void* p, uint64_t n32) {
 // Run extraction + sort for a plan whose per-segment arrays are set; the
 // sorted keys end in keysA or keysB (returned).  Host knows n_keys_total and
 // the sort tile count upper bound.
+// onesweep variants (keys per thread), selected by gkc_tuning.sort_items
+template <typename K>
+using OnesweepFn = void (*)(Plan*, SegArrays, const K*, K*, uint32_t*, int);
+template <typename K> static OnesweepFn<K> onesweep_variant(int items);
+template <> OnesweepFn<uint64_t> onesweep_variant<uint64_t>(int items) {
+    switch (items) {
+        case 8: return k_onesweep<uint64_t, 8>;
+        case 12: return k_onesweep<uint64_t, 12>;
+        case 20: return k_onesweep<uint64_t, 20>;
+        case 24: return k_onesweep<uint64_t, 24>;
+        default: return k_onesweep<uint64_t, 16>;
+    }
+}
+template <> OnesweepFn<u128k> onesweep_variant<u128k>(int items) {
+    switch (items) {
+        case 4: return k_onesweep<u128k, 4>;
+        case 12: return k_onesweep<u128k, 12>;
+        default: return k_onesweep<u128k, 8>;
+    }
+}
+template <typename K> static int sort_items(gkc_ctx* c) {
+    const int it = c->tune.sort_items;
+    if (sizeof(K) == 8) return (it == 8 || it == 12 || it == 16 || it == 20 || it == 24) ? it : 16;
+    return (it == 4 || it == 8 || it == 12) ? it : 8;
+}
+template <typename K> static int sort_grid(gkc_ctx* c) {
+    OnesweepFn<K> fn = onesweep_variant<K>(sort_items<K>(c));
+    int g = occupancy_blocks((const void*)fn, SORT_NT, c->n_cu);
+    if (c->tune.sort_blocks_per_cu > 0) g = std::min(g, c->tune.sort_blocks_per_cu * c->n_cu);
+    return std::max(g, 1);
+}
+template <typename K>
+static void launch_onesweep(gkc_ctx* c, int grid, Plan* plan, SegArrays sa, const K* src, K* dst, int pass) {
+    OnesweepFn<K> fn = onesweep_variant<K>(sort_items<K>(c));
+    uint32_t* lb = c->lookback.as<uint32_t>();
+    void* args[] = {&plan, &sa, &src, &dst, &lb, &pass};
+    HIPX(hipLaunchKernel((const void*)fn, dim3(grid), dim3(SORT_NT), args, 0, c->st));
+}
+
 template <typename K>
 static K* sort_segments(gkc_ctx* c, SegArrays sa, const uint64_t* hdr_base, const uint64_t* word_base,
                         uint32_t n_segs, uint64_t n_keys, uint64_t n_hdrs, float* ms_ex, float* ms_sort, float* ms_pass_sum) {
     Plan* plan = c->d_plan.as<Plan>();
-    const int TILE = SORT_NT * (sizeof(K) == 8 ? 16 : 8);
+    const int TILE = SORT_NT * sort_items<K>(c);
     hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(64), 0, c->st, plan, n_segs, (uint32_t)c->npass);
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, c->st, plan, sa, TILE);
     HIPX(hipGetLastError());
@@ -769,13 +827,13 @@ static K* sort_segments(gkc_ctx* c, SegArrays sa, const uint64_t* hdr_base, cons
     HIPX(hipGetLastError());
     HIPX(hipEventRecord(c->ev[3], c->st));
     const uint64_t tiles = n_keys / TILE + n_segs;
-    int g_s = (int)std::min<uint64_t>(tiles, occupancy_blocks((const void*)k_onesweep<K>, SORT_NT, c->n_cu));
+    int g_s = (int)std::min<uint64_t>(tiles, sort_grid<K>(c));
     K* src = c->keysA.as<K>();
     K* dst = c->keysB.as<K>();
     for (int ps = 0; ps < c->npass; ps++) {
         zero(c, c->lookback.p, tiles * RADIX);
         HIPX(hipEventRecord(c->evp[2 * ps], c->st));
-        hipLaunchKernelGGL(k_onesweep<K>, dim3(std::max(g_s, 1)), dim3(SORT_NT), 0, c->st, plan, sa, src, dst, c->lookback.as<uint32_t>(), ps);
+        launch_onesweep<K>(c, std::max(g_s, 1), plan, sa, src, dst, ps);
         HIPX(hipGetLastError());
         HIPX(hipEventRecord(c->evp[2 * ps + 1], c->st));
         std::swap(src, dst);
@@ -1000,7 +1058,7 @@ template <typename K>
 static K* enqueue_sort_all(gkc_ctx* c, SegArrays sa, uint64_t tiles_cap, bool timed) {
     const uint32_t P = c->prm.n_parts;
     Plan* plan = c->d_plan.as<Plan>();
-    const int TILE = sort_tile<K>();
+    const int TILE = SORT_NT * sort_items<K>(c);
     hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(64), 0, c->st, plan, P, (uint32_t)c->npass);
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, c->st, plan, sa, TILE);
     zero(c, sa.hist, (uint64_t)P * MAX_PASS * RADIX);
@@ -1011,13 +1069,13 @@ static K* enqueue_sort_all(gkc_ctx* c, SegArrays sa, uint64_t tiles_cap, bool ti
                        c->keysA.as<K>(), c->prm.k);
     hipLaunchKernelGGL(k_hist_scan, dim3((P * MAX_PASS + 3) / 4), dim3(256), 0, c->st, plan, sa.hist);
     if (timed) HIPX(hipEventRecord(c->ev[3], c->st));
-    const int g_s = occupancy_blocks((const void*)k_onesweep<K>, SORT_NT, c->n_cu);
+    const int g_s = sort_grid<K>(c);
     K* src = c->keysA.as<K>();
     K* dst = c->keysB.as<K>();
     for (int ps = 0; ps < c->npass; ps++) {
         zero(c, c->lookback.p, tiles_cap * RADIX);
         if (timed) HIPX(hipEventRecord(c->evp[2 * ps], c->st));
-        hipLaunchKernelGGL(k_onesweep<K>, dim3(g_s), dim3(SORT_NT), 0, c->st, plan, sa, src, dst, c->lookback.as<uint32_t>(), ps);
+        launch_onesweep<K>(c, g_s, plan, sa, src, dst, ps);
         if (timed) HIPX(hipEventRecord(c->evp[2 * ps + 1], c->st));
         std::swap(src, dst);
     }
@@ -1080,7 +1138,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     c->stats.n_kmers = tot_keys; c->stats.n_superkmers = tot_hdrs; c->stats.n_words = tot_words;
     c->stats.n_pass = c->npass;
     ensure_count_buffers<K>(c, tot_keys, P);
-    const uint64_t tiles_cap = tot_keys / sort_tile<K>() + P + 1;
+    const uint64_t tiles_cap = tot_keys / (SORT_NT * sort_items<K>(c)) + P + 1;
     const uint64_t rl_tiles_cap = tot_keys / RL_TILE + P + 1;
     SegArrays sa;
     sa.seg_keys = c->d_kmer_cur.as<uint32_t>(); sa.seg_hdrs = c->d_hdr_cur.as<uint32_t>();
@@ -1092,11 +1150,12 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     // table capacities: predicted (presized) or a count pass (count-then-allocate)
     const bool presized = (flags & GKC_PRESIZED) && c->presize_caps.size() == P;
     (void)occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu);
-    (void)occupancy_blocks((const void*)k_onesweep<K>, SORT_NT, c->n_cu);
+    (void)sort_grid<K>(c);
     (void)occupancy_blocks((const void*)k_runlength<K>, SORT_NT, c->n_cu);
     HIPX(hipEventRecord(c->ev[0], c->st));
     std::vector<const void*> key = {c->keysA.p, c->keysB.p, c->lookback.p, c->words.p, c->hdrs.p,
-                                    (const void*)(uintptr_t)tiles_cap, (const void*)(uintptr_t)c->npass};
+                                    (const void*)(uintptr_t)tiles_cap, (const void*)(uintptr_t)c->npass,
+                                    (const void*)(uintptr_t)sort_items<K>(c), (const void*)(uintptr_t)sort_grid<K>(c)};
     if (timed) {
         enqueue_sort_all<K>(c, sa, tiles_cap, true);
     } else {

@@ -187,10 +187,9 @@ __global__ __launch_bounds__(256) void k_hist_scan(Plan* __restrict__ plan, uint
 // One onesweep pass over all segments.  Tile = SORT_NT * ITEMS keys of one
 // segment; tiles are claimed in order from a counter so every tile's
 // predecessors are already running (forward progress of the look-back).
-template <typename K>
+template <typename K, int ITEMS>
 __global__ __launch_bounds__(SORT_NT) void k_onesweep(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ in,
                                                       K* __restrict__ out, uint32_t* __restrict__ lookback, int pass) {
-    constexpr int ITEMS = KeyTraits<K>::ITEMS;
     constexpr int TILE = SORT_NT * ITEMS;
     constexpr int NWAVE = SORT_NT / GKC_WAVE;
     __shared__ K s_keys[TILE];
@@ -439,8 +438,10 @@ __global__ __launch_bounds__(SORT_NT) void k_runlength(Plan* __restrict__ plan, 
 
 template __global__ void k_extract<uint64_t>(Plan*, SegArrays, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, uint64_t*, int);
 template __global__ void k_extract<u128k>(Plan*, SegArrays, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, u128k*, int);
-template __global__ void k_onesweep<uint64_t>(Plan*, SegArrays, const uint64_t*, uint64_t*, uint32_t*, int);
-template __global__ void k_onesweep<u128k>(Plan*, SegArrays, const u128k*, u128k*, uint32_t*, int);
+#define GKC_OS(K, I) template __global__ void k_onesweep<K, I>(Plan*, SegArrays, const K*, K*, uint32_t*, int);
+GKC_OS(uint64_t, 8) GKC_OS(uint64_t, 12) GKC_OS(uint64_t, 16) GKC_OS(uint64_t, 20) GKC_OS(uint64_t, 24)
+GKC_OS(u128k, 4) GKC_OS(u128k, 8) GKC_OS(u128k, 12)
+#undef GKC_OS
 template __global__ void k_runlength<uint64_t>(Plan*, SegArrays, const uint64_t*, TableOut, uint32_t*);
 template __global__ void k_runlength<u128k>(Plan*, SegArrays, const u128k*, TableOut, uint32_t*);
 

@@ -70,6 +70,14 @@ typedef struct gkc_stats {
     uint32_t partition_fallbacks;  /* presized batched scatters redone by count-then-allocate since gkc_reset */
 } gkc_stats;
 
+/* Kernel variants (picked by gkc_amd/autotune.py, keyed on k, read-length
+ * distribution and partition size). */
+typedef struct gkc_tuning {
+    int32_t sort_items;          /* keys per thread of a onesweep tile: 8,12,16,20,24 (u64) / 4,8,12 (u128) */
+    int32_t sort_blocks_per_cu;  /* persistent workgroups per CU in the sort passes (0: occupancy limit) */
+    int32_t reserved0, reserved1;
+} gkc_tuning;
+
 const char *gkc_last_error(void);
 int gkc_version(void);
 
@@ -156,6 +164,8 @@ int gkc_device_select(int device);
  * holding log1p(count) of the 4-mer at every 16th window start of each read,
  * by the window's partition. */
 int gkc_sketch(gkc_ctx *ctx, float *out);
+int gkc_set_tuning(gkc_ctx *ctx, const gkc_tuning *t);
+int gkc_get_tuning(gkc_ctx *ctx, gkc_tuning *t);
 /* Table capacities used by gkc_count(GKC_PRESIZED): caps[P] entries. */
 int gkc_set_table_caps(gkc_ctx *ctx, const uint32_t *caps);
 /* Per-partition k-mer totals of the bins (known exactly after gkc_partition). */
