@@ -46,6 +46,8 @@ def main() -> int:
     ap.add_argument("--cpu-reads", type=int, default=1_500_000, help="reads in the CPU baseline sample")
     ap.add_argument("--forms", default="plain,batched")
     ap.add_argument("--tune", action="store_true", help="run the autotuner on this workload and cache the result")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="end with a 1 GiB device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -65,7 +67,10 @@ def main() -> int:
     rs = W.generate(w)
     N = rs.kmers(w.k)
     _log(f"{w.name}: {rs.n} reads, {rs.bases} bases, {N} k-mers (generated in {time.time() - t0:.1f}s)")
-    chunks = W.chunk_bounds(rs, seed=W.genome_seed(w.config_id) + 777) if w.chunked else [(0, rs.n)]
+    if w.chunked:
+        chunks = W.chunk_bounds(rs, seed=W.genome_seed(w.config_id) + 777)
+    else:   # one batch holds < 2^31 bases: equal chunks of at most 1 GiB
+        chunks = W.chunk_bounds(rs, seed=0, lo_bytes=1 << 30, hi_bytes=1 << 30, fasta_overhead=False)
     _log(f"{len(chunks)} chunks: " + ", ".join(f"{int(rs.offs[b] - rs.offs[a]) >> 20}MB" for a, b in chunks))
 
     dev = torch.device("cuda:0")
@@ -199,6 +204,13 @@ def main() -> int:
                                           f"oracle phase1 {ot.times[0]:.2f}s + phase2 {ot.times[1]:.2f}s"}
         line["bit_exact_sample"] = exact
     ctx.close()
+    if args.calib_copy:
+        a = torch.ones(1 << 28, dtype=torch.int32, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        line["calib_copy_bytes"] = {"read": a.numel() * 4, "write": b.numel() * 4}
     print(json.dumps(line), flush=True)
     return 0
 

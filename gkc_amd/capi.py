@@ -153,7 +153,7 @@ class Counter:
     def partition(self, reads, offs, n: int | None = None, flags: int = 0) -> None:
         """reads/offs: host numpy arrays, or device addresses (int) with DEVICE_INPUT."""
         if flags & DEVICE_INPUT:
-            _check(lib().gkc_partition(self.h, C.c_void_p(reads), C.c_void_p(offs), n, flags))
+            _check(lib().gkc_partition(self.h, C.c_void_p(int(reads)), C.c_void_p(int(offs)), int(n), flags))
         else:
             reads = np.ascontiguousarray(reads, np.uint8)
             offs = np.ascontiguousarray(offs, np.uint64)

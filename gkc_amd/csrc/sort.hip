@@ -305,7 +305,7 @@ __global__ __launch_bounds__(SORT_NT) void k_onesweep(Plan* __restrict__ plan, S
 template <typename K>
 __global__ __launch_bounds__(SORT_NT) void k_runlength(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys,
                                                        TableOut to, uint32_t* __restrict__ lookback) {
-    __shared__ uint32_t s_first[SORT_NT];       // first head index (tile-local) of each thread, or RL_TILE
+    __shared__ uint32_t s_first[SORT_NT];       // first head index (tile-local) of each thread, or NO_HEAD
     __shared__ uint32_t s_scan[SORT_NT];
     __shared__ uint32_t s_tile, s_end, s_nd;
     __shared__ uint64_t s_excl;
@@ -341,12 +341,14 @@ __global__ __launch_bounds__(SORT_NT) void k_runlength(Plan* __restrict__ plan, 
             }
         }
         // first head of each thread -> next-head lookup across threads
-        s_first[threadIdx.x] = headmask ? my0 + __builtin_ctz(headmask) : RL_TILE;
+        // NO_HEAD must exceed every run end, including ends past the tile
+        constexpr uint32_t NO_HEAD = 0xFFFFFFFFu;
+        s_first[threadIdx.x] = headmask ? my0 + __builtin_ctz(headmask) : NO_HEAD;
         if (threadIdx.x == 0) { s_end = tn; s_nd = 0; }
         __syncthreads();
         // suffix minimum over threads of first-head positions
         for (int off = 1; off < SORT_NT; off <<= 1) {
-            const uint32_t y = (threadIdx.x + off < SORT_NT) ? s_first[threadIdx.x + off] : RL_TILE;
+            const uint32_t y = (threadIdx.x + off < SORT_NT) ? s_first[threadIdx.x + off] : NO_HEAD;
             __syncthreads();
             s_first[threadIdx.x] = min(s_first[threadIdx.x], y);
             __syncthreads();
@@ -370,7 +372,7 @@ __global__ __launch_bounds__(SORT_NT) void k_runlength(Plan* __restrict__ plan, 
         uint32_t keep = 0, nheads = 0;
         uint32_t cnt[RL_ITEMS];
         {
-            const uint32_t nxt_thread = (threadIdx.x + 1 < SORT_NT) ? s_first[threadIdx.x + 1] : RL_TILE;
+            const uint32_t nxt_thread = (threadIdx.x + 1 < SORT_NT) ? s_first[threadIdx.x + 1] : NO_HEAD;
             uint32_t next = min(nxt_thread, s_end);
 #pragma unroll
             for (int i = RL_ITEMS - 1; i >= 0; i--) {
