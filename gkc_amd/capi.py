@@ -36,7 +36,7 @@ class Params(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("sort_items", C.c_int32), ("sort_blocks_per_cu", C.c_int32), ("reserved0", C.c_int32),
+    _fields_ = [("sort_items", C.c_int32), ("sort_blocks_per_cu", C.c_int32), ("single_list", C.c_int32),
                 ("reserved1", C.c_int32)]
 
 
@@ -198,14 +198,14 @@ class Counter:
         caps = np.ascontiguousarray(caps, np.uint32)
         _check(lib().gkc_set_table_caps(self.h, _p(caps, _u32p)))
 
-    def set_tuning(self, sort_items: int = 0, sort_blocks_per_cu: int = 0) -> None:
-        t = Tuning(sort_items, sort_blocks_per_cu, 0, 0)
+    def set_tuning(self, sort_items: int = 0, sort_blocks_per_cu: int = 0, single_list: int = 0) -> None:
+        t = Tuning(sort_items, sort_blocks_per_cu, single_list, 0)
         _check(lib().gkc_set_tuning(self.h, C.byref(t)))
 
     def get_tuning(self) -> dict:
         t = Tuning()
         _check(lib().gkc_get_tuning(self.h, C.byref(t)))
-        return {"sort_items": t.sort_items, "sort_blocks_per_cu": t.sort_blocks_per_cu}
+        return {"sort_items": t.sort_items, "sort_blocks_per_cu": t.sort_blocks_per_cu, "single_list": t.single_list}
 
     def part_kmers(self) -> np.ndarray:
         out = np.zeros(self.n_parts, np.uint64)

@@ -93,6 +93,9 @@ struct gkc_ctx {
     uint64_t bases_done = 0;                   // bases partitioned since the last reset
     uint32_t p1_fallbacks = 0;                 // presized scatters redone since the last reset
     gkc_tuning tune{};                         // kernel variants picked by the autotuner
+    DevBuf d_wl;                               // work lists of the phase-2 kernels
+    std::vector<uint32_t> h_wl;
+    WorkLists wl{};
     hipGraphExec_t g_p1 = nullptr;             // captured presized scatter of one chunk
     std::vector<const void*> g_p1_key;
     uint64_t win_n = 0;
@@ -771,7 +774,7 @@ static void zero(gkc_ctx* c, void* p, uint64_t n32) {
 // the sort tile count upper bound.
 // onesweep variants (keys per thread), selected by gkc_tuning.sort_items
 template <typename K>
-using OnesweepFn = void (*)(Plan*, SegArrays, const K*, K*, uint32_t*, int);
+using OnesweepFn = void (*)(Plan*, SegArrays, WorkLists, const K*, K*, uint32_t*, int);
 template <typename K> static OnesweepFn<K> onesweep_variant(int items);
 template <> OnesweepFn<uint64_t> onesweep_variant<uint64_t>(int items) {
     switch (items) {
@@ -800,11 +803,59 @@ template <typename K> static int sort_grid(gkc_ctx* c) {
     if (c->tune.sort_blocks_per_cu > 0) g = std::min(g, c->tune.sort_blocks_per_cu * c->n_cu);
     return std::max(g, 1);
 }
+// Work lists of one counting launch: n_lists lists (8 = one per XCD in the
+// batched form), segments assigned longest-first to the list with the fewest
+// sort tiles so far, each list's segments in increasing id order.
+static void set_lists(gkc_ctx* c, uint32_t n_segs, const uint32_t* keys, const uint32_t* hdrs, uint32_t n_lists,
+                      int sort_tile_keys) {
+    const uint32_t n = n_segs;
+    std::vector<uint32_t> st(n), rt(n), it(n), owner(n);
+    for (uint32_t s = 0; s < n; s++) {
+        st[s] = (uint32_t)(((uint64_t)keys[s] + sort_tile_keys - 1) / sort_tile_keys);
+        rt[s] = (uint32_t)(((uint64_t)keys[s] + RL_TILE - 1) / RL_TILE);
+        it[s] = (hdrs[s] + EX_HDRS - 1) / EX_HDRS;
+    }
+    std::vector<uint32_t> byload(n);
+    for (uint32_t s = 0; s < n; s++) byload[s] = s;
+    std::stable_sort(byload.begin(), byload.end(), [&](uint32_t a, uint32_t b) { return st[a] > st[b]; });
+    std::vector<uint64_t> load(n_lists, 0);
+    for (uint32_t s : byload) {
+        uint32_t best = 0;
+        for (uint32_t L = 1; L < n_lists; L++) if (load[L] < load[best]) best = L;
+        owner[s] = best;
+        load[best] += st[s] + 1;
+    }
+    std::vector<uint32_t>& w = c->h_wl;
+    w.assign(33 + 4 * (size_t)n, 0);
+    uint32_t pos = 0;
+    for (uint32_t L = 0; L < n_lists; L++) {
+        w[24 + L] = pos;
+        uint64_t ts = 0, tr = 0, ti = 0;
+        for (uint32_t s = 0; s < n; s++) {
+            if (owner[s] != L) continue;
+            w[33 + pos] = s;
+            w[33 + n + pos] = (uint32_t)ts;
+            w[33 + 2 * n + pos] = (uint32_t)tr;
+            w[33 + 3 * n + pos] = (uint32_t)ti;
+            ts += st[s]; tr += rt[s]; ti += it[s];
+            pos++;
+        }
+        w[L] = (uint32_t)ts; w[8 + L] = (uint32_t)tr; w[16 + L] = (uint32_t)ti;
+    }
+    for (uint32_t L = n_lists; L <= 8; L++) w[24 + L] = pos;
+    c->d_wl.ensure(w.size() * 4);
+    HIPX(hipMemcpyAsync(c->d_wl.p, w.data(), w.size() * 4, hipMemcpyHostToDevice, c->st));
+    c->wl.w = c->d_wl.as<uint32_t>();
+    c->wl.n_lists = n_lists;
+    c->wl.n_segs = n;
+}
+
 template <typename K>
 static void launch_onesweep(gkc_ctx* c, int grid, Plan* plan, SegArrays sa, const K* src, K* dst, int pass) {
     OnesweepFn<K> fn = onesweep_variant<K>(sort_items<K>(c));
     uint32_t* lb = c->lookback.as<uint32_t>();
-    void* args[] = {&plan, &sa, &src, &dst, &lb, &pass};
+    WorkLists wl = c->wl;
+    void* args[] = {&plan, &sa, &wl, &src, &dst, &lb, &pass};
     HIPX(hipLaunchKernel((const void*)fn, dim3(grid), dim3(SORT_NT), args, 0, c->st));
 }
 
@@ -813,6 +864,10 @@ static K* sort_segments(gkc_ctx* c, SegArrays sa, const uint64_t* hdr_base, cons
                         uint32_t n_segs, uint64_t n_keys, uint64_t n_hdrs, float* ms_ex, float* ms_sort, float* ms_pass_sum) {
     Plan* plan = c->d_plan.as<Plan>();
     const int TILE = SORT_NT * sort_items<K>(c);
+    {
+        const uint32_t nk = (uint32_t)n_keys, nh = (uint32_t)n_hdrs;
+        set_lists(c, 1, &nk, &nh, 1, TILE);
+    }
     hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(64), 0, c->st, plan, n_segs, (uint32_t)c->npass);
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, c->st, plan, sa, TILE);
     HIPX(hipGetLastError());
@@ -820,7 +875,7 @@ static K* sort_segments(gkc_ctx* c, SegArrays sa, const uint64_t* hdr_base, cons
     HIPX(hipEventRecord(c->ev[2], c->st));
     const uint64_t items = n_hdrs / EX_HDRS + n_segs;
     int g_ex = (int)std::min<uint64_t>(items, occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu));
-    hipLaunchKernelGGL(k_extract<K>, dim3(std::max(g_ex, 1)), dim3(SORT_NT), 0, c->st, plan, sa, c->hdrs.as<SkmHdr>(), hdr_base,
+    hipLaunchKernelGGL(k_extract<K>, dim3(std::max(g_ex, 1)), dim3(SORT_NT), 0, c->st, plan, sa, c->wl, c->hdrs.as<SkmHdr>(), hdr_base,
                        c->words.as<uint32_t>(), word_base, c->keysA.as<K>(), c->prm.k);
     HIPX(hipGetLastError());
     hipLaunchKernelGGL(k_hist_scan, dim3(std::max<uint32_t>(1, (n_segs * MAX_PASS + 3) / 4)), dim3(256), 0, c->st, plan, sa.hist);
@@ -894,7 +949,7 @@ static void runlength_one(gkc_ctx* c, SegArrays sa, const K* sorted, uint64_t nk
     zero(c, c->d_nent.p, 1); zero(c, c->d_ndist.p, 1); zero(c, c->d_toverflow.p, 1);
     zero(c, c->lookback.p, rl_tiles);
     zero(c, &plan->ctr[1 + MAX_PASS], 1);
-    hipLaunchKernelGGL(k_runlength<K>, dim3(g_r), dim3(SORT_NT), 0, c->st, plan, sa, sorted, to, c->lookback.as<uint32_t>());
+    hipLaunchKernelGGL(k_runlength<K>, dim3(g_r), dim3(SORT_NT), 0, c->st, plan, sa, c->wl, sorted, to, c->lookback.as<uint32_t>());
     HIPX(hipGetLastError());
     uint32_t ne = 0, nd = 0;
     HIPX(hipMemcpyAsync(&ne, c->d_nent.p, 4, hipMemcpyDeviceToHost, c->st));
@@ -908,7 +963,7 @@ static void runlength_one(gkc_ctx* c, SegArrays sa, const K* sorted, uint64_t nk
     zero(c, c->d_nent.p, 1); zero(c, c->d_ndist.p, 1);
     zero(c, c->lookback.p, rl_tiles);
     zero(c, &plan->ctr[1 + MAX_PASS], 1);
-    hipLaunchKernelGGL(k_runlength<K>, dim3(g_r), dim3(SORT_NT), 0, c->st, plan, sa, sorted, to, c->lookback.as<uint32_t>());
+    hipLaunchKernelGGL(k_runlength<K>, dim3(g_r), dim3(SORT_NT), 0, c->st, plan, sa, c->wl, sorted, to, c->lookback.as<uint32_t>());
     HIPX(hipGetLastError());
     HIPX(hipEventRecord(c->ev[3], c->st));
     HIPX(hipEventSynchronize(c->ev[3]));
@@ -1064,7 +1119,7 @@ static K* enqueue_sort_all(gkc_ctx* c, SegArrays sa, uint64_t tiles_cap, bool ti
     zero(c, sa.hist, (uint64_t)P * MAX_PASS * RADIX);
     if (timed) HIPX(hipEventRecord(c->ev[2], c->st));
     const int g_ex = occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu);
-    hipLaunchKernelGGL(k_extract<K>, dim3(g_ex), dim3(SORT_NT), 0, c->st, plan, sa, c->hdrs.as<SkmHdr>(),
+    hipLaunchKernelGGL(k_extract<K>, dim3(g_ex), dim3(SORT_NT), 0, c->st, plan, sa, c->wl, c->hdrs.as<SkmHdr>(),
                        c->d_hdr_base.as<uint64_t>(), c->words.as<uint32_t>(), c->d_word_base.as<uint64_t>(),
                        c->keysA.as<K>(), c->prm.k);
     hipLaunchKernelGGL(k_hist_scan, dim3((P * MAX_PASS + 3) / 4), dim3(256), 0, c->st, plan, sa.hist);
@@ -1099,7 +1154,7 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const K* sorted, uin
     zero(c, c->lookback.p, rl_tiles_cap);
     zero(c, &plan->ctr[1 + MAX_PASS], 1);
     const int g_r = occupancy_blocks((const void*)k_runlength<K>, SORT_NT, c->n_cu);
-    hipLaunchKernelGGL(k_runlength<K>, dim3(g_r), dim3(SORT_NT), 0, c->st, plan, sa, sorted, to, c->lookback.as<uint32_t>());
+    hipLaunchKernelGGL(k_runlength<K>, dim3(g_r), dim3(SORT_NT), 0, c->st, plan, sa, c->wl, sorted, to, c->lookback.as<uint32_t>());
     HIPX(hipGetLastError());
 }
 
@@ -1146,6 +1201,8 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     sa.sort_tile_start = c->d_tilestart.as<uint32_t>(); sa.rl_tile_start = c->d_rlstart.as<uint32_t>();
     sa.hist = c->d_hist.as<uint32_t>();
     const K* sorted = (c->npass % 2 == 0) ? c->keysA.as<K>() : c->keysB.as<K>();
+    set_lists(c, P, c->h_kmer_cur.data(), c->h_hdr_cur.data(), c->tune.single_list == 1 ? 1 : NLIST,
+              SORT_NT * sort_items<K>(c));
 
     // table capacities: predicted (presized) or a count pass (count-then-allocate)
     const bool presized = (flags & GKC_PRESIZED) && c->presize_caps.size() == P;
@@ -1155,7 +1212,8 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     HIPX(hipEventRecord(c->ev[0], c->st));
     std::vector<const void*> key = {c->keysA.p, c->keysB.p, c->lookback.p, c->words.p, c->hdrs.p,
                                     (const void*)(uintptr_t)tiles_cap, (const void*)(uintptr_t)c->npass,
-                                    (const void*)(uintptr_t)sort_items<K>(c), (const void*)(uintptr_t)sort_grid<K>(c)};
+                                    (const void*)(uintptr_t)sort_items<K>(c), (const void*)(uintptr_t)sort_grid<K>(c),
+                                    c->d_wl.p, (const void*)(uintptr_t)c->wl.n_lists};
     if (timed) {
         enqueue_sort_all<K>(c, sa, tiles_cap, true);
     } else {

@@ -51,6 +51,11 @@ constexpr int SORT_NT = 256;
 constexpr int RL_ITEMS = 16;
 constexpr int RL_TILE = SORT_NT * RL_ITEMS;
 constexpr int EX_HDRS = 256;
+constexpr int NLIST = 8;
+constexpr int NCTR = NLIST * (MAX_PASS + 2);
+__host__ __device__ constexpr int ctr_extract() { return 0; }
+__host__ __device__ constexpr int ctr_pass(int p) { return NLIST * (1 + p); }
+__host__ __device__ constexpr int ctr_rl() { return NLIST * (1 + MAX_PASS); }
 
 // Device-resident plan of one counting launch (all sizes live here so the
 // launch sequence can be captured once in a hipGraph and replayed).
@@ -62,7 +67,24 @@ struct Plan {
     uint32_t tot_rl_tiles;
     uint32_t pad;
     uint64_t tot_keys;
-    uint32_t ctr[48];           // work counters: [0] extract, [1+pass] sort passes, [1+MAX_PASS] run-length
+    uint32_t ctr[NCTR];         // work counters, NLIST per stage: see ctr_extract / ctr_pass / ctr_rl
+};
+
+// Work lists.  Segments are grouped into lists (one per XCD in the batched
+// form, a single list in the plain form); the tiles of a list are claimed in
+// order from the list's own counter, so consecutive tiles of a segment run on
+// one XCD (the partial cache lines at digit-run boundaries meet in one L2, and
+// look-back neighbours sit on the same die).  A workgroup starts on list
+// blockIdx % n_lists and moves to the next list when its list is empty, so
+// every list drains even if an XCD gets no workgroup.
+// Packed layout of w (n = n_segs):
+//   [0, 8)        sort tiles per list        [8, 16)   run-length tiles per list
+//   [16, 24)      extraction items per list  [24, 33)  first order index of each list
+//   [33, 33+n)    segment order              then per order index: sort tile offset in
+//   its list [n], run-length tile offset [n], extraction item offset [n]
+struct WorkLists {
+    const uint32_t* w;
+    uint32_t n_lists, n_segs;
 };
 
 struct SegArrays {
@@ -107,15 +129,15 @@ __global__ void k_set_u32(uint32_t* p, uint32_t v);
 __global__ void k_plan_init(Plan* plan, uint32_t n_segs, uint32_t n_pass);
 __global__ void k_set_chunk(ChunkDesc* dst, ChunkDesc v);
 template <typename K>
-__global__ void k_extract(Plan* __restrict__ plan, SegArrays sa, const SkmHdr* __restrict__ hdrs,
+__global__ void k_extract(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const SkmHdr* __restrict__ hdrs,
                           const uint64_t* __restrict__ hdr_base, const uint32_t* __restrict__ words,
                           const uint64_t* __restrict__ word_base, K* __restrict__ keys, int k);
 __global__ void k_hist_scan(Plan* __restrict__ plan, uint32_t* __restrict__ hist);
 template <typename K, int ITEMS>
-__global__ void k_onesweep(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ in, K* __restrict__ out,
+__global__ void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ in, K* __restrict__ out,
                            uint32_t* __restrict__ lookback, int pass);
 template <typename K>
-__global__ void k_runlength(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys, TableOut to,
+__global__ void k_runlength(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ keys, TableOut to,
                             uint32_t* __restrict__ lookback);
 
 }  // namespace gkc

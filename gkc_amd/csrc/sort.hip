@@ -25,6 +25,35 @@ __device__ __forceinline__ uint32_t find_seg(const uint32_t* start, uint32_t n_s
     return lo;
 }
 
+// Claim the next unit of work of kind KIND (0 sort tile, 1 run-length tile,
+// 2 extraction item) from the work lists; run by one thread.  `list` is the
+// workgroup's current list (advanced when a list is empty).  Returns false when
+// every list is exhausted.
+template <int KIND>
+__device__ __forceinline__ bool claim(const WorkLists& wl, uint32_t* ctr, uint32_t& list, uint32_t& seg, uint32_t& idx) {
+    const uint32_t n = wl.n_segs;
+    for (uint32_t tries = 0; tries < wl.n_lists; tries++) {
+        const uint32_t L = list;
+        const uint32_t tot = wl.w[KIND * 8 + L];
+        if (tot) {
+            const uint32_t t = atomicAdd(&ctr[L], 1u);
+            if (t < tot) {
+                const uint32_t* off = wl.w + 33 + n + KIND * n;
+                uint32_t lo = wl.w[24 + L], hi = wl.w[24 + L + 1];   // order indices of list L
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (off[mid] <= t) lo = mid; else hi = mid;
+                }
+                seg = wl.w[33 + lo];
+                idx = t - off[lo];
+                return true;
+            }
+        }
+        list = (L + 1) % wl.n_lists;
+    }
+    return false;
+}
+
 // Build the segment layout from per-segment key and header counts (one block).
 __global__ __launch_bounds__(1024) void k_plan(Plan* plan, SegArrays sa, int sort_tile_keys) {
     __shared__ uint64_t s_k[1024];
@@ -76,7 +105,7 @@ __global__ void k_plan_init(Plan* plan, uint32_t n_segs, uint32_t n_pass) {
     const int t = threadIdx.x;
     if (t == 0) { plan->n_segs = n_segs; plan->n_pass = n_pass; plan->tot_items = 0; plan->tot_sort_tiles = 0;
                   plan->tot_rl_tiles = 0; plan->pad = 0; plan->tot_keys = 0; }
-    if (t < 48) plan->ctr[t] = 0;
+    for (int i = t; i < NCTR; i += blockDim.x) plan->ctr[i] = 0;
 }
 __global__ void k_set_chunk(ChunkDesc* dst, ChunkDesc v) { *dst = v; }
 
@@ -124,22 +153,25 @@ __device__ __forceinline__ u128k canon_of(u128k x, int k) {
 }
 
 template <typename K>
-__global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, SegArrays sa, const SkmHdr* __restrict__ hdrs,
+__global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const SkmHdr* __restrict__ hdrs,
                                                      const uint64_t* __restrict__ hdr_base,
                                                      const uint32_t* __restrict__ words,
                                                      const uint64_t* __restrict__ word_base, K* __restrict__ keys, int k) {
     __shared__ uint32_t s_hist[MAX_PASS * RADIX];
-    __shared__ uint32_t s_item;
+    __shared__ uint32_t s_seg, s_idx, s_ok;
     const int npass = plan->n_pass;
-    const uint32_t tot = plan->tot_items, S = plan->n_segs;
+    uint32_t list = blockIdx.x % wl.n_lists;
     for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(&plan->ctr[0], 1u);
+        if (threadIdx.x == 0) {
+            uint32_t sg = 0, ix = 0;
+            s_ok = claim<2>(wl, &plan->ctr[ctr_extract()], list, sg, ix);
+            s_seg = sg; s_idx = ix;
+        }
         for (int i = threadIdx.x; i < npass * RADIX; i += SORT_NT) s_hist[i] = 0;
         __syncthreads();
-        const uint32_t item = s_item;
-        if (item >= tot) break;
-        const uint32_t seg = find_seg(sa.item_start, S, item);
-        const uint32_t h = (item - sa.item_start[seg]) * EX_HDRS + threadIdx.x;
+        if (!s_ok) break;
+        const uint32_t seg = s_seg;
+        const uint32_t h = s_idx * EX_HDRS + threadIdx.x;
         if (h < sa.seg_hdrs[seg]) {
             const SkmHdr hd = hdrs[hdr_base[seg] + h];
             const uint32_t* wds = words + word_base[seg] + hd.word_off;
@@ -188,7 +220,7 @@ __global__ __launch_bounds__(256) void k_hist_scan(Plan* __restrict__ plan, uint
 // segment; tiles are claimed in order from a counter so every tile's
 // predecessors are already running (forward progress of the look-back).
 template <typename K, int ITEMS>
-__global__ __launch_bounds__(SORT_NT) void k_onesweep(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ in,
+__global__ __launch_bounds__(SORT_NT) void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ in,
                                                       K* __restrict__ out, uint32_t* __restrict__ lookback, int pass) {
     constexpr int TILE = SORT_NT * ITEMS;
     constexpr int NWAVE = SORT_NT / GKC_WAVE;
@@ -196,19 +228,22 @@ __global__ __launch_bounds__(SORT_NT) void k_onesweep(Plan* __restrict__ plan, S
     __shared__ uint32_t s_whist[NWAVE][RADIX];
     __shared__ uint32_t s_dstart[RADIX];
     __shared__ uint64_t s_gbase[RADIX];
-    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_seg, s_idx, s_ok;
     const int sh = pass * RADIX_BITS;
-    const uint32_t tot = plan->tot_sort_tiles, S = plan->n_segs;
     const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE;
+    uint32_t list = blockIdx.x % wl.n_lists;
     for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(&plan->ctr[1 + pass], 1u);
+        if (threadIdx.x == 0) {
+            uint32_t sg = 0, ix = 0;
+            s_ok = claim<0>(wl, &plan->ctr[ctr_pass(pass)], list, sg, ix);
+            s_seg = sg; s_idx = ix;
+        }
         for (int i = threadIdx.x; i < NWAVE * RADIX; i += SORT_NT) (&s_whist[0][0])[i] = 0;
         __syncthreads();
-        const uint32_t tile = s_tile;
-        if (tile >= tot) break;
-        const uint32_t seg = find_seg(sa.sort_tile_start, S, tile);
-        const uint32_t first_tile = sa.sort_tile_start[seg];
-        const uint32_t ti = tile - first_tile;
+        if (!s_ok) break;
+        const uint32_t seg = s_seg;
+        const uint32_t ti = s_idx;
+        const uint32_t tile = sa.sort_tile_start[seg] + ti;      // look-back slot
         const uint64_t kbase = sa.key_start[seg];
         const uint32_t nseg = sa.seg_keys[seg];
         const uint32_t t0 = ti * TILE;
@@ -303,20 +338,24 @@ __global__ __launch_bounds__(SORT_NT) void k_onesweep(Plan* __restrict__ plan, S
 // decoupled look-back (one value per tile).  Thread t of a tile holds keys
 // [t0 + 16t, t0 + 16t + 16); the tile's last run is followed past the tile.
 template <typename K>
-__global__ __launch_bounds__(SORT_NT) void k_runlength(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys,
+__global__ __launch_bounds__(SORT_NT) void k_runlength(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ keys,
                                                        TableOut to, uint32_t* __restrict__ lookback) {
     __shared__ uint32_t s_first[SORT_NT];       // first head index (tile-local) of each thread, or NO_HEAD
     __shared__ uint32_t s_scan[SORT_NT];
-    __shared__ uint32_t s_tile, s_end, s_nd;
+    __shared__ uint32_t s_end, s_nd, s_seg, s_idx, s_ok;
     __shared__ uint64_t s_excl;
-    const uint32_t tot = plan->tot_rl_tiles, S = plan->n_segs;
+    uint32_t list = blockIdx.x % wl.n_lists;
     for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(&plan->ctr[1 + MAX_PASS], 1u);
+        if (threadIdx.x == 0) {
+            uint32_t sg = 0, ix = 0;
+            s_ok = claim<1>(wl, &plan->ctr[ctr_rl()], list, sg, ix);
+            s_seg = sg; s_idx = ix;
+        }
         __syncthreads();
-        const uint32_t tile = s_tile;
-        if (tile >= tot) break;
-        const uint32_t seg = find_seg(sa.rl_tile_start, S, tile);
-        const uint32_t ti = tile - sa.rl_tile_start[seg];
+        if (!s_ok) break;
+        const uint32_t seg = s_seg;
+        const uint32_t ti = s_idx;
+        const uint32_t tile = sa.rl_tile_start[seg] + ti;        // look-back slot
         const K* kp = keys + sa.key_start[seg];
         const uint32_t nseg = sa.seg_keys[seg];
         const uint32_t t0 = ti * RL_TILE;
@@ -438,13 +477,13 @@ __global__ __launch_bounds__(SORT_NT) void k_runlength(Plan* __restrict__ plan, 
     }
 }
 
-template __global__ void k_extract<uint64_t>(Plan*, SegArrays, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, uint64_t*, int);
-template __global__ void k_extract<u128k>(Plan*, SegArrays, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, u128k*, int);
-#define GKC_OS(K, I) template __global__ void k_onesweep<K, I>(Plan*, SegArrays, const K*, K*, uint32_t*, int);
+template __global__ void k_extract<uint64_t>(Plan*, SegArrays, WorkLists, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, uint64_t*, int);
+template __global__ void k_extract<u128k>(Plan*, SegArrays, WorkLists, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, u128k*, int);
+#define GKC_OS(K, I) template __global__ void k_onesweep<K, I>(Plan*, SegArrays, WorkLists, const K*, K*, uint32_t*, int);
 GKC_OS(uint64_t, 8) GKC_OS(uint64_t, 12) GKC_OS(uint64_t, 16) GKC_OS(uint64_t, 20) GKC_OS(uint64_t, 24)
 GKC_OS(u128k, 4) GKC_OS(u128k, 8) GKC_OS(u128k, 12)
 #undef GKC_OS
-template __global__ void k_runlength<uint64_t>(Plan*, SegArrays, const uint64_t*, TableOut, uint32_t*);
-template __global__ void k_runlength<u128k>(Plan*, SegArrays, const u128k*, TableOut, uint32_t*);
+template __global__ void k_runlength<uint64_t>(Plan*, SegArrays, WorkLists, const uint64_t*, TableOut, uint32_t*);
+template __global__ void k_runlength<u128k>(Plan*, SegArrays, WorkLists, const u128k*, TableOut, uint32_t*);
 
 }  // namespace gkc

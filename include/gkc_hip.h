@@ -75,7 +75,8 @@ typedef struct gkc_stats {
 typedef struct gkc_tuning {
     int32_t sort_items;          /* keys per thread of a onesweep tile: 8,12,16,20,24 (u64) / 4,8,12 (u128) */
     int32_t sort_blocks_per_cu;  /* persistent workgroups per CU in the sort passes (0: occupancy limit) */
-    int32_t reserved0, reserved1;
+    int32_t single_list;         /* 1: batched tiles in one global order instead of one work list per XCD */
+    int32_t reserved1;
 } gkc_tuning;
 
 const char *gkc_last_error(void);
