@@ -948,7 +948,7 @@ static void runlength_one(gkc_ctx* c, SegArrays sa, const K* sorted, uint64_t nk
     Plan* plan = c->d_plan.as<Plan>();
     zero(c, c->d_nent.p, 1); zero(c, c->d_ndist.p, 1); zero(c, c->d_toverflow.p, 1);
     zero(c, c->lookback.p, rl_tiles);
-    zero(c, &plan->ctr[1 + MAX_PASS], 1);
+    zero(c, &plan->ctr[ctr_rl()], NLIST);
     hipLaunchKernelGGL(k_runlength<K>, dim3(g_r), dim3(SORT_NT), 0, c->st, plan, sa, c->wl, sorted, to, c->lookback.as<uint32_t>());
     HIPX(hipGetLastError());
     uint32_t ne = 0, nd = 0;
@@ -962,7 +962,7 @@ static void runlength_one(gkc_ctx* c, SegArrays sa, const K* sorted, uint64_t nk
     to.emit = 1; to.keys = c->tkeys.p; to.counts = c->tcnts.as<uint32_t>();
     zero(c, c->d_nent.p, 1); zero(c, c->d_ndist.p, 1);
     zero(c, c->lookback.p, rl_tiles);
-    zero(c, &plan->ctr[1 + MAX_PASS], 1);
+    zero(c, &plan->ctr[ctr_rl()], NLIST);
     hipLaunchKernelGGL(k_runlength<K>, dim3(g_r), dim3(SORT_NT), 0, c->st, plan, sa, c->wl, sorted, to, c->lookback.as<uint32_t>());
     HIPX(hipGetLastError());
     HIPX(hipEventRecord(c->ev[3], c->st));
@@ -1152,7 +1152,7 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const K* sorted, uin
     to.min_cnt = c->prm.min_count; to.max_cnt = c->prm.max_count; to.emit = emit;
     zero(c, c->d_nent.p, P); zero(c, c->d_ndist.p, P); zero(c, c->d_toverflow.p, 1);
     zero(c, c->lookback.p, rl_tiles_cap);
-    zero(c, &plan->ctr[1 + MAX_PASS], 1);
+    zero(c, &plan->ctr[ctr_rl()], NLIST);
     const int g_r = occupancy_blocks((const void*)k_runlength<K>, SORT_NT, c->n_cu);
     hipLaunchKernelGGL(k_runlength<K>, dim3(g_r), dim3(SORT_NT), 0, c->st, plan, sa, c->wl, sorted, to, c->lookback.as<uint32_t>());
     HIPX(hipGetLastError());
