@@ -152,14 +152,22 @@ __device__ __forceinline__ u128k canon_of(u128k x, int k) {
     return less128(r, x) ? r : x;
 }
 
+// Work item = EX_HDRS consecutive headers of one segment; each wave takes 64
+// of them, prefix-sums their k-mer counts and strides its lanes over the
+// combined k-mers, so consecutive lanes write consecutive keys (a scatter
+// block's superkmers of one partition have consecutive k-mer offsets).
 template <typename K>
 __global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const SkmHdr* __restrict__ hdrs,
                                                      const uint64_t* __restrict__ hdr_base,
                                                      const uint32_t* __restrict__ words,
                                                      const uint64_t* __restrict__ word_base, K* __restrict__ keys, int k) {
+    constexpr int NWAVE = SORT_NT / GKC_WAVE;
     __shared__ uint32_t s_hist[MAX_PASS * RADIX];
+    __shared__ uint32_t s_pre[NWAVE][GKC_WAVE + 1];     // inclusive k-mer prefix of the wave's headers
+    __shared__ uint32_t s_koff[NWAVE][GKC_WAVE], s_woff[NWAVE][GKC_WAVE];
     __shared__ uint32_t s_seg, s_idx, s_ok;
     const int npass = plan->n_pass;
+    const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE;
     uint32_t list = blockIdx.x % wl.n_lists;
     for (;;) {
         if (threadIdx.x == 0) {
@@ -172,16 +180,37 @@ __global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, Se
         if (!s_ok) break;
         const uint32_t seg = s_seg;
         const uint32_t h = s_idx * EX_HDRS + threadIdx.x;
+        uint32_t n = 0;
         if (h < sa.seg_hdrs[seg]) {
             const SkmHdr hd = hdrs[hdr_base[seg] + h];
-            const uint32_t* wds = words + word_base[seg] + hd.word_off;
-            K* out = keys + sa.key_start[seg] + hd.kmer_off;
-            for (uint32_t j = 0; j < hd.n_kmers; j++) {
-                const K c = canon_of(kmer_at<K>(wds, j, k), k);
-                out[j] = c;
-                for (int ps = 0; ps < npass; ps++)
-                    atomicAdd(&s_hist[ps * RADIX + KeyTraits<K>::digit(c, ps * RADIX_BITS)], 1u);
+            n = hd.n_kmers;
+            s_koff[wv][lane] = hd.kmer_off;
+            s_woff[wv][lane] = hd.word_off;
+        }
+        uint32_t inc = n;
+#pragma unroll
+        for (int off = 1; off < GKC_WAVE; off <<= 1) {
+            const uint32_t y = __shfl_up(inc, off);
+            if ((int)lane >= off) inc += y;
+        }
+        s_pre[wv][lane + 1] = inc;
+        if (lane == 0) s_pre[wv][0] = 0;
+        const uint32_t T = __shfl(inc, GKC_WAVE - 1);
+        __syncthreads();
+        const uint32_t* wseg = words + word_base[seg];
+        K* kseg = keys + sa.key_start[seg];
+        for (uint32_t q = lane; q < T; q += GKC_WAVE) {
+            // header of k-mer q: largest j with pre[j] <= q
+            uint32_t lo = 0, hi = GKC_WAVE;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_pre[wv][mid] <= q) lo = mid; else hi = mid;
             }
+            const uint32_t o = q - s_pre[wv][lo];
+            const K c = canon_of(kmer_at<K>(wseg + s_woff[wv][lo], o, k), k);
+            kseg[s_koff[wv][lo] + o] = c;
+            for (int ps = 0; ps < npass; ps++)
+                atomicAdd(&s_hist[ps * RADIX + KeyTraits<K>::digit(c, ps * RADIX_BITS)], 1u);
         }
         __syncthreads();
         uint32_t* gh = sa.hist + (size_t)seg * MAX_PASS * RADIX;
@@ -217,8 +246,17 @@ __global__ __launch_bounds__(256) void k_hist_scan(Plan* __restrict__ plan, uint
 }
 
 // One onesweep pass over all segments.  Tile = SORT_NT * ITEMS keys of one
-// segment; tiles are claimed in order from a counter so every tile's
-// predecessors are already running (forward progress of the look-back).
+// segment; tiles are claimed in order from the work lists, so every tile's
+// predecessors are claimed by running workgroups (forward progress of the
+// look-back).  Per tile:
+//   1. digits and the tile's digit histogram (LDS atomics, no returns)
+//   2. publish the aggregate at once, so successors can look back early
+//   3. prefetch the keys of the next claimed tile (in flight during 4-6)
+//   4. stable rank within each wave in (item, lane) order: lanes with equal
+//      digits are matched with RADIX_BITS ballots, the running per-wave digit
+//      counters live in LDS
+//   5. decoupled look-back per digit across the segment's earlier tiles
+//   6. reorder the tile in LDS by digit and write each digit's run out
 template <typename K, int ITEMS>
 __global__ __launch_bounds__(SORT_NT) void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ in,
                                                       K* __restrict__ out, uint32_t* __restrict__ lookback, int pass) {
@@ -226,81 +264,110 @@ __global__ __launch_bounds__(SORT_NT) void k_onesweep(Plan* __restrict__ plan, S
     constexpr int NWAVE = SORT_NT / GKC_WAVE;
     __shared__ K s_keys[TILE];
     __shared__ uint32_t s_whist[NWAVE][RADIX];
+    __shared__ uint32_t s_hist[RADIX];
     __shared__ uint32_t s_dstart[RADIX];
     __shared__ uint64_t s_gbase[RADIX];
     __shared__ uint32_t s_seg, s_idx, s_ok;
     const int sh = pass * RADIX_BITS;
     const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE;
+    const uint32_t d = threadIdx.x;     // SORT_NT == RADIX: one digit per thread
+    uint32_t* ctr = &plan->ctr[ctr_pass(pass)];
     uint32_t list = blockIdx.x % wl.n_lists;
+    const uint32_t wbase = wv * GKC_WAVE * ITEMS;   // wave wv owns tile keys [wbase, wbase + 64*ITEMS)
+
+    // prologue: first tile
+    if (threadIdx.x == 0) {
+        uint32_t sg = 0, ix = 0;
+        s_ok = claim<0>(wl, ctr, list, sg, ix);
+        s_seg = sg; s_idx = ix;
+    }
+    __syncthreads();
+    if (!s_ok) return;
+    uint32_t seg = s_seg, ti = s_idx;
+    uint32_t nseg = sa.seg_keys[seg];
+    uint64_t kbase = sa.key_start[seg];
+    uint32_t tn = min((uint32_t)TILE, nseg - ti * TILE);
+    K key[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; i++) {
+        const uint32_t j = wbase + i * GKC_WAVE + lane;
+        if (j < tn) key[i] = in[kbase + (uint64_t)ti * TILE + j];
+    }
     for (;;) {
+        // claim the next tile while this one is being processed
+        __syncthreads();
         if (threadIdx.x == 0) {
             uint32_t sg = 0, ix = 0;
-            s_ok = claim<0>(wl, &plan->ctr[ctr_pass(pass)], list, sg, ix);
+            s_ok = claim<0>(wl, ctr, list, sg, ix);
             s_seg = sg; s_idx = ix;
         }
+        s_hist[d] = 0;
         for (int i = threadIdx.x; i < NWAVE * RADIX; i += SORT_NT) (&s_whist[0][0])[i] = 0;
         __syncthreads();
-        if (!s_ok) break;
-        const uint32_t seg = s_seg;
-        const uint32_t ti = s_idx;
+        const bool have_next = s_ok;
+        const uint32_t nxt_seg = s_seg, nxt_ti = s_idx;
         const uint32_t tile = sa.sort_tile_start[seg] + ti;      // look-back slot
-        const uint64_t kbase = sa.key_start[seg];
-        const uint32_t nseg = sa.seg_keys[seg];
-        const uint32_t t0 = ti * TILE;
-        const uint32_t tn = min((uint32_t)TILE, nseg - t0);
-
-        // load: wave wv owns tile keys [wv*64*ITEMS, ...), item i of lane l at +i*64+l
-        K key[ITEMS];
-        uint32_t rank[ITEMS];
+        // 1. digits + tile histogram
         uint32_t dig[ITEMS];
-        const uint32_t wbase = wv * GKC_WAVE * ITEMS;
 #pragma unroll
         for (int i = 0; i < ITEMS; i++) {
             const uint32_t j = wbase + i * GKC_WAVE + lane;
-            if (j < tn) key[i] = in[kbase + t0 + j];
             dig[i] = (j < tn) ? KeyTraits<K>::digit(key[i], sh) : RADIX;
+            if (dig[i] < RADIX) atomicAdd(&s_hist[dig[i]], 1u);
         }
-        // rank within the wave in (item, lane) order: match lanes of equal digit
-        // with RADIX_BITS ballots, count the lower lanes, chain per-wave counters
+        __syncthreads();
+        // 2. aggregate out
+        const uint32_t total = s_hist[d];
+        uint32_t* lb = lookback + (size_t)tile * RADIX;
+        st_agent(&lb[d], (ti == 0 ? LB_INC : LB_AGG) | total);
+        // 3. prefetch the next tile
+        K nkey[ITEMS];
+        uint32_t n_nseg = 0, n_tn = 0;
+        uint64_t n_kbase = 0;
+        if (have_next) {
+            n_nseg = sa.seg_keys[nxt_seg];
+            n_kbase = sa.key_start[nxt_seg];
+            n_tn = min((uint32_t)TILE, n_nseg - nxt_ti * TILE);
+#pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                const uint32_t j = wbase + i * GKC_WAVE + lane;
+                if (j < n_tn) nkey[i] = in[n_kbase + (uint64_t)nxt_ti * TILE + j];
+            }
+        }
+        // 4. stable rank within the wave
+        uint32_t rank[ITEMS];
 #pragma unroll
         for (int i = 0; i < ITEMS; i++) {
-            const uint32_t d = dig[i];
-            uint64_t peers = __ballot(d < RADIX);
+            const uint32_t dd = dig[i];
+            uint64_t peers = __ballot(dd < RADIX);
 #pragma unroll
             for (int b = 0; b < RADIX_BITS; b++) {
-                const uint64_t bal = __ballot((d >> b) & 1u);
-                peers &= ((d >> b) & 1u) ? bal : ~bal;
+                const uint64_t bal = __ballot((dd >> b) & 1u);
+                peers &= ((dd >> b) & 1u) ? bal : ~bal;
             }
-            if (d < RADIX) {
-                const uint32_t before = s_whist[wv][d];
+            if (dd < RADIX) {
+                const uint32_t before = s_whist[wv][dd];
                 const uint64_t lower = peers & lanemask_lt();
                 rank[i] = before + (uint32_t)__popcll(lower);
-                if (lower == 0) s_whist[wv][d] = before + (uint32_t)__popcll(peers);
+                if (lower == 0) s_whist[wv][dd] = before + (uint32_t)__popcll(peers);
             }
         }
-        __syncthreads();
-        // per digit: wave prefix, tile total, publish aggregate
-        const uint32_t d = threadIdx.x;     // SORT_NT == RADIX
-        uint32_t total = 0;
-#pragma unroll
-        for (int w = 0; w < NWAVE; w++) { const uint32_t c = s_whist[w][d]; s_whist[w][d] = total; total += c; }
-        uint32_t* lb = lookback + (size_t)tile * RADIX;
-        if (ti == 0) st_agent(&lb[d], LB_INC | total);
-        else st_agent(&lb[d], LB_AGG | total);
-        // tile-local digit starts (block exclusive scan over digits)
+        // tile-local digit starts (exclusive scan over digits)
         s_dstart[d] = total;
         __syncthreads();
-        {
-            // Hillis-Steele over 256 entries
-            for (int off = 1; off < RADIX; off <<= 1) {
-                const uint32_t x = d >= (uint32_t)off ? s_dstart[d - off] : 0;
-                __syncthreads();
-                s_dstart[d] += x;
-                __syncthreads();
-            }
+        for (int off = 1; off < RADIX; off <<= 1) {
+            const uint32_t x = d >= (uint32_t)off ? s_dstart[d - off] : 0;
+            __syncthreads();
+            s_dstart[d] += x;
+            __syncthreads();
         }
         const uint32_t dstart = s_dstart[d] - total;
-        // decoupled look-back across the segment's earlier tiles
+        {   // per digit: exclusive prefix over the waves
+            uint32_t acc = 0;
+#pragma unroll
+            for (int w = 0; w < NWAVE; w++) { const uint32_t c = s_whist[w][d]; s_whist[w][d] = acc; acc += c; }
+        }
+        // 5. decoupled look-back
         uint32_t excl = 0;
         if (ti > 0) {
             int64_t j = (int64_t)tile - 1;
@@ -318,7 +385,7 @@ __global__ __launch_bounds__(SORT_NT) void k_onesweep(Plan* __restrict__ plan, S
         __syncthreads();
         s_dstart[d] = dstart;
         __syncthreads();
-        // scatter into LDS in digit order, then write runs out
+        // 6. reorder in LDS by digit, write each digit's run out
 #pragma unroll
         for (int i = 0; i < ITEMS; i++) {
             const uint32_t dd = dig[i];
@@ -329,10 +396,12 @@ __global__ __launch_bounds__(SORT_NT) void k_onesweep(Plan* __restrict__ plan, S
             const K x = s_keys[j];
             out[s_gbase[KeyTraits<K>::digit(x, sh)] + j] = x;
         }
-        __syncthreads();
+        if (!have_next) break;
+        seg = nxt_seg; ti = nxt_ti; nseg = n_nseg; kbase = n_kbase; tn = n_tn;
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) key[i] = nkey[i];
     }
 }
-
 
 // Run-length count + clamp + threshold on sorted segments, compacted with a
 // decoupled look-back (one value per tile).  Thread t of a tile holds keys
