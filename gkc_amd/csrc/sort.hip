@@ -266,6 +266,7 @@ __global__ __launch_bounds__(SORT_NT) void k_onesweep(Plan* __restrict__ plan, S
     __shared__ uint32_t s_whist[NWAVE][RADIX];
     __shared__ uint32_t s_hist[RADIX];
     __shared__ uint32_t s_dstart[RADIX];
+    __shared__ uint32_t s_wsum[NWAVE];
     __shared__ uint64_t s_gbase[RADIX];
     __shared__ uint32_t s_seg, s_idx, s_ok;
     const int sh = pass * RADIX_BITS;
@@ -352,39 +353,56 @@ __global__ __launch_bounds__(SORT_NT) void k_onesweep(Plan* __restrict__ plan, S
                 if (lower == 0) s_whist[wv][dd] = before + (uint32_t)__popcll(peers);
             }
         }
-        // tile-local digit starts (exclusive scan over digits)
-        s_dstart[d] = total;
-        __syncthreads();
-        for (int off = 1; off < RADIX; off <<= 1) {
-            const uint32_t x = d >= (uint32_t)off ? s_dstart[d - off] : 0;
-            __syncthreads();
-            s_dstart[d] += x;
-            __syncthreads();
+        // tile-local digit starts: exclusive scan over the 256 digits, one
+        // wave-level shuffle scan per 64 digits plus the 4 wave sums
+        uint32_t inc = total;
+#pragma unroll
+        for (int off = 1; off < GKC_WAVE; off <<= 1) {
+            const uint32_t y = __shfl_up(inc, off);
+            if ((int)lane >= off) inc += y;
         }
-        const uint32_t dstart = s_dstart[d] - total;
-        {   // per digit: exclusive prefix over the waves
+        if (lane == GKC_WAVE - 1) s_wsum[wv] = inc;
+        __syncthreads();                                         // ranking done, wave sums visible
+        uint32_t wsum = 0;
+#pragma unroll
+        for (int w = 0; w < NWAVE; w++) wsum += (w < (int)wv) ? s_wsum[w] : 0u;
+        s_dstart[d] = wsum + inc - total;
+        {   // per digit: exclusive prefix over the waves' running counters
             uint32_t acc = 0;
 #pragma unroll
             for (int w = 0; w < NWAVE; w++) { const uint32_t c = s_whist[w][d]; s_whist[w][d] = acc; acc += c; }
         }
-        // 5. decoupled look-back
-        uint32_t excl = 0;
-        if (ti > 0) {
-            int64_t j = (int64_t)tile - 1;
-            for (;;) {
-                uint32_t v = ld_agent(&lookback[(size_t)j * RADIX + d]);
-                while ((v & ~LB_VAL) == 0) { __builtin_amdgcn_s_sleep(1); v = ld_agent(&lookback[(size_t)j * RADIX + d]); }
-                excl += v & LB_VAL;
-                if (v & LB_INC) break;
-                j--;
+        __syncthreads();
+        // 5. decoupled look-back by wave 0 for all digits (4 per lane), which
+        //    also sets the global bases; the other waves go straight on to 6
+        if (wv == 0) {
+            const uint32_t d0 = lane * 4;
+            uint32_t exc[4] = {0, 0, 0, 0};
+            if (ti > 0) {
+                uint32_t todo = 0xF;
+                int64_t jj[4] = {(int64_t)tile - 1, (int64_t)tile - 1, (int64_t)tile - 1, (int64_t)tile - 1};
+                while (todo) {
+                    uint32_t v[4];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) v[e] = ((todo >> e) & 1) ? ld_agent(&lookback[(size_t)jj[e] * RADIX + d0 + e]) : 0u;
+                    bool wait = false;
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        if (!((todo >> e) & 1)) continue;
+                        if ((v[e] & ~LB_VAL) == 0) { wait = true; continue; }
+                        exc[e] += v[e] & LB_VAL;
+                        if (v[e] & LB_INC) todo &= ~(1u << e);
+                        else jj[e]--;
+                    }
+                    if (wait) __builtin_amdgcn_s_sleep(1);
+                }
+#pragma unroll
+                for (int e = 0; e < 4; e++) st_agent(&lb[d0 + e], LB_INC | (exc[e] + s_hist[d0 + e]));
             }
-            st_agent(&lb[d], LB_INC | (excl + total));
+            const uint32_t* hs = sa.hist + ((size_t)seg * MAX_PASS + pass) * RADIX;
+#pragma unroll
+            for (int e = 0; e < 4; e++) s_gbase[d0 + e] = kbase + hs[d0 + e] + exc[e] - s_dstart[d0 + e];
         }
-        const uint32_t* hs = sa.hist + ((size_t)seg * MAX_PASS + pass) * RADIX;
-        s_gbase[d] = kbase + hs[d] + excl - dstart;
-        __syncthreads();
-        s_dstart[d] = dstart;
-        __syncthreads();
         // 6. reorder in LDS by digit, write each digit's run out
 #pragma unroll
         for (int i = 0; i < ITEMS; i++) {
