@@ -373,36 +373,21 @@ __global__ __launch_bounds__(SORT_NT) void k_onesweep(Plan* __restrict__ plan, S
             for (int w = 0; w < NWAVE; w++) { const uint32_t c = s_whist[w][d]; s_whist[w][d] = acc; acc += c; }
         }
         __syncthreads();
-        // 5. decoupled look-back by wave 0 for all digits (4 per lane), which
-        //    also sets the global bases; the other waves go straight on to 6
-        if (wv == 0) {
-            const uint32_t d0 = lane * 4;
-            uint32_t exc[4] = {0, 0, 0, 0};
-            if (ti > 0) {
-                uint32_t todo = 0xF;
-                int64_t jj[4] = {(int64_t)tile - 1, (int64_t)tile - 1, (int64_t)tile - 1, (int64_t)tile - 1};
-                while (todo) {
-                    uint32_t v[4];
-#pragma unroll
-                    for (int e = 0; e < 4; e++) v[e] = ((todo >> e) & 1) ? ld_agent(&lookback[(size_t)jj[e] * RADIX + d0 + e]) : 0u;
-                    bool wait = false;
-#pragma unroll
-                    for (int e = 0; e < 4; e++) {
-                        if (!((todo >> e) & 1)) continue;
-                        if ((v[e] & ~LB_VAL) == 0) { wait = true; continue; }
-                        exc[e] += v[e] & LB_VAL;
-                        if (v[e] & LB_INC) todo &= ~(1u << e);
-                        else jj[e]--;
-                    }
-                    if (wait) __builtin_amdgcn_s_sleep(1);
-                }
-#pragma unroll
-                for (int e = 0; e < 4; e++) st_agent(&lb[d0 + e], LB_INC | (exc[e] + s_hist[d0 + e]));
+        // 5. decoupled look-back, one digit per thread
+        uint32_t excl = 0;
+        if (ti > 0) {
+            int64_t j = (int64_t)tile - 1;
+            for (;;) {
+                uint32_t v = ld_agent(&lookback[(size_t)j * RADIX + d]);
+                while ((v & ~LB_VAL) == 0) { __builtin_amdgcn_s_sleep(1); v = ld_agent(&lookback[(size_t)j * RADIX + d]); }
+                excl += v & LB_VAL;
+                if (v & LB_INC) break;
+                j--;
             }
-            const uint32_t* hs = sa.hist + ((size_t)seg * MAX_PASS + pass) * RADIX;
-#pragma unroll
-            for (int e = 0; e < 4; e++) s_gbase[d0 + e] = kbase + hs[d0 + e] + exc[e] - s_dstart[d0 + e];
+            st_agent(&lb[d], LB_INC | (excl + total));
         }
+        const uint32_t* hs = sa.hist + ((size_t)seg * MAX_PASS + pass) * RADIX;
+        s_gbase[d] = kbase + hs[d] + excl - s_dstart[d];
         // 6. reorder in LDS by digit, write each digit's run out
 #pragma unroll
         for (int i = 0; i < ITEMS; i++) {
