@@ -800,7 +800,7 @@ template <typename K> static int sort_items(gkc_ctx* c) {
 template <typename K> static int sort_grid(gkc_ctx* c) {
     OnesweepFn<K> fn = onesweep_variant<K>(sort_items<K>(c));
     int g = occupancy_blocks((const void*)fn, SORT_NT, c->n_cu);
-    if (c->tune.sort_blocks_per_cu > 0) g = std::min(g, c->tune.sort_blocks_per_cu * c->n_cu);
+    if (c->tune.sort_blocks_per_cu > 0) g = c->tune.sort_blocks_per_cu * c->n_cu;   // explicit (may exceed residency)
     return std::max(g, 1);
 }
 // Work lists of one counting launch: n_lists lists (8 = one per XCD in the

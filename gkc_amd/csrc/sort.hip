@@ -245,6 +245,9 @@ __global__ __launch_bounds__(256) void k_hist_scan(Plan* __restrict__ plan, uint
     }
 }
 
+#ifndef ONESWEEP_WAVES_PER_EU
+#define ONESWEEP_WAVES_PER_EU 4
+#endif
 // One onesweep pass over all segments.  Tile = SORT_NT * ITEMS keys of one
 // segment; tiles are claimed in order from the work lists, so every tile's
 // predecessors are claimed by running workgroups (forward progress of the
@@ -258,7 +261,7 @@ __global__ __launch_bounds__(256) void k_hist_scan(Plan* __restrict__ plan, uint
 //   5. decoupled look-back per digit across the segment's earlier tiles
 //   6. reorder the tile in LDS by digit and write each digit's run out
 template <typename K, int ITEMS>
-__global__ __launch_bounds__(SORT_NT) void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ in,
+__global__ __launch_bounds__(SORT_NT, ONESWEEP_WAVES_PER_EU) void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ in,
                                                       K* __restrict__ out, uint32_t* __restrict__ lookback, int pass) {
     constexpr int TILE = SORT_NT * ITEMS;
     constexpr int NWAVE = SORT_NT / GKC_WAVE;
@@ -309,7 +312,7 @@ __global__ __launch_bounds__(SORT_NT) void k_onesweep(Plan* __restrict__ plan, S
         const uint32_t nxt_seg = s_seg, nxt_ti = s_idx;
         const uint32_t tile = sa.sort_tile_start[seg] + ti;      // look-back slot
         // 1. digits + tile histogram
-        uint32_t dig[ITEMS];
+        uint32_t dig[ITEMS];          // digit in bits [8:0] (RADIX = invalid), rank in [31:9]
 #pragma unroll
         for (int i = 0; i < ITEMS; i++) {
             const uint32_t j = wbase + i * GKC_WAVE + lane;
@@ -336,7 +339,6 @@ __global__ __launch_bounds__(SORT_NT) void k_onesweep(Plan* __restrict__ plan, S
             }
         }
         // 4. stable rank within the wave
-        uint32_t rank[ITEMS];
 #pragma unroll
         for (int i = 0; i < ITEMS; i++) {
             const uint32_t dd = dig[i];
@@ -349,7 +351,7 @@ __global__ __launch_bounds__(SORT_NT) void k_onesweep(Plan* __restrict__ plan, S
             if (dd < RADIX) {
                 const uint32_t before = s_whist[wv][dd];
                 const uint64_t lower = peers & lanemask_lt();
-                rank[i] = before + (uint32_t)__popcll(lower);
+                dig[i] = dd | ((before + (uint32_t)__popcll(lower)) << 9);
                 if (lower == 0) s_whist[wv][dd] = before + (uint32_t)__popcll(peers);
             }
         }
@@ -391,8 +393,8 @@ __global__ __launch_bounds__(SORT_NT) void k_onesweep(Plan* __restrict__ plan, S
         // 6. reorder in LDS by digit, write each digit's run out
 #pragma unroll
         for (int i = 0; i < ITEMS; i++) {
-            const uint32_t dd = dig[i];
-            if (dd < RADIX) s_keys[s_dstart[dd] + s_whist[wv][dd] + rank[i]] = key[i];
+            const uint32_t dd = dig[i] & 511u;
+            if (dd < RADIX) s_keys[s_dstart[dd] + s_whist[wv][dd] + (dig[i] >> 9)] = key[i];
         }
         __syncthreads();
         for (uint32_t j = threadIdx.x; j < tn; j += SORT_NT) {
