@@ -166,7 +166,8 @@ def collect_pairs(cids, genome_indices, scale=1.0, device_flags=0, log=print):
             rs = W.generate(w, g=g)
             with capi.Counter(w.k, w.p, w.n_parts, hpc=w.hpc, filter_mode=w.filter_mode, part_mode=w.part_mode,
                               min_count=1) as c:
-                c.partition(rs.reads, rs.offs)
+                for a, b in W.chunk_bounds(rs, seed=0, lo_bytes=1 << 30, hi_bytes=1 << 30, fasta_overhead=False):
+                    c.partition(rs.reads[int(rs.offs[a]):int(rs.offs[b])], rs.offs[a:b + 1])
                 x = c.sketch()
                 c.count(device_flags)
                 po, pd, pk = c.table_info()
