@@ -35,6 +35,23 @@ def _log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def learned_report(predictor, ctx, part_distinct, part_kmers) -> dict:
+    """Predicted vs exact per-partition counts of the last batched run (the
+    bins still hold its superkmers and its sketch)."""
+    import numpy as np
+
+    from gkc_amd import learned
+    tt = part_kmers.astype(np.float64)
+    tot_p, dist_p = predictor.predict(ctx.sketch(), tt)
+    td = part_distinct.astype(np.float64)
+    caps = predictor.caps(ctx).astype(np.float64)
+    return {"mape_total_pct": learned.mape(tot_p, tt), "mape_distinct_pct": learned.mape(dist_p, td),
+            "under_predicted_partitions": int(np.sum(caps < td)), "partitions": int(len(td)),
+            "table_capacity_over_exact": float(caps.sum() / max(td.sum(), 1)),
+            "table_capacity_over_total_bound": float(caps.sum() / max(tt.sum(), 1)),
+            "margin": predictor.margin, "trained_on": predictor.info.get("pairs")}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,6 +192,7 @@ def main() -> int:
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_launch": pass_bytes, "avg_launch_ms": pass_ms},
         "full_size_invariants_ok": full_ok,
+        "learned": learned_report(predictor, ctx, pd, pk) if predictor is not None else None,
         "tuning": {"key": tkey, **ctx.get_tuning()},
     }
 

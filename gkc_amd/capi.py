@@ -37,7 +37,7 @@ class Params(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("sort_items", C.c_int32), ("sort_blocks_per_cu", C.c_int32), ("single_list", C.c_int32),
-                ("reserved1", C.c_int32)]
+                ("count_mode", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -45,7 +45,8 @@ class Stats(C.Structure):
                 ("ms_sort", C.c_double), ("ms_runlength", C.c_double), ("ms_sort_pass_avg", C.c_double),
                 ("n_kmers", C.c_uint64), ("n_superkmers", C.c_uint64), ("n_words", C.c_uint64),
                 ("n_distinct", C.c_uint64), ("n_entries", C.c_uint64), ("n_pass", C.c_uint32),
-                ("fallbacks", C.c_uint32), ("host_syncs", C.c_uint32), ("partition_fallbacks", C.c_uint32)]
+                ("fallbacks", C.c_uint32), ("host_syncs", C.c_uint32), ("partition_fallbacks", C.c_uint32),
+                ("heavy_fallbacks", C.c_uint32), ("binned", C.c_uint32)]
 
     def asdict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -198,14 +199,17 @@ class Counter:
         caps = np.ascontiguousarray(caps, np.uint32)
         _check(lib().gkc_set_table_caps(self.h, _p(caps, _u32p)))
 
-    def set_tuning(self, sort_items: int = 0, sort_blocks_per_cu: int = 0, single_list: int = 0) -> None:
-        t = Tuning(sort_items, sort_blocks_per_cu, single_list, 0)
+    def set_tuning(self, sort_items: int = 0, sort_blocks_per_cu: int = 0, single_list: int = 0,
+                   count_mode: int = 0) -> None:
+        """count_mode (batched form): 0 binned counting, 1 full LSD sort + run-length."""
+        t = Tuning(sort_items, sort_blocks_per_cu, single_list, count_mode)
         _check(lib().gkc_set_tuning(self.h, C.byref(t)))
 
     def get_tuning(self) -> dict:
         t = Tuning()
         _check(lib().gkc_get_tuning(self.h, C.byref(t)))
-        return {"sort_items": t.sort_items, "sort_blocks_per_cu": t.sort_blocks_per_cu, "single_list": t.single_list}
+        return {"sort_items": t.sort_items, "sort_blocks_per_cu": t.sort_blocks_per_cu, "single_list": t.single_list,
+                "count_mode": t.count_mode}
 
     def part_kmers(self) -> np.ndarray:
         out = np.zeros(self.n_parts, np.uint64)

@@ -314,7 +314,7 @@ static P1Params p1params(gkc_ctx* c, int scatter) {
 template <typename K>
 static void count_impl(gkc_ctx* c, uint32_t flags);
 template <typename K>
-static void count_batched(gkc_ctx* c, uint32_t flags);
+static void count_batched(gkc_ctx* c, uint32_t flags, bool force_lsd);
 
 extern "C" {
 
@@ -616,8 +616,8 @@ int gkc_partition(gkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n
 int gkc_count(gkc_ctx* c, uint32_t flags) {
     return guarded(c, [&] {
         if (flags & GKC_BATCHED) {
-            if (c->keyw == 8) count_batched<uint64_t>(c, flags);
-            else count_batched<u128k>(c, flags);
+            if (c->keyw == 8) count_batched<uint64_t>(c, flags, false);
+            else count_batched<u128k>(c, flags, false);
         } else {
             if (c->keyw == 8) count_impl<uint64_t>(c, flags);
             else count_impl<u128k>(c, flags);
@@ -710,6 +710,7 @@ int gkc_sketch(gkc_ctx* c, float* out) {
 int gkc_set_tuning(gkc_ctx* c, const gkc_tuning* t) {
     return guarded(c, [&] {
         if (!t) throw ArgError{"null tuning", GKC_EINVAL};
+        if (t->count_mode < 0 || t->count_mode > 1) throw ArgError{"count_mode must be 0 or 1", GKC_EINVAL};
         c->tune = *t;
         return 0;
     });
@@ -807,12 +808,12 @@ template <typename K> static int sort_grid(gkc_ctx* c) {
 // batched form), segments assigned longest-first to the list with the fewest
 // sort tiles so far, each list's segments in increasing id order.
 static void set_lists(gkc_ctx* c, uint32_t n_segs, const uint32_t* keys, const uint32_t* hdrs, uint32_t n_lists,
-                      int sort_tile_keys) {
+                      int sort_tile_keys, int rl_tile_keys = RL_TILE) {
     const uint32_t n = n_segs;
     std::vector<uint32_t> st(n), rt(n), it(n), owner(n);
     for (uint32_t s = 0; s < n; s++) {
         st[s] = (uint32_t)(((uint64_t)keys[s] + sort_tile_keys - 1) / sort_tile_keys);
-        rt[s] = (uint32_t)(((uint64_t)keys[s] + RL_TILE - 1) / RL_TILE);
+        rt[s] = (uint32_t)(((uint64_t)keys[s] + rl_tile_keys - 1) / rl_tile_keys);
         it[s] = (hdrs[s] + EX_HDRS - 1) / EX_HDRS;
     }
     std::vector<uint32_t> byload(n);
@@ -868,8 +869,8 @@ static K* sort_segments(gkc_ctx* c, SegArrays sa, const uint64_t* hdr_base, cons
         const uint32_t nk = (uint32_t)n_keys, nh = (uint32_t)n_hdrs;
         set_lists(c, 1, &nk, &nh, 1, TILE);
     }
-    hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(64), 0, c->st, plan, n_segs, (uint32_t)c->npass);
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, c->st, plan, sa, TILE);
+    hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(64), 0, c->st, plan, n_segs, (uint32_t)c->npass, 0u);
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, c->st, plan, sa, TILE, RL_TILE);
     HIPX(hipGetLastError());
     zero(c, sa.hist, (uint64_t)n_segs * MAX_PASS * RADIX);
     HIPX(hipEventRecord(c->ev[2], c->st));
@@ -1106,16 +1107,33 @@ extern "C" int gkc_count_packed(gkc_ctx* c, const uint32_t* words, uint64_t n_wo
 
 // ---------------------------------------------------------------- batched form
 
-// Enqueue extraction + all onesweep passes over every partition at once.
+// Shape of one batched counting launch.  Binned (the default when k >= 8):
+// two onesweep passes over key bits [2k-16, 2k) group each segment's keys by
+// bin and k_bincount sorts + counts tiles of whole bins in LDS.  Otherwise the
+// full LSD sort (ceil(2k/8) passes) and k_runlength.
+struct BatchShape {
+    bool binned;
+    int npass;
+    uint32_t sh0;
+    int rl_tile;
+};
+template <typename K>
+static BatchShape batch_shape(const gkc_ctx* c) {
+    const bool binned = c->tune.count_mode != 1 && c->prm.k >= 8;
+    if (binned) return BatchShape{true, 2, (uint32_t)(2 * c->prm.k - 16), bc_tile<K>()};
+    return BatchShape{false, c->npass, 0u, RL_TILE};
+}
+
+// Enqueue extraction + the onesweep passes over every partition at once.
 // Sizes are read by the kernels from the device plan, so the sequence can be
 // captured once and replayed.  With `timed`, per-pass events are recorded.
 template <typename K>
-static K* enqueue_sort_all(gkc_ctx* c, SegArrays sa, uint64_t tiles_cap, bool timed) {
+static K* enqueue_sort_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs, uint64_t tiles_cap, bool timed) {
     const uint32_t P = c->prm.n_parts;
     Plan* plan = c->d_plan.as<Plan>();
     const int TILE = SORT_NT * sort_items<K>(c);
-    hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(64), 0, c->st, plan, P, (uint32_t)c->npass);
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, c->st, plan, sa, TILE);
+    hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(64), 0, c->st, plan, P, (uint32_t)bs.npass, bs.sh0);
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, c->st, plan, sa, TILE, bs.rl_tile);
     zero(c, sa.hist, (uint64_t)P * MAX_PASS * RADIX);
     if (timed) HIPX(hipEventRecord(c->ev[2], c->st));
     const int g_ex = occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu);
@@ -1127,7 +1145,7 @@ static K* enqueue_sort_all(gkc_ctx* c, SegArrays sa, uint64_t tiles_cap, bool ti
     const int g_s = sort_grid<K>(c);
     K* src = c->keysA.as<K>();
     K* dst = c->keysB.as<K>();
-    for (int ps = 0; ps < c->npass; ps++) {
+    for (int ps = 0; ps < bs.npass; ps++) {
         zero(c, c->lookback.p, tiles_cap * RADIX);
         if (timed) HIPX(hipEventRecord(c->evp[2 * ps], c->st));
         launch_onesweep<K>(c, g_s, plan, sa, src, dst, ps);
@@ -1136,12 +1154,15 @@ static K* enqueue_sort_all(gkc_ctx* c, SegArrays sa, uint64_t tiles_cap, bool ti
     }
     if (timed) HIPX(hipEventRecord(c->ev[6], c->st));
     HIPX(hipGetLastError());
-    (void)TILE;
     return src;
 }
 
+// Count (emit = 0) or count and write the table (emit = 1) from the keys the
+// sort left in `sorted`.  Flags: d_toverflow[0] table capacity exceeded,
+// d_toverflow[1] a bin too large for a k_bincount tile.
 template <typename K>
-static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const K* sorted, uint64_t rl_tiles_cap, int emit) {
+static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs, const K* sorted,
+                                  uint64_t rl_tiles_cap, int emit) {
     const uint32_t P = c->prm.n_parts;
     Plan* plan = c->d_plan.as<Plan>();
     TableOut to;
@@ -1150,11 +1171,18 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const K* sorted, uin
     to.n_entries = c->d_nent.as<uint32_t>(); to.n_distinct = c->d_ndist.as<uint32_t>();
     to.overflow = c->d_toverflow.as<uint32_t>();
     to.min_cnt = c->prm.min_count; to.max_cnt = c->prm.max_count; to.emit = emit;
-    zero(c, c->d_nent.p, P); zero(c, c->d_ndist.p, P); zero(c, c->d_toverflow.p, 1);
+    zero(c, c->d_nent.p, P); zero(c, c->d_ndist.p, P); zero(c, c->d_toverflow.p, 2);
     zero(c, c->lookback.p, rl_tiles_cap);
     zero(c, &plan->ctr[ctr_rl()], NLIST);
-    const int g_r = occupancy_blocks((const void*)k_runlength<K>, SORT_NT, c->n_cu);
-    hipLaunchKernelGGL(k_runlength<K>, dim3(g_r), dim3(SORT_NT), 0, c->st, plan, sa, c->wl, sorted, to, c->lookback.as<uint32_t>());
+    if (bs.binned) {
+        const int g_b = occupancy_blocks((const void*)k_bincount<K>, BC_NT, c->n_cu);
+        hipLaunchKernelGGL(k_bincount<K>, dim3(g_b), dim3(BC_NT), 0, c->st, plan, sa, c->wl, sorted, to,
+                           c->lookback.as<uint32_t>(), c->d_toverflow.as<uint32_t>() + 1, c->prm.k);
+    } else {
+        const int g_r = occupancy_blocks((const void*)k_runlength<K>, SORT_NT, c->n_cu);
+        hipLaunchKernelGGL(k_runlength<K>, dim3(g_r), dim3(SORT_NT), 0, c->st, plan, sa, c->wl, sorted, to,
+                           c->lookback.as<uint32_t>());
+    }
     HIPX(hipGetLastError());
 }
 
@@ -1178,9 +1206,11 @@ hipGraphExec_t capture(gkc_ctx* c, F&& enqueue) {
 }
 
 template <typename K>
-static void count_batched(gkc_ctx* c, uint32_t flags) {
+static void count_batched(gkc_ctx* c, uint32_t flags, bool force_lsd) {
     const uint32_t P = c->prm.n_parts;
     const bool timed = (flags & GKC_SYNC_ONLY) == 0 && getenv("GKC_NO_GRAPH") != nullptr;
+    BatchShape bs = batch_shape<K>(c);
+    if (force_lsd) bs = BatchShape{false, c->npass, 0u, RL_TILE};
     c->part_off.assign(P + 1, 0);
     c->part_distinct.assign(P, 0);
     c->part_kmers.assign(P, 0);
@@ -1189,40 +1219,47 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
         c->part_kmers[q] = c->h_kmer_cur[q];
         tot_keys += c->h_kmer_cur[q]; tot_hdrs += c->h_hdr_cur[q]; tot_words += c->h_word_cur[q];
     }
+    const uint32_t heavy_before = force_lsd ? c->stats.heavy_fallbacks : 0;
+    const uint32_t syncs_before = force_lsd ? c->stats.host_syncs : 0;
     c->stats = gkc_stats{};
     c->stats.n_kmers = tot_keys; c->stats.n_superkmers = tot_hdrs; c->stats.n_words = tot_words;
-    c->stats.n_pass = c->npass;
+    c->stats.n_pass = bs.npass;
+    c->stats.binned = bs.binned ? 1 : 0;
+    c->stats.heavy_fallbacks = heavy_before;
+    c->stats.host_syncs = syncs_before;
     ensure_count_buffers<K>(c, tot_keys, P);
     const uint64_t tiles_cap = tot_keys / (SORT_NT * sort_items<K>(c)) + P + 1;
-    const uint64_t rl_tiles_cap = tot_keys / RL_TILE + P + 1;
+    const uint64_t rl_tiles_cap = tot_keys / bs.rl_tile + P + 1;
     SegArrays sa;
     sa.seg_keys = c->d_kmer_cur.as<uint32_t>(); sa.seg_hdrs = c->d_hdr_cur.as<uint32_t>();
     sa.key_start = c->d_keystart.as<uint64_t>(); sa.item_start = c->d_itemstart.as<uint32_t>();
     sa.sort_tile_start = c->d_tilestart.as<uint32_t>(); sa.rl_tile_start = c->d_rlstart.as<uint32_t>();
     sa.hist = c->d_hist.as<uint32_t>();
-    const K* sorted = (c->npass % 2 == 0) ? c->keysA.as<K>() : c->keysB.as<K>();
+    const K* sorted = (bs.npass % 2 == 0) ? c->keysA.as<K>() : c->keysB.as<K>();
     set_lists(c, P, c->h_kmer_cur.data(), c->h_hdr_cur.data(), c->tune.single_list == 1 ? 1 : NLIST,
-              SORT_NT * sort_items<K>(c));
+              SORT_NT * sort_items<K>(c), bs.rl_tile);
 
     // table capacities: predicted (presized) or a count pass (count-then-allocate)
     const bool presized = (flags & GKC_PRESIZED) && c->presize_caps.size() == P;
     (void)occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu);
     (void)sort_grid<K>(c);
     (void)occupancy_blocks((const void*)k_runlength<K>, SORT_NT, c->n_cu);
+    (void)occupancy_blocks((const void*)k_bincount<K>, BC_NT, c->n_cu);
     HIPX(hipEventRecord(c->ev[0], c->st));
     std::vector<const void*> key = {c->keysA.p, c->keysB.p, c->lookback.p, c->words.p, c->hdrs.p,
-                                    (const void*)(uintptr_t)tiles_cap, (const void*)(uintptr_t)c->npass,
+                                    (const void*)(uintptr_t)tiles_cap, (const void*)(uintptr_t)bs.npass,
                                     (const void*)(uintptr_t)sort_items<K>(c), (const void*)(uintptr_t)sort_grid<K>(c),
-                                    c->d_wl.p, (const void*)(uintptr_t)c->wl.n_lists};
+                                    c->d_wl.p, (const void*)(uintptr_t)c->wl.n_lists,
+                                    (const void*)(uintptr_t)(bs.binned ? 1 : 0)};
     if (timed) {
-        enqueue_sort_all<K>(c, sa, tiles_cap, true);
+        enqueue_sort_all<K>(c, sa, bs, tiles_cap, true);
     } else {
         if (!c->g_sort || c->g_key != key) {
             if (c->g_sort) { HIPX(hipGraphExecDestroy(c->g_sort)); c->g_sort = nullptr; }
             if (c->g_cnt) { HIPX(hipGraphExecDestroy(c->g_cnt)); c->g_cnt = nullptr; }
             if (c->g_emit) { HIPX(hipGraphExecDestroy(c->g_emit)); c->g_emit = nullptr; }
-            c->g_sort = capture(c, [&] { enqueue_sort_all<K>(c, sa, tiles_cap, false); });
-            c->g_cnt = capture(c, [&] { enqueue_runlength_all<K>(c, sa, sorted, rl_tiles_cap, 0); });
+            c->g_sort = capture(c, [&] { enqueue_sort_all<K>(c, sa, bs, tiles_cap, false); });
+            c->g_cnt = capture(c, [&] { enqueue_runlength_all<K>(c, sa, bs, sorted, rl_tiles_cap, 0); });
             c->g_key = key;
         }
         HIPX(hipGraphLaunch(c->g_sort, c->st));
@@ -1230,14 +1267,21 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     HIPX(hipEventRecord(c->ev[4], c->st));
     std::vector<uint32_t>& caps = c->h_caps;
     caps.assign(P, 0);
+    uint32_t flags_dev[2] = {0, 0};
     if (presized) {
         caps = c->presize_caps;
     } else {
-        if (timed) enqueue_runlength_all<K>(c, sa, sorted, rl_tiles_cap, 0);
+        if (timed) enqueue_runlength_all<K>(c, sa, bs, sorted, rl_tiles_cap, 0);
         else HIPX(hipGraphLaunch(c->g_cnt, c->st));
         HIPX(hipMemcpyAsync(caps.data(), c->d_nent.p, P * 4, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipMemcpyAsync(flags_dev, c->d_toverflow.p, 8, hipMemcpyDeviceToHost, c->st));
         HIPX(hipStreamSynchronize(c->st));
         c->stats.host_syncs++;
+        if (flags_dev[1]) {         // a bin exceeds the LDS tile: count with the full LSD sort
+            c->stats.heavy_fallbacks++;
+            count_batched<K>(c, flags, true);
+            return;
+        }
     }
     // table placement (physical starts) and capacity
     c->h_tstart.assign(P, 0);
@@ -1249,11 +1293,11 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     HIPX(hipMemcpyAsync(c->d_tstart.p, c->h_tstart.data(), P * 8, hipMemcpyHostToDevice, c->st));
     HIPX(hipMemcpyAsync(c->d_tcap.p, caps.data(), P * 4, hipMemcpyHostToDevice, c->st));
     if (timed) {
-        enqueue_runlength_all<K>(c, sa, sorted, rl_tiles_cap, 1);
+        enqueue_runlength_all<K>(c, sa, bs, sorted, rl_tiles_cap, 1);
     } else {
         if (!c->g_emit || old_tk != c->tkeys.p) {
             if (c->g_emit) { HIPX(hipGraphExecDestroy(c->g_emit)); c->g_emit = nullptr; }
-            c->g_emit = capture(c, [&] { enqueue_runlength_all<K>(c, sa, sorted, rl_tiles_cap, 1); });
+            c->g_emit = capture(c, [&] { enqueue_runlength_all<K>(c, sa, bs, sorted, rl_tiles_cap, 1); });
         }
         HIPX(hipGraphLaunch(c->g_emit, c->st));
     }
@@ -1261,13 +1305,17 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     std::vector<uint32_t>& nent = c->h_nent;
     nent.assign(P, 0);
     std::vector<uint32_t> ndist(P);
-    uint32_t ovf = 0;
     HIPX(hipMemcpyAsync(nent.data(), c->d_nent.p, P * 4, hipMemcpyDeviceToHost, c->st));
     HIPX(hipMemcpyAsync(ndist.data(), c->d_ndist.p, P * 4, hipMemcpyDeviceToHost, c->st));
-    HIPX(hipMemcpyAsync(&ovf, c->d_toverflow.p, 4, hipMemcpyDeviceToHost, c->st));
+    HIPX(hipMemcpyAsync(flags_dev, c->d_toverflow.p, 8, hipMemcpyDeviceToHost, c->st));
     HIPX(hipStreamSynchronize(c->st));
     c->stats.host_syncs++;
-    if (ovf) {
+    if (flags_dev[1]) {
+        c->stats.heavy_fallbacks++;
+        count_batched<K>(c, flags, true);
+        return;
+    }
+    if (flags_dev[0]) {
         // under-prediction caught on device: count-then-allocate emission with
         // the exact per-partition entry counts the emit pass just measured
         c->stats.fallbacks++;
@@ -1281,15 +1329,15 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
         HIPX(hipMemcpyAsync(c->d_tcap.p, caps.data(), P * 4, hipMemcpyHostToDevice, c->st));
         if (timed || before != c->tkeys.p) {
             if (!timed && c->g_emit) { HIPX(hipGraphExecDestroy(c->g_emit)); c->g_emit = nullptr; }
-            enqueue_runlength_all<K>(c, sa, sorted, rl_tiles_cap, 1);
+            enqueue_runlength_all<K>(c, sa, bs, sorted, rl_tiles_cap, 1);
         } else {
             HIPX(hipGraphLaunch(c->g_emit, c->st));
         }
         HIPX(hipMemcpyAsync(nent.data(), c->d_nent.p, P * 4, hipMemcpyDeviceToHost, c->st));
-        HIPX(hipMemcpyAsync(&ovf, c->d_toverflow.p, 4, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipMemcpyAsync(flags_dev, c->d_toverflow.p, 8, hipMemcpyDeviceToHost, c->st));
         HIPX(hipStreamSynchronize(c->st));
         c->stats.host_syncs++;
-        if (ovf) throw ArgError{"internal: table overflow after exact sizing", GKC_ESTATE};
+        if (flags_dev[0]) throw ArgError{"internal: table overflow after exact sizing", GKC_ESTATE};
     }
     HIPX(hipEventRecord(c->ev[1], c->st));
     HIPX(hipEventSynchronize(c->ev[1]));
@@ -1310,13 +1358,13 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
         float a = 0, b = 0, pass_sum = 0;
         HIPX(hipEventElapsedTime(&a, c->ev[2], c->ev[3]));
         HIPX(hipEventElapsedTime(&b, c->ev[3], c->ev[6]));
-        for (int ps = 0; ps < c->npass; ps++) {
+        for (int ps = 0; ps < bs.npass; ps++) {
             float m = 0;
             HIPX(hipEventElapsedTime(&m, c->evp[2 * ps], c->evp[2 * ps + 1]));
             pass_sum += m;
         }
         c->stats.ms_extract = a; c->stats.ms_sort = b;
-        c->stats.ms_sort_pass_avg = pass_sum / c->npass;
+        c->stats.ms_sort_pass_avg = pass_sum / bs.npass;
     } else {
         c->stats.ms_sort = s1;
     }

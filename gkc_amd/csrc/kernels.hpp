@@ -65,7 +65,7 @@ struct Plan {
     uint32_t tot_items;         // extraction work items (EX_HDRS headers each)
     uint32_t tot_sort_tiles;
     uint32_t tot_rl_tiles;
-    uint32_t pad;
+    uint32_t sh0;               // bit position of pass 0's digit (0: full LSD sort)
     uint64_t tot_keys;
     uint32_t ctr[NCTR];         // work counters, NLIST per stage: see ctr_extract / ctr_pass / ctr_rl
 };
@@ -116,17 +116,32 @@ template <> struct KeyTraits<uint64_t> {
 };
 template <> struct KeyTraits<u128k> {
     static constexpr int ITEMS = 8;
-    __device__ static inline uint32_t digit(const u128k& x, int sh) {
-        return sh < 64 ? (uint32_t)(x.lo >> sh) & (RADIX - 1) : (uint32_t)(x.hi >> (sh - 64)) & (RADIX - 1);
+    __device__ static inline uint32_t digit(const u128k& x, int sh) {   // bits [sh, sh+8), may straddle bit 64
+        if (sh >= 64) return (uint32_t)(x.hi >> (sh - 64)) & (RADIX - 1);
+        uint64_t v = x.lo >> sh;
+        if (sh > 64 - RADIX_BITS) v |= x.hi << (64 - sh);
+        return (uint32_t)v & (RADIX - 1);
     }
 };
 template <typename K> constexpr int sort_tile() { return SORT_NT * KeyTraits<K>::ITEMS; }
 
-__global__ void k_plan(Plan* plan, SegArrays sa, int sort_tile_keys);
+// Binned counting (batched form): after two onesweep passes over key bits
+// [2k-16, 2k) the keys of a segment are grouped by their top 16 bits ("bin").
+// A k_bincount tile owns the bins that start in its BC_TILE-key window, sorts
+// them in LDS on the bits that still vary and run-length counts them.  A tile
+// holds up to bc_cap keys, so bins of up to bc_cap - bc_tile + 1 keys fit.
+constexpr int BC_NT = 1024;
+template <typename K> struct BinTraits;
+template <> struct BinTraits<uint64_t> { static constexpr int ITEMS = 16; };
+template <> struct BinTraits<u128k> { static constexpr int ITEMS = 8; };
+template <typename K> constexpr int bc_cap() { return BC_NT * BinTraits<K>::ITEMS; }
+template <typename K> constexpr int bc_tile() { return bc_cap<K>() / 2; }
+
+__global__ void k_plan(Plan* plan, SegArrays sa, int sort_tile_keys, int rl_tile_keys);
 __global__ void k_zero32(uint32_t* p, uint64_t n);
 __global__ void k_set_u64(uint64_t* p, uint64_t v);
 __global__ void k_set_u32(uint32_t* p, uint32_t v);
-__global__ void k_plan_init(Plan* plan, uint32_t n_segs, uint32_t n_pass);
+__global__ void k_plan_init(Plan* plan, uint32_t n_segs, uint32_t n_pass, uint32_t sh0);
 __global__ void k_set_chunk(ChunkDesc* dst, ChunkDesc v);
 template <typename K>
 __global__ void k_extract(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const SkmHdr* __restrict__ hdrs,
@@ -139,5 +154,8 @@ __global__ void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, 
 template <typename K>
 __global__ void k_runlength(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ keys, TableOut to,
                             uint32_t* __restrict__ lookback);
+template <typename K>
+__global__ void k_bincount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ keys, TableOut to,
+                           uint32_t* __restrict__ lookback, uint32_t* __restrict__ heavy, int k);
 
 }  // namespace gkc

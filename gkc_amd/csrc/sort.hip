@@ -55,7 +55,7 @@ __device__ __forceinline__ bool claim(const WorkLists& wl, uint32_t* ctr, uint32
 }
 
 // Build the segment layout from per-segment key and header counts (one block).
-__global__ __launch_bounds__(1024) void k_plan(Plan* plan, SegArrays sa, int sort_tile_keys) {
+__global__ __launch_bounds__(1024) void k_plan(Plan* plan, SegArrays sa, int sort_tile_keys, int rl_tile_keys) {
     __shared__ uint64_t s_k[1024];
     __shared__ uint32_t s_i[1024], s_t[1024], s_r[1024];
     const uint32_t S = plan->n_segs;
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(1024) void k_plan(Plan* plan, SegArrays sa, int sor
             nk = sa.seg_keys[s];
             ni = (sa.seg_hdrs[s] + EX_HDRS - 1) / EX_HDRS;
             nt = (uint32_t)((nk + sort_tile_keys - 1) / sort_tile_keys);
-            nr = (uint32_t)((nk + RL_TILE - 1) / RL_TILE);
+            nr = (uint32_t)((nk + rl_tile_keys - 1) / rl_tile_keys);
         }
         s_k[threadIdx.x] = nk; s_i[threadIdx.x] = ni; s_t[threadIdx.x] = nt; s_r[threadIdx.x] = nr;
         __syncthreads();
@@ -101,10 +101,10 @@ __global__ __launch_bounds__(1024) void k_plan(Plan* plan, SegArrays sa, int sor
 // launch, so no host buffer has to outlive an asynchronous copy).
 __global__ void k_set_u64(uint64_t* p, uint64_t v) { *p = v; }
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
-__global__ void k_plan_init(Plan* plan, uint32_t n_segs, uint32_t n_pass) {
+__global__ void k_plan_init(Plan* plan, uint32_t n_segs, uint32_t n_pass, uint32_t sh0) {
     const int t = threadIdx.x;
     if (t == 0) { plan->n_segs = n_segs; plan->n_pass = n_pass; plan->tot_items = 0; plan->tot_sort_tiles = 0;
-                  plan->tot_rl_tiles = 0; plan->pad = 0; plan->tot_keys = 0; }
+                  plan->tot_rl_tiles = 0; plan->sh0 = sh0; plan->tot_keys = 0; }
     for (int i = t; i < NCTR; i += blockDim.x) plan->ctr[i] = 0;
 }
 __global__ void k_set_chunk(ChunkDesc* dst, ChunkDesc v) { *dst = v; }
@@ -116,7 +116,8 @@ __global__ void k_zero32(uint32_t* p, uint64_t n) {
 }
 
 // Extract the canonical k-mers of each superkmer into the segment's key
-// range and accumulate the digit histograms of all passes.
+// range and accumulate the digit histograms of all passes (pass p's digit is
+// key bits [sh0 + 8p, sh0 + 8p + 8)).
 // Bases are 2-bit, 16 per u32 word, MSB first.
 template <typename K>
 __device__ __forceinline__ K kmer_at(const uint32_t* wds, uint32_t o, int k);
@@ -167,6 +168,7 @@ __global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, Se
     __shared__ uint32_t s_koff[NWAVE][GKC_WAVE], s_woff[NWAVE][GKC_WAVE];
     __shared__ uint32_t s_seg, s_idx, s_ok;
     const int npass = plan->n_pass;
+    const int sh0 = plan->sh0;
     const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE;
     uint32_t list = blockIdx.x % wl.n_lists;
     for (;;) {
@@ -210,7 +212,7 @@ __global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, Se
             const K c = canon_of(kmer_at<K>(wseg + s_woff[wv][lo], o, k), k);
             kseg[s_koff[wv][lo] + o] = c;
             for (int ps = 0; ps < npass; ps++)
-                atomicAdd(&s_hist[ps * RADIX + KeyTraits<K>::digit(c, ps * RADIX_BITS)], 1u);
+                atomicAdd(&s_hist[ps * RADIX + KeyTraits<K>::digit(c, sh0 + ps * RADIX_BITS)], 1u);
         }
         __syncthreads();
         uint32_t* gh = sa.hist + (size_t)seg * MAX_PASS * RADIX;
@@ -272,7 +274,7 @@ __global__ __launch_bounds__(SORT_NT, ONESWEEP_WAVES_PER_EU) void k_onesweep(Pla
     __shared__ uint32_t s_wsum[NWAVE];
     __shared__ uint64_t s_gbase[RADIX];
     __shared__ uint32_t s_seg, s_idx, s_ok;
-    const int sh = pass * RADIX_BITS;
+    const int sh = (int)plan->sh0 + pass * RADIX_BITS;
     const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE;
     const uint32_t d = threadIdx.x;     // SORT_NT == RADIX: one digit per thread
     uint32_t* ctr = &plan->ctr[ctr_pass(pass)];
@@ -550,6 +552,317 @@ __global__ __launch_bounds__(SORT_NT) void k_runlength(Plan* __restrict__ plan, 
         __syncthreads();
     }
 }
+
+// ---------------------------------------------------------------- binned count
+// Keys of a segment grouped by bin (key bits [2k-16, 2k), i.e. sorted by
+// them) are counted tile by tile in LDS.  Tile t of a segment owns the bins
+// whose first key lies in [t*TILE, (t+1)*TILE): its range [a, b) starts at the
+// first bin start at or after t*TILE and ends at the first bin start at or
+// after (t+1)*TILE.  The range is sorted in LDS by an LSD radix sort on the
+// bits that vary inside it (ranking by wave64 ballots as in k_onesweep, no
+// look-back: the whole range is in the workgroup), run-length counted and
+// thresholded, and written at its place in the segment's table via a
+// decoupled look-back over the tiles (one value per tile).  A range larger
+// than the LDS capacity (a bin of more than CAP - TILE + 1 keys) sets *heavy
+// and contributes nothing; the caller then counts with the full LSD sort.
+template <typename K>
+__device__ __forceinline__ uint32_t bin_of(const K& x, int bsh) {
+    return KeyTraits<K>::digit(x, bsh) | (KeyTraits<K>::digit(x, bsh + RADIX_BITS) << RADIX_BITS);
+}
+__device__ __forceinline__ void or_diff(uint64_t& lo, uint64_t&, uint64_t a, uint64_t b) { lo |= a ^ b; }
+__device__ __forceinline__ void or_diff(uint64_t& lo, uint64_t& hi, const u128k& a, const u128k& b) {
+    lo |= a.lo ^ b.lo; hi |= a.hi ^ b.hi;
+}
+
+template <typename K>
+__global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl,
+                                                    const K* __restrict__ keys, TableOut to,
+                                                    uint32_t* __restrict__ lookback, uint32_t* __restrict__ heavy, int k) {
+    constexpr int ITEMS = BinTraits<K>::ITEMS;
+    constexpr uint32_t CAP = bc_cap<K>();
+    constexpr uint32_t TILE = bc_tile<K>();
+    constexpr int NWAVE = BC_NT / GKC_WAVE;
+    constexpr int NCH = CAP / GKC_WAVE;          // 64-key chunks of a range, in key order
+    constexpr int NCW = NCH / GKC_WAVE;          // waves that scan the chunk values
+    constexpr uint32_t INF = 0xFFFFFFFFu;
+    static_assert(NCH % GKC_WAVE == 0 && NCW <= NWAVE && RADIX == 4 * GKC_WAVE, "tile shape");
+    __shared__ K s_keys[CAP];
+    __shared__ uint32_t s_whist[NWAVE][RADIX];
+    __shared__ uint32_t s_dstart[RADIX];
+    __shared__ uint64_t s_hb[NCH];               // run-head ballot of each chunk
+    __shared__ uint32_t s_ch[NCH];               // first head of each chunk, then the next head after it
+    __shared__ uint32_t s_ck[NCH];               // kept entries of each chunk, then their exclusive prefix
+    __shared__ uint32_t s_red[NWAVE];
+    __shared__ uint32_t s_orv[NWAVE][4];
+    __shared__ uint32_t s_a, s_b, s_seg, s_idx, s_ok, s_nd, s_tot;
+    __shared__ uint32_t s_excl;
+    const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE, tid = threadIdx.x;
+    const uint32_t wbase = wv * GKC_WAVE * ITEMS;     // wave wv holds range keys [wbase, wbase + 64*ITEMS)
+    const int bsh = 2 * k - 2 * RADIX_BITS;
+    uint32_t list = blockIdx.x % wl.n_lists;
+    for (;;) {
+        if (tid == 0) {
+            uint32_t sg = 0, ix = 0;
+            s_ok = claim<1>(wl, &plan->ctr[ctr_rl()], list, sg, ix);
+            s_seg = sg; s_idx = ix; s_nd = 0;
+        }
+        __syncthreads();
+        if (!s_ok) break;
+        const uint32_t seg = s_seg, ti = s_idx;
+        // per-tile copy of the lane's base index, opaque to the compiler: the
+        // per-item addresses are recomputed in the loop instead of being
+        // hoisted out of it (which would exceed the 128 VGPRs of 16 waves/CU)
+        uint32_t jb = wbase + lane;
+        asm volatile("" : "+v"(jb));
+        const uint32_t tile = sa.rl_tile_start[seg] + ti;        // look-back slot
+        const K* kp = keys + sa.key_start[seg];
+        const uint32_t nseg = sa.seg_keys[seg];
+        const uint32_t lo = ti * TILE, hi = min(lo + TILE, nseg);
+        // 1. range: wave 0 finds the first bin start in [lo, hi) (hi: none),
+        //    wave 1 the first bin start at or after hi
+        if (wv == 0) {
+            uint32_t a = lo;
+            if (lo > 0) {
+                const uint32_t pb = bin_of(kp[lo - 1], bsh);
+                for (;; a += GKC_WAVE) {
+                    const uint32_t j = a + lane;
+                    const bool stop = j >= hi || bin_of(kp[j], bsh) != pb;
+                    const uint64_t m = __ballot(stop);
+                    if (m) { a += (uint32_t)__builtin_ctzll(m); break; }
+                }
+            }
+            if (lane == 0) s_a = a;
+        } else if (wv == 1) {
+            uint32_t b = hi;
+            if (hi < nseg) {
+                const uint32_t pb = bin_of(kp[hi - 1], bsh);
+                for (;;) {
+                    const uint32_t j = b + lane;
+                    const bool stop = j >= nseg || bin_of(kp[j], bsh) != pb;
+                    const uint64_t m = __ballot(stop);
+                    if (m) { b += (uint32_t)__builtin_ctzll(m); break; }
+                    b += GKC_WAVE;
+                    if (b - hi > CAP) break;                     // too large either way
+                }
+            }
+            if (lane == 0) s_b = b;
+        }
+        __syncthreads();
+        const uint32_t a = s_a;
+        uint32_t n = a < hi ? s_b - a : 0;
+        if (n > CAP) {
+            if (tid == 0) atomicOr(heavy, 1u);
+            n = 0;
+        }
+        // 2. load the range (wave-blocked, lane-striped) and find the bits that vary
+        K key[ITEMS];
+        uint64_t olo = 0, ohi = 0;
+        K ref{};
+        if (n) ref = kp[a];
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            const uint32_t j = jb + i * GKC_WAVE;
+            if (j < n) { key[i] = kp[a + j]; or_diff(olo, ohi, key[i], ref); }
+        }
+        uint32_t o0 = (uint32_t)olo, o1 = (uint32_t)(olo >> 32), o2 = (uint32_t)ohi, o3 = (uint32_t)(ohi >> 32);
+#pragma unroll
+        for (int off = 1; off < GKC_WAVE; off <<= 1) {
+            o0 |= __shfl_xor(o0, off); o1 |= __shfl_xor(o1, off);
+            o2 |= __shfl_xor(o2, off); o3 |= __shfl_xor(o3, off);
+        }
+        if (lane == 0) { s_orv[wv][0] = o0; s_orv[wv][1] = o1; s_orv[wv][2] = o2; s_orv[wv][3] = o3; }
+        __syncthreads();
+        o0 = o1 = o2 = o3 = 0;
+#pragma unroll
+        for (int w = 0; w < NWAVE; w++) { o0 |= s_orv[w][0]; o1 |= s_orv[w][1]; o2 |= s_orv[w][2]; o3 |= s_orv[w][3]; }
+        const uint64_t vlo = ((uint64_t)o1 << 32) | o0, vhi = ((uint64_t)o3 << 32) | o2;
+        const int nbits = vhi ? 128 - __builtin_clzll(vhi) : (vlo ? 64 - __builtin_clzll(vlo) : 0);
+        const int npass = (nbits + RADIX_BITS - 1) / RADIX_BITS;
+        // 3. LSD radix sort of the range in LDS on bits [0, nbits)
+        for (int ps = 0; ps < npass; ps++) {
+            const int sh = ps * RADIX_BITS;
+            for (int i = tid; i < NWAVE * RADIX; i += BC_NT) (&s_whist[0][0])[i] = 0;
+            __syncthreads();
+            uint32_t dig[ITEMS];     // digit in bits [8:0] (RADIX = none), rank in the wave in [31:9]
+#pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                const uint32_t j = jb + i * GKC_WAVE;
+                const uint32_t dd = (j < n) ? KeyTraits<K>::digit(key[i], sh) : RADIX;
+                uint64_t peers = __ballot(dd < RADIX);
+#pragma unroll
+                for (int bb = 0; bb < RADIX_BITS; bb++) {
+                    const uint64_t bal = __ballot((dd >> bb) & 1u);
+                    peers &= ((dd >> bb) & 1u) ? bal : ~bal;
+                }
+                dig[i] = dd;
+                if (dd < RADIX) {
+                    const uint32_t before = s_whist[wv][dd];
+                    const uint64_t lower = peers & lanemask_lt();
+                    dig[i] = dd | ((before + (uint32_t)__popcll(lower)) << 9);
+                    if (lower == 0) s_whist[wv][dd] = before + (uint32_t)__popcll(peers);
+                }
+            }
+            __syncthreads();
+            // digit starts (digit-major, then wave): threads 0..255 own one digit
+            uint32_t tot = 0, inc = 0;
+            if (tid < RADIX) {
+#pragma unroll
+                for (int w = 0; w < NWAVE; w++) { const uint32_t c = s_whist[w][tid]; s_whist[w][tid] = tot; tot += c; }
+                inc = tot;
+#pragma unroll
+                for (int off = 1; off < GKC_WAVE; off <<= 1) {
+                    const uint32_t y = __shfl_up(inc, off);
+                    if ((int)lane >= off) inc += y;
+                }
+                if (lane == GKC_WAVE - 1) s_red[wv] = inc;
+            }
+            __syncthreads();
+            if (tid < RADIX) {
+                uint32_t off = 0;
+                for (uint32_t w = 0; w < wv; w++) off += s_red[w];
+                s_dstart[tid] = off + inc - tot;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                const uint32_t dd = dig[i] & 511u;
+                if (dd < RADIX) s_keys[s_dstart[dd] + s_whist[wv][dd] + (dig[i] >> 9)] = key[i];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                const uint32_t j = jb + i * GKC_WAVE;
+                if (j < n) key[i] = s_keys[j];
+            }
+        }
+        if (npass == 0) {           // 0 or 1 key, or all keys equal: already in order
+#pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                const uint32_t j = jb + i * GKC_WAVE;
+                if (j < n) s_keys[j] = key[i];
+            }
+            __syncthreads();
+        }
+        // 4. run heads; first head of each 64-key chunk
+        uint32_t headmask = 0;
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            const uint32_t j = jb + i * GKC_WAVE;
+            const bool head = j < n && (j == 0 || !(key[i] == s_keys[j - 1]));
+            const uint64_t hb = __ballot(head);
+            headmask |= head ? 1u << i : 0u;
+            if (lane == 0) {
+                s_hb[wv * ITEMS + i] = hb;
+                s_ch[wv * ITEMS + i] = hb ? wbase + i * GKC_WAVE + (uint32_t)__builtin_ctzll(hb) : INF;
+                if (hb) atomicAdd(&s_nd, (uint32_t)__popcll(hb));
+            }
+        }
+        __syncthreads();
+        // next head after each chunk: exclusive suffix minimum over the chunks
+        uint32_t sfx = INF;
+        if (tid < (uint32_t)NCH) {
+            sfx = s_ch[tid];
+#pragma unroll
+            for (int off = 1; off < GKC_WAVE; off <<= 1) {
+                const uint32_t y = __shfl_down(sfx, off);
+                if (lane + off < GKC_WAVE) sfx = min(sfx, y);
+            }
+            if (lane == 0) s_red[wv] = sfx;
+        }
+        __syncthreads();
+        if (tid < (uint32_t)NCH) {
+            uint32_t e = __shfl_down(sfx, 1);
+            if (lane == GKC_WAVE - 1) e = INF;
+            for (int w = wv + 1; w < NCW; w++) e = min(e, s_red[w]);
+            s_ch[tid] = e;
+        }
+        __syncthreads();
+        // 5. counts, threshold, kept entries per chunk
+        uint32_t cnt[ITEMS];
+        uint32_t keepmask = 0;
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            const uint32_t c0 = wbase + i * GKC_WAVE;
+            cnt[i] = 0;
+            bool keep = false;
+            if ((headmask >> i) & 1u) {
+                const uint64_t hb = s_hb[wv * ITEMS + i];
+                const uint64_t above = lane == GKC_WAVE - 1 ? 0ull : hb & (~0ull << (lane + 1));
+                uint32_t nh = above ? c0 + (uint32_t)__builtin_ctzll(above) : s_ch[wv * ITEMS + i];
+                if (nh == INF) nh = n;
+                cnt[i] = nh - (c0 + lane);
+                keep = cnt[i] >= to.min_cnt && (to.max_cnt == 0 || cnt[i] <= to.max_cnt);
+            }
+            keepmask |= keep ? 1u << i : 0u;
+            const uint64_t kb = __ballot(keep);
+            if (lane == 0) s_ck[wv * ITEMS + i] = (uint32_t)__popcll(kb);
+        }
+        __syncthreads();
+        // exclusive prefix of the kept entries over the chunks
+        if (tid < (uint32_t)NCH) {
+            const uint32_t v = s_ck[tid];
+            uint32_t inc = v;
+#pragma unroll
+            for (int off = 1; off < GKC_WAVE; off <<= 1) {
+                const uint32_t y = __shfl_up(inc, off);
+                if ((int)lane >= off) inc += y;
+            }
+            s_ck[tid] = inc - v;
+            if (lane == GKC_WAVE - 1) s_red[wv] = inc;
+        }
+        __syncthreads();
+        if (tid < (uint32_t)NCH) {
+            uint32_t off = 0;
+            for (uint32_t w = 0; w < wv; w++) off += s_red[w];
+            s_ck[tid] += off;
+        }
+        if (tid == 0) {
+            uint32_t tot = 0;
+            for (int w = 0; w < NCW; w++) tot += s_red[w];
+            // 6. decoupled look-back for the tile's place in the segment's table
+            uint32_t* lb = lookback + tile;
+            uint32_t excl = 0;
+            if (ti == 0) st_agent(lb, LB_INC | tot);
+            else {
+                st_agent(lb, LB_AGG | tot);
+                int64_t j = (int64_t)tile - 1;
+                for (;;) {
+                    uint32_t v = ld_agent(&lookback[j]);
+                    while ((v & ~LB_VAL) == 0) { __builtin_amdgcn_s_sleep(1); v = ld_agent(&lookback[j]); }
+                    excl += v & LB_VAL;
+                    if (v & LB_INC) break;
+                    j--;
+                }
+                st_agent(lb, LB_INC | (excl + tot));
+            }
+            s_excl = excl; s_tot = tot;
+            if (s_nd) atomicAdd(&to.n_distinct[seg], s_nd);
+            if (tot) atomicAdd(&to.n_entries[seg], tot);
+        }
+        __syncthreads();
+        // 7. write the kept (k-mer, count) entries
+        if (to.emit && s_tot) {
+            const uint32_t cap = to.table_cap[seg];
+            K* tk = reinterpret_cast<K*>(to.keys) + to.table_start[seg];
+            uint32_t* tc = to.counts + to.table_start[seg];
+            const uint32_t base = s_excl;
+#pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                const bool keep = (keepmask >> i) & 1u;
+                const uint64_t kb = __ballot(keep);
+                if (keep) {
+                    const uint32_t idx = base + s_ck[wv * ITEMS + i] + (uint32_t)__popcll(kb & lanemask_lt());
+                    if (idx < cap) { tk[idx] = key[i]; tc[idx] = cnt[i]; }
+                    else atomicOr(to.overflow, 1u);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template __global__ void k_bincount<uint64_t>(Plan*, SegArrays, WorkLists, const uint64_t*, TableOut, uint32_t*, uint32_t*, int);
+template __global__ void k_bincount<u128k>(Plan*, SegArrays, WorkLists, const u128k*, TableOut, uint32_t*, uint32_t*, int);
 
 template __global__ void k_extract<uint64_t>(Plan*, SegArrays, WorkLists, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, uint64_t*, int);
 template __global__ void k_extract<u128k>(Plan*, SegArrays, WorkLists, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, u128k*, int);

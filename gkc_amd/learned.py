@@ -3,7 +3,9 @@
 A small MLP (256 -> 128 -> 128 -> 2, ReLU) reads the per-partition 4-mer
 sketch the partition kernel takes (log1p of the 4-mer counts at every 16th
 window start, by the window's partition) and predicts the partition's log
-total and log distinct k-mer counts.  The predicted distinct count sizes the
+total k-mer count and its log distinct/total ratio.  The exact total is known
+once the superkmers are scattered (the bins hold it), so the distinct count
+is the exact total times the predicted ratio; that count sizes the
 (k-mer, count) table of the batched counter before counting, so the
 run-length pass writes the table directly (no count pass, one host sync);
 an under-prediction is caught on the device and falls back to
@@ -135,18 +137,23 @@ class Predictor:
     def normalise(self, sketch: np.ndarray) -> np.ndarray:
         return ((sketch - self.x_mean) / self.x_std).astype(np.float32)
 
-    def predict(self, sketch: np.ndarray, bf16: bool = True) -> tuple[np.ndarray, np.ndarray]:
-        """-> (predicted total, predicted distinct) k-mers per partition."""
+    def predict(self, sketch: np.ndarray, total: np.ndarray | None = None,
+                bf16: bool = True) -> tuple[np.ndarray, np.ndarray]:
+        """-> (predicted total, predicted distinct) k-mers per partition.  The
+        distinct count is the predicted ratio times `total` (the exact totals)
+        when given, else times the predicted total."""
         x = self.normalise(sketch)
         y = self.mlp.forward_bf16(x) if bf16 else self.mlp.forward(x)
         logs = y * self.t_std + self.t_mean
-        return np.expm1(logs[:, 0]).clip(0), np.expm1(logs[:, 1]).clip(0)
+        tot = np.expm1(logs[:, 0]).clip(0)
+        base = tot if total is None else np.asarray(total, np.float64)
+        return tot, base * np.exp(np.minimum(logs[:, 1], 0.0))
 
     def caps(self, counter) -> np.ndarray:
-        """Table capacities for counter's current bins: the predicted distinct
-        count times the margin, never above the exact k-mer total."""
-        _, dist = self.predict(counter.sketch())
+        """Table capacities for counter's current bins: the exact k-mer total
+        times the predicted ratio times the margin, never above the total."""
         total = counter.part_kmers().astype(np.float64)
+        _, dist = self.predict(counter.sketch(), total)
         caps = np.minimum(np.ceil(dist * self.margin) + 16, total)
         return caps.astype(np.uint32)
 
@@ -157,7 +164,8 @@ def mape(pred: np.ndarray, true: np.ndarray) -> float:
 
 
 def collect_pairs(cids, genome_indices, scale=1.0, device_flags=0, log=print):
-    """(sketch, exact totals) pairs from the plain counter on seeded genomes."""
+    """(sketch, [log1p total, log(distinct / total)]) pairs per non-empty
+    partition, from the plain counter on seeded genomes."""
     from . import workloads as W
     xs, ts = [], []
     for cid in cids:
@@ -173,6 +181,7 @@ def collect_pairs(cids, genome_indices, scale=1.0, device_flags=0, log=print):
                 po, pd, pk = c.table_info()
             keep = pk > 0
             xs.append(x[keep])
-            ts.append(np.stack([np.log1p(pk[keep].astype(np.float64)), np.log1p(pd[keep].astype(np.float64))], 1))
+            tk, td = pk[keep].astype(np.float64), pd[keep].astype(np.float64)
+            ts.append(np.stack([np.log1p(tk), np.log(td / tk)], 1))
             log(f"pairs C{cid} g={g}: {int(keep.sum())} partitions, {int(pk.sum())} k-mers")
-    return np.concatenate(xs).astype(np.float32), np.concatenate(ts).astype(np.float32)
+    return np.concatenate(xs).astype(np.float32), np.concatenate(ts)

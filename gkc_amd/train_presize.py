@@ -2,13 +2,14 @@
 
     python -m gkc_amd.train_presize [--seeds 20] [--scale 1.0] [--epochs 300] [--out PATH]
 
-Training pairs: (sketch, exact log total / log distinct) per partition, from
+Training pairs: (sketch, exact log total / log distinct-over-total) per partition, from
 the plain counter on the short-read (C1), chunked short-read (C2) and
 long-read HPC (C3) workloads over genome seeds 1..S.  Held-out evaluation:
 seeds 101..105 of the same workloads and of the skewed-GC workload (C5).
-The margin is the 99th percentile of true/predicted distinct on the
-training pairs, so that about 1 % of the partitions take the
-count-then-allocate fallback.
+The margin is the 99.5th percentile of true/predicted distinct on the
+training pairs times a safety factor (1.05) for inputs unlike the training
+workloads; a partition still under-predicted takes the count-then-allocate
+fallback.
 """
 from __future__ import annotations
 
@@ -32,6 +33,9 @@ def main() -> int:
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", default=learned.DEFAULT_MODEL)
     ap.add_argument("--report", default=None)
+    ap.add_argument("--margin-quantile", type=float, default=0.995)
+    ap.add_argument("--margin-safety", type=float, default=1.05,
+                    help="extra factor on the margin for inputs unlike the training workloads")
     args = ap.parse_args()
     log = lambda *a: print("[train]", *a, file=sys.stderr, flush=True)  # noqa: E731
     t0 = time.time()
@@ -46,16 +50,17 @@ def main() -> int:
     losses = mlp.train(Xn, Tn, epochs=args.epochs, batch=256, lr=args.lr, seed=99)
     log(f"trained {args.epochs} epochs in {time.time() - t1:.1f}s, loss {losses[0]:.4f} -> {losses[-1]:.5f}")
     pred = learned.Predictor(mlp.get_params(), x_mean, x_std, t_mean, t_std, margin=1.0)
-    # margin from the training distribution of true / predicted distinct
+    # margin from the training distribution of true / predicted distinct ratio
     y = mlp.forward_bf16(Xn) * t_std + t_mean
-    ratio = np.expm1(T[:, 1]) / np.maximum(np.expm1(y[:, 1]), 1.0)
-    pred.margin = float(max(1.0, np.quantile(ratio, 0.99)))
+    ratio = np.exp(T[:, 1] - np.minimum(y[:, 1], 0.0))
+    pred.margin = float(max(1.0, np.quantile(ratio, args.margin_quantile)) * args.margin_safety)
     report = {"pairs": int(len(X)), "epochs": args.epochs, "loss_first": float(losses[0]),
               "loss_last": float(losses[-1]), "margin": pred.margin, "eval": {}}
     for cid in [1, 2, 3, 5]:
         Xe, Te = learned.collect_pairs([cid], range(101, 106), scale=args.eval_scale, log=log)
-        tot, dist = pred.predict(Xe)
-        tt, td = np.expm1(Te[:, 0]), np.expm1(Te[:, 1])
+        tt = np.rint(np.expm1(Te[:, 0]))
+        td = np.rint(tt * np.exp(Te[:, 1]))
+        tot, dist = pred.predict(Xe, tt)
         caps = np.minimum(np.ceil(dist * pred.margin) + 16, tt)
         report["eval"][f"C{cid}"] = {
             "partitions": int(len(Xe)),

@@ -68,6 +68,8 @@ typedef struct gkc_stats {
     uint32_t fallbacks;        /* presize overflows that fell back to count-then-allocate */
     uint32_t host_syncs;       /* host synchronisations inside the last gkc_count */
     uint32_t partition_fallbacks;  /* presized batched scatters redone by count-then-allocate since gkc_reset */
+    uint32_t heavy_fallbacks;  /* binned counts redone with the full LSD sort (a bin larger than an LDS tile) */
+    uint32_t binned;           /* 1 when the last gkc_count used the binned kernels */
 } gkc_stats;
 
 /* Kernel variants (picked by gkc_amd/autotune.py, keyed on k, read-length
@@ -76,7 +78,8 @@ typedef struct gkc_tuning {
     int32_t sort_items;          /* keys per thread of a onesweep tile: 8,12,16,20,24 (u64) / 4,8,12 (u128) */
     int32_t sort_blocks_per_cu;  /* persistent workgroups per CU in the sort passes (0: occupancy limit) */
     int32_t single_list;         /* 1: batched tiles in one global order instead of one work list per XCD */
-    int32_t reserved1;
+    int32_t count_mode;          /* batched form: 0 binned (2 onesweep passes on the top 16 key bits, then
+                                    LDS sort + count per tile of bins; default for k >= 8), 1 full LSD sort */
 } gkc_tuning;
 
 const char *gkc_last_error(void);

@@ -1,0 +1,130 @@
+"""Binned counting (batched form): two onesweep passes on the top 16 key bits,
+then an LDS sort + run-length count per tile of whole bins.  Bit-exact
+against the oracle and against the full LSD sort (count_mode 1), including
+the edges of the tiling: bins straddling tile windows, ranges of one key or
+of one repeated key, bins too large for a tile (fallback to the LSD sort),
+digits straddling bit 64 of 128-bit keys, and thresholds."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from gkc_amd import capi
+from gkc_amd import workloads as W
+from oracle import oracle as O
+from tests.helpers import assert_tables_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[gpu-case]", *a, flush=True)
+
+
+def _count(reads, offs, k, p, P, mode, hpc=False, min_count=1, max_count=0, batches=2, caps=None):
+    n = len(offs) - 1
+    with capi.Counter(k, p, P, hpc=hpc, min_count=min_count, max_count=max_count) as c:
+        c.set_tuning(count_mode=mode)
+        bounds = np.linspace(0, n, batches + 1).astype(int)
+        for a, b in zip(bounds[:-1], bounds[1:]):
+            if b > a:
+                c.partition(reads[int(offs[a]):int(offs[b])], offs[a:b + 1] - offs[a], flags=capi.BATCHED)
+        flags = capi.BATCHED
+        if caps is not None:
+            c.set_table_caps(caps)
+            flags |= capi.PRESIZED
+        c.count(flags)
+        return c.table(), c.stats()
+
+
+def _reads(seqs):
+    reads = np.frombuffer(b"".join(seqs), np.uint8).copy()
+    offs = np.concatenate([[0], np.cumsum([len(s) for s in seqs])]).astype(np.uint64)
+    return reads, offs
+
+
+@pytest.mark.parametrize("cid,scale,k,p,P", [
+    (1, 0.05, 31, 11, 2),       # ~300 k keys per segment: many tiles, bins straddling windows
+    (2, 0.004, 31, 11, 64),
+    (1, 0.02, 8, 4, 3),         # bins = the whole key (2k - 16 = 0)
+    (1, 0.02, 9, 5, 5),
+    (1, 0.02, 32, 16, 4),       # all 64 bits used
+    (1, 0.02, 33, 17, 4),       # 128-bit keys, bin digits straddle bit 64
+    (1, 0.02, 40, 13, 8),
+    (4, 0.002, 63, 19, 16),
+])
+def test_binned_matches_oracle_and_lsd(gpu, cid, scale, k, p, P):
+    w = W.workload(cid, scale=scale)
+    rs = W.generate(w)
+    ot = O.count(rs.reads, rs.offs, k, p, P)
+    _say("binned", w.name, k, p, P)
+    gt, st = _count(rs.reads, rs.offs, k, p, P, mode=0)
+    assert st["binned"] == 1 and st["heavy_fallbacks"] == 0 and st["n_pass"] == 2
+    assert_tables_equal(gt, ot, P, k)
+    _say("lsd", w.name, k, p, P)
+    lt, sl = _count(rs.reads, rs.offs, k, p, P, mode=1)
+    assert sl["binned"] == 0 and sl["n_pass"] == (2 * k + 7) // 8
+    assert_tables_equal(lt, ot, P, k)
+
+
+@pytest.mark.parametrize("minc,maxc", [(2, 0), (1, 3), (3, 40)])
+def test_binned_thresholds(gpu, minc, maxc):
+    w = W.workload(1, scale=0.03)
+    rs = W.generate(w)
+    ot = O.count(rs.reads, rs.offs, 31, 11, 4, min_count=minc, max_count=maxc)
+    _say("thresholds", minc, maxc)
+    gt, st = _count(rs.reads, rs.offs, 31, 11, 4, mode=0, min_count=minc, max_count=maxc)
+    assert st["binned"] == 1
+    assert_tables_equal(gt, ot, 4, 31)
+
+
+def test_binned_hpc_long_reads(gpu):
+    w = W.workload(3, scale=0.002)
+    rs = W.generate(w)
+    ot = O.count(rs.reads, rs.offs, w.k, w.p, 16, hpc=True)
+    _say("binned hpc")
+    gt, st = _count(rs.reads, rs.offs, w.k, w.p, 16, mode=0, hpc=True)
+    assert st["binned"] == 1
+    assert_tables_equal(gt, ot, 16, w.k)
+
+
+@pytest.mark.parametrize("k,p,copies", [(31, 11, 300), (40, 13, 200)])
+def test_heavy_bin_falls_back_to_lsd(gpu, k, p, copies):
+    """A k-mer occurring more often than a tile holds (poly-A: k-mer 0) makes
+    its bin too large for LDS; the count is redone with the full LSD sort."""
+    rng = np.random.default_rng(5)
+    seqs = [b"A" * 150] * copies
+    seqs += [bytes(rng.choice(list(b"ACGT"), 150).tolist()) for _ in range(400)]
+    reads, offs = _reads(seqs)
+    ot = O.count(reads, offs, k, p, 4)
+    _say("heavy", k, copies)
+    gt, st = _count(reads, offs, k, p, 4, mode=0)
+    assert st["heavy_fallbacks"] == 1 and st["binned"] == 0
+    assert_tables_equal(gt, ot, 4, k)
+
+
+def test_binned_small_ranges(gpu):
+    """Ranges of a single key, of one repeated key (no radix pass), empty
+    partitions, reads of exactly k bases."""
+    seqs = [b"ACGTTGCAAGGCTTAACCGGTTAAGGCCTTA", b"A" * 60, b"C" * 45, b"GATTACA" * 12,
+            b"", b"ACGTTGCAAGGCTTAACCGGTTAAGGCCTTA", b"AC" * 40]
+    reads, offs = _reads(seqs)
+    ot = O.count(reads, offs, 31, 11, 9)
+    _say("small ranges")
+    gt, st = _count(reads, offs, 31, 11, 9, mode=0, batches=1)
+    assert st["binned"] == 1
+    assert_tables_equal(gt, ot, 9, 31)
+    bf = O.brute_force_table(reads, offs, 31)
+    assert dict(zip(gt[0].tolist(), gt[2].tolist())) == bf
+
+
+def test_binned_presized_fallback(gpu):
+    w = W.workload(1, scale=0.03)
+    rs = W.generate(w)
+    ot = O.count(rs.reads, rs.offs, 31, 11, 8)
+    exact = np.diff(ot.part_off).astype(np.uint32)
+    for name, caps, fb in [("exact", exact, 0), ("under", np.maximum(exact // 3, 1), 1)]:
+        _say("binned presized", name)
+        gt, st = _count(rs.reads, rs.offs, 31, 11, 8, mode=0, caps=caps)
+        assert st["binned"] == 1 and st["fallbacks"] == fb and st["host_syncs"] == 1 + fb
+        assert_tables_equal(gt, ot, 8, 31)
