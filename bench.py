@@ -134,7 +134,9 @@ def main() -> int:
         ms = el / args.steps * 1e3
         results[name] = {"kmers_per_s": N / (ms / 1e3), "bases_per_s": rs.bases / (ms / 1e3), "ms_per_step": ms,
                          "ms_count": st["ms_count"], "host_syncs": st["host_syncs"], "fallbacks": st["fallbacks"],
-                         "n_distinct": st["n_distinct"], "n_entries": st["n_entries"]}
+                         "partition_fallbacks": st["partition_fallbacks"], "binned": st["binned"],
+                         "heavy_ranges": st["heavy_ranges"], "n_distinct": st["n_distinct"],
+                         "n_entries": st["n_entries"]}
         _log(f"{name}: {ms:.2f} ms/step, {N / ms / 1e6:.2f} G k-mers/s, count {st['ms_count']:.2f} ms, "
              f"syncs {st['host_syncs']}, fallbacks {st['fallbacks']}")
         return st
@@ -160,6 +162,11 @@ def main() -> int:
     run_step(ctx, capi.BATCHED, capi.BATCHED)
     st = ctx.stats()
     os.environ.pop("GKC_NO_GRAPH")
+    stages = {"ms_partition_last_chunk": st["ms_partition"], "ms_extract": st["ms_extract"],
+              "ms_sort": st["ms_sort"], "ms_sort_pass_avg": st["ms_sort_pass_avg"], "n_pass": st["n_pass"],
+              "ms_count_emit": st["ms_runlength"], "ms_count_total": st["ms_count"], "binned": st["binned"],
+              "heavy_ranges": st["heavy_ranges"]}
+    _log(f"instrumented batched count: {stages}")
     W_key = 8 if w.k <= 32 else 16
     pass_bytes = 2 * W_key * N                   # key read + key write per pass
     pass_ms = st["ms_sort_pass_avg"]
@@ -191,6 +198,7 @@ def main() -> int:
         "roofline": {"kernel": "k_onesweep<u64>" if W_key == 8 else "k_onesweep<u128k>", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_launch": pass_bytes, "avg_launch_ms": pass_ms},
+        "stages_instrumented": stages,
         "full_size_invariants_ok": full_ok,
         "learned": learned_report(predictor, ctx, pd, pk) if predictor is not None else None,
         "tuning": {"key": tkey, **ctx.get_tuning()},

@@ -46,7 +46,7 @@ class Stats(C.Structure):
                 ("n_kmers", C.c_uint64), ("n_superkmers", C.c_uint64), ("n_words", C.c_uint64),
                 ("n_distinct", C.c_uint64), ("n_entries", C.c_uint64), ("n_pass", C.c_uint32),
                 ("fallbacks", C.c_uint32), ("host_syncs", C.c_uint32), ("partition_fallbacks", C.c_uint32),
-                ("heavy_fallbacks", C.c_uint32), ("binned", C.c_uint32)]
+                ("heavy_ranges", C.c_uint32), ("binned", C.c_uint32)]
 
     def asdict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -104,12 +104,15 @@ struct gkc_ctx {
     // counting
     DevBuf keysA, keysB, lookback, d_plan, d_segk, d_segh, d_keystart, d_itemstart, d_tilestart, d_rlstart, d_hist;
     DevBuf d_tstart, d_tcap, d_nent, d_ndist, d_toverflow;
+    DevBuf d_ranges;                           // k_bincount tile ranges {a, n}
     DevBuf tkeys, tcnts;
     std::vector<uint64_t> part_off, part_distinct, part_kmers, part_start;   // part_start: physical table offsets
     std::vector<uint32_t> h_caps, h_nent;
     std::vector<uint64_t> h_tstart;
     // captured launch sequence of the batched form (re-captured when buffers move)
     hipGraphExec_t g_sort = nullptr, g_emit = nullptr, g_cnt = nullptr;
+    const void* g_emit_tk = nullptr;           // table buffers g_emit was captured with
+    const void* g_emit_tc = nullptr;
     std::vector<const void*> g_key;
     std::vector<uint32_t> presize_caps;
     bool counted = false;
@@ -314,7 +317,7 @@ static P1Params p1params(gkc_ctx* c, int scatter) {
 template <typename K>
 static void count_impl(gkc_ctx* c, uint32_t flags);
 template <typename K>
-static void count_batched(gkc_ctx* c, uint32_t flags, bool force_lsd);
+static void count_batched(gkc_ctx* c, uint32_t flags);
 
 extern "C" {
 
@@ -616,8 +619,8 @@ int gkc_partition(gkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n
 int gkc_count(gkc_ctx* c, uint32_t flags) {
     return guarded(c, [&] {
         if (flags & GKC_BATCHED) {
-            if (c->keyw == 8) count_batched<uint64_t>(c, flags, false);
-            else count_batched<u128k>(c, flags, false);
+            if (c->keyw == 8) count_batched<uint64_t>(c, flags);
+            else count_batched<u128k>(c, flags);
         } else {
             if (c->keyw == 8) count_impl<uint64_t>(c, flags);
             else count_impl<u128k>(c, flags);
@@ -1158,10 +1161,11 @@ static K* enqueue_sort_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs, uint6
 }
 
 // Count (emit = 0) or count and write the table (emit = 1) from the keys the
-// sort left in `sorted`.  Flags: d_toverflow[0] table capacity exceeded,
-// d_toverflow[1] a bin too large for a k_bincount tile.
+// sort left in `sorted`.  d_toverflow[0]: table capacity exceeded;
+// d_toverflow[1]: k_bincount ranges larger than LDS (sorted through global
+// memory).
 template <typename K>
-static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs, const K* sorted,
+static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs, K* sorted,
                                   uint64_t rl_tiles_cap, int emit) {
     const uint32_t P = c->prm.n_parts;
     Plan* plan = c->d_plan.as<Plan>();
@@ -1175,9 +1179,13 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs
     zero(c, c->lookback.p, rl_tiles_cap);
     zero(c, &plan->ctr[ctr_rl()], NLIST);
     if (bs.binned) {
+        K* alt = sorted == c->keysA.as<K>() ? c->keysB.as<K>() : c->keysA.as<K>();
+        uint2* ranges = c->d_ranges.as<uint2>();
+        const int g_r = occupancy_blocks((const void*)k_binranges<K>, 256, c->n_cu);
+        hipLaunchKernelGGL(k_binranges<K>, dim3(g_r), dim3(256), 0, c->st, plan, sa, (const K*)sorted, ranges, c->prm.k);
         const int g_b = occupancy_blocks((const void*)k_bincount<K>, BC_NT, c->n_cu);
-        hipLaunchKernelGGL(k_bincount<K>, dim3(g_b), dim3(BC_NT), 0, c->st, plan, sa, c->wl, sorted, to,
-                           c->lookback.as<uint32_t>(), c->d_toverflow.as<uint32_t>() + 1, c->prm.k);
+        hipLaunchKernelGGL(k_bincount<K>, dim3(g_b), dim3(BC_NT), 0, c->st, plan, sa, c->wl, sorted, alt,
+                           (const uint2*)ranges, to, c->lookback.as<uint32_t>(), c->d_toverflow.as<uint32_t>() + 1);
     } else {
         const int g_r = occupancy_blocks((const void*)k_runlength<K>, SORT_NT, c->n_cu);
         hipLaunchKernelGGL(k_runlength<K>, dim3(g_r), dim3(SORT_NT), 0, c->st, plan, sa, c->wl, sorted, to,
@@ -1206,11 +1214,10 @@ hipGraphExec_t capture(gkc_ctx* c, F&& enqueue) {
 }
 
 template <typename K>
-static void count_batched(gkc_ctx* c, uint32_t flags, bool force_lsd) {
+static void count_batched(gkc_ctx* c, uint32_t flags) {
     const uint32_t P = c->prm.n_parts;
     const bool timed = (flags & GKC_SYNC_ONLY) == 0 && getenv("GKC_NO_GRAPH") != nullptr;
-    BatchShape bs = batch_shape<K>(c);
-    if (force_lsd) bs = BatchShape{false, c->npass, 0u, RL_TILE};
+    const BatchShape bs = batch_shape<K>(c);
     c->part_off.assign(P + 1, 0);
     c->part_distinct.assign(P, 0);
     c->part_kmers.assign(P, 0);
@@ -1219,14 +1226,10 @@ static void count_batched(gkc_ctx* c, uint32_t flags, bool force_lsd) {
         c->part_kmers[q] = c->h_kmer_cur[q];
         tot_keys += c->h_kmer_cur[q]; tot_hdrs += c->h_hdr_cur[q]; tot_words += c->h_word_cur[q];
     }
-    const uint32_t heavy_before = force_lsd ? c->stats.heavy_fallbacks : 0;
-    const uint32_t syncs_before = force_lsd ? c->stats.host_syncs : 0;
     c->stats = gkc_stats{};
     c->stats.n_kmers = tot_keys; c->stats.n_superkmers = tot_hdrs; c->stats.n_words = tot_words;
     c->stats.n_pass = bs.npass;
     c->stats.binned = bs.binned ? 1 : 0;
-    c->stats.heavy_fallbacks = heavy_before;
-    c->stats.host_syncs = syncs_before;
     ensure_count_buffers<K>(c, tot_keys, P);
     const uint64_t tiles_cap = tot_keys / (SORT_NT * sort_items<K>(c)) + P + 1;
     const uint64_t rl_tiles_cap = tot_keys / bs.rl_tile + P + 1;
@@ -1235,7 +1238,8 @@ static void count_batched(gkc_ctx* c, uint32_t flags, bool force_lsd) {
     sa.key_start = c->d_keystart.as<uint64_t>(); sa.item_start = c->d_itemstart.as<uint32_t>();
     sa.sort_tile_start = c->d_tilestart.as<uint32_t>(); sa.rl_tile_start = c->d_rlstart.as<uint32_t>();
     sa.hist = c->d_hist.as<uint32_t>();
-    const K* sorted = (bs.npass % 2 == 0) ? c->keysA.as<K>() : c->keysB.as<K>();
+    K* sorted = (bs.npass % 2 == 0) ? c->keysA.as<K>() : c->keysB.as<K>();
+    c->d_ranges.ensure(rl_tiles_cap * sizeof(uint2));
     set_lists(c, P, c->h_kmer_cur.data(), c->h_hdr_cur.data(), c->tune.single_list == 1 ? 1 : NLIST,
               SORT_NT * sort_items<K>(c), bs.rl_tile);
 
@@ -1245,8 +1249,9 @@ static void count_batched(gkc_ctx* c, uint32_t flags, bool force_lsd) {
     (void)sort_grid<K>(c);
     (void)occupancy_blocks((const void*)k_runlength<K>, SORT_NT, c->n_cu);
     (void)occupancy_blocks((const void*)k_bincount<K>, BC_NT, c->n_cu);
+    (void)occupancy_blocks((const void*)k_binranges<K>, 256, c->n_cu);
     HIPX(hipEventRecord(c->ev[0], c->st));
-    std::vector<const void*> key = {c->keysA.p, c->keysB.p, c->lookback.p, c->words.p, c->hdrs.p,
+    std::vector<const void*> key = {c->keysA.p, c->keysB.p, c->lookback.p, c->words.p, c->hdrs.p, c->d_ranges.p,
                                     (const void*)(uintptr_t)tiles_cap, (const void*)(uintptr_t)bs.npass,
                                     (const void*)(uintptr_t)sort_items<K>(c), (const void*)(uintptr_t)sort_grid<K>(c),
                                     c->d_wl.p, (const void*)(uintptr_t)c->wl.n_lists,
@@ -1277,30 +1282,27 @@ static void count_batched(gkc_ctx* c, uint32_t flags, bool force_lsd) {
         HIPX(hipMemcpyAsync(flags_dev, c->d_toverflow.p, 8, hipMemcpyDeviceToHost, c->st));
         HIPX(hipStreamSynchronize(c->st));
         c->stats.host_syncs++;
-        if (flags_dev[1]) {         // a bin exceeds the LDS tile: count with the full LSD sort
-            c->stats.heavy_fallbacks++;
-            count_batched<K>(c, flags, true);
-            return;
-        }
     }
     // table placement (physical starts) and capacity
     c->h_tstart.assign(P, 0);
     uint64_t tot_cap = 0;
     for (uint32_t q = 0; q < P; q++) { c->h_tstart[q] = tot_cap; tot_cap += caps[q]; }
-    const void* old_tk = c->tkeys.p;
     c->tkeys.ensure((tot_cap + 1) * sizeof(K));
     c->tcnts.ensure((tot_cap + 1) * 4);
     HIPX(hipMemcpyAsync(c->d_tstart.p, c->h_tstart.data(), P * 8, hipMemcpyHostToDevice, c->st));
     HIPX(hipMemcpyAsync(c->d_tcap.p, caps.data(), P * 4, hipMemcpyHostToDevice, c->st));
-    if (timed) {
-        enqueue_runlength_all<K>(c, sa, bs, sorted, rl_tiles_cap, 1);
-    } else {
-        if (!c->g_emit || old_tk != c->tkeys.p) {
+    // emit pass: replay the captured graph only if it writes the current table
+    // buffers (either may have moved when the table grew)
+    auto emit_pass = [&]() {
+        if (timed) { enqueue_runlength_all<K>(c, sa, bs, sorted, rl_tiles_cap, 1); return; }
+        if (!c->g_emit || c->g_emit_tk != c->tkeys.p || c->g_emit_tc != c->tcnts.p) {
             if (c->g_emit) { HIPX(hipGraphExecDestroy(c->g_emit)); c->g_emit = nullptr; }
             c->g_emit = capture(c, [&] { enqueue_runlength_all<K>(c, sa, bs, sorted, rl_tiles_cap, 1); });
+            c->g_emit_tk = c->tkeys.p; c->g_emit_tc = c->tcnts.p;
         }
         HIPX(hipGraphLaunch(c->g_emit, c->st));
-    }
+    };
+    emit_pass();
     HIPX(hipEventRecord(c->ev[5], c->st));
     std::vector<uint32_t>& nent = c->h_nent;
     nent.assign(P, 0);
@@ -1310,11 +1312,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags, bool force_lsd) {
     HIPX(hipMemcpyAsync(flags_dev, c->d_toverflow.p, 8, hipMemcpyDeviceToHost, c->st));
     HIPX(hipStreamSynchronize(c->st));
     c->stats.host_syncs++;
-    if (flags_dev[1]) {
-        c->stats.heavy_fallbacks++;
-        count_batched<K>(c, flags, true);
-        return;
-    }
+    c->stats.heavy_ranges = flags_dev[1];
     if (flags_dev[0]) {
         // under-prediction caught on device: count-then-allocate emission with
         // the exact per-partition entry counts the emit pass just measured
@@ -1322,17 +1320,11 @@ static void count_batched(gkc_ctx* c, uint32_t flags, bool force_lsd) {
         caps = nent;
         tot_cap = 0;
         for (uint32_t q = 0; q < P; q++) { c->h_tstart[q] = tot_cap; tot_cap += caps[q]; }
-        const void* before = c->tkeys.p;
         c->tkeys.ensure((tot_cap + 1) * sizeof(K));
         c->tcnts.ensure((tot_cap + 1) * 4);
         HIPX(hipMemcpyAsync(c->d_tstart.p, c->h_tstart.data(), P * 8, hipMemcpyHostToDevice, c->st));
         HIPX(hipMemcpyAsync(c->d_tcap.p, caps.data(), P * 4, hipMemcpyHostToDevice, c->st));
-        if (timed || before != c->tkeys.p) {
-            if (!timed && c->g_emit) { HIPX(hipGraphExecDestroy(c->g_emit)); c->g_emit = nullptr; }
-            enqueue_runlength_all<K>(c, sa, bs, sorted, rl_tiles_cap, 1);
-        } else {
-            HIPX(hipGraphLaunch(c->g_emit, c->st));
-        }
+        emit_pass();
         HIPX(hipMemcpyAsync(nent.data(), c->d_nent.p, P * 4, hipMemcpyDeviceToHost, c->st));
         HIPX(hipMemcpyAsync(flags_dev, c->d_toverflow.p, 8, hipMemcpyDeviceToHost, c->st));
         HIPX(hipStreamSynchronize(c->st));

@@ -127,13 +127,14 @@ template <typename K> constexpr int sort_tile() { return SORT_NT * KeyTraits<K>:
 
 // Binned counting (batched form): after two onesweep passes over key bits
 // [2k-16, 2k) the keys of a segment are grouped by their top 16 bits ("bin").
-// A k_bincount tile owns the bins that start in its BC_TILE-key window, sorts
-// them in LDS on the bits that still vary and run-length counts them.  A tile
-// holds up to bc_cap keys, so bins of up to bc_cap - bc_tile + 1 keys fit.
+// A k_bincount tile owns the bins that start in its bc_tile-key window, sorts
+// them on the bits that still vary and run-length counts them: in LDS when
+// they hold at most bc_cap keys, else in bc_cap-key chunks through global
+// memory.
 constexpr int BC_NT = 1024;
 template <typename K> struct BinTraits;
-template <> struct BinTraits<uint64_t> { static constexpr int ITEMS = 16; };
-template <> struct BinTraits<u128k> { static constexpr int ITEMS = 8; };
+template <> struct BinTraits<uint64_t> { static constexpr int ITEMS = 8; };   // 128 VGPRs, no spills
+template <> struct BinTraits<u128k> { static constexpr int ITEMS = 4; };
 template <typename K> constexpr int bc_cap() { return BC_NT * BinTraits<K>::ITEMS; }
 template <typename K> constexpr int bc_tile() { return bc_cap<K>() / 2; }
 
@@ -155,7 +156,11 @@ template <typename K>
 __global__ void k_runlength(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ keys, TableOut to,
                             uint32_t* __restrict__ lookback);
 template <typename K>
-__global__ void k_bincount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ keys, TableOut to,
-                           uint32_t* __restrict__ lookback, uint32_t* __restrict__ heavy, int k);
+__global__ void k_binranges(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys, uint2* __restrict__ ranges,
+                            int k);
+template <typename K>
+__global__ void k_bincount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, K* __restrict__ keys, K* __restrict__ alt,
+                           const uint2* __restrict__ ranges, TableOut to, uint32_t* __restrict__ lookback,
+                           uint32_t* __restrict__ n_heavy);
 
 }  // namespace gkc

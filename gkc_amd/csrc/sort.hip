@@ -555,16 +555,18 @@ __global__ __launch_bounds__(SORT_NT) void k_runlength(Plan* __restrict__ plan, 
 
 // ---------------------------------------------------------------- binned count
 // Keys of a segment grouped by bin (key bits [2k-16, 2k), i.e. sorted by
-// them) are counted tile by tile in LDS.  Tile t of a segment owns the bins
-// whose first key lies in [t*TILE, (t+1)*TILE): its range [a, b) starts at the
+// them) are counted tile by tile.  Tile t of a segment owns the bins whose
+// first key lies in [t*TILE, (t+1)*TILE): its range [a, a+n) starts at the
 // first bin start at or after t*TILE and ends at the first bin start at or
-// after (t+1)*TILE.  The range is sorted in LDS by an LSD radix sort on the
-// bits that vary inside it (ranking by wave64 ballots as in k_onesweep, no
-// look-back: the whole range is in the workgroup), run-length counted and
-// thresholded, and written at its place in the segment's table via a
-// decoupled look-back over the tiles (one value per tile).  A range larger
-// than the LDS capacity (a bin of more than CAP - TILE + 1 keys) sets *heavy
-// and contributes nothing; the caller then counts with the full LSD sort.
+// after (t+1)*TILE (k_binranges; computed before any tile reorders its keys).
+// k_bincount sorts a range on the bits that vary inside it (LSD radix,
+// ranking by wave64 ballots as in k_onesweep), run-length counts and
+// thresholds it, and writes it at its place in the segment's table via a
+// decoupled look-back over the tiles (one value per tile).  A range of up to
+// CAP keys is sorted in LDS; a larger one (a bin of more than CAP - TILE + 1
+// keys: skewed composition or a high-copy k-mer) is sorted by the same
+// workgroup in CAP-key chunks through global memory, ping-ponging with the
+// spare key buffer, and counted chunk by chunk.
 template <typename K>
 __device__ __forceinline__ uint32_t bin_of(const K& x, int bsh) {
     return KeyTraits<K>::digit(x, bsh) | (KeyTraits<K>::digit(x, bsh + RADIX_BITS) << RADIX_BITS);
@@ -574,19 +576,113 @@ __device__ __forceinline__ void or_diff(uint64_t& lo, uint64_t& hi, const u128k&
     lo |= a.lo ^ b.lo; hi |= a.hi ^ b.hi;
 }
 
+// One wave per tile: ranges[t] = {a, n} (n = 0 when no bin starts in the window).
+template <typename K>
+__global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys,
+                                                   uint2* __restrict__ ranges, int k) {
+    constexpr uint32_t TILE = bc_tile<K>();
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / GKC_WAVE;
+    const uint32_t nw = gridDim.x * blockDim.x / GKC_WAVE;
+    const uint32_t total = plan->tot_rl_tiles, S = plan->n_segs;
+    const int bsh = 2 * k - 2 * RADIX_BITS;
+    for (uint32_t t = wave; t < total; t += nw) {
+        const uint32_t seg = find_seg(sa.rl_tile_start, S, t);
+        const uint32_t ti = t - sa.rl_tile_start[seg];
+        const K* kp = keys + sa.key_start[seg];
+        const uint32_t nseg = sa.seg_keys[seg];
+        const uint32_t lo = ti * TILE, hi = min(lo + TILE, nseg);
+        uint32_t a = lo;
+        if (lo > 0) {
+            const uint32_t pb = bin_of(kp[lo - 1], bsh);
+            for (;; a += GKC_WAVE) {
+                const uint32_t j = a + lane;
+                const bool stop = j >= hi || bin_of(kp[j], bsh) != pb;
+                const uint64_t m = __ballot(stop);
+                if (m) { a += (uint32_t)__builtin_ctzll(m); break; }
+            }
+        }
+        uint32_t n = 0;
+        if (a < hi) {
+            uint32_t b = hi;
+            if (hi < nseg) {
+                const uint32_t pb = bin_of(kp[hi - 1], bsh);
+                for (;; b += GKC_WAVE) {
+                    const uint32_t j = b + lane;
+                    const bool stop = j >= nseg || bin_of(kp[j], bsh) != pb;
+                    const uint64_t m = __ballot(stop);
+                    if (m) { b += (uint32_t)__builtin_ctzll(m); break; }
+                }
+            }
+            n = b - a;
+        }
+        if (lane == 0) ranges[t] = make_uint2(a, n);
+    }
+}
+
+// Stable rank of the wave's valid keys (index jb + 64 i < nv) by digit `sh`,
+// continuing the wave's running digit counters s_whist[wv][*].  wb is the
+// wave's first index (jb - lane): items the whole wave has past nv are skipped.
+template <typename K, int ITEMS>
+__device__ __forceinline__ void rank_in_wave(const K (&key)[ITEMS], uint32_t jb, uint32_t wb, uint32_t nv, int sh,
+                                             uint32_t* __restrict__ whist, uint32_t (&dig)[ITEMS]) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; i++) {
+        dig[i] = RADIX;
+        if (wb + i * GKC_WAVE >= nv) continue;
+        const uint32_t j = jb + i * GKC_WAVE;
+        const uint32_t dd = (j < nv) ? KeyTraits<K>::digit(key[i], sh) : RADIX;
+        uint64_t peers = __ballot(dd < RADIX);
+#pragma unroll
+        for (int bb = 0; bb < RADIX_BITS; bb++) {
+            const uint64_t bal = __ballot((dd >> bb) & 1u);
+            peers &= ((dd >> bb) & 1u) ? bal : ~bal;
+        }
+        dig[i] = dd;
+        if (dd < RADIX) {
+            const uint32_t before = whist[dd];
+            const uint64_t lower = peers & lanemask_lt();
+            dig[i] = dd | ((before + (uint32_t)__popcll(lower)) << 9);
+            if (lower == 0) whist[dd] = before + (uint32_t)__popcll(peers);
+        }
+    }
+}
+
+// Exclusive scan over the 256 digits of `v` (threads 0..255, four full
+// waves); s_red holds the four wave sums.  Returns the exclusive prefix;
+// *total gets the sum (valid in every thread after the call).
+__device__ __forceinline__ uint32_t digit_scan(uint32_t v, uint32_t tid, uint32_t lane, uint32_t wv,
+                                               uint32_t* s_red, uint32_t* total) {
+    uint32_t inc = v;
+    if (tid < RADIX) {
+#pragma unroll
+        for (int off = 1; off < GKC_WAVE; off <<= 1) {
+            const uint32_t y = __shfl_up(inc, off);
+            if ((int)lane >= off) inc += y;
+        }
+        if (lane == GKC_WAVE - 1) s_red[wv] = inc;
+    }
+    __syncthreads();
+    uint32_t off = 0;
+    if (tid < RADIX) for (uint32_t w = 0; w < wv; w++) off += s_red[w];
+    *total = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+    return off + inc - v;
+}
+
 template <typename K>
 __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl,
-                                                    const K* __restrict__ keys, TableOut to,
-                                                    uint32_t* __restrict__ lookback, uint32_t* __restrict__ heavy, int k) {
+                                                    K* __restrict__ keys, K* __restrict__ alt,
+                                                    const uint2* __restrict__ ranges, TableOut to,
+                                                    uint32_t* __restrict__ lookback, uint32_t* __restrict__ n_heavy) {
     constexpr int ITEMS = BinTraits<K>::ITEMS;
     constexpr uint32_t CAP = bc_cap<K>();
-    constexpr uint32_t TILE = bc_tile<K>();
     constexpr int NWAVE = BC_NT / GKC_WAVE;
     constexpr int NCH = CAP / GKC_WAVE;          // 64-key chunks of a range, in key order
     constexpr int NCW = NCH / GKC_WAVE;          // waves that scan the chunk values
     constexpr uint32_t INF = 0xFFFFFFFFu;
     static_assert(NCH % GKC_WAVE == 0 && NCW <= NWAVE && RADIX == 4 * GKC_WAVE, "tile shape");
     __shared__ K s_keys[CAP];
+    __shared__ uint32_t s_cnt[sizeof(K) == 8 ? CAP : 1];   // hash-table counts, then entry counts
     __shared__ uint32_t s_whist[NWAVE][RADIX];
     __shared__ uint32_t s_dstart[RADIX];
     __shared__ uint64_t s_hb[NCH];               // run-head ballot of each chunk
@@ -594,17 +690,15 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
     __shared__ uint32_t s_ck[NCH];               // kept entries of each chunk, then their exclusive prefix
     __shared__ uint32_t s_red[NWAVE];
     __shared__ uint32_t s_orv[NWAVE][4];
-    __shared__ uint32_t s_a, s_b, s_seg, s_idx, s_ok, s_nd, s_tot;
-    __shared__ uint32_t s_excl;
+    __shared__ uint32_t s_seg, s_idx, s_ok, s_nd, s_out, s_nh, s_excl, s_tot;
     const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE, tid = threadIdx.x;
-    const uint32_t wbase = wv * GKC_WAVE * ITEMS;     // wave wv holds range keys [wbase, wbase + 64*ITEMS)
-    const int bsh = 2 * k - 2 * RADIX_BITS;
+    const uint32_t wbase = wv * GKC_WAVE * ITEMS;     // wave wv holds chunk keys [wbase, wbase + 64*ITEMS)
     uint32_t list = blockIdx.x % wl.n_lists;
     for (;;) {
         if (tid == 0) {
             uint32_t sg = 0, ix = 0;
             s_ok = claim<1>(wl, &plan->ctr[ctr_rl()], list, sg, ix);
-            s_seg = sg; s_idx = ix; s_nd = 0;
+            s_seg = sg; s_idx = ix; s_nd = 0; s_out = 0; s_nh = 0;
         }
         __syncthreads();
         if (!s_ok) break;
@@ -615,54 +709,40 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
         uint32_t jb = wbase + lane;
         asm volatile("" : "+v"(jb));
         const uint32_t tile = sa.rl_tile_start[seg] + ti;        // look-back slot
-        const K* kp = keys + sa.key_start[seg];
-        const uint32_t nseg = sa.seg_keys[seg];
-        const uint32_t lo = ti * TILE, hi = min(lo + TILE, nseg);
-        // 1. range: wave 0 finds the first bin start in [lo, hi) (hi: none),
-        //    wave 1 the first bin start at or after hi
-        if (wv == 0) {
-            uint32_t a = lo;
-            if (lo > 0) {
-                const uint32_t pb = bin_of(kp[lo - 1], bsh);
-                for (;; a += GKC_WAVE) {
-                    const uint32_t j = a + lane;
-                    const bool stop = j >= hi || bin_of(kp[j], bsh) != pb;
-                    const uint64_t m = __ballot(stop);
-                    if (m) { a += (uint32_t)__builtin_ctzll(m); break; }
-                }
-            }
-            if (lane == 0) s_a = a;
-        } else if (wv == 1) {
-            uint32_t b = hi;
-            if (hi < nseg) {
-                const uint32_t pb = bin_of(kp[hi - 1], bsh);
+        const uint2 rg = ranges[tile];
+        const uint32_t n = rg.y;
+        const bool heavy = n > CAP;
+        K* src = keys + sa.key_start[seg] + rg.x;
+        K* dst = alt + sa.key_start[seg] + rg.x;
+        if (heavy && tid == 0) {
+            // base of a heavy range first: its total is known only at the end
+            uint32_t excl = 0;
+            if (ti > 0) {
+                int64_t j = (int64_t)tile - 1;
                 for (;;) {
-                    const uint32_t j = b + lane;
-                    const bool stop = j >= nseg || bin_of(kp[j], bsh) != pb;
-                    const uint64_t m = __ballot(stop);
-                    if (m) { b += (uint32_t)__builtin_ctzll(m); break; }
-                    b += GKC_WAVE;
-                    if (b - hi > CAP) break;                     // too large either way
+                    uint32_t v = ld_agent(&lookback[j]);
+                    while ((v & ~LB_VAL) == 0) { __builtin_amdgcn_s_sleep(1); v = ld_agent(&lookback[j]); }
+                    excl += v & LB_VAL;
+                    if (v & LB_INC) break;
+                    j--;
                 }
             }
-            if (lane == 0) s_b = b;
+            s_excl = excl;
+            atomicAdd(n_heavy, 1u);
         }
-        __syncthreads();
-        const uint32_t a = s_a;
-        uint32_t n = a < hi ? s_b - a : 0;
-        if (n > CAP) {
-            if (tid == 0) atomicOr(heavy, 1u);
-            n = 0;
-        }
-        // 2. load the range (wave-blocked, lane-striped) and find the bits that vary
+        // 1. bits that vary inside the range
         K key[ITEMS];
         uint64_t olo = 0, ohi = 0;
-        K ref{};
-        if (n) ref = kp[a];
+        {
+            K ref{};
+            if (n) ref = src[0];
+            for (uint32_t c0 = 0; c0 < n; c0 += CAP) {
 #pragma unroll
-        for (int i = 0; i < ITEMS; i++) {
-            const uint32_t j = jb + i * GKC_WAVE;
-            if (j < n) { key[i] = kp[a + j]; or_diff(olo, ohi, key[i], ref); }
+                for (int i = 0; i < ITEMS; i++) {
+                    const uint32_t j = c0 + jb + i * GKC_WAVE;
+                    if (j < n) { key[i] = src[j]; or_diff(olo, ohi, key[i], ref); }
+                }
+            }
         }
         uint32_t o0 = (uint32_t)olo, o1 = (uint32_t)(olo >> 32), o2 = (uint32_t)ohi, o3 = (uint32_t)(ohi >> 32);
 #pragma unroll
@@ -678,191 +758,347 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
         const uint64_t vlo = ((uint64_t)o1 << 32) | o0, vhi = ((uint64_t)o3 << 32) | o2;
         const int nbits = vhi ? 128 - __builtin_clzll(vhi) : (vlo ? 64 - __builtin_clzll(vlo) : 0);
         const int npass = (nbits + RADIX_BITS - 1) / RADIX_BITS;
-        // 3. LSD radix sort of the range in LDS on bits [0, nbits)
-        for (int ps = 0; ps < npass; ps++) {
-            const int sh = ps * RADIX_BITS;
-            for (int i = tid; i < NWAVE * RADIX; i += BC_NT) (&s_whist[0][0])[i] = 0;
-            __syncthreads();
-            uint32_t dig[ITEMS];     // digit in bits [8:0] (RADIX = none), rank in the wave in [31:9]
+        uint32_t dig[ITEMS];
+        uint32_t cnt[ITEMS];
+        // 64-bit keys in LDS: equal keys are counted first in an LDS hash table,
+        // so only the distinct keys are sorted (with their counts)
+        const bool hashed = sizeof(K) == 8 && !heavy;
+        uint32_t nsort = n;                    // entries sorted in LDS
+        if constexpr (sizeof(K) == 8) {
+            if (hashed) {
+                constexpr uint64_t EMPTY = ~0ull;  // never canonical: its reverse complement (0) is smaller
+                constexpr int LOG_CAP = __builtin_ctz(CAP);
+                for (uint32_t i = tid; i < CAP; i += BC_NT) { s_keys[i] = EMPTY; s_cnt[i] = 0; }
+                __syncthreads();
 #pragma unroll
-            for (int i = 0; i < ITEMS; i++) {
-                const uint32_t j = jb + i * GKC_WAVE;
-                const uint32_t dd = (j < n) ? KeyTraits<K>::digit(key[i], sh) : RADIX;
-                uint64_t peers = __ballot(dd < RADIX);
-#pragma unroll
-                for (int bb = 0; bb < RADIX_BITS; bb++) {
-                    const uint64_t bal = __ballot((dd >> bb) & 1u);
-                    peers &= ((dd >> bb) & 1u) ? bal : ~bal;
+                for (int i = 0; i < ITEMS; i++) {
+                    if (wbase + i * GKC_WAVE >= n) continue;
+                    const uint32_t j = jb + i * GKC_WAVE;
+                    if (j < n) {
+                        const unsigned long long x = key[i];
+                        uint32_t h = (uint32_t)((x * 0x9E3779B97F4A7C15ull) >> (64 - LOG_CAP));
+                        for (;;) {     // n <= CAP slots: a free or matching slot exists
+                            const unsigned long long old =
+                                atomicCAS(reinterpret_cast<unsigned long long*>(&s_keys[h]), (unsigned long long)EMPTY, x);
+                            if (old == EMPTY || old == x) { atomicAdd(&s_cnt[h], 1u); break; }
+                            h = (h + 1) & (CAP - 1);
+                        }
+                    }
                 }
-                dig[i] = dd;
-                if (dd < RADIX) {
-                    const uint32_t before = s_whist[wv][dd];
-                    const uint64_t lower = peers & lanemask_lt();
-                    dig[i] = dd | ((before + (uint32_t)__popcll(lower)) << 9);
-                    if (lower == 0) s_whist[wv][dd] = before + (uint32_t)__popcll(peers);
-                }
-            }
-            __syncthreads();
-            // digit starts (digit-major, then wave): threads 0..255 own one digit
-            uint32_t tot = 0, inc = 0;
-            if (tid < RADIX) {
+                __syncthreads();
+                // compact the occupied slots (slot i*BC_NT + tid of thread tid)
+                uint32_t occ = 0;
 #pragma unroll
-                for (int w = 0; w < NWAVE; w++) { const uint32_t c = s_whist[w][tid]; s_whist[w][tid] = tot; tot += c; }
-                inc = tot;
+                for (int i = 0; i < ITEMS; i++) {
+                    key[i] = s_keys[i * BC_NT + tid];
+                    cnt[i] = s_cnt[i * BC_NT + tid];
+                    occ += key[i] != EMPTY ? 1u : 0u;
+                }
+                uint32_t inc = occ;
 #pragma unroll
                 for (int off = 1; off < GKC_WAVE; off <<= 1) {
                     const uint32_t y = __shfl_up(inc, off);
                     if ((int)lane >= off) inc += y;
                 }
                 if (lane == GKC_WAVE - 1) s_red[wv] = inc;
+                __syncthreads();
+                uint32_t pos = inc - occ, dtot = 0;
+#pragma unroll
+                for (int w = 0; w < NWAVE; w++) { const uint32_t v = s_red[w]; pos += (uint32_t)w < wv ? v : 0u; dtot += v; }
+#pragma unroll
+                for (int i = 0; i < ITEMS; i++)
+                    if (key[i] != EMPTY) { s_keys[pos] = key[i]; s_cnt[pos] = cnt[i]; pos++; }
+                nsort = dtot;
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < ITEMS; i++) {
+                    const uint32_t j = jb + i * GKC_WAVE;
+                    if (j < nsort) { key[i] = s_keys[j]; cnt[i] = s_cnt[j]; }
+                }
+            }
+        }
+        if (!heavy) {
+            // 2a. LSD radix sort of the range (or of its distinct keys) in LDS
+            //     (entries in registers, j = jb + 64 i)
+            for (int ps = 0; ps < npass; ps++) {
+                const int sh = ps * RADIX_BITS;
+                for (int i = tid; i < NWAVE * RADIX; i += BC_NT) (&s_whist[0][0])[i] = 0;
+                __syncthreads();
+                rank_in_wave<K, ITEMS>(key, jb, wbase, nsort, sh, s_whist[wv], dig);
+                __syncthreads();
+                uint32_t tot = 0;
+                if (tid < RADIX) {
+#pragma unroll
+                    for (int w = 0; w < NWAVE; w++) { const uint32_t c = s_whist[w][tid]; s_whist[w][tid] = tot; tot += c; }
+                }
+                uint32_t all;
+                const uint32_t ex = digit_scan(tot, tid, lane, wv, s_red, &all);
+                if (tid < RADIX) s_dstart[tid] = ex;
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < ITEMS; i++) {
+                    const uint32_t dd = dig[i] & 511u;
+                    if (dd < RADIX) {
+                        const uint32_t p = s_dstart[dd] + s_whist[wv][dd] + (dig[i] >> 9);
+                        s_keys[p] = key[i];
+                        if (hashed) s_cnt[p] = cnt[i];
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < ITEMS; i++) {
+                    const uint32_t j = jb + i * GKC_WAVE;
+                    if (wbase + i * GKC_WAVE < nsort && j < nsort) {
+                        key[i] = s_keys[j];
+                        if (hashed) cnt[i] = s_cnt[j];
+                    }
+                }
+            }
+            if (npass == 0 && !hashed) {   // 0 or 1 key, or all keys equal: already in order
+#pragma unroll
+                for (int i = 0; i < ITEMS; i++) {
+                    const uint32_t j = jb + i * GKC_WAVE;
+                    if (j < n) s_keys[j] = key[i];
+                }
+                __syncthreads();
+            }
+        } else {
+            // 2b. LSD radix sort of the range through global memory, CAP keys
+            //     at a time, stable across chunks (running digit positions)
+            for (int ps = 0; ps < npass; ps++) {
+                const int sh = ps * RADIX_BITS;
+                if (tid < RADIX) s_dstart[tid] = 0;
+                __syncthreads();
+                for (uint32_t c0 = 0; c0 < n; c0 += CAP) {
+#pragma unroll
+                    for (int i = 0; i < ITEMS; i++) {
+                        const uint32_t j = c0 + jb + i * GKC_WAVE;
+                        if (j < n) atomicAdd(&s_dstart[KeyTraits<K>::digit(src[j], sh)], 1u);
+                    }
+                }
+                __syncthreads();
+                uint32_t all;
+                const uint32_t h = tid < RADIX ? s_dstart[tid] : 0;
+                const uint32_t ex = digit_scan(h, tid, lane, wv, s_red, &all);
+                if (tid < RADIX) s_dstart[tid] = ex;
+                for (uint32_t c0 = 0; c0 < n; c0 += CAP) {
+                    const uint32_t nv = min(CAP, n - c0);
+                    for (int i = tid; i < NWAVE * RADIX; i += BC_NT) (&s_whist[0][0])[i] = 0;
+                    __syncthreads();
+#pragma unroll
+                    for (int i = 0; i < ITEMS; i++) {
+                        const uint32_t j = jb + i * GKC_WAVE;
+                        if (j < nv) key[i] = src[c0 + j];
+                    }
+                    rank_in_wave<K, ITEMS>(key, jb, wbase, nv, sh, s_whist[wv], dig);
+                    __syncthreads();
+                    uint32_t tot = 0;
+                    if (tid < RADIX) {
+#pragma unroll
+                        for (int w = 0; w < NWAVE; w++) { const uint32_t c = s_whist[w][tid]; s_whist[w][tid] = tot; tot += c; }
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int i = 0; i < ITEMS; i++) {
+                        const uint32_t dd = dig[i] & 511u;
+                        if (dd < RADIX) dst[s_dstart[dd] + s_whist[wv][dd] + (dig[i] >> 9)] = key[i];
+                    }
+                    __syncthreads();
+                    if (tid < RADIX) s_dstart[tid] += tot;
+                }
+                K* t = src; src = dst; dst = t;
+                __syncthreads();
+            }
+            if (npass & 1) {
+                // the sorted range ended in the spare buffer: copy it back, so
+                // the key buffer stays grouped by bin for a later pass over it
+                for (uint32_t j = tid; j < n; j += BC_NT) dst[j] = src[j];
+                K* t = src; src = dst; dst = t;
+                __syncthreads();
+            }
+        }
+        // 3. run-length count of the sorted range, CAP keys at a time (one
+        //    chunk when the range was sorted in LDS), emitted in key order
+        for (uint32_t c0 = 0; c0 == 0 || c0 < n; c0 += CAP) {
+            const uint32_t nv = hashed ? nsort : (n > c0 ? min(CAP, n - c0) : 0);
+            uint32_t keepmask = 0;
+            if (hashed) {
+                // entries are distinct and counted: threshold them
+#pragma unroll
+                for (int i = 0; i < ITEMS; i++) {
+                    if (wbase + i * GKC_WAVE >= nv) {
+                        if (lane == 0) s_ck[wv * ITEMS + i] = 0;
+                        continue;
+                    }
+                    const uint32_t j = jb + i * GKC_WAVE;
+                    const bool keep = j < nv && cnt[i] >= to.min_cnt && (to.max_cnt == 0 || cnt[i] <= to.max_cnt);
+                    keepmask |= keep ? 1u << i : 0u;
+                    const uint64_t kb = __ballot(keep);
+                    if (lane == 0) s_ck[wv * ITEMS + i] = (uint32_t)__popcll(kb);
+                }
+                if (tid == 0) s_nd = nv;
+            } else {
+            if (heavy) {
+#pragma unroll
+                for (int i = 0; i < ITEMS; i++) {
+                    const uint32_t j = jb + i * GKC_WAVE;
+                    if (j < nv) key[i] = src[c0 + j];
+                }
+                // next head at or after the chunk's end (wave 0, kept across chunks)
+                if (wv == 0 && s_nh < c0 + nv) {
+                    uint32_t e = c0 + nv;
+                    if (e < n) {
+                        for (;; e += GKC_WAVE) {
+                            const uint32_t j = e + lane;
+                            const bool stop = j >= n || !(src[j] == src[j - 1]);
+                            const uint64_t m = __ballot(stop);
+                            if (m) { e += (uint32_t)__builtin_ctzll(m); break; }
+                        }
+                    }
+                    if (lane == 0) s_nh = e;
+                }
+            }
+            uint32_t headmask = 0;
+#pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                if (wbase + i * GKC_WAVE >= nv) {          // the whole wave is past the range
+                    if (lane == 0) { s_hb[wv * ITEMS + i] = 0; s_ch[wv * ITEMS + i] = INF; }
+                    continue;
+                }
+                const uint32_t j = jb + i * GKC_WAVE;
+                bool head = false;
+                if (j < nv) {
+                    const uint32_t g = c0 + j;
+                    head = g == 0 || !(key[i] == (heavy ? src[g - 1] : s_keys[j - 1]));
+                }
+                const uint64_t hb = __ballot(head);
+                headmask |= head ? 1u << i : 0u;
+                if (lane == 0) {
+                    s_hb[wv * ITEMS + i] = hb;
+                    s_ch[wv * ITEMS + i] = hb ? wbase + i * GKC_WAVE + (uint32_t)__builtin_ctzll(hb) : INF;
+                    if (hb) atomicAdd(&s_nd, (uint32_t)__popcll(hb));
+                }
             }
             __syncthreads();
-            if (tid < RADIX) {
+            // next head after each 64-key chunk: exclusive suffix minimum
+            uint32_t sfx = INF;
+            if (tid < (uint32_t)NCH) {
+                sfx = s_ch[tid];
+#pragma unroll
+                for (int off = 1; off < GKC_WAVE; off <<= 1) {
+                    const uint32_t y = __shfl_down(sfx, off);
+                    if (lane + off < GKC_WAVE) sfx = min(sfx, y);
+                }
+                if (lane == 0) s_red[wv] = sfx;
+            }
+            __syncthreads();
+            if (tid < (uint32_t)NCH) {
+                uint32_t e = __shfl_down(sfx, 1);
+                if (lane == GKC_WAVE - 1) e = INF;
+                for (int w = wv + 1; w < NCW; w++) e = min(e, s_red[w]);
+                s_ch[tid] = e;
+            }
+            __syncthreads();
+            // counts (chunk-local positions; the run after the chunk's last
+            // head ends at the chunk end in LDS mode, at s_nh otherwise)
+            const uint32_t tail = heavy ? s_nh - c0 : nv;
+#pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                const uint32_t q0 = wbase + i * GKC_WAVE;
+                cnt[i] = 0;
+                if (q0 >= nv) {
+                    if (lane == 0) s_ck[wv * ITEMS + i] = 0;
+                    continue;
+                }
+                bool keep = false;
+                if ((headmask >> i) & 1u) {
+                    const uint64_t hb = s_hb[wv * ITEMS + i];
+                    const uint64_t above = lane == GKC_WAVE - 1 ? 0ull : hb & (~0ull << (lane + 1));
+                    uint32_t nh = above ? q0 + (uint32_t)__builtin_ctzll(above) : s_ch[wv * ITEMS + i];
+                    if (nh == INF) nh = tail;
+                    cnt[i] = nh - (q0 + lane);
+                    keep = cnt[i] >= to.min_cnt && (to.max_cnt == 0 || cnt[i] <= to.max_cnt);
+                }
+                keepmask |= keep ? 1u << i : 0u;
+                const uint64_t kb = __ballot(keep);
+                if (lane == 0) s_ck[wv * ITEMS + i] = (uint32_t)__popcll(kb);
+            }
+            }
+            __syncthreads();
+            // exclusive prefix of the kept entries over the 64-key chunks
+            if (tid < (uint32_t)NCH) {
+                const uint32_t v = s_ck[tid];
+                uint32_t inc = v;
+#pragma unroll
+                for (int off = 1; off < GKC_WAVE; off <<= 1) {
+                    const uint32_t y = __shfl_up(inc, off);
+                    if ((int)lane >= off) inc += y;
+                }
+                s_ck[tid] = inc - v;
+                if (lane == GKC_WAVE - 1) s_red[wv] = inc;
+            }
+            __syncthreads();
+            if (tid < (uint32_t)NCH) {
                 uint32_t off = 0;
                 for (uint32_t w = 0; w < wv; w++) off += s_red[w];
-                s_dstart[tid] = off + inc - tot;
+                s_ck[tid] += off;
+            }
+            uint32_t ctot = 0;
+            for (int w = 0; w < NCW; w++) ctot += s_red[w];
+            if (!heavy && tid == 0) {
+                // 4. decoupled look-back for the tile's place in the segment's table
+                uint32_t* lb = lookback + tile;
+                uint32_t excl = 0;
+                if (ti == 0) st_agent(lb, LB_INC | ctot);
+                else {
+                    st_agent(lb, LB_AGG | ctot);
+                    int64_t j = (int64_t)tile - 1;
+                    for (;;) {
+                        uint32_t v = ld_agent(&lookback[j]);
+                        while ((v & ~LB_VAL) == 0) { __builtin_amdgcn_s_sleep(1); v = ld_agent(&lookback[j]); }
+                        excl += v & LB_VAL;
+                        if (v & LB_INC) break;
+                        j--;
+                    }
+                    st_agent(lb, LB_INC | (excl + ctot));
+                }
+                s_excl = excl;
             }
             __syncthreads();
+            // 5. write the kept (k-mer, count) entries
+            if (to.emit && ctot) {
+                const uint32_t cap = to.table_cap[seg];
+                K* tk = reinterpret_cast<K*>(to.keys) + to.table_start[seg];
+                uint32_t* tc = to.counts + to.table_start[seg];
+                const uint32_t base = s_excl + s_out;
 #pragma unroll
-            for (int i = 0; i < ITEMS; i++) {
-                const uint32_t dd = dig[i] & 511u;
-                if (dd < RADIX) s_keys[s_dstart[dd] + s_whist[wv][dd] + (dig[i] >> 9)] = key[i];
+                for (int i = 0; i < ITEMS; i++) {
+                    if (wbase + i * GKC_WAVE >= nv) continue;
+                    const bool keep = (keepmask >> i) & 1u;
+                    const uint64_t kb = __ballot(keep);
+                    if (keep) {
+                        const uint32_t idx = base + s_ck[wv * ITEMS + i] + (uint32_t)__popcll(kb & lanemask_lt());
+                        if (idx < cap) { tk[idx] = key[i]; tc[idx] = cnt[i]; }
+                        else atomicOr(to.overflow, 1u);
+                    }
+                }
             }
             __syncthreads();
-#pragma unroll
-            for (int i = 0; i < ITEMS; i++) {
-                const uint32_t j = jb + i * GKC_WAVE;
-                if (j < n) key[i] = s_keys[j];
-            }
-        }
-        if (npass == 0) {           // 0 or 1 key, or all keys equal: already in order
-#pragma unroll
-            for (int i = 0; i < ITEMS; i++) {
-                const uint32_t j = jb + i * GKC_WAVE;
-                if (j < n) s_keys[j] = key[i];
-            }
-            __syncthreads();
-        }
-        // 4. run heads; first head of each 64-key chunk
-        uint32_t headmask = 0;
-#pragma unroll
-        for (int i = 0; i < ITEMS; i++) {
-            const uint32_t j = jb + i * GKC_WAVE;
-            const bool head = j < n && (j == 0 || !(key[i] == s_keys[j - 1]));
-            const uint64_t hb = __ballot(head);
-            headmask |= head ? 1u << i : 0u;
-            if (lane == 0) {
-                s_hb[wv * ITEMS + i] = hb;
-                s_ch[wv * ITEMS + i] = hb ? wbase + i * GKC_WAVE + (uint32_t)__builtin_ctzll(hb) : INF;
-                if (hb) atomicAdd(&s_nd, (uint32_t)__popcll(hb));
-            }
-        }
-        __syncthreads();
-        // next head after each chunk: exclusive suffix minimum over the chunks
-        uint32_t sfx = INF;
-        if (tid < (uint32_t)NCH) {
-            sfx = s_ch[tid];
-#pragma unroll
-            for (int off = 1; off < GKC_WAVE; off <<= 1) {
-                const uint32_t y = __shfl_down(sfx, off);
-                if (lane + off < GKC_WAVE) sfx = min(sfx, y);
-            }
-            if (lane == 0) s_red[wv] = sfx;
-        }
-        __syncthreads();
-        if (tid < (uint32_t)NCH) {
-            uint32_t e = __shfl_down(sfx, 1);
-            if (lane == GKC_WAVE - 1) e = INF;
-            for (int w = wv + 1; w < NCW; w++) e = min(e, s_red[w]);
-            s_ch[tid] = e;
-        }
-        __syncthreads();
-        // 5. counts, threshold, kept entries per chunk
-        uint32_t cnt[ITEMS];
-        uint32_t keepmask = 0;
-#pragma unroll
-        for (int i = 0; i < ITEMS; i++) {
-            const uint32_t c0 = wbase + i * GKC_WAVE;
-            cnt[i] = 0;
-            bool keep = false;
-            if ((headmask >> i) & 1u) {
-                const uint64_t hb = s_hb[wv * ITEMS + i];
-                const uint64_t above = lane == GKC_WAVE - 1 ? 0ull : hb & (~0ull << (lane + 1));
-                uint32_t nh = above ? c0 + (uint32_t)__builtin_ctzll(above) : s_ch[wv * ITEMS + i];
-                if (nh == INF) nh = n;
-                cnt[i] = nh - (c0 + lane);
-                keep = cnt[i] >= to.min_cnt && (to.max_cnt == 0 || cnt[i] <= to.max_cnt);
-            }
-            keepmask |= keep ? 1u << i : 0u;
-            const uint64_t kb = __ballot(keep);
-            if (lane == 0) s_ck[wv * ITEMS + i] = (uint32_t)__popcll(kb);
-        }
-        __syncthreads();
-        // exclusive prefix of the kept entries over the chunks
-        if (tid < (uint32_t)NCH) {
-            const uint32_t v = s_ck[tid];
-            uint32_t inc = v;
-#pragma unroll
-            for (int off = 1; off < GKC_WAVE; off <<= 1) {
-                const uint32_t y = __shfl_up(inc, off);
-                if ((int)lane >= off) inc += y;
-            }
-            s_ck[tid] = inc - v;
-            if (lane == GKC_WAVE - 1) s_red[wv] = inc;
-        }
-        __syncthreads();
-        if (tid < (uint32_t)NCH) {
-            uint32_t off = 0;
-            for (uint32_t w = 0; w < wv; w++) off += s_red[w];
-            s_ck[tid] += off;
+            if (tid == 0) s_out += ctot;
         }
         if (tid == 0) {
-            uint32_t tot = 0;
-            for (int w = 0; w < NCW; w++) tot += s_red[w];
-            // 6. decoupled look-back for the tile's place in the segment's table
-            uint32_t* lb = lookback + tile;
-            uint32_t excl = 0;
-            if (ti == 0) st_agent(lb, LB_INC | tot);
-            else {
-                st_agent(lb, LB_AGG | tot);
-                int64_t j = (int64_t)tile - 1;
-                for (;;) {
-                    uint32_t v = ld_agent(&lookback[j]);
-                    while ((v & ~LB_VAL) == 0) { __builtin_amdgcn_s_sleep(1); v = ld_agent(&lookback[j]); }
-                    excl += v & LB_VAL;
-                    if (v & LB_INC) break;
-                    j--;
-                }
-                st_agent(lb, LB_INC | (excl + tot));
-            }
-            s_excl = excl; s_tot = tot;
+            if (heavy) st_agent(lookback + tile, LB_INC | (s_excl + s_out));
             if (s_nd) atomicAdd(&to.n_distinct[seg], s_nd);
-            if (tot) atomicAdd(&to.n_entries[seg], tot);
-        }
-        __syncthreads();
-        // 7. write the kept (k-mer, count) entries
-        if (to.emit && s_tot) {
-            const uint32_t cap = to.table_cap[seg];
-            K* tk = reinterpret_cast<K*>(to.keys) + to.table_start[seg];
-            uint32_t* tc = to.counts + to.table_start[seg];
-            const uint32_t base = s_excl;
-#pragma unroll
-            for (int i = 0; i < ITEMS; i++) {
-                const bool keep = (keepmask >> i) & 1u;
-                const uint64_t kb = __ballot(keep);
-                if (keep) {
-                    const uint32_t idx = base + s_ck[wv * ITEMS + i] + (uint32_t)__popcll(kb & lanemask_lt());
-                    if (idx < cap) { tk[idx] = key[i]; tc[idx] = cnt[i]; }
-                    else atomicOr(to.overflow, 1u);
-                }
-            }
+            if (s_out) atomicAdd(&to.n_entries[seg], s_out);
         }
         __syncthreads();
     }
 }
 
-template __global__ void k_bincount<uint64_t>(Plan*, SegArrays, WorkLists, const uint64_t*, TableOut, uint32_t*, uint32_t*, int);
-template __global__ void k_bincount<u128k>(Plan*, SegArrays, WorkLists, const u128k*, TableOut, uint32_t*, uint32_t*, int);
+template __global__ void k_binranges<uint64_t>(Plan*, SegArrays, const uint64_t*, uint2*, int);
+template __global__ void k_binranges<u128k>(Plan*, SegArrays, const u128k*, uint2*, int);
+template __global__ void k_bincount<uint64_t>(Plan*, SegArrays, WorkLists, uint64_t*, uint64_t*, const uint2*, TableOut, uint32_t*, uint32_t*);
+template __global__ void k_bincount<u128k>(Plan*, SegArrays, WorkLists, u128k*, u128k*, const uint2*, TableOut, uint32_t*, uint32_t*);
 
 template __global__ void k_extract<uint64_t>(Plan*, SegArrays, WorkLists, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, uint64_t*, int);
 template __global__ void k_extract<u128k>(Plan*, SegArrays, WorkLists, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, u128k*, int);

@@ -68,7 +68,7 @@ typedef struct gkc_stats {
     uint32_t fallbacks;        /* presize overflows that fell back to count-then-allocate */
     uint32_t host_syncs;       /* host synchronisations inside the last gkc_count */
     uint32_t partition_fallbacks;  /* presized batched scatters redone by count-then-allocate since gkc_reset */
-    uint32_t heavy_fallbacks;  /* binned counts redone with the full LSD sort (a bin larger than an LDS tile) */
+    uint32_t heavy_ranges;     /* binned tiles too large for LDS, sorted through global memory (last gkc_count) */
     uint32_t binned;           /* 1 when the last gkc_count used the binned kernels */
 } gkc_stats;
 

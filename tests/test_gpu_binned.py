@@ -2,8 +2,9 @@
 then an LDS sort + run-length count per tile of whole bins.  Bit-exact
 against the oracle and against the full LSD sort (count_mode 1), including
 the edges of the tiling: bins straddling tile windows, ranges of one key or
-of one repeated key, bins too large for a tile (fallback to the LSD sort),
-digits straddling bit 64 of 128-bit keys, and thresholds."""
+of one repeated key, ranges too large for LDS (sorted and counted in chunks
+through global memory), digits straddling bit 64 of 128-bit keys, and
+thresholds."""
 from __future__ import annotations
 
 import numpy as np
@@ -59,7 +60,7 @@ def test_binned_matches_oracle_and_lsd(gpu, cid, scale, k, p, P):
     ot = O.count(rs.reads, rs.offs, k, p, P)
     _say("binned", w.name, k, p, P)
     gt, st = _count(rs.reads, rs.offs, k, p, P, mode=0)
-    assert st["binned"] == 1 and st["heavy_fallbacks"] == 0 and st["n_pass"] == 2
+    assert st["binned"] == 1 and st["n_pass"] == 2
     assert_tables_equal(gt, ot, P, k)
     _say("lsd", w.name, k, p, P)
     lt, sl = _count(rs.reads, rs.offs, k, p, P, mode=1)
@@ -89,18 +90,40 @@ def test_binned_hpc_long_reads(gpu):
 
 
 @pytest.mark.parametrize("k,p,copies", [(31, 11, 300), (40, 13, 200)])
-def test_heavy_bin_falls_back_to_lsd(gpu, k, p, copies):
-    """A k-mer occurring more often than a tile holds (poly-A: k-mer 0) makes
-    its bin too large for LDS; the count is redone with the full LSD sort."""
+def test_heavy_range_repeated_kmer(gpu, k, p, copies):
+    """A k-mer occurring more often than LDS holds (poly-A: k-mer 0) makes its
+    bin's range too large for LDS; it is sorted and counted in chunks through
+    global memory (runs crossing chunk ends)."""
     rng = np.random.default_rng(5)
     seqs = [b"A" * 150] * copies
     seqs += [bytes(rng.choice(list(b"ACGT"), 150).tolist()) for _ in range(400)]
     reads, offs = _reads(seqs)
     ot = O.count(reads, offs, k, p, 4)
-    _say("heavy", k, copies)
+    _say("heavy repeated", k, copies)
     gt, st = _count(reads, offs, k, p, 4, mode=0)
-    assert st["heavy_fallbacks"] == 1 and st["binned"] == 0
+    assert st["binned"] == 1 and st["heavy_ranges"] >= 1
     assert_tables_equal(gt, ot, 4, k)
+
+
+@pytest.mark.parametrize("minc,nrand", [(1, 22), (2, 22), (1, 11), (2, 12)])
+def test_heavy_range_distinct_kmers(gpu, minc, nrand):
+    """One bin holding ~40 k keys of ~20 k distinct k-mers (fixed prefix, then
+    nrand random bases, then A, so canonical = forward; one k-mer per read): a
+    multi-chunk global sort over 2*nrand varying bits (11 bases: 3 passes,
+    the odd case), counted twice (count pass, then emit pass)."""
+    rng = np.random.default_rng(11)
+    body = rng.choice(list(b"ACGT"), (20000, nrand)).astype(np.uint8)
+    prefix = (b"ACGTACGTACGTACGTACGTACGTACGTACGT")[:30 - nrand]
+    seqs = []
+    for r in range(len(body)):
+        s = prefix + bytes(body[r].tolist()) + b"A"
+        seqs += [s] * (1 + r % 3)
+    reads, offs = _reads(seqs)
+    ot = O.count(reads, offs, 31, 11, 1, min_count=minc)
+    _say("heavy distinct", minc)
+    gt, st = _count(reads, offs, 31, 11, 1, mode=0, min_count=minc)
+    assert st["binned"] == 1 and st["heavy_ranges"] >= 1
+    assert_tables_equal(gt, ot, 1, 31)
 
 
 def test_binned_small_ranges(gpu):
