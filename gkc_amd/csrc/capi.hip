@@ -104,7 +104,9 @@ struct gkc_ctx {
     // counting
     DevBuf keysA, keysB, lookback, d_plan, d_segk, d_segh, d_keystart, d_itemstart, d_tilestart, d_rlstart, d_hist;
     DevBuf d_tstart, d_tcap, d_nent, d_ndist, d_toverflow;
-    DevBuf d_ranges;                           // k_bincount tile ranges {a, n}
+    DevBuf d_ranges;                           // binned-count tile ranges {a, n}
+    DevBuf d_hlist, d_hres;                    // heavy tiles of a wave-level count and their {kept, distinct}
+    DevBuf d_wkept;                            // kept entries per window, then window offsets
     DevBuf tkeys, tcnts;
     std::vector<uint64_t> part_off, part_distinct, part_kmers, part_start;   // part_start: physical table offsets
     std::vector<uint32_t> h_caps, h_nent;
@@ -113,6 +115,9 @@ struct gkc_ctx {
     hipGraphExec_t g_sort = nullptr, g_emit = nullptr, g_cnt = nullptr;
     const void* g_emit_tk = nullptr;           // table buffers g_emit was captured with
     const void* g_emit_tc = nullptr;
+    hipGraphExec_t g_emit_p = nullptr;         // emit pass that also prepares (no count pass before it)
+    const void* g_emit_p_tk = nullptr;
+    const void* g_emit_p_tc = nullptr;
     std::vector<const void*> g_key;
     std::vector<uint32_t> presize_caps;
     bool counted = false;
@@ -389,6 +394,7 @@ void gkc_destroy(gkc_ctx* c) {
     if (c->st) (void)hipStreamSynchronize(c->st);
     if (c->g_sort) (void)hipGraphExecDestroy(c->g_sort);
     if (c->g_emit) (void)hipGraphExecDestroy(c->g_emit);
+    if (c->g_emit_p) (void)hipGraphExecDestroy(c->g_emit_p);
     if (c->g_cnt) (void)hipGraphExecDestroy(c->g_cnt);
     if (c->g_p1) (void)hipGraphExecDestroy(c->g_p1);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
@@ -1110,21 +1116,33 @@ extern "C" int gkc_count_packed(gkc_ctx* c, const uint32_t* words, uint64_t n_wo
 
 // ---------------------------------------------------------------- batched form
 
-// Shape of one batched counting launch.  Binned (the default when k >= 8):
-// two onesweep passes over key bits [2k-16, 2k) group each segment's keys by
-// bin and k_bincount sorts + counts tiles of whole bins in LDS.  Otherwise the
-// full LSD sort (ceil(2k/8) passes) and k_runlength.
+// Shape of one batched counting launch (count_mode 0, k >= 8: binned).
+//   64-bit keys: three onesweep passes over key bits [2k-24, 2k) group each
+//     segment's keys by 24-bit bin; k_wavecount counts WC_TILE-key windows of
+//     whole bins per wave, after k_bincount<., true> has counted the ranges
+//     larger than WC_CAP into staging.
+//   128-bit keys: two passes over [2k-16, 2k); k_bincount sorts + counts
+//     tiles of whole bins per workgroup (in LDS, or in chunks through global
+//     memory when larger).
+//   Otherwise (count_mode 1, k < 8): the full LSD sort (ceil(2k/8) passes)
+//   and k_runlength.
 struct BatchShape {
     bool binned;
+    bool wave;
     int npass;
     uint32_t sh0;
     int rl_tile;
+    int bin_bits;
 };
 template <typename K>
 static BatchShape batch_shape(const gkc_ctx* c) {
-    const bool binned = c->tune.count_mode != 1 && c->prm.k >= 8;
-    if (binned) return BatchShape{true, 2, (uint32_t)(2 * c->prm.k - 16), bc_tile<K>()};
-    return BatchShape{false, c->npass, 0u, RL_TILE};
+    const int kb = 2 * c->prm.k;
+    if (c->tune.count_mode == 1 || c->prm.k < 8) return BatchShape{false, false, c->npass, 0u, RL_TILE, 0};
+    if (sizeof(K) == 8) {
+        const int np = std::min(3, (kb + 7) / 8);
+        return BatchShape{true, true, np, (uint32_t)std::max(0, kb - 8 * np), WC_TILE, 8 * np};
+    }
+    return BatchShape{true, false, 2, (uint32_t)(kb - 16), bc_tile<K>(), 16};
 }
 
 // Enqueue extraction + the onesweep passes over every partition at once.
@@ -1162,11 +1180,16 @@ static K* enqueue_sort_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs, uint6
 
 // Count (emit = 0) or count and write the table (emit = 1) from the keys the
 // sort left in `sorted`.  d_toverflow[0]: table capacity exceeded;
-// d_toverflow[1]: k_bincount ranges larger than LDS (sorted through global
-// memory).
+// d_toverflow[1]: ranges too large for the counting kernel (sorted through
+// global memory; in the wave-level count, counted ahead into staging);
+// d_toverflow[2]: claim counter of the heavy list.
+// Wave-level count: `prep` (the first pass over the sorted keys of a
+// gkc_count) counts every window into staging and scans the windows' kept
+// counts into table offsets; an emit pass then only copies staging to the
+// table (again with new capacities after an under-prediction).
 template <typename K>
 static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs, K* sorted,
-                                  uint64_t rl_tiles_cap, int emit) {
+                                  uint64_t rl_tiles_cap, int emit, bool prep) {
     const uint32_t P = c->prm.n_parts;
     Plan* plan = c->d_plan.as<Plan>();
     TableOut to;
@@ -1175,17 +1198,51 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs
     to.n_entries = c->d_nent.as<uint32_t>(); to.n_distinct = c->d_ndist.as<uint32_t>();
     to.overflow = c->d_toverflow.as<uint32_t>();
     to.min_cnt = c->prm.min_count; to.max_cnt = c->prm.max_count; to.emit = emit;
-    zero(c, c->d_nent.p, P); zero(c, c->d_ndist.p, P); zero(c, c->d_toverflow.p, 2);
+    K* alt = sorted == c->keysA.as<K>() ? c->keysB.as<K>() : c->keysA.as<K>();
+    uint2* ranges = c->d_ranges.as<uint2>();
+    HeavyList hl{c->d_hlist.as<uint32_t>(), c->d_toverflow.as<uint32_t>() + 1, c->d_toverflow.as<uint32_t>() + 2,
+                 c->d_hres.as<uint2>(), c->d_wkept.as<uint32_t>()};
+    if (bs.wave) {
+        if constexpr (sizeof(K) == 8) {
+            if (prep) {
+                zero(c, c->d_nent.p, P); zero(c, c->d_ndist.p, P);
+                zero(c, c->d_toverflow.as<uint32_t>() + 1, 2);
+                zero(c, &plan->ctr[ctr_rl()], NLIST);
+                const int g_r = occupancy_blocks((const void*)k_binranges<K>, 256, c->n_cu);
+                hipLaunchKernelGGL(k_binranges<K>, dim3(g_r), dim3(256), 0, c->st, plan, sa, (const K*)sorted, ranges,
+                                   bs.rl_tile, (int)bs.sh0, bs.bin_bits, (uint32_t)WC_CAP, hl);
+                const int g_h = occupancy_blocks((const void*)k_bincount<K, true>, BC_NT, c->n_cu);
+                hipLaunchKernelGGL((k_bincount<K, true>), dim3(g_h), dim3(BC_NT), 0, c->st, plan, sa, c->wl, sorted, alt,
+                                   (const uint2*)ranges, to, c->lookback.as<uint32_t>(), c->d_toverflow.as<uint32_t>() + 1,
+                                   hl);
+                const int g_w = occupancy_blocks((const void*)k_wavecount, WC_NT, c->n_cu);
+                hipLaunchKernelGGL(k_wavecount, dim3(g_w), dim3(WC_NT), 0, c->st, plan, sa, c->wl, (uint64_t*)sorted,
+                                   (uint64_t*)alt, (const uint2*)ranges, hl, to);
+                hipLaunchKernelGGL(k_tilescan, dim3(std::min<uint32_t>(P, 4 * c->n_cu)), dim3(1024), 0, c->st, plan, sa,
+                                   hl.kept);
+            }
+            if (emit) {
+                zero(c, c->d_toverflow.p, 1);
+                const int g_c = occupancy_blocks((const void*)k_wavecopy, 256, c->n_cu);
+                hipLaunchKernelGGL(k_wavecopy, dim3(g_c), dim3(256), 0, c->st, plan, sa, (const uint64_t*)sorted,
+                                   (const uint64_t*)alt, (const uint2*)ranges, hl, to);
+            }
+        }
+        HIPX(hipGetLastError());
+        return;
+    }
+    zero(c, c->d_nent.p, P); zero(c, c->d_ndist.p, P); zero(c, c->d_toverflow.p, 3);
     zero(c, c->lookback.p, rl_tiles_cap);
     zero(c, &plan->ctr[ctr_rl()], NLIST);
     if (bs.binned) {
-        K* alt = sorted == c->keysA.as<K>() ? c->keysB.as<K>() : c->keysA.as<K>();
-        uint2* ranges = c->d_ranges.as<uint2>();
+        const HeavyList none{nullptr, nullptr, nullptr, nullptr, nullptr};
         const int g_r = occupancy_blocks((const void*)k_binranges<K>, 256, c->n_cu);
-        hipLaunchKernelGGL(k_binranges<K>, dim3(g_r), dim3(256), 0, c->st, plan, sa, (const K*)sorted, ranges, c->prm.k);
-        const int g_b = occupancy_blocks((const void*)k_bincount<K>, BC_NT, c->n_cu);
-        hipLaunchKernelGGL(k_bincount<K>, dim3(g_b), dim3(BC_NT), 0, c->st, plan, sa, c->wl, sorted, alt,
-                           (const uint2*)ranges, to, c->lookback.as<uint32_t>(), c->d_toverflow.as<uint32_t>() + 1);
+        hipLaunchKernelGGL(k_binranges<K>, dim3(g_r), dim3(256), 0, c->st, plan, sa, (const K*)sorted, ranges,
+                           bs.rl_tile, (int)bs.sh0, bs.bin_bits, 0xFFFFFFFFu, none);
+        const int g_b = occupancy_blocks((const void*)k_bincount<K, false>, BC_NT, c->n_cu);
+        hipLaunchKernelGGL((k_bincount<K, false>), dim3(g_b), dim3(BC_NT), 0, c->st, plan, sa, c->wl, sorted, alt,
+                           (const uint2*)ranges, to, c->lookback.as<uint32_t>(), c->d_toverflow.as<uint32_t>() + 1,
+                           none);
     } else {
         const int g_r = occupancy_blocks((const void*)k_runlength<K>, SORT_NT, c->n_cu);
         hipLaunchKernelGGL(k_runlength<K>, dim3(g_r), dim3(SORT_NT), 0, c->st, plan, sa, c->wl, sorted, to,
@@ -1240,6 +1297,9 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     sa.hist = c->d_hist.as<uint32_t>();
     K* sorted = (bs.npass % 2 == 0) ? c->keysA.as<K>() : c->keysB.as<K>();
     c->d_ranges.ensure(rl_tiles_cap * sizeof(uint2));
+    c->d_hlist.ensure(rl_tiles_cap * sizeof(uint32_t));
+    c->d_hres.ensure(rl_tiles_cap * sizeof(uint2));
+    c->d_wkept.ensure(rl_tiles_cap * sizeof(uint32_t));
     set_lists(c, P, c->h_kmer_cur.data(), c->h_hdr_cur.data(), c->tune.single_list == 1 ? 1 : NLIST,
               SORT_NT * sort_items<K>(c), bs.rl_tile);
 
@@ -1248,14 +1308,20 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     (void)occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu);
     (void)sort_grid<K>(c);
     (void)occupancy_blocks((const void*)k_runlength<K>, SORT_NT, c->n_cu);
-    (void)occupancy_blocks((const void*)k_bincount<K>, BC_NT, c->n_cu);
+    (void)occupancy_blocks((const void*)k_bincount<K, false>, BC_NT, c->n_cu);
     (void)occupancy_blocks((const void*)k_binranges<K>, 256, c->n_cu);
+    if constexpr (sizeof(K) == 8) {
+        (void)occupancy_blocks((const void*)k_bincount<K, true>, BC_NT, c->n_cu);
+        (void)occupancy_blocks((const void*)k_wavecount, WC_NT, c->n_cu);
+        (void)occupancy_blocks((const void*)k_wavecopy, 256, c->n_cu);
+    }
     HIPX(hipEventRecord(c->ev[0], c->st));
     std::vector<const void*> key = {c->keysA.p, c->keysB.p, c->lookback.p, c->words.p, c->hdrs.p, c->d_ranges.p,
+                                    c->d_hlist.p, c->d_hres.p, c->d_wkept.p,
                                     (const void*)(uintptr_t)tiles_cap, (const void*)(uintptr_t)bs.npass,
                                     (const void*)(uintptr_t)sort_items<K>(c), (const void*)(uintptr_t)sort_grid<K>(c),
                                     c->d_wl.p, (const void*)(uintptr_t)c->wl.n_lists,
-                                    (const void*)(uintptr_t)(bs.binned ? 1 : 0)};
+                                    (const void*)(uintptr_t)(bs.binned ? 1 : 0), (const void*)(uintptr_t)(bs.wave ? 1 : 0)};
     if (timed) {
         enqueue_sort_all<K>(c, sa, bs, tiles_cap, true);
     } else {
@@ -1263,8 +1329,9 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
             if (c->g_sort) { HIPX(hipGraphExecDestroy(c->g_sort)); c->g_sort = nullptr; }
             if (c->g_cnt) { HIPX(hipGraphExecDestroy(c->g_cnt)); c->g_cnt = nullptr; }
             if (c->g_emit) { HIPX(hipGraphExecDestroy(c->g_emit)); c->g_emit = nullptr; }
+            if (c->g_emit_p) { HIPX(hipGraphExecDestroy(c->g_emit_p)); c->g_emit_p = nullptr; }
             c->g_sort = capture(c, [&] { enqueue_sort_all<K>(c, sa, bs, tiles_cap, false); });
-            c->g_cnt = capture(c, [&] { enqueue_runlength_all<K>(c, sa, bs, sorted, rl_tiles_cap, 0); });
+            c->g_cnt = capture(c, [&] { enqueue_runlength_all<K>(c, sa, bs, sorted, rl_tiles_cap, 0, true); });
             c->g_key = key;
         }
         HIPX(hipGraphLaunch(c->g_sort, c->st));
@@ -1276,7 +1343,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     if (presized) {
         caps = c->presize_caps;
     } else {
-        if (timed) enqueue_runlength_all<K>(c, sa, bs, sorted, rl_tiles_cap, 0);
+        if (timed) enqueue_runlength_all<K>(c, sa, bs, sorted, rl_tiles_cap, 0, true);
         else HIPX(hipGraphLaunch(c->g_cnt, c->st));
         HIPX(hipMemcpyAsync(caps.data(), c->d_nent.p, P * 4, hipMemcpyDeviceToHost, c->st));
         HIPX(hipMemcpyAsync(flags_dev, c->d_toverflow.p, 8, hipMemcpyDeviceToHost, c->st));
@@ -1292,15 +1359,22 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     HIPX(hipMemcpyAsync(c->d_tstart.p, c->h_tstart.data(), P * 8, hipMemcpyHostToDevice, c->st));
     HIPX(hipMemcpyAsync(c->d_tcap.p, caps.data(), P * 4, hipMemcpyHostToDevice, c->st));
     // emit pass: replay the captured graph only if it writes the current table
-    // buffers (either may have moved when the table grew)
+    // buffers (either may have moved when the table grew).  The first pass
+    // over the sorted keys of this count prepares (see enqueue_runlength_all).
+    bool prepared = !presized;
     auto emit_pass = [&]() {
-        if (timed) { enqueue_runlength_all<K>(c, sa, bs, sorted, rl_tiles_cap, 1); return; }
-        if (!c->g_emit || c->g_emit_tk != c->tkeys.p || c->g_emit_tc != c->tcnts.p) {
-            if (c->g_emit) { HIPX(hipGraphExecDestroy(c->g_emit)); c->g_emit = nullptr; }
-            c->g_emit = capture(c, [&] { enqueue_runlength_all<K>(c, sa, bs, sorted, rl_tiles_cap, 1); });
-            c->g_emit_tk = c->tkeys.p; c->g_emit_tc = c->tcnts.p;
+        const bool prep = !prepared;
+        prepared = true;
+        if (timed) { enqueue_runlength_all<K>(c, sa, bs, sorted, rl_tiles_cap, 1, prep); return; }
+        hipGraphExec_t& g = prep ? c->g_emit_p : c->g_emit;
+        const void*& gk = prep ? c->g_emit_p_tk : c->g_emit_tk;
+        const void*& gc = prep ? c->g_emit_p_tc : c->g_emit_tc;
+        if (!g || gk != c->tkeys.p || gc != c->tcnts.p) {
+            if (g) { HIPX(hipGraphExecDestroy(g)); g = nullptr; }
+            g = capture(c, [&] { enqueue_runlength_all<K>(c, sa, bs, sorted, rl_tiles_cap, 1, prep); });
+            gk = c->tkeys.p; gc = c->tcnts.p;
         }
-        HIPX(hipGraphLaunch(c->g_emit, c->st));
+        HIPX(hipGraphLaunch(g, c->st));
     };
     emit_pass();
     HIPX(hipEventRecord(c->ev[5], c->st));

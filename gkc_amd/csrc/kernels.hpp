@@ -155,12 +155,39 @@ __global__ void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, 
 template <typename K>
 __global__ void k_runlength(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ keys, TableOut to,
                             uint32_t* __restrict__ lookback);
+// Wave-level count (64-bit keys): after three onesweep passes over key bits
+// [sh0, sh0 + 24) each wave counts WC_TILE-key windows of whole 24-bit bins
+// on its own (LDS hash table, rank of the distinct keys by comparison) into
+// staging, with no workgroup barriers and no wait on other windows; a scan of
+// the windows' kept counts gives their table offsets and a copy writes the
+// table.  Ranges of more than WC_CAP keys are counted beforehand by
+// k_bincount<., true> into the same staging.
+constexpr int WC_NT = 256;
+constexpr int WC_TILE = 256;
+constexpr int WC_CAP = 512;
+constexpr uint32_t WC_BATCH = 16;   // windows claimed per atomic
+
+// Heavy-range list of a binned count: tiles whose range exceeds the
+// capacity of the counting kernel, counted ahead by k_bincount<., true>.
+struct HeavyList {
+    uint32_t* tiles;          // [*count] tile ids
+    uint32_t* count;          // appended by k_binranges
+    uint32_t* next;           // claim counter of k_bincount<., true>
+    uint2* res;               // [tile] {kept entries, distinct keys} of each heavy tile
+    uint32_t* kept;           // [tile] kept entries of each window, then their offsets (k_tilescan)
+};
+
 template <typename K>
 __global__ void k_binranges(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys, uint2* __restrict__ ranges,
-                            int k);
-template <typename K>
+                            int tile_keys, int bsh, int bin_bits, uint32_t heavy_cap, HeavyList hl);
+template <typename K, bool STAGED>
 __global__ void k_bincount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, K* __restrict__ keys, K* __restrict__ alt,
                            const uint2* __restrict__ ranges, TableOut to, uint32_t* __restrict__ lookback,
-                           uint32_t* __restrict__ n_heavy);
+                           uint32_t* __restrict__ n_heavy, HeavyList hl);
+__global__ void k_wavecount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, uint64_t* __restrict__ keys,
+                            uint64_t* __restrict__ alt, const uint2* __restrict__ ranges, HeavyList hl, TableOut to);
+__global__ void k_tilescan(Plan* __restrict__ plan, SegArrays sa, uint32_t* __restrict__ kept);
+__global__ void k_wavecopy(Plan* __restrict__ plan, SegArrays sa, const uint64_t* __restrict__ keys,
+                           const uint64_t* __restrict__ alt, const uint2* __restrict__ ranges, HeavyList hl, TableOut to);
 
 }  // namespace gkc

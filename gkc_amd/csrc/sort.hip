@@ -54,6 +54,39 @@ __device__ __forceinline__ bool claim(const WorkLists& wl, uint32_t* ctr, uint32
     return false;
 }
 
+// Claim a batch of up to G consecutive units of kind KIND (one atomic per
+// batch, so a counter is not hit once per small unit); run by one lane.
+// Returns the batch's list-order units [t0, t1) of list `list`, or false
+// when every list is exhausted.
+template <int KIND>
+__device__ __forceinline__ bool claim_batch(const WorkLists& wl, uint32_t* ctr, uint32_t& list, uint32_t G,
+                                            uint32_t& t0, uint32_t& t1) {
+    for (uint32_t tries = 0; tries < wl.n_lists; tries++) {
+        const uint32_t L = list;
+        const uint32_t tot = wl.w[KIND * 8 + L];
+        if (tot) {
+            const uint32_t b = atomicAdd(&ctr[L], 1u);
+            if ((uint64_t)b * G < tot) { t0 = b * G; t1 = min(t0 + G, tot); return true; }
+        }
+        list = (L + 1) % wl.n_lists;
+    }
+    return false;
+}
+
+// Segment and in-segment index of unit t (list order) of list L.
+template <int KIND>
+__device__ __forceinline__ void unit_at(const WorkLists& wl, uint32_t L, uint32_t t, uint32_t& seg, uint32_t& idx) {
+    const uint32_t n = wl.n_segs;
+    const uint32_t* off = wl.w + 33 + n + KIND * n;
+    uint32_t lo = wl.w[24 + L], hi = wl.w[24 + L + 1];
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= t) lo = mid; else hi = mid;
+    }
+    seg = wl.w[33 + lo];
+    idx = t - off[lo];
+}
+
 // Build the segment layout from per-segment key and header counts (one block).
 __global__ __launch_bounds__(1024) void k_plan(Plan* plan, SegArrays sa, int sort_tile_keys, int rl_tile_keys) {
     __shared__ uint64_t s_k[1024];
@@ -567,25 +600,33 @@ __global__ __launch_bounds__(SORT_NT) void k_runlength(Plan* __restrict__ plan, 
 // keys: skewed composition or a high-copy k-mer) is sorted by the same
 // workgroup in CAP-key chunks through global memory, ping-ponging with the
 // spare key buffer, and counted chunk by chunk.
-template <typename K>
-__device__ __forceinline__ uint32_t bin_of(const K& x, int bsh) {
-    return KeyTraits<K>::digit(x, bsh) | (KeyTraits<K>::digit(x, bsh + RADIX_BITS) << RADIX_BITS);
+// Key bits [bsh, bsh + nb), nb <= 32 (bits above the key width read as 0).
+__device__ __forceinline__ uint32_t bits_of(uint64_t x, int bsh, int nb) {
+    return (uint32_t)(x >> bsh) & (uint32_t)((1ull << nb) - 1);
+}
+__device__ __forceinline__ uint32_t bits_of(const u128k& x, int bsh, int nb) {
+    uint64_t v;
+    if (bsh >= 64) v = x.hi >> (bsh - 64);
+    else v = (x.lo >> bsh) | (bsh ? x.hi << (64 - bsh) : 0ull);
+    return (uint32_t)v & (uint32_t)((1ull << nb) - 1);
 }
 __device__ __forceinline__ void or_diff(uint64_t& lo, uint64_t&, uint64_t a, uint64_t b) { lo |= a ^ b; }
 __device__ __forceinline__ void or_diff(uint64_t& lo, uint64_t& hi, const u128k& a, const u128k& b) {
     lo |= a.lo ^ b.lo; hi |= a.hi ^ b.hi;
 }
 
-// One wave per tile: ranges[t] = {a, n} (n = 0 when no bin starts in the window).
+// One wave per tile: ranges[t] = {a, n} (n = 0 when no bin starts in the
+// window); bins are key bits [bsh, bsh + bin_bits).  Tiles whose range holds
+// more than heavy_cap keys are appended to the heavy list (when it is given).
 template <typename K>
 __global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys,
-                                                   uint2* __restrict__ ranges, int k) {
-    constexpr uint32_t TILE = bc_tile<K>();
+                                                   uint2* __restrict__ ranges, int tile_keys, int bsh, int bin_bits,
+                                                   uint32_t heavy_cap, HeavyList hl) {
+    const uint32_t TILE = (uint32_t)tile_keys;
     const uint32_t lane = __lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / GKC_WAVE;
     const uint32_t nw = gridDim.x * blockDim.x / GKC_WAVE;
     const uint32_t total = plan->tot_rl_tiles, S = plan->n_segs;
-    const int bsh = 2 * k - 2 * RADIX_BITS;
     for (uint32_t t = wave; t < total; t += nw) {
         const uint32_t seg = find_seg(sa.rl_tile_start, S, t);
         const uint32_t ti = t - sa.rl_tile_start[seg];
@@ -594,10 +635,10 @@ __global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegA
         const uint32_t lo = ti * TILE, hi = min(lo + TILE, nseg);
         uint32_t a = lo;
         if (lo > 0) {
-            const uint32_t pb = bin_of(kp[lo - 1], bsh);
+            const uint32_t pb = bits_of(kp[lo - 1], bsh, bin_bits);
             for (;; a += GKC_WAVE) {
                 const uint32_t j = a + lane;
-                const bool stop = j >= hi || bin_of(kp[j], bsh) != pb;
+                const bool stop = j >= hi || bits_of(kp[j], bsh, bin_bits) != pb;
                 const uint64_t m = __ballot(stop);
                 if (m) { a += (uint32_t)__builtin_ctzll(m); break; }
             }
@@ -606,17 +647,20 @@ __global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegA
         if (a < hi) {
             uint32_t b = hi;
             if (hi < nseg) {
-                const uint32_t pb = bin_of(kp[hi - 1], bsh);
+                const uint32_t pb = bits_of(kp[hi - 1], bsh, bin_bits);
                 for (;; b += GKC_WAVE) {
                     const uint32_t j = b + lane;
-                    const bool stop = j >= nseg || bin_of(kp[j], bsh) != pb;
+                    const bool stop = j >= nseg || bits_of(kp[j], bsh, bin_bits) != pb;
                     const uint64_t m = __ballot(stop);
                     if (m) { b += (uint32_t)__builtin_ctzll(m); break; }
                 }
             }
             n = b - a;
         }
-        if (lane == 0) ranges[t] = make_uint2(a, n);
+        if (lane == 0) {
+            ranges[t] = make_uint2(a, n);
+            if (hl.tiles && n > heavy_cap) hl.tiles[atomicAdd(hl.count, 1u)] = t;
+        }
     }
 }
 
@@ -669,11 +713,16 @@ __device__ __forceinline__ uint32_t digit_scan(uint32_t v, uint32_t tid, uint32_
     return off + inc - v;
 }
 
-template <typename K>
+// STAGED: count the tiles of the heavy list instead (no look-back); the
+// kept entries of a range at a go to staging, keys at alt[a + i] and counts
+// as u32 over the range's own keys (they trail the reads: entry i never
+// overtakes key 2i), and {kept, distinct} to hl.res[tile].
+template <typename K, bool STAGED>
 __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl,
                                                     K* __restrict__ keys, K* __restrict__ alt,
                                                     const uint2* __restrict__ ranges, TableOut to,
-                                                    uint32_t* __restrict__ lookback, uint32_t* __restrict__ n_heavy) {
+                                                    uint32_t* __restrict__ lookback, uint32_t* __restrict__ n_heavy,
+                                                    HeavyList hl) {
     constexpr int ITEMS = BinTraits<K>::ITEMS;
     constexpr uint32_t CAP = bc_cap<K>();
     constexpr int NWAVE = BC_NT / GKC_WAVE;
@@ -697,8 +746,19 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
     for (;;) {
         if (tid == 0) {
             uint32_t sg = 0, ix = 0;
-            s_ok = claim<1>(wl, &plan->ctr[ctr_rl()], list, sg, ix);
-            s_seg = sg; s_idx = ix; s_nd = 0; s_out = 0; s_nh = 0;
+            bool ok;
+            if constexpr (STAGED) {
+                const uint32_t q = atomicAdd(hl.next, 1u);
+                ok = q < *hl.count;
+                if (ok) {
+                    const uint32_t t = hl.tiles[q];
+                    sg = find_seg(sa.rl_tile_start, plan->n_segs, t);
+                    ix = t - sa.rl_tile_start[sg];
+                }
+            } else {
+                ok = claim<1>(wl, &plan->ctr[ctr_rl()], list, sg, ix);
+            }
+            s_ok = ok; s_seg = sg; s_idx = ix; s_nd = 0; s_out = 0; s_nh = 0; s_excl = 0;
         }
         __syncthreads();
         if (!s_ok) break;
@@ -714,7 +774,7 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
         const bool heavy = n > CAP;
         K* src = keys + sa.key_start[seg] + rg.x;
         K* dst = alt + sa.key_start[seg] + rg.x;
-        if (heavy && tid == 0) {
+        if (!STAGED && heavy && tid == 0) {
             // base of a heavy range first: its total is known only at the end
             uint32_t excl = 0;
             if (ti > 0) {
@@ -1045,7 +1105,7 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
             }
             uint32_t ctot = 0;
             for (int w = 0; w < NCW; w++) ctot += s_red[w];
-            if (!heavy && tid == 0) {
+            if (!STAGED && !heavy && tid == 0) {
                 // 4. decoupled look-back for the tile's place in the segment's table
                 uint32_t* lb = lookback + tile;
                 uint32_t excl = 0;
@@ -1066,10 +1126,11 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
             }
             __syncthreads();
             // 5. write the kept (k-mer, count) entries
-            if (to.emit && ctot) {
-                const uint32_t cap = to.table_cap[seg];
-                K* tk = reinterpret_cast<K*>(to.keys) + to.table_start[seg];
-                uint32_t* tc = to.counts + to.table_start[seg];
+            if ((STAGED || to.emit) && ctot) {
+                const uint32_t cap = STAGED ? 0xFFFFFFFFu : to.table_cap[seg];
+                K* tk = STAGED ? alt + sa.key_start[seg] + rg.x : reinterpret_cast<K*>(to.keys) + to.table_start[seg];
+                uint32_t* tc = STAGED ? reinterpret_cast<uint32_t*>(keys + sa.key_start[seg] + rg.x)
+                                      : to.counts + to.table_start[seg];
                 const uint32_t base = s_excl + s_out;
 #pragma unroll
                 for (int i = 0; i < ITEMS; i++) {
@@ -1087,18 +1148,188 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
             if (tid == 0) s_out += ctot;
         }
         if (tid == 0) {
-            if (heavy) st_agent(lookback + tile, LB_INC | (s_excl + s_out));
-            if (s_nd) atomicAdd(&to.n_distinct[seg], s_nd);
-            if (s_out) atomicAdd(&to.n_entries[seg], s_out);
+            if constexpr (STAGED) {
+                hl.res[tile] = make_uint2(s_out, s_nd);
+            } else {
+                if (heavy) st_agent(lookback + tile, LB_INC | (s_excl + s_out));
+                if (s_nd) atomicAdd(&to.n_distinct[seg], s_nd);
+                if (s_out) atomicAdd(&to.n_entries[seg], s_out);
+            }
         }
         __syncthreads();
     }
 }
 
-template __global__ void k_binranges<uint64_t>(Plan*, SegArrays, const uint64_t*, uint2*, int);
-template __global__ void k_binranges<u128k>(Plan*, SegArrays, const u128k*, uint2*, int);
-template __global__ void k_bincount<uint64_t>(Plan*, SegArrays, WorkLists, uint64_t*, uint64_t*, const uint2*, TableOut, uint32_t*, uint32_t*);
-template __global__ void k_bincount<u128k>(Plan*, SegArrays, WorkLists, u128k*, u128k*, const uint2*, TableOut, uint32_t*, uint32_t*);
+// Wave-level binned count of 64-bit keys (see WC_TILE): every wave claims
+// batches of windows and, per window range [a, a + n):
+//   n <= WC_CAP: counts equal keys in its LDS hash table and ranks the kept
+//                distinct keys by comparison (their place in key order), then
+//                stages them like k_bincount<., true> (keys at alt[a + r],
+//                counts as u32 over the range's own keys, read already);
+//   n >  WC_CAP: takes {kept, distinct} k_bincount<., true> staged;
+// and records the window's kept count in hl.kept[tile].  The per-segment
+// distinct and kept totals are accumulated.  No window waits for another:
+// k_tilescan turns the kept counts into table offsets, k_wavecopy writes.
+__global__ __launch_bounds__(WC_NT) void k_wavecount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl,
+                                                     uint64_t* __restrict__ keys, uint64_t* __restrict__ alt,
+                                                     const uint2* __restrict__ ranges, HeavyList hl, TableOut to) {
+    constexpr int NW = WC_NT / GKC_WAVE;
+    constexpr int IT = WC_CAP / GKC_WAVE;
+    constexpr int LOG_CAP = __builtin_ctz(WC_CAP);
+    constexpr uint64_t EMPTY = ~0ull;     // never canonical: its reverse complement (0) is smaller
+    static_assert((WC_CAP & (WC_CAP - 1)) == 0, "table size");
+    __shared__ uint64_t s_tk[NW][WC_CAP];
+    __shared__ uint32_t s_tc[NW][WC_CAP];
+    const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE;
+    uint64_t* tk = s_tk[wv];
+    uint32_t* tc = s_tc[wv];
+    uint32_t list = (blockIdx.x * NW + wv) % wl.n_lists;
+    uint32_t acc_seg = 0xFFFFFFFFu, acc_d = 0, acc_m = 0;     // per-segment totals, flushed on change
+    uint32_t t = 0, t1 = 0;                                   // claimed batch [t, t1) of list `list`
+    for (;;) {
+        uint32_t seg = 0, ti = 0, ok = 1;
+        if (t >= t1) {
+            uint32_t a0 = 0, a1 = 0;
+            if (lane == 0) ok = claim_batch<1>(wl, &plan->ctr[ctr_rl()], list, WC_BATCH, a0, a1) ? 1u : 0u;
+            ok = __shfl(ok, 0); list = __shfl(list, 0); t = __shfl(a0, 0); t1 = __shfl(a1, 0);
+        }
+        if (ok) { unit_at<1>(wl, list, t, seg, ti); t++; }
+        if (seg != acc_seg || !ok) {
+            if (lane == 0 && acc_seg != 0xFFFFFFFFu) {
+                if (acc_d) atomicAdd(&to.n_distinct[acc_seg], acc_d);
+                if (acc_m) atomicAdd(&to.n_entries[acc_seg], acc_m);
+            }
+            acc_seg = seg; acc_d = 0; acc_m = 0;
+        }
+        if (!ok) break;
+        const uint32_t tile = sa.rl_tile_start[seg] + ti;
+        const uint2 rg = ranges[tile];
+        const uint32_t n = rg.y;
+        uint64_t* kp = keys + sa.key_start[seg] + rg.x;
+        uint64_t* sk = alt + sa.key_start[seg] + rg.x;
+        uint32_t* sc = reinterpret_cast<uint32_t*>(kp);
+        uint32_t d = 0, m = 0;
+        if (n > (uint32_t)WC_CAP) {
+            const uint2 hr = hl.res[tile];
+            m = hr.x; d = hr.y;
+        } else if (n) {
+            // hash-count the range in the wave's table
+            uint64_t x[IT];
+#pragma unroll
+            for (int i = 0; i < IT; i++) {
+                const uint32_t j = i * GKC_WAVE + lane;
+                x[i] = (i * GKC_WAVE < (int)n && j < n) ? kp[j] : EMPTY;
+                tk[j] = EMPTY; tc[j] = 0;
+            }
+#pragma unroll
+            for (int i = 0; i < IT; i++) {
+                if (x[i] == EMPTY) continue;
+                const unsigned long long xv = x[i];
+                uint32_t h = (uint32_t)((xv * 0x9E3779B97F4A7C15ull) >> (64 - LOG_CAP));
+                for (;;) {     // n <= WC_CAP slots: a free or matching slot exists
+                    const unsigned long long old =
+                        atomicCAS(reinterpret_cast<unsigned long long*>(&tk[h]), (unsigned long long)EMPTY, xv);
+                    if (old == EMPTY || old == xv) { atomicAdd(&tc[h], 1u); break; }
+                    h = (h + 1) & (WC_CAP - 1);
+                }
+            }
+            // compact the kept distinct keys to the front of the table
+#pragma unroll
+            for (int i = 0; i < IT; i++) {
+                const uint64_t key = tk[i * GKC_WAVE + lane];
+                const uint32_t c = tc[i * GKC_WAVE + lane];
+                const bool occ = key != EMPTY;
+                const bool keep = occ && c >= to.min_cnt && (to.max_cnt == 0 || c <= to.max_cnt);
+                d += (uint32_t)__popcll(__ballot(occ));
+                const uint64_t kb = __ballot(keep);
+                x[i] = key;
+                if (keep) {
+                    const uint32_t p = m + (uint32_t)__popcll(kb & lanemask_lt());
+                    tk[p] = key; tc[p] = c;     // p <= this slot's index: reads of later slots are not disturbed
+                }
+                m += (uint32_t)__popcll(kb);
+            }
+            // rank of each kept key among the kept keys = its place in key order; stage
+#pragma unroll
+            for (int i = 0; i < IT; i++) {
+                if (i * GKC_WAVE >= (int)m) continue;
+                const uint32_t e = i * GKC_WAVE + lane;
+                const uint64_t key = e < m ? tk[e] : EMPTY;
+                uint32_t r = 0;
+                for (uint32_t q = 0; q < m; q++) r += tk[q] < key ? 1u : 0u;
+                if (e < m) { sk[r] = key; sc[r] = tc[e]; }
+            }
+        }
+        if (lane == 0) hl.kept[tile] = m;
+        acc_d += d; acc_m += m;
+    }
+}
+
+// In-place exclusive scan of the kept counts of each segment's windows (one
+// workgroup per segment): window offsets within the segment's table.
+__global__ __launch_bounds__(1024) void k_tilescan(Plan* __restrict__ plan, SegArrays sa, uint32_t* __restrict__ kept) {
+    __shared__ uint32_t s_w[1024 / GKC_WAVE];
+    const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE;
+    for (uint32_t seg = blockIdx.x; seg < plan->n_segs; seg += gridDim.x) {
+        const uint32_t b = sa.rl_tile_start[seg], e = sa.rl_tile_start[seg + 1];
+        uint32_t carry = 0;
+        for (uint32_t base = b; base < e; base += 1024) {
+            const uint32_t t = base + threadIdx.x;
+            const uint32_t v = t < e ? kept[t] : 0u;
+            uint32_t inc = v;
+#pragma unroll
+            for (int off = 1; off < GKC_WAVE; off <<= 1) {
+                const uint32_t y = __shfl_up(inc, off);
+                if ((int)lane >= off) inc += y;
+            }
+            if (lane == GKC_WAVE - 1) s_w[wv] = inc;
+            __syncthreads();
+            uint32_t off = 0, tot = 0;
+#pragma unroll
+            for (int w = 0; w < 1024 / GKC_WAVE; w++) { const uint32_t x = s_w[w]; off += (uint32_t)w < wv ? x : 0u; tot += x; }
+            if (t < e) kept[t] = carry + off + inc - v;
+            carry += tot;
+            __syncthreads();
+        }
+    }
+}
+
+// Write each window's staged entries at its table offset (one wave per
+// window): the binned count's emission; repeatable with other capacities.
+__global__ __launch_bounds__(256) void k_wavecopy(Plan* __restrict__ plan, SegArrays sa, const uint64_t* __restrict__ keys,
+                                                  const uint64_t* __restrict__ alt, const uint2* __restrict__ ranges,
+                                                  HeavyList hl, TableOut to) {
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / GKC_WAVE;
+    const uint32_t nw = gridDim.x * blockDim.x / GKC_WAVE;
+    const uint32_t total = plan->tot_rl_tiles, S = plan->n_segs;
+    uint32_t seg = 0;
+    for (uint32_t t = wave; t < total; t += nw) {
+        while (seg + 1 < S && sa.rl_tile_start[seg + 1] <= t) seg++;      // t only grows
+        const uint32_t last = sa.rl_tile_start[seg + 1] - 1;
+        const uint32_t off = hl.kept[t];
+        const uint32_t end = t < last ? hl.kept[t + 1] : to.n_entries[seg];
+        const uint32_t m = end - off;
+        if (!m) continue;
+        const uint32_t a = ranges[t].x;
+        const uint64_t* sk = alt + sa.key_start[seg] + a;
+        const uint32_t* sc = reinterpret_cast<const uint32_t*>(keys + sa.key_start[seg] + a);
+        const uint32_t cap = to.table_cap[seg];
+        uint64_t* okey = reinterpret_cast<uint64_t*>(to.keys) + to.table_start[seg];
+        uint32_t* ocnt = to.counts + to.table_start[seg];
+        for (uint32_t e = lane; e < m; e += GKC_WAVE) {
+            const uint32_t idx = off + e;
+            if (idx < cap) { okey[idx] = sk[e]; ocnt[idx] = sc[e]; }
+            else atomicOr(to.overflow, 1u);
+        }
+    }
+}
+
+template __global__ void k_binranges<uint64_t>(Plan*, SegArrays, const uint64_t*, uint2*, int, int, int, uint32_t, HeavyList);
+template __global__ void k_binranges<u128k>(Plan*, SegArrays, const u128k*, uint2*, int, int, int, uint32_t, HeavyList);
+template __global__ void k_bincount<uint64_t, false>(Plan*, SegArrays, WorkLists, uint64_t*, uint64_t*, const uint2*, TableOut, uint32_t*, uint32_t*, HeavyList);
+template __global__ void k_bincount<uint64_t, true>(Plan*, SegArrays, WorkLists, uint64_t*, uint64_t*, const uint2*, TableOut, uint32_t*, uint32_t*, HeavyList);
+template __global__ void k_bincount<u128k, false>(Plan*, SegArrays, WorkLists, u128k*, u128k*, const uint2*, TableOut, uint32_t*, uint32_t*, HeavyList);
 
 template __global__ void k_extract<uint64_t>(Plan*, SegArrays, WorkLists, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, uint64_t*, int);
 template __global__ void k_extract<u128k>(Plan*, SegArrays, WorkLists, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, u128k*, int);

@@ -1,5 +1,7 @@
-"""Binned counting (batched form): two onesweep passes on the top 16 key bits,
-then an LDS sort + run-length count per tile of whole bins.  Bit-exact
+"""Binned counting (batched form): onesweep passes on the top 24 (64-bit keys)
+or 16 (128-bit keys) key bits, then per window of whole bins a count in LDS
+(wave-level hash count for 64-bit keys, workgroup LDS sort + run-length count
+for 128-bit keys).  Bit-exact
 against the oracle and against the full LSD sort (count_mode 1), including
 the edges of the tiling: bins straddling tile windows, ranges of one key or
 of one repeated key, ranges too large for LDS (sorted and counted in chunks
@@ -60,7 +62,7 @@ def test_binned_matches_oracle_and_lsd(gpu, cid, scale, k, p, P):
     ot = O.count(rs.reads, rs.offs, k, p, P)
     _say("binned", w.name, k, p, P)
     gt, st = _count(rs.reads, rs.offs, k, p, P, mode=0)
-    assert st["binned"] == 1 and st["n_pass"] == 2
+    assert st["binned"] == 1 and st["n_pass"] == (min(3, (2 * k + 7) // 8) if k <= 32 else 2)
     assert_tables_equal(gt, ot, P, k)
     _say("lsd", w.name, k, p, P)
     lt, sl = _count(rs.reads, rs.offs, k, p, P, mode=1)
@@ -122,7 +124,9 @@ def test_heavy_range_distinct_kmers(gpu, minc, nrand):
     ot = O.count(reads, offs, 31, 11, 1, min_count=minc)
     _say("heavy distinct", minc)
     gt, st = _count(reads, offs, 31, 11, 1, mode=0, min_count=minc)
-    assert st["binned"] == 1 and st["heavy_ranges"] >= 1
+    assert st["binned"] == 1
+    if len(prefix) >= 12:       # the whole 24-bit bin is the fixed prefix: one range of ~40 k keys
+        assert st["heavy_ranges"] >= 1
     assert_tables_equal(gt, ot, 1, 31)
 
 
