@@ -35,6 +35,22 @@ def _log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload: str, items: int):
+    """HBM bytes per onesweep launch from the committed rocprofv3 counter runs
+    (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel's readings) of the same
+    workload and kernel variant, or (None, None)."""
+    import glob
+    import re
+    here = os.path.dirname(os.path.abspath(__file__))
+    for f in sorted(glob.glob(os.path.join(here, "profiles", "*traffic*.md"))):
+        m = re.search(r"<!-- traffic-json: (\{.*\}) -->", open(f).read())
+        if m:
+            d = json.loads(m.group(1))
+            if d.get("workload") == workload and d.get("items") == items:
+                return int(d["traffic_bytes"]), os.path.relpath(f, here)
+    return None, None
+
+
 def learned_report(predictor, ctx, part_distinct, part_kmers) -> dict:
     """Predicted vs exact per-partition counts of the last batched run (the
     bins still hold its superkmers and its sketch)."""
@@ -176,8 +192,13 @@ def main() -> int:
     s_w = 4.0 * st["n_words"] / N
     h_b = 16.0 * st["n_superkmers"] / N
     delta = st["n_distinct"] / N
-    A = r + (s_w + h_b) + (s_w + h_b + W_key) + st["n_pass"] * 2 * W_key + (W_key + (W_key + 4) * delta)
+    # partition (ASCII in, 2-bit superkmers + headers out), extraction (bins in,
+    # keys out), onesweep passes (read + write), count (keys in, entries out)
+    # and, binned, the copy of the staged entries into the table
+    A = (r + (s_w + h_b) + (s_w + h_b + W_key) + st["n_pass"] * 2 * W_key + (W_key + (W_key + 4) * delta)
+         + (2 * (W_key + 4) * delta if st["binned"] else 0.0))
     roof_kmers = HBM_PEAK_GBS * 1e9 / A
+    traffic, traffic_src = measured_traffic(w.name, ctx.get_tuning()["sort_items"])
 
     line = {
         "metric": METRIC,
@@ -197,7 +218,8 @@ def main() -> int:
                                    / roof_kmers)},
         "roofline": {"kernel": "k_onesweep<u64>" if W_key == 8 else "k_onesweep<u128k>", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_launch": pass_bytes, "avg_launch_ms": pass_ms},
+                     "traffic": traffic, "traffic_source": traffic_src, "bytes_per_launch": pass_bytes,
+                     "avg_launch_ms": pass_ms},
         "stages_instrumented": stages,
         "full_size_invariants_ok": full_ok,
         "learned": learned_report(predictor, ctx, pd, pk) if predictor is not None else None,
