@@ -105,6 +105,8 @@ struct gkc_ctx {
     DevBuf keysA, keysB, lookback, d_plan, d_segk, d_segh, d_keystart, d_itemstart, d_tilestart, d_rlstart, d_hist;
     DevBuf d_tstart, d_tcap, d_nent, d_ndist, d_toverflow;
     DevBuf d_ranges;                           // binned-count tile ranges {a, n}
+    DevBuf d_tiler0;                           // read holding the first position of each phase-1 tile
+    DevBuf d_chunk_s;                          // descriptor of a chunk's sample (first-batch presizing)
     DevBuf d_hlist, d_hres;                    // heavy tiles of a wave-level count and their {kept, distinct}
     DevBuf d_wkept;                            // kept entries per window, then window offsets
     DevBuf tkeys, tcnts;
@@ -284,13 +286,19 @@ static uint64_t stage_batch(gkc_ctx* c, const char* reads, const uint64_t* offs,
         cd.reads = c->d_hpc_reads.as<uint8_t>();
         cd.lens = c->d_lens.as<int32_t>();
     }
+    c->d_tiler0.ensure(((size_t)cd.n_tiles + 1) * 4);
+    cd.tile_r0 = c->d_tiler0.as<uint32_t>();
+    hipLaunchKernelGGL(k_tile_reads, dim3(std::max<uint64_t>(1, (n + 255) / 256)), dim3(256), 0, c->st, doffs,
+                       (uint32_t)n, cd.base, cd.n_tiles, c->d_tiler0.as<uint32_t>());
     c->d_chunk.ensure(sizeof(ChunkDesc));
     hipLaunchKernelGGL(k_set_chunk, dim3(1), dim3(1), 0, c->st, c->d_chunk.as<ChunkDesc>(), cd);
     HIPX(hipGetLastError());
     return B;
 }
 
-static void launch_superkmers(gkc_ctx* c, const ChunkDesc& cd, P1Params prm, BinSet bs, bool persistent = false) {
+static void launch_superkmers(gkc_ctx* c, const ChunkDesc& cd, P1Params prm, BinSet bs, bool persistent = false,
+                              const ChunkDesc* dcd = nullptr) {
+    if (!dcd) dcd = c->d_chunk.as<ChunkDesc>();
     c->d_ctr.ensure(64);
     zero(c, c->d_ctr.p, 1);
     if (cd.n_tiles == 0 && !persistent) return;
@@ -301,11 +309,11 @@ static void launch_superkmers(gkc_ctx* c, const ChunkDesc& cd, P1Params prm, Bin
     int grid = occupancy_blocks(fn, P1_NT, c->n_cu);
     if (!persistent) grid = std::min<int>(std::max<uint32_t>(cd.n_tiles, 1), grid);
     if (win)
-        hipLaunchKernelGGL((k_superkmers<uint64_t, true>), dim3(grid), dim3(P1_NT), 0, c->st, c->d_chunk.as<ChunkDesc>(), prm, bs, c->d_ctr.as<uint32_t>());
+        hipLaunchKernelGGL((k_superkmers<uint64_t, true>), dim3(grid), dim3(P1_NT), 0, c->st, dcd, prm, bs, c->d_ctr.as<uint32_t>());
     else if (wide)
-        hipLaunchKernelGGL((k_superkmers<uint64_t, false>), dim3(grid), dim3(P1_NT), 0, c->st, c->d_chunk.as<ChunkDesc>(), prm, bs, c->d_ctr.as<uint32_t>());
+        hipLaunchKernelGGL((k_superkmers<uint64_t, false>), dim3(grid), dim3(P1_NT), 0, c->st, dcd, prm, bs, c->d_ctr.as<uint32_t>());
     else
-        hipLaunchKernelGGL((k_superkmers<uint32_t, false>), dim3(grid), dim3(P1_NT), 0, c->st, c->d_chunk.as<ChunkDesc>(), prm, bs, c->d_ctr.as<uint32_t>());
+        hipLaunchKernelGGL((k_superkmers<uint32_t, false>), dim3(grid), dim3(P1_NT), 0, c->st, dcd, prm, bs, c->d_ctr.as<uint32_t>());
     HIPX(hipGetLastError());
 }
 
@@ -544,18 +552,38 @@ __global__ void k_restore_u32(const uint32_t* __restrict__ src, uint32_t* __rest
 }
 
 // Batched phase 1 of one batch: bins presized from the per-partition shares of
-// the batches already partitioned, a single scatter pass replayed from a
-// captured graph (the chunk descriptor lives in device memory), one host
-// sync.  Returns false when there is no history yet or a bin overflowed (the
-// cursors are then restored and the caller runs the plain form).
+// the batches already partitioned (for the first batch, from a count pass
+// over its first 1/16: reads come in no genome order, so that is a random
+// sample), a single scatter pass replayed from a captured graph (the chunk
+// descriptor lives in device memory), one host sync.  Returns false when a
+// bin overflowed (the cursors are then restored and the caller runs the
+// plain form).
 static bool partition_presized(gkc_ctx* c, const ChunkDesc& cd) {
     const uint32_t P = c->prm.n_parts;
-    if (c->bases_done == 0) return false;
-    const double f = (double)cd.n_bases / (double)c->bases_done;
     std::vector<uint32_t> nw(P), nh(P);
-    for (uint32_t q = 0; q < P; q++) {
-        nw[q] = (uint32_t)std::min<double>(std::ceil(c->h_word_cur[q] * f * 1.25) + 256, 4.0e9);
-        nh[q] = (uint32_t)std::min<double>(std::ceil(c->h_hdr_cur[q] * f * 1.25) + 64, 4.0e9);
+    if (c->bases_done == 0) {
+        ChunkDesc sc = cd;
+        sc.n_tiles = std::max<uint32_t>(1, cd.n_tiles / 16);
+        sc.n_bases = std::min<uint32_t>(cd.n_bases, sc.n_tiles * (uint32_t)P1_T);
+        c->d_chunk_s.ensure(sizeof(ChunkDesc));
+        hipLaunchKernelGGL(k_set_chunk, dim3(1), dim3(1), 0, c->st, c->d_chunk_s.as<ChunkDesc>(), sc);
+        zero(c, c->d_tmp_hdr.p, P); zero(c, c->d_tmp_word.p, P); zero(c, c->d_tmp_kmer.p, P);
+        launch_superkmers(c, sc, p1params(c, 0), binset(c, true), false, c->d_chunk_s.as<ChunkDesc>());
+        std::vector<uint32_t> sw(P), sh(P);
+        HIPX(hipMemcpyAsync(sw.data(), c->d_tmp_word.p, P * 4, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipMemcpyAsync(sh.data(), c->d_tmp_hdr.p, P * 4, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipStreamSynchronize(c->st));
+        const double f = (double)cd.n_bases / (double)sc.n_bases;
+        for (uint32_t q = 0; q < P; q++) {
+            nw[q] = (uint32_t)std::min<double>(std::ceil(sw[q] * f * 1.25) + 1024, 4.0e9);
+            nh[q] = (uint32_t)std::min<double>(std::ceil(sh[q] * f * 1.25) + 256, 4.0e9);
+        }
+    } else {
+        const double f = (double)cd.n_bases / (double)c->bases_done;
+        for (uint32_t q = 0; q < P; q++) {
+            nw[q] = (uint32_t)std::min<double>(std::ceil(c->h_word_cur[q] * f * 1.25) + 256, 4.0e9);
+            nh[q] = (uint32_t)std::min<double>(std::ceil(c->h_hdr_cur[q] * f * 1.25) + 64, 4.0e9);
+        }
     }
     bins_reserve(c, nw, nh);
     c->d_backup.ensure((size_t)3 * P * 4);
@@ -608,7 +636,7 @@ int gkc_partition(gkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n
         bool done = false, sketched = false;
         if ((flags & GKC_BATCHED) && !(flags & GKC_EMIT_WINDOWS)) {
             done = partition_presized(c, cd);
-            sketched = !done && c->bases_done > 0;   // a failed attempt already sketched this chunk
+            sketched = !done;                        // a failed attempt already sketched this chunk
         }
         if (!done) partition_plain(c, cd, flags, sketched);
         c->bases_done += cd.n_bases - cd.lead;

@@ -14,6 +14,7 @@ struct ChunkDesc {
     const uint8_t* reads;
     const uint64_t* offs;     // n_reads + 1
     const int32_t* lens;      // optional per-read valid length (HPC), else null
+    const uint32_t* tile_r0;  // [n_tiles] read holding each tile's first position (0 in the lead)
     uint64_t base;            // offs[0] - lead
     uint32_t n_reads;
     uint32_t n_bases;         // offs[n] - offs[0]
@@ -26,7 +27,7 @@ struct SkmHdr {
     uint32_t word_off;   // partition-local offset of its first 2-bit word
     uint32_t kmer_off;   // partition-local index of its first k-mer
     uint32_t n_kmers;    // number of k-mers (windows) it carries
-    uint32_t mm_lo;      // low 32 bits of its minimizer (diagnostics / sketch)
+    uint32_t reserved;   // 0
 };
 
 struct u128k {           // 128-bit key: hi holds the most significant bits

@@ -38,6 +38,8 @@ struct P1Params {
 
 template <typename MM, bool WIN>
 __global__ void k_superkmers(const ChunkDesc* __restrict__ cdp, P1Params prm, BinSet bins, uint32_t* __restrict__ tile_ctr);
+__global__ void k_tile_reads(const uint64_t* __restrict__ offs, uint32_t n_reads, uint64_t base, uint32_t n_tiles,
+                             uint32_t* __restrict__ tile_r0);
 __global__ void k_hpc(const uint8_t* __restrict__ reads, const uint64_t* __restrict__ offs, uint32_t n_reads,
                       uint8_t* __restrict__ out, int32_t* __restrict__ lens, int32_t* __restrict__ orig_pos);
 __global__ void k_skm_lists(const uint64_t* __restrict__ offs, const int32_t* __restrict__ lens, uint32_t n_reads, int k,

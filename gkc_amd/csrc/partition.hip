@@ -24,12 +24,26 @@ __device__ __forceinline__ uint32_t find_read(const uint64_t* offs, uint32_t n, 
     return lo;
 }
 
+// Read holding the first position of each tile (one thread per read; a read
+// covering positions [s, e) of the chunk owns the tiles whose first position
+// lies in it).  Tile 0 may start in the alignment lead, before read 0.
+__global__ __launch_bounds__(256) void k_tile_reads(const uint64_t* __restrict__ offs, uint32_t n_reads, uint64_t base,
+                                                    uint32_t n_tiles, uint32_t* __restrict__ tile_r0) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r == 0) tile_r0[0] = 0;
+    if (r >= n_reads) return;
+    const uint64_t s = offs[r] - base, e = offs[r + 1] - base;
+    for (uint64_t t = (s + P1_T - 1) / P1_T; t * P1_T < e && t < n_tiles; t++) tile_r0[t] = r;
+}
+
+constexpr uint32_t P1_RO = 512;     // read offsets a tile stages in LDS (more: global search)
+
 // With WIN, every candidate carries its position: the window minimum runs
 // over composites (value << 14 | no-candidate << 13 | 8191 - start), so ties
 // go to the later p-mer start (the reference's update order, gkc_cuda.cu:281-284)
 // and the chosen start and strand are recovered per window.
 template <typename MM, bool WIN>
-__global__ __launch_bounds__(P1_NT) void k_superkmers(const ChunkDesc* __restrict__ cdp, P1Params prm,
+__global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __restrict__ cdp, P1Params prm,
                                                        BinSet bins, uint32_t* __restrict__ tile_ctr) {
     constexpr int CSH = WIN ? 14 : 0;
     __shared__ int8_t s_str[WIN ? P1_T + MAXK : 1];
@@ -39,10 +53,12 @@ __global__ __launch_bounds__(P1_NT) void k_superkmers(const ChunkDesc* __restric
     __shared__ uint16_t s_start[P1_T];
     __shared__ uint16_t s_wend[P1_T];
     __shared__ uint16_t s_part[P1_T];
-    __shared__ uint32_t s_mmlo[P1_T];
-    __shared__ uint32_t s_scan[P1_NT];
-    __shared__ unsigned long long s_loc[1024];   // per partition: [63:48] headers, [47:24] k-mers, [23:0] words
-    __shared__ uint32_t s_gh[1024], s_gk[1024], s_gw[1024];
+    __shared__ uint32_t s_scan[P1_NT / GKC_WAVE];
+    __shared__ uint32_t s_ro[P1_RO];              // offsets (chunk positions) of reads r0 .. r0 + n_ro - 1
+    // per partition: block-local [63:48] headers, [47:24] k-mers, [23:0] words;
+    // then the block's global bases [63:32] words, [31:0] headers
+    __shared__ unsigned long long s_loc[1024];
+    __shared__ uint32_t s_gk[1024];
     __shared__ uint32_t s_tile, s_ns, s_ovf;
 
     const ChunkDesc cd = *cdp;
@@ -52,14 +68,21 @@ __global__ __launch_bounds__(P1_NT) void k_superkmers(const ChunkDesc* __restric
     const MM MC = WIN ? (MM)((M << CSH) | (MM)0x3FFF) : M;     // largest composite
     const uint32_t P = prm.P;
 
+    if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    __syncthreads();
     for (;;) {
-        if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
-        __syncthreads();
         const uint32_t tile = s_tile;
         if (tile >= cd.n_tiles) break;
+        uint32_t next_tile = 0;
+        if (tid == 0) next_tile = atomicAdd(tile_ctr, 1u);      // claimed now, read at the end of this tile
         const uint32_t t0 = tile * P1_T;
         const uint32_t tlen = min((uint32_t)P1_T, cd.n_bases - t0);      // window starts owned
         const uint32_t blen = min((uint32_t)(P1_T + MAXK + 32), cd.n_bases - t0);  // bytes staged
+        // reads holding this tile's window starts: r0 .. r1 (r1 holds the next tile's first position)
+        const uint32_t r0 = cd.tile_r0[tile];
+        const uint32_t r1 = tile + 1 < cd.n_tiles ? cd.tile_r0[tile + 1] : cd.n_reads - 1;
+        const uint32_t n_ro = r1 - r0 + 2;                                 // offsets r0 .. r1 + 1
+        const bool ro_lds = n_ro <= P1_RO;
 
         // 1. stage bytes [t0, t0+blen) in LDS with 16-byte loads (t0 is 16-aligned)
         {
@@ -71,6 +94,8 @@ __global__ __launch_bounds__(P1_NT) void k_superkmers(const ChunkDesc* __restric
                 s_bytes[i] = (i < blen) ? cd.reads[t0 + i] : (uint8_t)'A';
             for (int i = tid; i < 1024 && i < (int)P; i += P1_NT) s_loc[i] = 0;
             if (tid == 0) s_ovf = 0;
+            if (ro_lds)
+                for (uint32_t i = tid; i < n_ro; i += P1_NT) s_ro[i] = (uint32_t)(cd.offs[r0 + i] - cd.base);
         }
         __syncthreads();
 
@@ -156,9 +181,19 @@ __global__ __launch_bounds__(P1_NT) void k_superkmers(const ChunkDesc* __restric
             uint32_t r = 0;
             uint64_t rbeg = 0, rnext = 0, rend = 0;
             if (tid * P1_V < (int)tlen) {
-                r = find_read(cd.offs, cd.n_reads, cd.base + max(g0, cd.lead));
-                rbeg = cd.offs[r] - cd.base;
-                rnext = cd.offs[r + 1] - cd.base;
+                const uint32_t gs = max(g0, cd.lead);
+                if (ro_lds) {           // largest i with s_ro[i] <= gs, in LDS
+                    uint32_t lo = 0, hi = n_ro - 1;
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (s_ro[mid] <= gs) lo = mid; else hi = mid;
+                    }
+                    r = r0 + lo;
+                } else {
+                    r = find_read(cd.offs, cd.n_reads, cd.base + gs);
+                }
+                rbeg = ro_lds ? s_ro[r - r0] : cd.offs[r] - cd.base;
+                rnext = ro_lds ? s_ro[r - r0 + 1] : cd.offs[r + 1] - cd.base;
                 rend = cd.lens ? rbeg + (uint32_t)cd.lens[r] : rnext;
             }
             MM prev = tid > 0 ? s_last[tid - 1] : (MM)0;
@@ -175,7 +210,7 @@ __global__ __launch_bounds__(P1_NT) void k_superkmers(const ChunkDesc* __restric
                     while (g >= rnext) {            // advance (skips empty reads)
                         r++;
                         rbeg = rnext;
-                        rnext = cd.offs[r + 1] - cd.base;
+                        rnext = ro_lds ? s_ro[r - r0 + 1] : cd.offs[r + 1] - cd.base;
                         rend = cd.lens ? rbeg + (uint32_t)cd.lens[r] : rnext;
                     }
                     const bool valid = g >= rbeg && (uint64_t)g + k <= rend;
@@ -207,26 +242,28 @@ __global__ __launch_bounds__(P1_NT) void k_superkmers(const ChunkDesc* __restric
             }
         }
 
-        // 5. compact the superkmer starts in tile order (block scan of counts)
+        // 5. compact the superkmer starts in tile order (wave scans + wave sums)
         {
             const uint32_t c = __popc(bmask);
-            s_scan[tid] = c;
-            __syncthreads();
-            for (int off = 1; off < P1_NT; off <<= 1) {
-                uint32_t x = tid >= off ? s_scan[tid - off] : 0;
-                __syncthreads();
-                s_scan[tid] += x;
-                __syncthreads();
+            const uint32_t lane = __lane_id(), wv = tid / GKC_WAVE;
+            uint32_t inc = c;
+#pragma unroll
+            for (int off = 1; off < GKC_WAVE; off <<= 1) {
+                const uint32_t y = __shfl_up(inc, off);
+                if ((int)lane >= off) inc += y;
             }
-            uint32_t o = s_scan[tid] - c;
-            if (tid == P1_NT - 1) s_ns = s_scan[tid];
+            if (lane == GKC_WAVE - 1) s_scan[wv] = inc;
+            __syncthreads();
+            uint32_t o = inc - c, tot = 0;
+#pragma unroll
+            for (int w2 = 0; w2 < P1_NT / GKC_WAVE; w2++) { const uint32_t x = s_scan[w2]; o += (uint32_t)w2 < wv ? x : 0u; tot += x; }
+            if (tid == 0) s_ns = tot;
 #pragma unroll
             for (int s = 0; s < P1_V; s++) {
                 if (bmask & (1u << s)) {
                     s_start[o] = (uint16_t)(tid * P1_V + s);
                     s_wend[o] = (uint16_t)wend[s];
                     s_part[o] = (uint16_t)part_of(mm[s], P, prm.pmode);
-                    s_mmlo[o] = (uint32_t)mm[s];
                     o++;
                 }
             }
@@ -263,7 +300,8 @@ __global__ __launch_bounds__(P1_NT) void k_superkmers(const ChunkDesc* __restric
                     s_ovf = 1;
                     atomicOr(bins.overflow, 1u);
                 }
-                s_gh[q] = gh; s_gw[q] = gw; s_gk[q] = gk;
+                s_loc[q] = ((unsigned long long)gw << 32) | gh;
+                s_gk[q] = gk;
             }
         }
         __syncthreads();
@@ -280,13 +318,14 @@ __global__ __launch_bounds__(P1_NT) void k_superkmers(const ChunkDesc* __restric
                     const uint32_t nb = n + k - 1;
                     const uint32_t nw = (nb + 15) >> 4;
                     const uint32_t part = s_part[i];
-                    const uint32_t wo = s_gw[part] + (uint32_t)(loc[q] & 0xFFFFFFu);
+                    const unsigned long long gb = s_loc[part];
+                    const uint32_t wo = (uint32_t)(gb >> 32) + (uint32_t)(loc[q] & 0xFFFFFFu);
                     SkmHdr h;
                     h.word_off = wo;
                     h.kmer_off = s_gk[part] + (uint32_t)((loc[q] >> 24) & 0xFFFFFFu);
                     h.n_kmers = n;
-                    h.mm_lo = s_mmlo[i];
-                    bins.hdrs[bins.hdr_base[part] + s_gh[part] + (uint32_t)(loc[q] >> 48)] = h;
+                    h.reserved = 0;
+                    bins.hdrs[bins.hdr_base[part] + (uint32_t)gb + (uint32_t)(loc[q] >> 48)] = h;
                     uint32_t* dst = bins.words + bins.word_base[part] + wo;
                     for (uint32_t wi = 0; wi < nw; wi++) {
                         uint32_t x = 0;
@@ -301,6 +340,7 @@ __global__ __launch_bounds__(P1_NT) void k_superkmers(const ChunkDesc* __restric
                 }
             }
         }
+        if (tid == 0) s_tile = next_tile;
         __syncthreads();
     }
 }
