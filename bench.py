@@ -98,7 +98,7 @@ def main() -> int:
     w = W.workload(args.config, scale=args.scale)
     t0 = time.time()
     rs = W.generate(w)
-    N = rs.kmers(w.k)
+    N = rs.kmers(w.k, hpc=w.hpc)
     _log(f"{w.name}: {rs.n} reads, {rs.bases} bases, {N} k-mers (generated in {time.time() - t0:.1f}s)")
     if w.chunked:
         chunks = W.chunk_bounds(rs, seed=W.genome_seed(w.config_id) + 777)

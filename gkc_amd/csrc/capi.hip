@@ -260,7 +260,6 @@ static uint64_t stage_batch(gkc_ctx* c, const char* reads, const uint64_t* offs,
         lead = (uint32_t)(reinterpret_cast<uintptr_t>(reads) & 15);
         dreads = reinterpret_cast<const uint8_t*>(reads) - lead;
         doffs = offs;
-        if (c->prm.hpc && lead) throw ArgError{"HPC needs 16-byte aligned device reads", GKC_EINVAL};
     } else {
         o0 = offs[0]; on = offs[n];
         const uint64_t B = on - o0;
@@ -276,12 +275,14 @@ static uint64_t stage_batch(gkc_ctx* c, const char* reads, const uint64_t* offs,
     cd.reads = dreads; cd.offs = doffs; cd.lens = nullptr; cd.base = o0 - lead;
     cd.n_reads = (uint32_t)n; cd.n_bases = (uint32_t)(B + lead); cd.n_tiles = cdiv(B + lead, P1_T); cd.lead = lead;
     if (c->prm.hpc && n > 0) {
-        c->d_hpc_reads.ensure(B + 64);
+        // compressed reads keep the input's layout, lead included: read r's runs
+        // start at the same position, so the tiles index them like the input
+        c->d_hpc_reads.ensure(B + lead + 64);
         c->d_lens.ensure((n + 1) * 4);
         c->d_hpc_pos.ensure((B + 1) * 4);
         int blocks = std::min<int64_t>((n + 3) / 4, 65535);
-        hipLaunchKernelGGL(k_hpc, dim3(blocks), dim3(256), 0, c->st, dreads, doffs, (uint32_t)n,
-                           c->d_hpc_reads.as<uint8_t>(), c->d_lens.as<int32_t>(), c->d_hpc_pos.as<int32_t>());
+        hipLaunchKernelGGL(k_hpc, dim3(blocks), dim3(256), 0, c->st, dreads + lead, doffs, (uint32_t)n,
+                           c->d_hpc_reads.as<uint8_t>() + lead, c->d_lens.as<int32_t>(), c->d_hpc_pos.as<int32_t>());
         HIPX(hipGetLastError());
         cd.reads = c->d_hpc_reads.as<uint8_t>();
         cd.lens = c->d_lens.as<int32_t>();

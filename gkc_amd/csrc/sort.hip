@@ -615,20 +615,26 @@ __device__ __forceinline__ void or_diff(uint64_t& lo, uint64_t& hi, const u128k&
     lo |= a.lo ^ b.lo; hi |= a.hi ^ b.hi;
 }
 
-// One wave per tile: ranges[t] = {a, n} (n = 0 when no bin starts in the
-// window); bins are key bits [bsh, bsh + bin_bits).  Tiles whose range holds
-// more than heavy_cap keys are appended to the heavy list (when it is given).
+// Groups of 16 lanes, each over a contiguous run of windows: ranges[t] =
+// {a, n} (n = 0 when no bin starts in the window); bins are key bits
+// [bsh, bsh + bin_bits).  Windows whose range holds more than heavy_cap keys
+// are appended to the heavy list (when it is given).
 template <typename K>
 __global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys,
                                                    uint2* __restrict__ ranges, int tile_keys, int bsh, int bin_bits,
                                                    uint32_t heavy_cap, HeavyList hl) {
+    constexpr uint32_t GL = 16;
     const uint32_t TILE = (uint32_t)tile_keys;
-    const uint32_t lane = __lane_id();
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / GKC_WAVE;
-    const uint32_t nw = gridDim.x * blockDim.x / GKC_WAVE;
+    const uint32_t lane = __lane_id(), grp = lane / GL, gl = lane % GL;
+    const uint32_t ngroups = gridDim.x * blockDim.x / GL;
+    const uint32_t gid = (blockIdx.x * blockDim.x + threadIdx.x) / GL;
     const uint32_t total = plan->tot_rl_tiles, S = plan->n_segs;
-    for (uint32_t t = wave; t < total; t += nw) {
-        const uint32_t seg = find_seg(sa.rl_tile_start, S, t);
+    const uint32_t per = (total + ngroups - 1) / ngroups;
+    const uint32_t tb = min(total, gid * per), te = min(total, tb + per);
+    const uint64_t gmask = ((1ull << GL) - 1) << (grp * GL);
+    uint32_t seg = tb < te ? find_seg(sa.rl_tile_start, S, tb) : 0;
+    for (uint32_t t = tb; t < te; t++) {
+        while (seg + 1 < S && sa.rl_tile_start[seg + 1] <= t) seg++;
         const uint32_t ti = t - sa.rl_tile_start[seg];
         const K* kp = keys + sa.key_start[seg];
         const uint32_t nseg = sa.seg_keys[seg];
@@ -636,10 +642,10 @@ __global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegA
         uint32_t a = lo;
         if (lo > 0) {
             const uint32_t pb = bits_of(kp[lo - 1], bsh, bin_bits);
-            for (;; a += GKC_WAVE) {
-                const uint32_t j = a + lane;
+            for (;; a += GL) {
+                const uint32_t j = a + gl;
                 const bool stop = j >= hi || bits_of(kp[j], bsh, bin_bits) != pb;
-                const uint64_t m = __ballot(stop);
+                const uint64_t m = (__ballot(stop) & gmask) >> (grp * GL);
                 if (m) { a += (uint32_t)__builtin_ctzll(m); break; }
             }
         }
@@ -648,16 +654,16 @@ __global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegA
             uint32_t b = hi;
             if (hi < nseg) {
                 const uint32_t pb = bits_of(kp[hi - 1], bsh, bin_bits);
-                for (;; b += GKC_WAVE) {
-                    const uint32_t j = b + lane;
+                for (;; b += GL) {
+                    const uint32_t j = b + gl;
                     const bool stop = j >= nseg || bits_of(kp[j], bsh, bin_bits) != pb;
-                    const uint64_t m = __ballot(stop);
+                    const uint64_t m = (__ballot(stop) & gmask) >> (grp * GL);
                     if (m) { b += (uint32_t)__builtin_ctzll(m); break; }
                 }
             }
             n = b - a;
         }
-        if (lane == 0) {
+        if (gl == 0) {
             ranges[t] = make_uint2(a, n);
             if (hl.tiles && n > heavy_cap) hl.tiles[atomicAdd(hl.count, 1u)] = t;
         }
@@ -1186,14 +1192,28 @@ __global__ __launch_bounds__(WC_NT) void k_wavecount(Plan* __restrict__ plan, Se
     uint32_t list = (blockIdx.x * NW + wv) % wl.n_lists;
     uint32_t acc_seg = 0xFFFFFFFFu, acc_d = 0, acc_m = 0;     // per-segment totals, flushed on change
     uint32_t t = 0, t1 = 0;                                   // claimed batch [t, t1) of list `list`
+    uint32_t oi = 0;                                          // order index of t's segment in the list
+    const uint32_t* uoff = wl.w + 33 + wl.n_segs + 1 * wl.n_segs;   // window offsets by order index
     for (;;) {
         uint32_t seg = 0, ti = 0, ok = 1;
         if (t >= t1) {
             uint32_t a0 = 0, a1 = 0;
             if (lane == 0) ok = claim_batch<1>(wl, &plan->ctr[ctr_rl()], list, WC_BATCH, a0, a1) ? 1u : 0u;
             ok = __shfl(ok, 0); list = __shfl(list, 0); t = __shfl(a0, 0); t1 = __shfl(a1, 0);
+            oi = 0xFFFFFFFFu;
         }
-        if (ok) { unit_at<1>(wl, list, t, seg, ti); t++; }
+        if (ok) {
+            if (oi == 0xFFFFFFFFu) {            // first window of the batch: locate by binary search
+                uint32_t lo = wl.w[24 + list], hi = wl.w[24 + list + 1];
+                while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (uoff[mid] <= t) lo = mid; else hi = mid; }
+                oi = lo;
+            } else {                            // later windows: the same or a following segment of the list
+                while (oi + 1 < wl.w[24 + list + 1] && uoff[oi + 1] <= t) oi++;
+            }
+            seg = wl.w[33 + oi];
+            ti = t - uoff[oi];
+            t++;
+        }
         if (seg != acc_seg || !ok) {
             if (lane == 0 && acc_seg != 0xFFFFFFFFu) {
                 if (acc_d) atomicAdd(&to.n_distinct[acc_seg], acc_d);

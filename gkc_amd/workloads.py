@@ -145,9 +145,39 @@ class ReadSet:
     def bases(self) -> int:
         return int(self.offs[-1] - self.offs[0])
 
-    def kmers(self, k: int) -> int:
+    def kmers(self, k: int, hpc: bool = False) -> int:
+        """k-mer occurrences: sum over reads of max(0, len - k + 1), with len
+        the homopolymer-compressed length (number of runs of equal bytes)
+        when hpc."""
         lens = np.diff(self.offs).astype(np.int64)
+        if hpc:
+            lens = self.hpc_lengths()
         return int(np.maximum(lens - k + 1, 0).sum())
+
+    def hpc_lengths(self) -> np.ndarray:
+        """Runs of identical bytes per read (the compressed read length)."""
+        n = self.n
+        lens = np.diff(self.offs).astype(np.int64)
+        out = np.zeros(n, np.int64)
+        step = 1 << 26                      # bounded memory: blocks of whole reads
+        r = 0
+        while r < n:
+            r1 = int(np.searchsorted(self.offs, self.offs[r] + step, side="right")) - 1
+            r1 = max(r1, r + 1)
+            a, b = int(self.offs[r]), int(self.offs[r1])
+            seg = self.reads[a:b]
+            change = np.empty(b - a, np.bool_)
+            if b > a:
+                change[0] = True
+                np.not_equal(seg[1:], seg[:-1], out=change[1:])
+                starts = (self.offs[r:r1] - a).astype(np.int64)
+                nz = lens[r:r1] > 0
+                change[starts[nz]] = True      # a read's first byte starts a run
+                cs = np.concatenate([[0], np.cumsum(change, dtype=np.int64)])
+                ends = (self.offs[r + 1:r1 + 1] - a).astype(np.int64)
+                out[r:r1] = cs[ends] - cs[starts]
+            r = r1
+        return out
 
 
 def generate(w: Workload, g: int = 0, threads: int | None = None, keep_genome: bool = False) -> ReadSet:

@@ -78,3 +78,24 @@ def test_device_input_unaligned(gpu):
                             flags=capi.DEVICE_INPUT | flags)
             c.count(flags)
             assert_tables_equal(c.table(), ot, 64, 31)
+
+
+def test_device_input_unaligned_hpc(gpu):
+    """HPC on device reads that start off a 16-byte boundary (the compressed
+    reads keep the input's layout, lead included)."""
+    import torch
+    w = W.workload(3, scale=0.001)
+    rs = W.generate(w)
+    ot = O.count(rs.reads, rs.offs, w.k, w.p, 32, hpc=True)
+    d_reads = torch.from_numpy(rs.reads).cuda()
+    d_offs = torch.from_numpy(rs.offs.view(np.int64)).cuda()
+    torch.cuda.synchronize()
+    parts = _chunks(rs.n, [3, 40, rs.n])
+    _say("device input hpc", parts)
+    for flags in (0, capi.BATCHED):
+        with capi.Counter(w.k, w.p, 32, hpc=True) as c:
+            for a, b in parts:
+                c.partition(d_reads.data_ptr() + int(rs.offs[a]), d_offs.data_ptr() + 8 * a, b - a,
+                            flags=capi.DEVICE_INPUT | flags)
+            c.count(flags)
+            assert_tables_equal(c.table(), ot, 32, w.k)
