@@ -1145,16 +1145,12 @@ extern "C" int gkc_count_packed(gkc_ctx* c, const uint32_t* words, uint64_t n_wo
 
 // ---------------------------------------------------------------- batched form
 
-// Shape of one batched counting launch (count_mode 0, k >= 8: binned).
-//   64-bit keys: three onesweep passes over key bits [2k-24, 2k) group each
-//     segment's keys by 24-bit bin; k_wavecount counts WC_TILE-key windows of
-//     whole bins per wave, after k_bincount<., true> has counted the ranges
-//     larger than WC_CAP into staging.
-//   128-bit keys: two passes over [2k-16, 2k); k_bincount sorts + counts
-//     tiles of whole bins per workgroup (in LDS, or in chunks through global
-//     memory when larger).
-//   Otherwise (count_mode 1, k < 8): the full LSD sort (ceil(2k/8) passes)
-//   and k_runlength.
+// Shape of one batched counting launch.  Binned (count_mode 0, k >= 8): up
+// to three onesweep passes over the top 24 key bits group each segment's
+// keys by 24-bit bin; k_wavecount counts WC_TILE-key windows of whole bins
+// per wave, after k_bincount has counted the ranges larger than
+// WC_CAP into staging.  Otherwise (count_mode 1, k < 8): the full LSD sort
+// (ceil(2k/8) passes) and k_runlength.
 struct BatchShape {
     bool binned;
     bool wave;
@@ -1167,11 +1163,8 @@ template <typename K>
 static BatchShape batch_shape(const gkc_ctx* c) {
     const int kb = 2 * c->prm.k;
     if (c->tune.count_mode == 1 || c->prm.k < 8) return BatchShape{false, false, c->npass, 0u, RL_TILE, 0};
-    if (sizeof(K) == 8) {
-        const int np = std::min(3, (kb + 7) / 8);
-        return BatchShape{true, true, np, (uint32_t)std::max(0, kb - 8 * np), WC_TILE, 8 * np};
-    }
-    return BatchShape{true, false, 2, (uint32_t)(kb - 16), bc_tile<K>(), 16};
+    const int np = std::min(3, (kb + 7) / 8);
+    return BatchShape{true, true, np, (uint32_t)std::max(0, kb - 8 * np), WC_TILE, 8 * np};
 }
 
 // Enqueue extraction + the onesweep passes over every partition at once.
@@ -1232,7 +1225,7 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs
     HeavyList hl{c->d_hlist.as<uint32_t>(), c->d_toverflow.as<uint32_t>() + 1, c->d_toverflow.as<uint32_t>() + 2,
                  c->d_hres.as<uint2>(), c->d_wkept.as<uint32_t>()};
     if (bs.wave) {
-        if constexpr (sizeof(K) == 8) {
+        {
             if (prep) {
                 zero(c, c->d_nent.p, P); zero(c, c->d_ndist.p, P);
                 zero(c, c->d_toverflow.as<uint32_t>() + 1, 2);
@@ -1240,21 +1233,20 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs
                 const int g_r = occupancy_blocks((const void*)k_binranges<K>, 256, c->n_cu);
                 hipLaunchKernelGGL(k_binranges<K>, dim3(g_r), dim3(256), 0, c->st, plan, sa, (const K*)sorted, ranges,
                                    bs.rl_tile, (int)bs.sh0, bs.bin_bits, (uint32_t)WC_CAP, hl);
-                const int g_h = occupancy_blocks((const void*)k_bincount<K, true>, BC_NT, c->n_cu);
-                hipLaunchKernelGGL((k_bincount<K, true>), dim3(g_h), dim3(BC_NT), 0, c->st, plan, sa, c->wl, sorted, alt,
-                                   (const uint2*)ranges, to, c->lookback.as<uint32_t>(), c->d_toverflow.as<uint32_t>() + 1,
-                                   hl);
-                const int g_w = occupancy_blocks((const void*)k_wavecount, WC_NT, c->n_cu);
-                hipLaunchKernelGGL(k_wavecount, dim3(g_w), dim3(WC_NT), 0, c->st, plan, sa, c->wl, (uint64_t*)sorted,
-                                   (uint64_t*)alt, (const uint2*)ranges, hl, to);
+                const int g_h = occupancy_blocks((const void*)k_bincount<K>, BC_NT, c->n_cu);
+                hipLaunchKernelGGL(k_bincount<K>, dim3(g_h), dim3(BC_NT), 0, c->st, plan, sa, sorted, alt,
+                                   (const uint2*)ranges, to, hl);
+                const int g_w = occupancy_blocks((const void*)k_wavecount<K>, WC_NT, c->n_cu);
+                hipLaunchKernelGGL(k_wavecount<K>, dim3(g_w), dim3(WC_NT), 0, c->st, plan, sa, c->wl, sorted, alt,
+                                   (const uint2*)ranges, hl, to);
                 hipLaunchKernelGGL(k_tilescan, dim3(std::min<uint32_t>(P, 4 * c->n_cu)), dim3(1024), 0, c->st, plan, sa,
                                    hl.kept);
             }
             if (emit) {
                 zero(c, c->d_toverflow.p, 1);
-                const int g_c = occupancy_blocks((const void*)k_wavecopy, 256, c->n_cu);
-                hipLaunchKernelGGL(k_wavecopy, dim3(g_c), dim3(256), 0, c->st, plan, sa, (const uint64_t*)sorted,
-                                   (const uint64_t*)alt, (const uint2*)ranges, hl, to);
+                const int g_c = occupancy_blocks((const void*)k_wavecopy<K>, 256, c->n_cu);
+                hipLaunchKernelGGL(k_wavecopy<K>, dim3(g_c), dim3(256), 0, c->st, plan, sa, (const K*)sorted,
+                                   (const K*)alt, (const uint2*)ranges, hl, to);
             }
         }
         HIPX(hipGetLastError());
@@ -1263,16 +1255,7 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs
     zero(c, c->d_nent.p, P); zero(c, c->d_ndist.p, P); zero(c, c->d_toverflow.p, 3);
     zero(c, c->lookback.p, rl_tiles_cap);
     zero(c, &plan->ctr[ctr_rl()], NLIST);
-    if (bs.binned) {
-        const HeavyList none{nullptr, nullptr, nullptr, nullptr, nullptr};
-        const int g_r = occupancy_blocks((const void*)k_binranges<K>, 256, c->n_cu);
-        hipLaunchKernelGGL(k_binranges<K>, dim3(g_r), dim3(256), 0, c->st, plan, sa, (const K*)sorted, ranges,
-                           bs.rl_tile, (int)bs.sh0, bs.bin_bits, 0xFFFFFFFFu, none);
-        const int g_b = occupancy_blocks((const void*)k_bincount<K, false>, BC_NT, c->n_cu);
-        hipLaunchKernelGGL((k_bincount<K, false>), dim3(g_b), dim3(BC_NT), 0, c->st, plan, sa, c->wl, sorted, alt,
-                           (const uint2*)ranges, to, c->lookback.as<uint32_t>(), c->d_toverflow.as<uint32_t>() + 1,
-                           none);
-    } else {
+    {
         const int g_r = occupancy_blocks((const void*)k_runlength<K>, SORT_NT, c->n_cu);
         hipLaunchKernelGGL(k_runlength<K>, dim3(g_r), dim3(SORT_NT), 0, c->st, plan, sa, c->wl, sorted, to,
                            c->lookback.as<uint32_t>());
@@ -1337,13 +1320,10 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     (void)occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu);
     (void)sort_grid<K>(c);
     (void)occupancy_blocks((const void*)k_runlength<K>, SORT_NT, c->n_cu);
-    (void)occupancy_blocks((const void*)k_bincount<K, false>, BC_NT, c->n_cu);
     (void)occupancy_blocks((const void*)k_binranges<K>, 256, c->n_cu);
-    if constexpr (sizeof(K) == 8) {
-        (void)occupancy_blocks((const void*)k_bincount<K, true>, BC_NT, c->n_cu);
-        (void)occupancy_blocks((const void*)k_wavecount, WC_NT, c->n_cu);
-        (void)occupancy_blocks((const void*)k_wavecopy, 256, c->n_cu);
-    }
+    (void)occupancy_blocks((const void*)k_bincount<K>, BC_NT, c->n_cu);
+    (void)occupancy_blocks((const void*)k_wavecount<K>, WC_NT, c->n_cu);
+    (void)occupancy_blocks((const void*)k_wavecopy<K>, 256, c->n_cu);
     HIPX(hipEventRecord(c->ev[0], c->st));
     std::vector<const void*> key = {c->keysA.p, c->keysB.p, c->lookback.p, c->words.p, c->hdrs.p, c->d_ranges.p,
                                     c->d_hlist.p, c->d_hres.p, c->d_wkept.p,

@@ -127,18 +127,16 @@ template <> struct KeyTraits<u128k> {
 };
 template <typename K> constexpr int sort_tile() { return SORT_NT * KeyTraits<K>::ITEMS; }
 
-// Binned counting (batched form): after two onesweep passes over key bits
-// [2k-16, 2k) the keys of a segment are grouped by their top 16 bits ("bin").
-// A k_bincount tile owns the bins that start in its bc_tile-key window, sorts
-// them on the bits that still vary and run-length counts them: in LDS when
-// they hold at most bc_cap keys, else in bc_cap-key chunks through global
-// memory.
+// Heavy windows of the binned count (ranges of more than WC_CAP keys): one
+// k_bincount workgroup counts a range in LDS when it holds at most bc_cap
+// keys (64-bit keys: hash count, then an LSD sort of the distinct keys;
+// 128-bit keys: LSD sort, run-length count), else sorts it in bc_cap-key
+// chunks through global memory and counts it chunk by chunk.
 constexpr int BC_NT = 1024;
 template <typename K> struct BinTraits;
 template <> struct BinTraits<uint64_t> { static constexpr int ITEMS = 8; };   // 128 VGPRs, no spills
 template <> struct BinTraits<u128k> { static constexpr int ITEMS = 4; };
 template <typename K> constexpr int bc_cap() { return BC_NT * BinTraits<K>::ITEMS; }
-template <typename K> constexpr int bc_tile() { return bc_cap<K>() / 2; }
 
 __global__ void k_plan(Plan* plan, SegArrays sa, int sort_tile_keys, int rl_tile_keys);
 __global__ void k_zero32(uint32_t* p, uint64_t n);
@@ -157,24 +155,24 @@ __global__ void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, 
 template <typename K>
 __global__ void k_runlength(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ keys, TableOut to,
                             uint32_t* __restrict__ lookback);
-// Wave-level count (64-bit keys): after three onesweep passes over key bits
+// Wave-level count: after three onesweep passes over key bits
 // [sh0, sh0 + 24) each wave counts WC_TILE-key windows of whole 24-bit bins
 // on its own (LDS hash table, rank of the distinct keys by comparison) into
 // staging, with no workgroup barriers and no wait on other windows; a scan of
 // the windows' kept counts gives their table offsets and a copy writes the
 // table.  Ranges of more than WC_CAP keys are counted beforehand by
-// k_bincount<., true> into the same staging.
+// k_bincount into the same staging.
 constexpr int WC_NT = 256;
 constexpr int WC_TILE = 256;
 constexpr int WC_CAP = 512;
 constexpr uint32_t WC_BATCH = 16;   // windows claimed per atomic
 
 // Heavy-range list of a binned count: tiles whose range exceeds the
-// capacity of the counting kernel, counted ahead by k_bincount<., true>.
+// capacity of the counting kernel, counted ahead by k_bincount.
 struct HeavyList {
     uint32_t* tiles;          // [*count] tile ids
     uint32_t* count;          // appended by k_binranges
-    uint32_t* next;           // claim counter of k_bincount<., true>
+    uint32_t* next;           // claim counter of k_bincount
     uint2* res;               // [tile] {kept entries, distinct keys} of each heavy tile
     uint32_t* kept;           // [tile] kept entries of each window, then their offsets (k_tilescan)
 };
@@ -182,14 +180,15 @@ struct HeavyList {
 template <typename K>
 __global__ void k_binranges(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys, uint2* __restrict__ ranges,
                             int tile_keys, int bsh, int bin_bits, uint32_t heavy_cap, HeavyList hl);
-template <typename K, bool STAGED>
-__global__ void k_bincount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, K* __restrict__ keys, K* __restrict__ alt,
-                           const uint2* __restrict__ ranges, TableOut to, uint32_t* __restrict__ lookback,
-                           uint32_t* __restrict__ n_heavy, HeavyList hl);
-__global__ void k_wavecount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, uint64_t* __restrict__ keys,
-                            uint64_t* __restrict__ alt, const uint2* __restrict__ ranges, HeavyList hl, TableOut to);
+template <typename K>
+__global__ void k_bincount(Plan* __restrict__ plan, SegArrays sa, K* __restrict__ keys, K* __restrict__ alt,
+                           const uint2* __restrict__ ranges, TableOut to, HeavyList hl);
+template <typename K>
+__global__ void k_wavecount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, K* __restrict__ keys,
+                            K* __restrict__ alt, const uint2* __restrict__ ranges, HeavyList hl, TableOut to);
 __global__ void k_tilescan(Plan* __restrict__ plan, SegArrays sa, uint32_t* __restrict__ kept);
-__global__ void k_wavecopy(Plan* __restrict__ plan, SegArrays sa, const uint64_t* __restrict__ keys,
-                           const uint64_t* __restrict__ alt, const uint2* __restrict__ ranges, HeavyList hl, TableOut to);
+template <typename K>
+__global__ void k_wavecopy(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys,
+                           const K* __restrict__ alt, const uint2* __restrict__ ranges, HeavyList hl, TableOut to);
 
 }  // namespace gkc

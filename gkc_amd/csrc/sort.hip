@@ -587,19 +587,18 @@ __global__ __launch_bounds__(SORT_NT) void k_runlength(Plan* __restrict__ plan, 
 }
 
 // ---------------------------------------------------------------- binned count
-// Keys of a segment grouped by bin (key bits [2k-16, 2k), i.e. sorted by
-// them) are counted tile by tile.  Tile t of a segment owns the bins whose
-// first key lies in [t*TILE, (t+1)*TILE): its range [a, a+n) starts at the
-// first bin start at or after t*TILE and ends at the first bin start at or
-// after (t+1)*TILE (k_binranges; computed before any tile reorders its keys).
-// k_bincount sorts a range on the bits that vary inside it (LSD radix,
-// ranking by wave64 ballots as in k_onesweep), run-length counts and
-// thresholds it, and writes it at its place in the segment's table via a
-// decoupled look-back over the tiles (one value per tile).  A range of up to
-// CAP keys is sorted in LDS; a larger one (a bin of more than CAP - TILE + 1
-// keys: skewed composition or a high-copy k-mer) is sorted by the same
-// workgroup in CAP-key chunks through global memory, ping-ponging with the
-// spare key buffer, and counted chunk by chunk.
+// Keys of a segment grouped by bin (their top 24 bits, i.e. sorted by them)
+// are counted window by window.  Window t of a segment owns the bins whose
+// first key lies in [t*WC_TILE, (t+1)*WC_TILE): its range [a, a+n) starts at
+// the first bin start at or after t*WC_TILE and ends at the first bin start
+// at or after (t+1)*WC_TILE (k_binranges, before any window's keys are
+// reused as staging).  k_wavecount counts ranges of up to WC_CAP keys, one
+// wave each; k_bincount the larger ones beforehand (a bin of more than
+// WC_CAP - WC_TILE + 1 keys: skewed composition or a high-copy k-mer), one
+// workgroup each: sorted on the bits that vary inside the range (LSD radix,
+// ranking by wave64 ballots as in k_onesweep) in LDS, or in CAP-key chunks
+// through global memory ping-ponging with the spare key buffer, then
+// run-length counted, thresholded and staged.
 // Key bits [bsh, bsh + nb), nb <= 32 (bits above the key width read as 0).
 __device__ __forceinline__ uint32_t bits_of(uint64_t x, int bsh, int nb) {
     return (uint32_t)(x >> bsh) & (uint32_t)((1ull << nb) - 1);
@@ -719,16 +718,14 @@ __device__ __forceinline__ uint32_t digit_scan(uint32_t v, uint32_t tid, uint32_
     return off + inc - v;
 }
 
-// STAGED: count the tiles of the heavy list instead (no look-back); the
-// kept entries of a range at a go to staging, keys at alt[a + i] and counts
-// as u32 over the range's own keys (they trail the reads: entry i never
-// overtakes key 2i), and {kept, distinct} to hl.res[tile].
-template <typename K, bool STAGED>
-__global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl,
-                                                    K* __restrict__ keys, K* __restrict__ alt,
-                                                    const uint2* __restrict__ ranges, TableOut to,
-                                                    uint32_t* __restrict__ lookback, uint32_t* __restrict__ n_heavy,
-                                                    HeavyList hl) {
+// Counts the windows of the heavy list, one workgroup per window at a time.
+// The kept entries of a range at a go to staging, keys at alt[a + i] and
+// counts as u32 over the range's own keys (they trail the reads: entry i
+// never overtakes key 2i), and {kept, distinct} to hl.res[tile].
+template <typename K>
+__global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, SegArrays sa, K* __restrict__ keys,
+                                                    K* __restrict__ alt, const uint2* __restrict__ ranges,
+                                                    TableOut to, HeavyList hl) {
     constexpr int ITEMS = BinTraits<K>::ITEMS;
     constexpr uint32_t CAP = bc_cap<K>();
     constexpr int NWAVE = BC_NT / GKC_WAVE;
@@ -745,26 +742,20 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
     __shared__ uint32_t s_ck[NCH];               // kept entries of each chunk, then their exclusive prefix
     __shared__ uint32_t s_red[NWAVE];
     __shared__ uint32_t s_orv[NWAVE][4];
-    __shared__ uint32_t s_seg, s_idx, s_ok, s_nd, s_out, s_nh, s_excl, s_tot;
+    __shared__ uint32_t s_seg, s_idx, s_ok, s_nd, s_out, s_nh;
     const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE, tid = threadIdx.x;
     const uint32_t wbase = wv * GKC_WAVE * ITEMS;     // wave wv holds chunk keys [wbase, wbase + 64*ITEMS)
-    uint32_t list = blockIdx.x % wl.n_lists;
     for (;;) {
         if (tid == 0) {
             uint32_t sg = 0, ix = 0;
-            bool ok;
-            if constexpr (STAGED) {
-                const uint32_t q = atomicAdd(hl.next, 1u);
-                ok = q < *hl.count;
-                if (ok) {
-                    const uint32_t t = hl.tiles[q];
-                    sg = find_seg(sa.rl_tile_start, plan->n_segs, t);
-                    ix = t - sa.rl_tile_start[sg];
-                }
-            } else {
-                ok = claim<1>(wl, &plan->ctr[ctr_rl()], list, sg, ix);
+            const uint32_t q = atomicAdd(hl.next, 1u);
+            const bool ok = q < *hl.count;
+            if (ok) {
+                const uint32_t t = hl.tiles[q];
+                sg = find_seg(sa.rl_tile_start, plan->n_segs, t);
+                ix = t - sa.rl_tile_start[sg];
             }
-            s_ok = ok; s_seg = sg; s_idx = ix; s_nd = 0; s_out = 0; s_nh = 0; s_excl = 0;
+            s_ok = ok; s_seg = sg; s_idx = ix; s_nd = 0; s_out = 0; s_nh = 0;
         }
         __syncthreads();
         if (!s_ok) break;
@@ -774,28 +765,12 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
         // hoisted out of it (which would exceed the 128 VGPRs of 16 waves/CU)
         uint32_t jb = wbase + lane;
         asm volatile("" : "+v"(jb));
-        const uint32_t tile = sa.rl_tile_start[seg] + ti;        // look-back slot
+        const uint32_t tile = sa.rl_tile_start[seg] + ti;
         const uint2 rg = ranges[tile];
         const uint32_t n = rg.y;
-        const bool heavy = n > CAP;
+        const bool heavy = n > CAP;                             // larger than LDS: chunks through global memory
         K* src = keys + sa.key_start[seg] + rg.x;
         K* dst = alt + sa.key_start[seg] + rg.x;
-        if (!STAGED && heavy && tid == 0) {
-            // base of a heavy range first: its total is known only at the end
-            uint32_t excl = 0;
-            if (ti > 0) {
-                int64_t j = (int64_t)tile - 1;
-                for (;;) {
-                    uint32_t v = ld_agent(&lookback[j]);
-                    while ((v & ~LB_VAL) == 0) { __builtin_amdgcn_s_sleep(1); v = ld_agent(&lookback[j]); }
-                    excl += v & LB_VAL;
-                    if (v & LB_INC) break;
-                    j--;
-                }
-            }
-            s_excl = excl;
-            atomicAdd(n_heavy, 1u);
-        }
         // 1. bits that vary inside the range
         K key[ITEMS];
         uint64_t olo = 0, ohi = 0;
@@ -1111,33 +1086,12 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
             }
             uint32_t ctot = 0;
             for (int w = 0; w < NCW; w++) ctot += s_red[w];
-            if (!STAGED && !heavy && tid == 0) {
-                // 4. decoupled look-back for the tile's place in the segment's table
-                uint32_t* lb = lookback + tile;
-                uint32_t excl = 0;
-                if (ti == 0) st_agent(lb, LB_INC | ctot);
-                else {
-                    st_agent(lb, LB_AGG | ctot);
-                    int64_t j = (int64_t)tile - 1;
-                    for (;;) {
-                        uint32_t v = ld_agent(&lookback[j]);
-                        while ((v & ~LB_VAL) == 0) { __builtin_amdgcn_s_sleep(1); v = ld_agent(&lookback[j]); }
-                        excl += v & LB_VAL;
-                        if (v & LB_INC) break;
-                        j--;
-                    }
-                    st_agent(lb, LB_INC | (excl + ctot));
-                }
-                s_excl = excl;
-            }
             __syncthreads();
-            // 5. write the kept (k-mer, count) entries
-            if ((STAGED || to.emit) && ctot) {
-                const uint32_t cap = STAGED ? 0xFFFFFFFFu : to.table_cap[seg];
-                K* tk = STAGED ? alt + sa.key_start[seg] + rg.x : reinterpret_cast<K*>(to.keys) + to.table_start[seg];
-                uint32_t* tc = STAGED ? reinterpret_cast<uint32_t*>(keys + sa.key_start[seg] + rg.x)
-                                      : to.counts + to.table_start[seg];
-                const uint32_t base = s_excl + s_out;
+            // 4. stage the kept (k-mer, count) entries
+            if (ctot) {
+                K* tk = alt + sa.key_start[seg] + rg.x;
+                uint32_t* tc = reinterpret_cast<uint32_t*>(keys + sa.key_start[seg] + rg.x);
+                const uint32_t base = s_out;
 #pragma unroll
                 for (int i = 0; i < ITEMS; i++) {
                     if (wbase + i * GKC_WAVE >= nv) continue;
@@ -1145,50 +1099,87 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
                     const uint64_t kb = __ballot(keep);
                     if (keep) {
                         const uint32_t idx = base + s_ck[wv * ITEMS + i] + (uint32_t)__popcll(kb & lanemask_lt());
-                        if (idx < cap) { tk[idx] = key[i]; tc[idx] = cnt[i]; }
-                        else atomicOr(to.overflow, 1u);
+                        tk[idx] = key[i]; tc[idx] = cnt[i];
                     }
                 }
             }
             __syncthreads();
             if (tid == 0) s_out += ctot;
         }
-        if (tid == 0) {
-            if constexpr (STAGED) {
-                hl.res[tile] = make_uint2(s_out, s_nd);
-            } else {
-                if (heavy) st_agent(lookback + tile, LB_INC | (s_excl + s_out));
-                if (s_nd) atomicAdd(&to.n_distinct[seg], s_nd);
-                if (s_out) atomicAdd(&to.n_entries[seg], s_out);
-            }
-        }
+        if (tid == 0) hl.res[tile] = make_uint2(s_out, s_nd);
         __syncthreads();
     }
 }
 
-// Wave-level binned count of 64-bit keys (see WC_TILE): every wave claims
-// batches of windows and, per window range [a, a + n):
+// One wave's LDS hash table of a window (k_wavecount).  64-bit keys: the key
+// word is the slot's state (EMPTY = all ones, never a canonical k-mer: its
+// reverse complement, 0, is smaller) and is claimed by a 64-bit CAS.
+// 128-bit keys: a 32-bit tag (0 = free, else a nonzero fingerprint of the
+// key) is claimed by CAS and the winner writes the key; every lane then
+// compares the full key.  All lanes of a wave run the loop in lockstep, so a
+// key written in an iteration is visible to the compares of that iteration.
+template <typename K> struct WaveTable;
+template <> struct WaveTable<uint64_t> {
+    uint64_t key[WC_CAP];
+    uint32_t cnt[WC_CAP];
+    static constexpr uint64_t EMPTY = ~0ull;
+    __device__ void clear(uint32_t i) { key[i] = EMPTY; cnt[i] = 0; }
+    __device__ bool used(uint32_t i) const { return key[i] != EMPTY; }
+    __device__ void insert(uint64_t x) {
+        uint32_t h = (uint32_t)(((unsigned long long)x * 0x9E3779B97F4A7C15ull) >> (64 - __builtin_ctz(WC_CAP)));
+        for (;;) {     // at most WC_CAP keys: a free or matching slot exists
+            const unsigned long long old = atomicCAS(reinterpret_cast<unsigned long long*>(&key[h]),
+                                                     (unsigned long long)EMPTY, (unsigned long long)x);
+            if (old == EMPTY || old == x) { atomicAdd(&cnt[h], 1u); return; }
+            h = (h + 1) & (WC_CAP - 1);
+        }
+    }
+};
+template <> struct WaveTable<u128k> {
+    u128k key[WC_CAP];
+    uint32_t cnt[WC_CAP];
+    uint32_t tag[WC_CAP];
+    __device__ void clear(uint32_t i) { tag[i] = 0; cnt[i] = 0; }
+    __device__ bool used(uint32_t i) const { return tag[i] != 0; }
+    __device__ void insert(const u128k& x) {
+        const unsigned long long hv = ((unsigned long long)x.lo ^ ((unsigned long long)x.hi * 0xC2B2AE3D27D4EB4Full)) *
+                                      0x9E3779B97F4A7C15ull;
+        uint32_t h = (uint32_t)(hv >> (64 - __builtin_ctz(WC_CAP)));
+        const uint32_t fp = (uint32_t)hv | 1u;
+        for (;;) {
+            const uint32_t old = atomicCAS(&tag[h], 0u, fp);
+            if (old == 0u) key[h] = x;
+            // the winners' key stores before any lane's compare: keeps the
+            // compiler from reading key[h] ahead of a store it does not make
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+            if ((old == 0u || old == fp) && key[h] == x) { atomicAdd(&cnt[h], 1u); return; }
+            h = (h + 1) & (WC_CAP - 1);
+        }
+    }
+};
+__device__ __forceinline__ bool key_less(uint64_t a, uint64_t b) { return a < b; }
+__device__ __forceinline__ bool key_less(const u128k& a, const u128k& b) { return less128(a, b); }
+
+// Wave-level binned count (see WC_TILE): every wave claims batches of
+// windows and, per window range [a, a + n):
 //   n <= WC_CAP: counts equal keys in its LDS hash table and ranks the kept
 //                distinct keys by comparison (their place in key order), then
-//                stages them like k_bincount<., true> (keys at alt[a + r],
+//                stages them like k_bincount (keys at alt[a + r],
 //                counts as u32 over the range's own keys, read already);
-//   n >  WC_CAP: takes {kept, distinct} k_bincount<., true> staged;
+//   n >  WC_CAP: takes {kept, distinct} k_bincount staged;
 // and records the window's kept count in hl.kept[tile].  The per-segment
 // distinct and kept totals are accumulated.  No window waits for another:
 // k_tilescan turns the kept counts into table offsets, k_wavecopy writes.
+template <typename K>
 __global__ __launch_bounds__(WC_NT) void k_wavecount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl,
-                                                     uint64_t* __restrict__ keys, uint64_t* __restrict__ alt,
+                                                     K* __restrict__ keys, K* __restrict__ alt,
                                                      const uint2* __restrict__ ranges, HeavyList hl, TableOut to) {
     constexpr int NW = WC_NT / GKC_WAVE;
     constexpr int IT = WC_CAP / GKC_WAVE;
-    constexpr int LOG_CAP = __builtin_ctz(WC_CAP);
-    constexpr uint64_t EMPTY = ~0ull;     // never canonical: its reverse complement (0) is smaller
     static_assert((WC_CAP & (WC_CAP - 1)) == 0, "table size");
-    __shared__ uint64_t s_tk[NW][WC_CAP];
-    __shared__ uint32_t s_tc[NW][WC_CAP];
+    __shared__ WaveTable<K> s_tab[NW];
     const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE;
-    uint64_t* tk = s_tk[wv];
-    uint32_t* tc = s_tc[wv];
+    WaveTable<K>& T = s_tab[wv];
     uint32_t list = (blockIdx.x * NW + wv) % wl.n_lists;
     uint32_t acc_seg = 0xFFFFFFFFu, acc_d = 0, acc_m = 0;     // per-segment totals, flushed on change
     uint32_t t = 0, t1 = 0;                                   // claimed batch [t, t1) of list `list`
@@ -1225,8 +1216,8 @@ __global__ __launch_bounds__(WC_NT) void k_wavecount(Plan* __restrict__ plan, Se
         const uint32_t tile = sa.rl_tile_start[seg] + ti;
         const uint2 rg = ranges[tile];
         const uint32_t n = rg.y;
-        uint64_t* kp = keys + sa.key_start[seg] + rg.x;
-        uint64_t* sk = alt + sa.key_start[seg] + rg.x;
+        K* kp = keys + sa.key_start[seg] + rg.x;
+        K* sk = alt + sa.key_start[seg] + rg.x;
         uint32_t* sc = reinterpret_cast<uint32_t*>(kp);
         uint32_t d = 0, m = 0;
         if (n > (uint32_t)WC_CAP) {
@@ -1234,50 +1225,46 @@ __global__ __launch_bounds__(WC_NT) void k_wavecount(Plan* __restrict__ plan, Se
             m = hr.x; d = hr.y;
         } else if (n) {
             // hash-count the range in the wave's table
-            uint64_t x[IT];
+            K x[IT];
 #pragma unroll
             for (int i = 0; i < IT; i++) {
                 const uint32_t j = i * GKC_WAVE + lane;
-                x[i] = (i * GKC_WAVE < (int)n && j < n) ? kp[j] : EMPTY;
-                tk[j] = EMPTY; tc[j] = 0;
+                if (i * GKC_WAVE < (int)n && j < n) x[i] = kp[j];
+                T.clear(j);
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // other lanes' slots cleared
 #pragma unroll
             for (int i = 0; i < IT; i++) {
-                if (x[i] == EMPTY) continue;
-                const unsigned long long xv = x[i];
-                uint32_t h = (uint32_t)((xv * 0x9E3779B97F4A7C15ull) >> (64 - LOG_CAP));
-                for (;;) {     // n <= WC_CAP slots: a free or matching slot exists
-                    const unsigned long long old =
-                        atomicCAS(reinterpret_cast<unsigned long long*>(&tk[h]), (unsigned long long)EMPTY, xv);
-                    if (old == EMPTY || old == xv) { atomicAdd(&tc[h], 1u); break; }
-                    h = (h + 1) & (WC_CAP - 1);
-                }
+                if (i * GKC_WAVE < (int)n && i * GKC_WAVE + lane < n) T.insert(x[i]);
             }
             // compact the kept distinct keys to the front of the table
 #pragma unroll
             for (int i = 0; i < IT; i++) {
-                const uint64_t key = tk[i * GKC_WAVE + lane];
-                const uint32_t c = tc[i * GKC_WAVE + lane];
-                const bool occ = key != EMPTY;
+                const uint32_t slot = i * GKC_WAVE + lane;
+                const bool occ = T.used(slot);
+                K key{};
+                uint32_t c = 0;
+                if (occ) { key = T.key[slot]; c = T.cnt[slot]; }
                 const bool keep = occ && c >= to.min_cnt && (to.max_cnt == 0 || c <= to.max_cnt);
                 d += (uint32_t)__popcll(__ballot(occ));
                 const uint64_t kb = __ballot(keep);
-                x[i] = key;
                 if (keep) {
                     const uint32_t p = m + (uint32_t)__popcll(kb & lanemask_lt());
-                    tk[p] = key; tc[p] = c;     // p <= this slot's index: reads of later slots are not disturbed
+                    T.key[p] = key; T.cnt[p] = c;     // p <= this slot's index: reads of later slots are not disturbed
                 }
                 m += (uint32_t)__popcll(kb);
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // compacted entries of every lane
             // rank of each kept key among the kept keys = its place in key order; stage
 #pragma unroll
             for (int i = 0; i < IT; i++) {
                 if (i * GKC_WAVE >= (int)m) continue;
                 const uint32_t e = i * GKC_WAVE + lane;
-                const uint64_t key = e < m ? tk[e] : EMPTY;
+                K key{};
+                if (e < m) key = T.key[e];
                 uint32_t r = 0;
-                for (uint32_t q = 0; q < m; q++) r += tk[q] < key ? 1u : 0u;
-                if (e < m) { sk[r] = key; sc[r] = tc[e]; }
+                for (uint32_t q = 0; q < m; q++) r += key_less(T.key[q], key) ? 1u : 0u;
+                if (e < m) { sk[r] = key; sc[r] = T.cnt[e]; }
             }
         }
         if (lane == 0) hl.kept[tile] = m;
@@ -1316,8 +1303,9 @@ __global__ __launch_bounds__(1024) void k_tilescan(Plan* __restrict__ plan, SegA
 
 // Write each window's staged entries at its table offset (one wave per
 // window): the binned count's emission; repeatable with other capacities.
-__global__ __launch_bounds__(256) void k_wavecopy(Plan* __restrict__ plan, SegArrays sa, const uint64_t* __restrict__ keys,
-                                                  const uint64_t* __restrict__ alt, const uint2* __restrict__ ranges,
+template <typename K>
+__global__ __launch_bounds__(256) void k_wavecopy(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys,
+                                                  const K* __restrict__ alt, const uint2* __restrict__ ranges,
                                                   HeavyList hl, TableOut to) {
     const uint32_t lane = __lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / GKC_WAVE;
@@ -1332,10 +1320,10 @@ __global__ __launch_bounds__(256) void k_wavecopy(Plan* __restrict__ plan, SegAr
         const uint32_t m = end - off;
         if (!m) continue;
         const uint32_t a = ranges[t].x;
-        const uint64_t* sk = alt + sa.key_start[seg] + a;
+        const K* sk = alt + sa.key_start[seg] + a;
         const uint32_t* sc = reinterpret_cast<const uint32_t*>(keys + sa.key_start[seg] + a);
         const uint32_t cap = to.table_cap[seg];
-        uint64_t* okey = reinterpret_cast<uint64_t*>(to.keys) + to.table_start[seg];
+        K* okey = reinterpret_cast<K*>(to.keys) + to.table_start[seg];
         uint32_t* ocnt = to.counts + to.table_start[seg];
         for (uint32_t e = lane; e < m; e += GKC_WAVE) {
             const uint32_t idx = off + e;
@@ -1347,9 +1335,12 @@ __global__ __launch_bounds__(256) void k_wavecopy(Plan* __restrict__ plan, SegAr
 
 template __global__ void k_binranges<uint64_t>(Plan*, SegArrays, const uint64_t*, uint2*, int, int, int, uint32_t, HeavyList);
 template __global__ void k_binranges<u128k>(Plan*, SegArrays, const u128k*, uint2*, int, int, int, uint32_t, HeavyList);
-template __global__ void k_bincount<uint64_t, false>(Plan*, SegArrays, WorkLists, uint64_t*, uint64_t*, const uint2*, TableOut, uint32_t*, uint32_t*, HeavyList);
-template __global__ void k_bincount<uint64_t, true>(Plan*, SegArrays, WorkLists, uint64_t*, uint64_t*, const uint2*, TableOut, uint32_t*, uint32_t*, HeavyList);
-template __global__ void k_bincount<u128k, false>(Plan*, SegArrays, WorkLists, u128k*, u128k*, const uint2*, TableOut, uint32_t*, uint32_t*, HeavyList);
+template __global__ void k_bincount<uint64_t>(Plan*, SegArrays, uint64_t*, uint64_t*, const uint2*, TableOut, HeavyList);
+template __global__ void k_bincount<u128k>(Plan*, SegArrays, u128k*, u128k*, const uint2*, TableOut, HeavyList);
+template __global__ void k_wavecount<uint64_t>(Plan*, SegArrays, WorkLists, uint64_t*, uint64_t*, const uint2*, HeavyList, TableOut);
+template __global__ void k_wavecount<u128k>(Plan*, SegArrays, WorkLists, u128k*, u128k*, const uint2*, HeavyList, TableOut);
+template __global__ void k_wavecopy<uint64_t>(Plan*, SegArrays, const uint64_t*, const uint64_t*, const uint2*, HeavyList, TableOut);
+template __global__ void k_wavecopy<u128k>(Plan*, SegArrays, const u128k*, const u128k*, const uint2*, HeavyList, TableOut);
 
 template __global__ void k_extract<uint64_t>(Plan*, SegArrays, WorkLists, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, uint64_t*, int);
 template __global__ void k_extract<u128k>(Plan*, SegArrays, WorkLists, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, u128k*, int);

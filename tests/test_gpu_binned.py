@@ -1,7 +1,6 @@
-"""Binned counting (batched form): onesweep passes on the top 24 (64-bit keys)
-or 16 (128-bit keys) key bits, then per window of whole bins a count in LDS
-(wave-level hash count for 64-bit keys, workgroup LDS sort + run-length count
-for 128-bit keys).  Bit-exact
+"""Binned counting (batched form): onesweep passes on the top 24 key bits,
+then per window of whole bins a wave-level count in LDS (hash count, rank of
+the distinct keys), larger windows per workgroup.  Bit-exact
 against the oracle and against the full LSD sort (count_mode 1), including
 the edges of the tiling: bins straddling tile windows, ranges of one key or
 of one repeated key, ranges too large for LDS (sorted and counted in chunks
@@ -62,7 +61,7 @@ def test_binned_matches_oracle_and_lsd(gpu, cid, scale, k, p, P):
     ot = O.count(rs.reads, rs.offs, k, p, P)
     _say("binned", w.name, k, p, P)
     gt, st = _count(rs.reads, rs.offs, k, p, P, mode=0)
-    assert st["binned"] == 1 and st["n_pass"] == (min(3, (2 * k + 7) // 8) if k <= 32 else 2)
+    assert st["binned"] == 1 and st["n_pass"] == min(3, (2 * k + 7) // 8)
     assert_tables_equal(gt, ot, P, k)
     _say("lsd", w.name, k, p, P)
     lt, sl = _count(rs.reads, rs.offs, k, p, P, mode=1)
