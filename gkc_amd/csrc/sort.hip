@@ -614,55 +614,82 @@ __device__ __forceinline__ void or_diff(uint64_t& lo, uint64_t& hi, const u128k&
     lo |= a.lo ^ b.lo; hi |= a.hi ^ b.hi;
 }
 
-// Groups of 16 lanes, each over a contiguous run of windows: ranges[t] =
-// {a, n} (n = 0 when no bin starts in the window); bins are key bits
-// [bsh, bsh + bin_bits).  Windows whose range holds more than heavy_cap keys
-// are appended to the heavy list (when it is given).
+// Window ranges: ranges[t] = {a, n} (n = 0 when no bin starts in the
+// window); bins are key bits [bsh, bsh + bin_bits).  Each lane takes one
+// window and scans its two boundaries key by key (bins are small: most scans
+// end within a key or two); a boundary not found within SCAN keys is then
+// found by the whole wave, 64 keys a step.  Windows whose range holds more
+// than heavy_cap keys are appended to the heavy list (when it is given).
 template <typename K>
 __global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys,
                                                    uint2* __restrict__ ranges, int tile_keys, int bsh, int bin_bits,
                                                    uint32_t heavy_cap, HeavyList hl) {
-    constexpr uint32_t GL = 16;
+    constexpr uint32_t SCAN = 32;
     const uint32_t TILE = (uint32_t)tile_keys;
-    const uint32_t lane = __lane_id(), grp = lane / GL, gl = lane % GL;
-    const uint32_t ngroups = gridDim.x * blockDim.x / GL;
-    const uint32_t gid = (blockIdx.x * blockDim.x + threadIdx.x) / GL;
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / GKC_WAVE;
+    const uint32_t nw = gridDim.x * blockDim.x / GKC_WAVE;
     const uint32_t total = plan->tot_rl_tiles, S = plan->n_segs;
-    const uint32_t per = (total + ngroups - 1) / ngroups;
-    const uint32_t tb = min(total, gid * per), te = min(total, tb + per);
-    const uint64_t gmask = ((1ull << GL) - 1) << (grp * GL);
-    uint32_t seg = tb < te ? find_seg(sa.rl_tile_start, S, tb) : 0;
-    for (uint32_t t = tb; t < te; t++) {
-        while (seg + 1 < S && sa.rl_tile_start[seg + 1] <= t) seg++;
-        const uint32_t ti = t - sa.rl_tile_start[seg];
-        const K* kp = keys + sa.key_start[seg];
-        const uint32_t nseg = sa.seg_keys[seg];
-        const uint32_t lo = ti * TILE, hi = min(lo + TILE, nseg);
-        uint32_t a = lo;
-        if (lo > 0) {
-            const uint32_t pb = bits_of(kp[lo - 1], bsh, bin_bits);
-            for (;; a += GL) {
-                const uint32_t j = a + gl;
-                const bool stop = j >= hi || bits_of(kp[j], bsh, bin_bits) != pb;
-                const uint64_t m = (__ballot(stop) & gmask) >> (grp * GL);
-                if (m) { a += (uint32_t)__builtin_ctzll(m); break; }
-            }
+    // wave w takes windows [w*64, w*64 + 64), [w*64 + 64*nw, ...), ...
+    for (uint32_t t0 = wave * GKC_WAVE; t0 < total; t0 += nw * GKC_WAVE) {
+        const uint32_t t = t0 + lane;
+        const bool act = t < total;
+        uint32_t seg = 0, nseg = 0, lo = 0, hi = 0;
+        const K* kp = keys;
+        if (act) {
+            seg = find_seg(sa.rl_tile_start, S, t);
+            kp = keys + sa.key_start[seg];
+            nseg = sa.seg_keys[seg];
+            lo = (t - sa.rl_tile_start[seg]) * TILE;
+            hi = min(lo + TILE, nseg);
         }
-        uint32_t n = 0;
-        if (a < hi) {
-            uint32_t b = hi;
-            if (hi < nseg) {
-                const uint32_t pb = bits_of(kp[hi - 1], bsh, bin_bits);
-                for (;; b += GL) {
-                    const uint32_t j = b + gl;
-                    const bool stop = j >= nseg || bits_of(kp[j], bsh, bin_bits) != pb;
-                    const uint64_t m = (__ballot(stop) & gmask) >> (grp * GL);
-                    if (m) { b += (uint32_t)__builtin_ctzll(m); break; }
+        // a: first bin start in [lo, hi) (hi when none); b: first bin start >= hi
+        uint32_t a = lo, b = hi;
+        bool a_done = !act || lo == 0, b_done = !act || hi >= nseg;
+        if (!a_done) {
+            const uint32_t pb = bits_of(kp[lo - 1], bsh, bin_bits);
+            for (uint32_t s2 = 0; s2 < SCAN && !a_done; s2++, a++)
+                if (a >= hi || bits_of(kp[a], bsh, bin_bits) != pb) a_done = true;
+            if (a_done) a--;             // undo the loop's last increment
+        }
+        if (a_done && act && a < hi && !b_done) {
+            const uint32_t pb = bits_of(kp[hi - 1], bsh, bin_bits);
+            for (uint32_t s2 = 0; s2 < SCAN && !b_done; s2++, b++)
+                if (b >= nseg || bits_of(kp[b], bsh, bin_bits) != pb) b_done = true;
+            if (b_done) b--;
+        }
+        // boundaries still open: the wave scans them one window at a time
+        uint64_t open = __ballot(act && (!a_done || (a < hi && !b_done)));
+        while (open) {
+            const uint32_t L = (uint32_t)__builtin_ctzll(open);
+            open &= open - 1;
+            const K* kq = reinterpret_cast<const K*>(__shfl(reinterpret_cast<uintptr_t>(kp), L));
+            const uint32_t qhi = __shfl(hi, L), qnseg = __shfl(nseg, L), qlo = __shfl(lo, L);
+            uint32_t qa = __shfl(a, L);
+            const bool qa_done = __shfl((uint32_t)a_done, L) != 0;
+            if (!qa_done) {              // continue the a-scan from qa
+                const uint32_t pb = bits_of(kq[qlo - 1], bsh, bin_bits);
+                for (;; qa += GKC_WAVE) {
+                    const uint32_t j = qa + lane;
+                    const bool stop = j >= qhi || bits_of(kq[j], bsh, bin_bits) != pb;
+                    const uint64_t m = __ballot(stop);
+                    if (m) { qa += (uint32_t)__builtin_ctzll(m); break; }
                 }
             }
-            n = b - a;
+            uint32_t qb = qa_done ? __shfl(b, L) : qhi;
+            if (qa < qhi && qhi < qnseg && !(qa_done && __shfl((uint32_t)b_done, L) != 0)) {
+                const uint32_t pb = bits_of(kq[qhi - 1], bsh, bin_bits);
+                for (;; qb += GKC_WAVE) {
+                    const uint32_t j = qb + lane;
+                    const bool stop = j >= qnseg || bits_of(kq[j], bsh, bin_bits) != pb;
+                    const uint64_t m = __ballot(stop);
+                    if (m) { qb += (uint32_t)__builtin_ctzll(m); break; }
+                }
+            }
+            if (lane == L) { a = qa; b = qb; }
         }
-        if (gl == 0) {
+        if (act) {
+            const uint32_t n = a < hi ? b - a : 0u;
             ranges[t] = make_uint2(a, n);
             if (hl.tiles && n > heavy_cap) hl.tiles[atomicAdd(hl.count, 1u)] = t;
         }
@@ -722,6 +749,8 @@ __device__ __forceinline__ uint32_t digit_scan(uint32_t v, uint32_t tid, uint32_
 // The kept entries of a range at a go to staging, keys at alt[a + i] and
 // counts as u32 over the range's own keys (they trail the reads: entry i
 // never overtakes key 2i), and {kept, distinct} to hl.res[tile].
+constexpr uint32_t RANK_MAX = 2048;      // distinct keys a heavy window ranks by comparison
+
 template <typename K>
 __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, SegArrays sa, K* __restrict__ keys,
                                                     K* __restrict__ alt, const uint2* __restrict__ ranges,
@@ -804,6 +833,7 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
         // 64-bit keys in LDS: equal keys are counted first in an LDS hash table,
         // so only the distinct keys are sorted (with their counts)
         const bool hashed = sizeof(K) == 8 && !heavy;
+        bool ranked = false;                   // staged already by rank of comparison
         uint32_t nsort = n;                    // entries sorted in LDS
         if constexpr (sizeof(K) == 8) {
             if (hashed) {
@@ -851,13 +881,41 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
                     if (key[i] != EMPTY) { s_keys[pos] = key[i]; s_cnt[pos] = cnt[i]; pos++; }
                 nsort = dtot;
                 __syncthreads();
+                if (nsort <= RANK_MAX) {
+                    // few distinct keys: rank each kept key among the kept keys by
+                    // comparison (its place in key order) and stage it, no sort
+                    K* tk = alt + sa.key_start[seg] + rg.x;
+                    uint32_t* tc = reinterpret_cast<uint32_t*>(keys + sa.key_start[seg] + rg.x);
+                    uint32_t kept = 0;
+                    for (uint32_t e = tid; e < nsort; e += BC_NT) {
+                        const uint64_t x = s_keys[e];
+                        const uint32_t c = s_cnt[e];
+                        if (c >= to.min_cnt && (to.max_cnt == 0 || c <= to.max_cnt)) {
+                            uint32_t r = 0;
+                            for (uint32_t q = 0; q < nsort; q++) {
+                                const uint32_t cq = s_cnt[q];
+                                r += (s_keys[q] < x && cq >= to.min_cnt && (to.max_cnt == 0 || cq <= to.max_cnt)) ? 1u : 0u;
+                            }
+                            tk[r] = x; tc[r] = c;
+                            kept++;
+                        }
+                    }
 #pragma unroll
-                for (int i = 0; i < ITEMS; i++) {
-                    const uint32_t j = jb + i * GKC_WAVE;
-                    if (j < nsort) { key[i] = s_keys[j]; cnt[i] = s_cnt[j]; }
+                    for (int off = 1; off < GKC_WAVE; off <<= 1) kept += __shfl_xor(kept, off);
+                    if (lane == 0 && kept) atomicAdd(&s_out, kept);
+                    if (tid == 0) s_nd = nsort;
+                    ranked = true;
+                    __syncthreads();
+                } else {
+#pragma unroll
+                    for (int i = 0; i < ITEMS; i++) {
+                        const uint32_t j = jb + i * GKC_WAVE;
+                        if (j < nsort) { key[i] = s_keys[j]; cnt[i] = s_cnt[j]; }
+                    }
                 }
             }
         }
+        if (!ranked) {
         if (!heavy) {
             // 2a. LSD radix sort of the range (or of its distinct keys) in LDS
             //     (entries in registers, j = jb + 64 i)
@@ -1106,6 +1164,7 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
             __syncthreads();
             if (tid == 0) s_out += ctot;
         }
+        }   // !ranked
         if (tid == 0) hl.res[tile] = make_uint2(s_out, s_nd);
         __syncthreads();
     }

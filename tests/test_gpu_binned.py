@@ -154,3 +154,23 @@ def test_binned_presized_fallback(gpu):
         gt, st = _count(rs.reads, rs.offs, 31, 11, 8, mode=0, caps=caps)
         assert st["binned"] == 1 and st["fallbacks"] == fb and st["host_syncs"] == 1 + fb
         assert_tables_equal(gt, ot, 8, 31)
+
+
+@pytest.mark.parametrize("n_distinct,copies,minc", [(1500, 2, 1), (1500, 3, 3), (3000, 2, 1), (3000, 2, 2)])
+def test_heavy_window_in_lds(gpu, n_distinct, copies, minc):
+    """One 24-bit bin of 512 < n <= 8192 keys: a heavy window counted in LDS
+    by a workgroup; up to 2048 distinct keys ranked by comparison, more by
+    the LSD sort of the distinct keys."""
+    rng = np.random.default_rng(n_distinct + copies)
+    body = rng.choice(list(b"ACGT"), (n_distinct, 11)).astype(np.uint8)
+    seqs = []
+    for r in range(n_distinct):
+        s = b"ACGTACGTACGTACGTACG" + bytes(body[r].tolist()) + b"A"     # canonical = forward
+        seqs += [s] * (1 + r % copies)
+    seqs += [bytes(rng.choice(list(b"ACGT"), 150).tolist()) for _ in range(300)]
+    reads, offs = _reads(seqs)
+    ot = O.count(reads, offs, 31, 11, 1, min_count=minc)
+    _say("heavy window in LDS", n_distinct, copies, minc)
+    gt, st = _count(reads, offs, 31, 11, 1, mode=0, min_count=minc)
+    assert st["binned"] == 1 and st["heavy_ranges"] >= 1
+    assert_tables_equal(gt, ot, 1, 31)
