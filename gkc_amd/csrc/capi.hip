@@ -149,17 +149,17 @@ static int guarded(gkc_ctx* c, F&& f) {
 
 #include <map>
 static std::mutex g_occ_mu;
-static std::map<const void*, int> g_occ;
+static std::map<std::pair<const void*, size_t>, int> g_occ;
 // Resident workgroups per CU for a kernel (cached, so no runtime query runs
 // while a stream is being captured) times the CU count.
-static int occupancy_blocks(const void* fn, int nt, int n_cu) {
+static int occupancy_blocks(const void* fn, int nt, int n_cu, size_t dyn_lds = 0) {
     std::lock_guard<std::mutex> lk(g_occ_mu);
-    auto it = g_occ.find(fn);
+    auto it = g_occ.find({fn, dyn_lds});
     int b = 0;
     if (it != g_occ.end()) b = it->second;
     else {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, nt, 0) != hipSuccess || b < 1) { (void)hipGetLastError(); b = 1; }
-        g_occ[fn] = b;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, nt, dyn_lds) != hipSuccess || b < 1) { (void)hipGetLastError(); b = 1; }
+        g_occ[{fn, dyn_lds}] = b;
     }
     return b * n_cu;
 }
@@ -300,21 +300,20 @@ static uint64_t stage_batch(gkc_ctx* c, const char* reads, const uint64_t* offs,
 static void launch_superkmers(gkc_ctx* c, const ChunkDesc& cd, P1Params prm, BinSet bs, bool persistent = false,
                               const ChunkDesc* dcd = nullptr) {
     if (!dcd) dcd = c->d_chunk.as<ChunkDesc>();
-    c->d_ctr.ensure(64);
-    zero(c, c->d_ctr.p, 1);
     if (cd.n_tiles == 0 && !persistent) return;
     const bool win = prm.win_mm != nullptr;
     const bool wide = c->prm.p > 16;
     const void* fn = win ? (const void*)k_superkmers<uint64_t, true>
                          : wide ? (const void*)k_superkmers<uint64_t, false> : (const void*)k_superkmers<uint32_t, false>;
-    int grid = occupancy_blocks(fn, P1_NT, c->n_cu);
+    const size_t dyn = (size_t)prm.P * 12;          // per-partition counters
+    int grid = occupancy_blocks(fn, P1_NT, c->n_cu, dyn);
     if (!persistent) grid = std::min<int>(std::max<uint32_t>(cd.n_tiles, 1), grid);
     if (win)
-        hipLaunchKernelGGL((k_superkmers<uint64_t, true>), dim3(grid), dim3(P1_NT), 0, c->st, dcd, prm, bs, c->d_ctr.as<uint32_t>());
+        hipLaunchKernelGGL((k_superkmers<uint64_t, true>), dim3(grid), dim3(P1_NT), dyn, c->st, dcd, prm, bs);
     else if (wide)
-        hipLaunchKernelGGL((k_superkmers<uint64_t, false>), dim3(grid), dim3(P1_NT), 0, c->st, dcd, prm, bs, c->d_ctr.as<uint32_t>());
+        hipLaunchKernelGGL((k_superkmers<uint64_t, false>), dim3(grid), dim3(P1_NT), dyn, c->st, dcd, prm, bs);
     else
-        hipLaunchKernelGGL((k_superkmers<uint32_t, false>), dim3(grid), dim3(P1_NT), 0, c->st, dcd, prm, bs, c->d_ctr.as<uint32_t>());
+        hipLaunchKernelGGL((k_superkmers<uint32_t, false>), dim3(grid), dim3(P1_NT), dyn, c->st, dcd, prm, bs);
     HIPX(hipGetLastError());
 }
 
@@ -599,7 +598,7 @@ static bool partition_presized(gkc_ctx* c, const ChunkDesc& cd) {
     };
     // occupancy is queried (and cached) outside the capture
     (void)occupancy_blocks(c->prm.p > 16 ? (const void*)k_superkmers<uint64_t, false> : (const void*)k_superkmers<uint32_t, false>,
-                           P1_NT, c->n_cu);
+                           P1_NT, c->n_cu, (size_t)c->prm.n_parts * 12);
     if (!c->g_p1 || c->g_p1_key != key) {
         if (c->g_p1) { HIPX(hipGraphExecDestroy(c->g_p1)); c->g_p1 = nullptr; }
         c->g_p1 = capture(c, enqueue);

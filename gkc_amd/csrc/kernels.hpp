@@ -37,7 +37,7 @@ struct P1Params {
 };
 
 template <typename MM, bool WIN>
-__global__ void k_superkmers(const ChunkDesc* __restrict__ cdp, P1Params prm, BinSet bins, uint32_t* __restrict__ tile_ctr);
+__global__ void k_superkmers(const ChunkDesc* __restrict__ cdp, P1Params prm, BinSet bins);
 __global__ void k_tile_reads(const uint64_t* __restrict__ offs, uint32_t n_reads, uint64_t base, uint32_t n_tiles,
                              uint32_t* __restrict__ tile_r0);
 __global__ void k_hpc(const uint8_t* __restrict__ reads, const uint64_t* __restrict__ offs, uint32_t n_reads,

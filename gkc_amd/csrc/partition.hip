@@ -38,82 +38,182 @@ __global__ __launch_bounds__(256) void k_tile_reads(const uint64_t* __restrict__
 
 constexpr uint32_t P1_RO = 512;     // read offsets a tile stages in LDS (more: global search)
 
+constexpr int P1_NCW = (P1_T + MAXK + 32) / 16 + 2;     // packed code words of a staged tile
+constexpr int P1_MT = 8;                                  // tiles held per flush
+constexpr int P1_RCAP = 3072;                             // superkmer records held per flush
+// A flush follows any tile that leaves more than P1_RCAP - P1_T records, so
+// the next tile's (at most P1_T) always fit.  Flush-local counts stay below
+// 2^16: records <= P1_RCAP, k-mers <= P1_MT * P1_T, words <= k-mers / 16 +
+// 6 per record.
+static_assert(P1_RCAP % P1_NT == 0 && P1_RCAP >= P1_T, "record capacity");
+static_assert(P1_MT * P1_T < 65536 && P1_MT * P1_T / 16 + 6 * P1_RCAP < 65536, "16-bit flush-local counts");
+static_assert(P1_T <= 2048 && P1_MT <= 32, "record positions are slot << 11 | start in 16 bits");
+
+// base_code of 4 bytes (byte 0 first) packed into 8 bits, byte 0 in the top two.
+__device__ __forceinline__ uint32_t pack4(uint32_t v) {
+    const uint32_t e = ((v >> 2) ^ (v >> 1)) & 0x03030303u;   // code of byte i at bits [8i, 8i + 2)
+    return (e * 0x40100401u) >> 24;                            // byte i's code to bits [30 - 2i, 32 - 2i)
+}
+__device__ __forceinline__ uint32_t pack16(const uint4 v) {
+    return (pack4(v.x) << 24) | (pack4(v.y) << 16) | (pack4(v.z) << 8) | pack4(v.w);
+}
+// Codes of tile bases [j, j + n), 1 <= n <= 16, in the low 2n bits, base j highest.
+__device__ __forceinline__ uint32_t tile_codes(const uint32_t* sc, uint32_t j, uint32_t n) {
+    const uint64_t x = ((uint64_t)sc[j >> 4] << 32) | sc[(j >> 4) + 1];
+    return (uint32_t)((x << (2 * (j & 15))) >> (64 - 2 * n));
+}
+// The p-mer at tile base j (p <= 16 in 32 bits, p <= 32 in 64 bits).
+template <typename MM>
+__device__ __forceinline__ MM pmer_at(const uint32_t* sc, uint32_t j, int p);
+template <>
+__device__ __forceinline__ uint32_t pmer_at<uint32_t>(const uint32_t* sc, uint32_t j, int p) {
+    return tile_codes(sc, j, (uint32_t)p);
+}
+template <>
+__device__ __forceinline__ uint64_t pmer_at<uint64_t>(const uint32_t* sc, uint32_t j, int p) {
+    const uint32_t q = j >> 4, sh = 2 * (j & 15);
+    uint64_t x = ((uint64_t)sc[q] << 32) | sc[q + 1];
+    if (sh) x = (x << sh) | (sc[q + 2] >> (32 - sh));
+    return x >> (64 - 2 * p);
+}
+// Reverse complement of a p-mer held in the low 2p bits.
+__device__ __forceinline__ uint32_t rc_pmer(uint32_t f, int p) {
+    uint32_t y = __builtin_bitreverse32(~f);                      // bases reversed, each code's bits swapped
+    y = ((y >> 1) & 0x55555555u) | ((y & 0x55555555u) << 1);
+    return y >> (32 - 2 * p);
+}
+__device__ __forceinline__ uint64_t rc_pmer(uint64_t f, int p) { return revcomp64(f, p); }
+// new_filter2 (gkc_cuda.cu:214-216): a p-mer whose first three bases are
+// AAA, ACA or AGA (codes 0, 4, 8) is not a candidate; mode 0 accepts all.
+template <typename MM>
+__device__ __forceinline__ bool pmer_ok(MM v, int p, int mode) {
+    const uint32_t top = (uint32_t)(v >> (2 * (p - 3)));
+    return mode == 0 || ((0xFFFFFFFFFFFFFEEEull >> top) & 1ull);
+}
+template <typename MM>
+__device__ __forceinline__ uint32_t part_mm(MM mm, uint32_t P, int mode) {
+    if constexpr (sizeof(MM) == 4) {
+        if (mode != 1) return mm % P;
+    }
+    return part_of((uint64_t)mm, P, mode);
+}
+
+// One tile = P1_T window starts, P1_V per thread.  The tile's bases are
+// staged in LDS as packed 2-bit codes; every thread takes its first p-mer
+// from them directly and rolls the rest.  Tiles go to workgroups round-robin
+// and each workgroup loads the next tile's bases and read offsets into
+// registers while it works on the current one.
+// A workgroup keeps the superkmers of up to P1_MT tiles (their codes and one
+// record each) in LDS and reserves bin space for all of them at once: one
+// global atomic per (flush, partition, cursor) instead of per tile, since
+// device-scope atomics, not the arithmetic, bound this kernel otherwise.
 // With WIN, every candidate carries its position: the window minimum runs
 // over composites (value << 14 | no-candidate << 13 | 8191 - start), so ties
 // go to the later p-mer start (the reference's update order, gkc_cuda.cu:281-284)
 // and the chosen start and strand are recovered per window.
+// Dynamic LDS: P x (8 + 4) bytes (per-partition counters).
 template <typename MM, bool WIN>
-__global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __restrict__ cdp, P1Params prm,
-                                                       BinSet bins, uint32_t* __restrict__ tile_ctr) {
+__global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __restrict__ cdp, P1Params prm, BinSet bins) {
     constexpr int CSH = WIN ? 14 : 0;
+    constexpr int NQ = P1_RCAP / P1_NT;             // records per thread in a flush
     __shared__ int8_t s_str[WIN ? P1_T + MAXK : 1];
-    __shared__ __align__(16) uint8_t s_bytes[P1_T + MAXK + 32];
+    __shared__ uint32_t s_code[P1_MT][P1_NCW];      // staged tiles, one slot each
     __shared__ MM s_v[P1_T + MAXK];
     __shared__ MM s_last[P1_NT];
-    __shared__ uint16_t s_start[P1_T];
-    __shared__ uint16_t s_wend[P1_T];
-    __shared__ uint16_t s_part[P1_T];
+    // superkmer records: slot << 11 | tile-local start; k-mers (window end
+    // until the tile's records are complete); partition
+    __shared__ uint16_t s_rpos[P1_RCAP], s_rn[P1_RCAP], s_rpart[P1_RCAP];
     __shared__ uint32_t s_scan[P1_NT / GKC_WAVE];
     __shared__ uint32_t s_ro[P1_RO];              // offsets (chunk positions) of reads r0 .. r0 + n_ro - 1
-    // per partition: block-local [63:48] headers, [47:24] k-mers, [23:0] words;
-    // then the block's global bases [63:32] words, [31:0] headers
-    __shared__ unsigned long long s_loc[1024];
-    __shared__ uint32_t s_gk[1024];
-    __shared__ uint32_t s_tile, s_ns, s_ovf;
+    __shared__ uint32_t s_ns, s_ovf;
+    // per partition: flush-local [47:32] headers, [31:16] k-mers, [15:0] words;
+    // then the flush's global bases [63:32] words, [31:0] headers
+    extern __shared__ unsigned long long s_loc[];
+    const uint32_t P = prm.P;
+    uint32_t* s_gk = reinterpret_cast<uint32_t*>(s_loc + P);
 
     const ChunkDesc cd = *cdp;
     const int tid = threadIdx.x;
     const int k = prm.k, p = prm.p, w = k - p + 1;
     const MM M = (MM)((p >= (int)(sizeof(MM) * 4)) ? ~(MM)0 : (((MM)1 << (2 * p)) - 1));
     const MM MC = WIN ? (MM)((M << CSH) | (MM)0x3FFF) : M;     // largest composite
-    const uint32_t P = prm.P;
+    const uint32_t G = gridDim.x;
+    uint32_t tile = blockIdx.x;
+    if (tile >= cd.n_tiles) return;
 
-    if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
-    __syncthreads();
+    // reads holding tile t's window starts: a .. b (b holds the next tile's first position)
+    auto tile_reads = [&](uint32_t t, uint32_t& a, uint32_t& b) {
+        uint32_t tv = t;
+        asm volatile("" : "+v"(tv));                // per-lane loads, off the scalar cache path
+        a = cd.tile_r0[tv];
+        b = tv + 1 < cd.n_tiles ? cd.tile_r0[tv + 1] : cd.n_reads - 1;
+    };
+    uint4 pf = make_uint4(0u, 0u, 0u, 0u);          // this thread's 16 bases of the tile
+    uint32_t pro0 = 0, pro1 = 0;                    // read offsets tid, tid + P1_NT of the tile
+    auto fetch = [&](uint32_t t, uint32_t q0, uint32_t q1) {
+        const uint32_t tt0 = t * P1_T;
+        const uint32_t bl = min((uint32_t)(P1_T + MAXK + 32), cd.n_bases - tt0);
+        if (tid < P1_NCW && (uint32_t)tid * 16 + 16 <= bl) pf = reinterpret_cast<const uint4*>(cd.reads + tt0)[tid];
+        const uint32_t n = q1 - q0 + 2;
+        if (n <= P1_RO) {
+            if ((uint32_t)tid < n) pro0 = (uint32_t)(cd.offs[q0 + tid] - cd.base);
+            if ((uint32_t)tid + P1_NT < n) pro1 = (uint32_t)(cd.offs[q0 + tid + P1_NT] - cd.base);
+        }
+    };
+    uint32_t c_r0, c_r1, n_r0 = 0, n_r1 = 0;
+    tile_reads(tile, c_r0, c_r1);
+    fetch(tile, c_r0, c_r1);
+    if (tile + G < cd.n_tiles) tile_reads(tile + G, n_r0, n_r1);
+    for (uint32_t i = tid; i < P; i += P1_NT) s_loc[i] = 0;
+    uint32_t acc = 0, slot = 0;                     // records and tiles held
+
     for (;;) {
-        const uint32_t tile = s_tile;
-        if (tile >= cd.n_tiles) break;
-        uint32_t next_tile = 0;
-        if (tid == 0) next_tile = atomicAdd(tile_ctr, 1u);      // claimed now, read at the end of this tile
         const uint32_t t0 = tile * P1_T;
         const uint32_t tlen = min((uint32_t)P1_T, cd.n_bases - t0);      // window starts owned
-        const uint32_t blen = min((uint32_t)(P1_T + MAXK + 32), cd.n_bases - t0);  // bytes staged
-        // reads holding this tile's window starts: r0 .. r1 (r1 holds the next tile's first position)
-        const uint32_t r0 = cd.tile_r0[tile];
-        const uint32_t r1 = tile + 1 < cd.n_tiles ? cd.tile_r0[tile + 1] : cd.n_reads - 1;
+        const uint32_t blen = min((uint32_t)(P1_T + MAXK + 32), cd.n_bases - t0);  // bases staged
+        const uint32_t r0 = c_r0, r1 = c_r1;
         const uint32_t n_ro = r1 - r0 + 2;                                 // offsets r0 .. r1 + 1
         const bool ro_lds = n_ro <= P1_RO;
+        uint32_t* const sc = s_code[slot];
 
-        // 1. stage bytes [t0, t0+blen) in LDS with 16-byte loads (t0 is 16-aligned)
-        {
-            const uint4* src = reinterpret_cast<const uint4*>(cd.reads + t0);
-            uint4* dst = reinterpret_cast<uint4*>(s_bytes);
-            const uint32_t nv = blen >> 4;
-            for (uint32_t i = tid; i < nv; i += P1_NT) dst[i] = src[i];
-            for (uint32_t i = (nv << 4) + tid; i < (uint32_t)(P1_T + MAXK + 32); i += P1_NT)
-                s_bytes[i] = (i < blen) ? cd.reads[t0 + i] : (uint8_t)'A';
-            for (int i = tid; i < 1024 && i < (int)P; i += P1_NT) s_loc[i] = 0;
-            if (tid == 0) s_ovf = 0;
-            if (ro_lds)
-                for (uint32_t i = tid; i < n_ro; i += P1_NT) s_ro[i] = (uint32_t)(cd.offs[r0 + i] - cd.base);
+        // 1. stage: packed codes of bases [t0, t0 + blen) (code 0 past blen), read offsets
+        if (tid < P1_NCW) {
+            const uint32_t b = tid * 16;
+            uint32_t wd = 0;
+            if (b + 16 <= blen) wd = pack16(pf);
+            else
+                for (uint32_t t = 0; t < 16; t++)
+                    if (b + t < blen) wd |= base_code(cd.reads[t0 + b + t]) << (30 - 2 * t);
+            sc[tid] = wd;
+        }
+        if (ro_lds) {
+            if ((uint32_t)tid < n_ro) s_ro[tid] = pro0;
+            if ((uint32_t)tid + P1_NT < n_ro) s_ro[tid + P1_NT] = pro1;
         }
         __syncthreads();
+        const uint32_t nt = tile + G;
+        const bool more = nt < cd.n_tiles;
+        if (more) {                                 // the next tile's inputs load during this one
+            fetch(nt, n_r0, n_r1);
+            c_r0 = n_r0; c_r1 = n_r1;
+            if (nt + G < cd.n_tiles) tile_reads(nt + G, n_r0, n_r1);
+        }
 
         // 2. candidate value per p-mer start j: min over the passing forward and
         //    reverse-complement p-mer (M if neither passes), rolled over V starts.
         auto fill_v = [&](int j0, int cnt) {
-            MM f = 0, r = 0;
+            MM f = pmer_at<MM>(sc, j0, p);
+            MM r = rc_pmer(f, p);
+            uint32_t rw = tile_codes(sc, j0 + p, 16);           // bases rolled in next
             const int rs = 2 * (p - 1);
-            for (int t = 0; t < p - 1; t++) {
-                uint32_t c = base_code(s_bytes[j0 + t]);
-                f = (f << 2) | c;
-                r = (r >> 2) | ((MM)(3u - c) << rs);
-            }
             for (int s = 0; s < cnt; s++) {
-                uint32_t c = base_code(s_bytes[j0 + s + p - 1]);
-                f = ((f << 2) | c) & M;
-                r = (r >> 2) | ((MM)(3u - c) << rs);
-                const bool fp = pmer_pass(f, p, prm.fmode), rp = pmer_pass(r, p, prm.fmode);
+                if (s > 0) {
+                    const uint32_t c = rw >> 30;
+                    rw <<= 2;
+                    f = ((f << 2) | c) & M;
+                    r = (r >> 2) | ((MM)(3u - c) << rs);
+                }
+                const bool fp = pmer_ok(f, p, prm.fmode), rp = pmer_ok(r, p, prm.fmode);
                 if constexpr (WIN) {
                     MM val; int sent = 0; int8_t st;
                     if (fp && rp) { val = f < r ? f : r; st = f < r ? 0 : (r < f ? 1 : -1); }
@@ -173,13 +273,12 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
         }
         __syncthreads();
 
-        // 4. validity and boundaries
+        // 4. validity and boundaries (chunk positions fit 32 bits)
         uint32_t bmask = 0;
         uint32_t wend[P1_V];
         {
             const uint32_t g0 = t0 + tid * P1_V;
-            uint32_t r = 0;
-            uint64_t rbeg = 0, rnext = 0, rend = 0;
+            uint32_t r = 0, rbeg = 0, rnext = 0, rend = 0;
             if (tid * P1_V < (int)tlen) {
                 const uint32_t gs = max(g0, cd.lead);
                 if (ro_lds) {           // largest i with s_ro[i] <= gs, in LDS
@@ -192,8 +291,8 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
                 } else {
                     r = find_read(cd.offs, cd.n_reads, cd.base + gs);
                 }
-                rbeg = ro_lds ? s_ro[r - r0] : cd.offs[r] - cd.base;
-                rnext = ro_lds ? s_ro[r - r0 + 1] : cd.offs[r + 1] - cd.base;
+                rbeg = ro_lds ? s_ro[r - r0] : (uint32_t)(cd.offs[r] - cd.base);
+                rnext = ro_lds ? s_ro[r - r0 + 1] : (uint32_t)(cd.offs[r + 1] - cd.base);
                 rend = cd.lens ? rbeg + (uint32_t)cd.lens[r] : rnext;
             }
             MM prev = tid > 0 ? s_last[tid - 1] : (MM)0;
@@ -210,15 +309,14 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
                     while (g >= rnext) {            // advance (skips empty reads)
                         r++;
                         rbeg = rnext;
-                        rnext = ro_lds ? s_ro[r - r0 + 1] : cd.offs[r + 1] - cd.base;
+                        rnext = ro_lds ? s_ro[r - r0 + 1] : (uint32_t)(cd.offs[r + 1] - cd.base);
                         rend = cd.lens ? rbeg + (uint32_t)cd.lens[r] : rnext;
                     }
-                    const bool valid = g >= rbeg && (uint64_t)g + k <= rend;
+                    const bool valid = g >= rbeg && g < rend && rend - g >= (uint32_t)k;
                     if (valid) {
                         const bool first = (g == rbeg) || (g == t0);
                         if (first || mm[s] != prev) bmask |= 1u << s;
-                        uint64_t we = rend - k + 1 - t0;           // tile-local end of windows
-                        wend[s] = (uint32_t)min(we, (uint64_t)tlen);
+                        wend[s] = min(rend + 1 - (uint32_t)k - t0, tlen);     // tile-local end of windows
                         if (prm.mm_out) prm.mm_out[g] = (uint32_t)mm[s];
                         if constexpr (WIN) {
                             if (prm.win_mm) {
@@ -230,11 +328,8 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
                             }
                         }
                         if (prm.sketch && ((g - rbeg) & 15u) == 0) {
-                            uint32_t q = 0;
-                            const int jl = g - t0;
-#pragma unroll
-                            for (int t = 0; t < 4; t++) q = (q << 2) | base_code(s_bytes[jl + t]);
-                            atomicAdd(&prm.sketch[(size_t)part_of(mm[s], P, prm.pmode) * 256 + q], 1u);
+                            const uint32_t q = tile_codes(sc, g - t0, 4);
+                            atomicAdd(&prm.sketch[(size_t)part_mm<MM>(mm[s], P, prm.pmode) * 256 + q], 1u);
                         }
                     }
                 }
@@ -242,7 +337,8 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
             }
         }
 
-        // 5. compact the superkmer starts in tile order (wave scans + wave sums)
+        // 5. append the tile's superkmer starts to the records, in tile order
+        //    (wave scans + wave sums)
         {
             const uint32_t c = __popc(bmask);
             const uint32_t lane = __lane_id(), wv = tid / GKC_WAVE;
@@ -254,94 +350,107 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
             }
             if (lane == GKC_WAVE - 1) s_scan[wv] = inc;
             __syncthreads();
-            uint32_t o = inc - c, tot = 0;
+            uint32_t o = acc + inc - c, tot = 0;
 #pragma unroll
             for (int w2 = 0; w2 < P1_NT / GKC_WAVE; w2++) { const uint32_t x = s_scan[w2]; o += (uint32_t)w2 < wv ? x : 0u; tot += x; }
             if (tid == 0) s_ns = tot;
 #pragma unroll
             for (int s = 0; s < P1_V; s++) {
                 if (bmask & (1u << s)) {
-                    s_start[o] = (uint16_t)(tid * P1_V + s);
-                    s_wend[o] = (uint16_t)wend[s];
-                    s_part[o] = (uint16_t)part_of(mm[s], P, prm.pmode);
+                    s_rpos[o] = (uint16_t)((slot << 11) | (tid * P1_V + s));
+                    s_rn[o] = (uint16_t)wend[s];
+                    s_rpart[o] = (uint16_t)part_mm<MM>(mm[s], P, prm.pmode);
                     o++;
                 }
             }
         }
         __syncthreads();
-
-        // 6. sizes and per-partition block-local offsets.  One 64-bit LDS atomic
-        //    per superkmer reserves its header, k-mer and word slots together, so
-        //    the three run in the same order inside a partition.
+        // 6. k-mers of each new record: up to the next superkmer's start or its read's last window
         const uint32_t ns = s_ns;
-        unsigned long long loc[P1_V];
-#pragma unroll
-        for (int q = 0; q < P1_V; q++) {
-            const uint32_t i = tid + q * P1_NT;
-            if (i < ns) {
-                const uint32_t a = s_start[i];
-                uint32_t e = (i + 1 < ns) ? (uint32_t)s_start[i + 1] : tlen;
-                e = min(e, (uint32_t)s_wend[i]);
-                const unsigned long long n = e - a;
-                const unsigned long long nw = (n + k - 1 + 15) >> 4;
-                loc[q] = atomicAdd(&s_loc[s_part[i]], (1ull << 48) | (n << 24) | nw);
-            }
+        for (uint32_t i = acc + tid; i < acc + ns; i += P1_NT) {
+            const uint32_t a = s_rpos[i] & 2047u;
+            uint32_t e = (i + 1 < acc + ns) ? (uint32_t)(s_rpos[i + 1] & 2047u) : tlen;
+            e = min(e, (uint32_t)s_rn[i]);
+            s_rn[i] = (uint16_t)(e - a);
         }
-        __syncthreads();
-        // 7. one global reservation per (block, partition)
-        for (uint32_t q = tid; q < P; q += P1_NT) {
-            const unsigned long long v = s_loc[q];
-            const uint32_t h = (uint32_t)(v >> 48), kk = (uint32_t)(v >> 24) & 0xFFFFFFu, ww = (uint32_t)v & 0xFFFFFFu;
-            if (h) {
-                const uint32_t gh = atomicAdd(&bins.hdr_cur[q], h);
-                const uint32_t gw = atomicAdd(&bins.word_cur[q], ww);
-                const uint32_t gk = atomicAdd(&bins.kmer_cur[q], kk);
-                if (prm.scatter && (gh + h > bins.hdr_cap[q] || gw + ww > bins.word_cap[q])) {
-                    s_ovf = 1;
-                    atomicOr(bins.overflow, 1u);
-                }
-                s_loc[q] = ((unsigned long long)gw << 32) | gh;
-                s_gk[q] = gk;
-            }
-        }
-        __syncthreads();
-        // 8. write headers and 2-bit packed bases
-        if (prm.scatter && !s_ovf) {
+        acc += ns;
+        slot++;
+
+        // 7. flush: reserve bin space for the held records, then write them
+        if (!more || slot == P1_MT || acc > P1_RCAP - P1_T) {
+            __syncthreads();
+            // flush-local offsets: one 64-bit LDS atomic per record reserves its
+            // header, k-mer and word slots together, so the three run in the
+            // same order inside a partition
+            unsigned long long loc[NQ];
 #pragma unroll
-            for (int q = 0; q < P1_V; q++) {
+            for (int q = 0; q < NQ; q++) {
                 const uint32_t i = tid + q * P1_NT;
-                if (i < ns) {
-                    const uint32_t a = s_start[i];
-                    uint32_t e = (i + 1 < ns) ? (uint32_t)s_start[i + 1] : tlen;
-                    e = min(e, (uint32_t)s_wend[i]);
-                    const uint32_t n = e - a;
-                    const uint32_t nb = n + k - 1;
-                    const uint32_t nw = (nb + 15) >> 4;
-                    const uint32_t part = s_part[i];
-                    const unsigned long long gb = s_loc[part];
-                    const uint32_t wo = (uint32_t)(gb >> 32) + (uint32_t)(loc[q] & 0xFFFFFFu);
-                    SkmHdr h;
-                    h.word_off = wo;
-                    h.kmer_off = s_gk[part] + (uint32_t)((loc[q] >> 24) & 0xFFFFFFu);
-                    h.n_kmers = n;
-                    h.reserved = 0;
-                    bins.hdrs[bins.hdr_base[part] + (uint32_t)gb + (uint32_t)(loc[q] >> 48)] = h;
-                    uint32_t* dst = bins.words + bins.word_base[part] + wo;
-                    for (uint32_t wi = 0; wi < nw; wi++) {
-                        uint32_t x = 0;
-                        const uint32_t b0 = a + wi * 16;
+                loc[q] = 0;
+                if (i < acc) {
+                    const unsigned long long n = s_rn[i];
+                    const unsigned long long nw = (n + k - 1 + 15) >> 4;
+                    loc[q] = atomicAdd(&s_loc[s_rpart[i]], (1ull << 32) | (n << 16) | nw);
+                }
+            }
+            if (tid == 0) s_ovf = 0;
+            __syncthreads();
+            // one global reservation per (flush, partition)
+            for (uint32_t q = tid; q < P; q += P1_NT) {
+                const unsigned long long v = s_loc[q];
+                const uint32_t h = (uint32_t)(v >> 32), kk = (uint32_t)(v >> 16) & 0xFFFFu, ww = (uint32_t)v & 0xFFFFu;
+                if (h) {
+                    const uint32_t gh = atomicAdd(&bins.hdr_cur[q], h);
+                    const uint32_t gw = atomicAdd(&bins.word_cur[q], ww);
+                    const uint32_t gk = atomicAdd(&bins.kmer_cur[q], kk);
+                    if (prm.scatter && (gh + h > bins.hdr_cap[q] || gw + ww > bins.word_cap[q])) {
+                        s_ovf = 1;
+                        atomicOr(bins.overflow, 1u);
+                    }
+                    s_loc[q] = ((unsigned long long)gw << 32) | gh;
+                    s_gk[q] = gk;
+                }
+            }
+            __syncthreads();
+            // headers and 2-bit packed bases (funnel shifts of the staged codes)
+            if (prm.scatter && !s_ovf) {
 #pragma unroll
-                        for (int t = 0; t < 16; t++) {
-                            uint32_t c = (b0 + t < a + nb) ? base_code(s_bytes[b0 + t]) : 0u;
-                            x |= c << (30 - 2 * t);
+                for (int q = 0; q < NQ; q++) {
+                    const uint32_t i = tid + q * P1_NT;
+                    if (i < acc) {
+                        const uint32_t rp = s_rpos[i];
+                        const uint32_t* csl = s_code[rp >> 11];
+                        const uint32_t a = rp & 2047u;
+                        const uint32_t n = s_rn[i];
+                        const uint32_t nb = n + k - 1;
+                        const uint32_t nw = (nb + 15) >> 4;
+                        const uint32_t part = s_rpart[i];
+                        const unsigned long long gb = s_loc[part];
+                        const uint32_t wo = (uint32_t)(gb >> 32) + (uint32_t)(loc[q] & 0xFFFFu);
+                        SkmHdr h;
+                        h.word_off = wo;
+                        h.kmer_off = s_gk[part] + (uint32_t)((loc[q] >> 16) & 0xFFFFu);
+                        h.n_kmers = n;
+                        h.reserved = 0;
+                        bins.hdrs[bins.hdr_base[part] + (uint32_t)gb + (uint32_t)(loc[q] >> 32)] = h;
+                        uint32_t* dst = bins.words + bins.word_base[part] + wo;
+                        for (uint32_t wi = 0; wi < nw; wi++) {
+                            uint32_t x = tile_codes(csl, a + wi * 16, 16);
+                            const uint32_t rem = nb - wi * 16;
+                            if (rem < 16) x &= ~0u << (32 - 2 * rem);     // zero past the last base
+                            dst[wi] = x;
                         }
-                        dst[wi] = x;
                     }
                 }
             }
+            __syncthreads();
+            for (uint32_t i = tid; i < P; i += P1_NT) s_loc[i] = 0;
+            acc = 0;
+            slot = 0;
         }
-        if (tid == 0) s_tile = next_tile;
         __syncthreads();
+        if (!more) break;
+        tile = nt;
     }
 }
 
@@ -414,8 +523,8 @@ __global__ __launch_bounds__(256) void k_skm_lists(const uint64_t* __restrict__ 
     }
 }
 
-template __global__ void k_superkmers<uint32_t, false>(const ChunkDesc*, P1Params, BinSet, uint32_t*);
-template __global__ void k_superkmers<uint64_t, false>(const ChunkDesc*, P1Params, BinSet, uint32_t*);
-template __global__ void k_superkmers<uint64_t, true>(const ChunkDesc*, P1Params, BinSet, uint32_t*);
+template __global__ void k_superkmers<uint32_t, false>(const ChunkDesc*, P1Params, BinSet);
+template __global__ void k_superkmers<uint64_t, false>(const ChunkDesc*, P1Params, BinSet);
+template __global__ void k_superkmers<uint64_t, true>(const ChunkDesc*, P1Params, BinSet);
 
 }  // namespace gkc
