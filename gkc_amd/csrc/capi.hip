@@ -748,6 +748,7 @@ int gkc_set_tuning(gkc_ctx* c, const gkc_tuning* t) {
     return guarded(c, [&] {
         if (!t) throw ArgError{"null tuning", GKC_EINVAL};
         if (t->count_mode < 0 || t->count_mode > 1) throw ArgError{"count_mode must be 0 or 1", GKC_EINVAL};
+        if (t->bin_passes < 0 || t->bin_passes > 3) throw ArgError{"bin_passes must be 0..3", GKC_EINVAL};
         c->tune = *t;
         return 0;
     });
@@ -1162,7 +1163,7 @@ template <typename K>
 static BatchShape batch_shape(const gkc_ctx* c) {
     const int kb = 2 * c->prm.k;
     if (c->tune.count_mode == 1 || c->prm.k < 8) return BatchShape{false, false, c->npass, 0u, RL_TILE, 0};
-    const int np = std::min(3, (kb + 7) / 8);
+    const int np = std::min(c->tune.bin_passes > 0 ? c->tune.bin_passes : 3, (kb + 7) / 8);
     return BatchShape{true, true, np, (uint32_t)std::max(0, kb - 8 * np), WC_TILE, 8 * np};
 }
 

@@ -78,8 +78,9 @@ typedef struct gkc_tuning {
     int32_t sort_items;          /* keys per thread of a onesweep tile: 8,12,16,20,24 (u64) / 4,8,12 (u128) */
     int32_t sort_blocks_per_cu;  /* persistent workgroups per CU in the sort passes (0: occupancy limit) */
     int32_t single_list;         /* 1: batched tiles in one global order instead of one work list per XCD */
-    int32_t count_mode;          /* batched form: 0 binned (2 onesweep passes on the top 16 key bits, then
-                                    LDS sort + count per tile of bins; default for k >= 8), 1 full LSD sort */
+    int32_t count_mode;          /* batched form: 0 binned (onesweep passes on the top key bits, then
+                                    per-window hash count of whole bins; default for k >= 8), 1 full LSD sort */
+    int32_t bin_passes;          /* binned form: 8-bit onesweep passes before the count, 1..3 (0: automatic) */
 } gkc_tuning;
 
 const char *gkc_last_error(void);
