@@ -190,6 +190,7 @@ __device__ __forceinline__ u128k canon_of(u128k x, int k) {
 // of them, prefix-sums their k-mer counts and strides its lanes over the
 // combined k-mers, so consecutive lanes write consecutive keys (a scatter
 // block's superkmers of one partition have consecutive k-mer offsets).
+constexpr int EX_U = 4;
 template <typename K>
 __global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const SkmHdr* __restrict__ hdrs,
                                                      const uint64_t* __restrict__ hdr_base,
@@ -234,18 +235,34 @@ __global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, Se
         __syncthreads();
         const uint32_t* wseg = words + word_base[seg];
         K* kseg = keys + sa.key_start[seg];
-        for (uint32_t q = lane; q < T; q += GKC_WAVE) {
-            // header of k-mer q: largest j with pre[j] <= q
-            uint32_t lo = 0, hi = GKC_WAVE;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_pre[wv][mid] <= q) lo = mid; else hi = mid;
+        // EX_U k-mers per lane at a time, in phases (header search, bases,
+        // store), so their memory latencies overlap
+        for (uint32_t q0 = lane; q0 < T; q0 += EX_U * GKC_WAVE) {
+            uint32_t hj[EX_U], ho[EX_U];
+#pragma unroll
+            for (int u = 0; u < EX_U; u++) {
+                const uint32_t q = min(q0 + u * GKC_WAVE, T - 1);
+                // header of k-mer q: largest j with pre[j] <= q
+                uint32_t lo = 0, hi = GKC_WAVE;
+#pragma unroll
+                for (int it = 0; it < 6; it++) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_pre[wv][mid] <= q) lo = mid; else hi = mid;
+                }
+                hj[u] = lo;
+                ho[u] = q - s_pre[wv][lo];
             }
-            const uint32_t o = q - s_pre[wv][lo];
-            const K c = canon_of(kmer_at<K>(wseg + s_woff[wv][lo], o, k), k);
-            kseg[s_koff[wv][lo] + o] = c;
-            for (int ps = 0; ps < npass; ps++)
-                atomicAdd(&s_hist[ps * RADIX + KeyTraits<K>::digit(c, sh0 + ps * RADIX_BITS)], 1u);
+            K c[EX_U];
+#pragma unroll
+            for (int u = 0; u < EX_U; u++) c[u] = canon_of(kmer_at<K>(wseg + s_woff[wv][hj[u]], ho[u], k), k);
+#pragma unroll
+            for (int u = 0; u < EX_U; u++) {
+                if (q0 + u * GKC_WAVE < T) {
+                    kseg[s_koff[wv][hj[u]] + ho[u]] = c[u];
+                    for (int ps = 0; ps < npass; ps++)
+                        atomicAdd(&s_hist[ps * RADIX + KeyTraits<K>::digit(c[u], sh0 + ps * RADIX_BITS)], 1u);
+                }
+            }
         }
         __syncthreads();
         uint32_t* gh = sa.hist + (size_t)seg * MAX_PASS * RADIX;
@@ -281,7 +298,7 @@ __global__ __launch_bounds__(256) void k_hist_scan(Plan* __restrict__ plan, uint
 }
 
 #ifndef ONESWEEP_WAVES_PER_EU
-#define ONESWEEP_WAVES_PER_EU 4
+#define ONESWEEP_WAVES_PER_EU 3
 #endif
 // One onesweep pass over all segments.  Tile = SORT_NT * ITEMS keys of one
 // segment; tiles are claimed in order from the work lists, so every tile's
@@ -374,18 +391,25 @@ __global__ __launch_bounds__(SORT_NT, ONESWEEP_WAVES_PER_EU) void k_onesweep(Pla
             }
         }
         // 4. stable rank within the wave
+        const uint64_t lt = lanemask_lt();
 #pragma unroll
         for (int i = 0; i < ITEMS; i++) {
             const uint32_t dd = dig[i];
-            uint64_t peers = __ballot(dd < RADIX);
+            // peers (lanes with this lane's digit): the complement of the OR
+            // over the digit bits of ballot(bit) ^ own bit, in 32-bit halves
+            const uint64_t v = __ballot(dd < RADIX);
+            uint32_t mlo = ~(uint32_t)v, mhi = ~(uint32_t)(v >> 32);
 #pragma unroll
             for (int b = 0; b < RADIX_BITS; b++) {
-                const uint64_t bal = __ballot((dd >> b) & 1u);
-                peers &= ((dd >> b) & 1u) ? bal : ~bal;
+                const uint32_t bm = (uint32_t)((int32_t)(dd << (31 - b)) >> 31);     // 0 or ~0
+                const uint64_t bal = __ballot(bm != 0u);
+                mlo |= (uint32_t)bal ^ bm;
+                mhi |= (uint32_t)(bal >> 32) ^ bm;
             }
             if (dd < RADIX) {
+                const uint64_t peers = ~(((uint64_t)mhi << 32) | mlo);
                 const uint32_t before = s_whist[wv][dd];
-                const uint64_t lower = peers & lanemask_lt();
+                const uint64_t lower = peers & lt;
                 dig[i] = dd | ((before + (uint32_t)__popcll(lower)) << 9);
                 if (lower == 0) s_whist[wv][dd] = before + (uint32_t)__popcll(peers);
             }

@@ -23,10 +23,10 @@ def _say(*a):
     print("[gpu-case]", *a, flush=True)
 
 
-def _count(reads, offs, k, p, P, mode, hpc=False, min_count=1, max_count=0, batches=2, caps=None):
+def _count(reads, offs, k, p, P, mode, hpc=False, min_count=1, max_count=0, batches=2, caps=None, bin_passes=0):
     n = len(offs) - 1
     with capi.Counter(k, p, P, hpc=hpc, min_count=min_count, max_count=max_count) as c:
-        c.set_tuning(count_mode=mode)
+        c.set_tuning(count_mode=mode, bin_passes=bin_passes)
         bounds = np.linspace(0, n, batches + 1).astype(int)
         for a, b in zip(bounds[:-1], bounds[1:]):
             if b > a:
@@ -67,6 +67,23 @@ def test_binned_matches_oracle_and_lsd(gpu, cid, scale, k, p, P):
     lt, sl = _count(rs.reads, rs.offs, k, p, P, mode=1)
     assert sl["binned"] == 0 and sl["n_pass"] == (2 * k + 7) // 8
     assert_tables_equal(lt, ot, P, k)
+
+
+@pytest.mark.parametrize("k,p", [(31, 11), (45, 15)])
+def test_bin_passes(gpu, k, p):
+    """Every number of onesweep passes before the count gives the oracle's
+    table: fewer passes leave larger bins, so more windows are heavy."""
+    w = W.workload(1, scale=0.03)
+    rs = W.generate(w)
+    ot = O.count(rs.reads, rs.offs, k, p, 4)
+    heavy = []
+    for npss in (1, 2, 3):
+        _say("bin_passes", npss, k)
+        gt, st = _count(rs.reads, rs.offs, k, p, 4, mode=0, bin_passes=npss)
+        assert st["binned"] == 1 and st["n_pass"] == npss
+        assert_tables_equal(gt, ot, 4, k)
+        heavy.append(st["heavy_ranges"])
+    assert heavy[0] >= heavy[1] >= heavy[2]
 
 
 @pytest.mark.parametrize("minc,maxc", [(2, 0), (1, 3), (3, 40)])
