@@ -773,7 +773,6 @@ __device__ __forceinline__ uint32_t digit_scan(uint32_t v, uint32_t tid, uint32_
 // The kept entries of a range at a go to staging, keys at alt[a + i] and
 // counts as u32 over the range's own keys (they trail the reads: entry i
 // never overtakes key 2i), and {kept, distinct} to hl.res[tile].
-constexpr uint32_t RANK_MAX = 2048;      // distinct keys a heavy window ranks by comparison
 
 template <typename K>
 __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, SegArrays sa, K* __restrict__ keys,
@@ -795,9 +794,12 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
     __shared__ uint32_t s_ck[NCH];               // kept entries of each chunk, then their exclusive prefix
     __shared__ uint32_t s_red[NWAVE];
     __shared__ uint32_t s_orv[NWAVE][4];
-    __shared__ uint32_t s_seg, s_idx, s_ok, s_nd, s_out, s_nh;
+    __shared__ uint32_t s_seg, s_idx, s_ok, s_nd, s_out, s_nh, s_occ, s_hfail;
     const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE, tid = threadIdx.x;
     const uint32_t wbase = wv * GKC_WAVE * ITEMS;     // wave wv holds chunk keys [wbase, wbase + 64*ITEMS)
+    constexpr uint64_t EMPTY = ~0ull;  // never canonical: its reverse complement (0) is smaller
+    constexpr int LOG_CAP = __builtin_ctz(CAP);
+    constexpr uint32_t HASH_LIMIT = CAP / 4 * 3;   // distinct keys a hashed range may hold
     for (;;) {
         if (tid == 0) {
             uint32_t sg = 0, ix = 0;
@@ -808,8 +810,14 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
                 sg = find_seg(sa.rl_tile_start, plan->n_segs, t);
                 ix = t - sa.rl_tile_start[sg];
             }
-            s_ok = ok; s_seg = sg; s_idx = ix; s_nd = 0; s_out = 0; s_nh = 0;
+            s_ok = ok; s_seg = sg; s_idx = ix; s_nd = 0; s_out = 0; s_nh = 0; s_occ = 0; s_hfail = 0;
         }
+        // 64-bit keys: equal keys are counted in an LDS hash table (while the
+        // range is read, CAP keys at a time), so only the distinct keys are
+        // sorted, with their counts; ranges with more than HASH_LIMIT distinct
+        // keys are sorted whole instead
+        if constexpr (sizeof(K) == 8)
+            for (uint32_t i = tid; i < CAP; i += BC_NT) { s_keys[i] = EMPTY; s_cnt[i] = 0; }
         __syncthreads();
         if (!s_ok) break;
         const uint32_t seg = s_seg, ti = s_idx;
@@ -824,17 +832,42 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
         const bool heavy = n > CAP;                             // larger than LDS: chunks through global memory
         K* src = keys + sa.key_start[seg] + rg.x;
         K* dst = alt + sa.key_start[seg] + rg.x;
-        // 1. bits that vary inside the range
+        // 1. bits that vary inside the range (and, for 64-bit keys, the hash count)
         K key[ITEMS];
         uint64_t olo = 0, ohi = 0;
         {
             K ref{};
             if (n) ref = src[0];
             for (uint32_t c0 = 0; c0 < n; c0 += CAP) {
-#pragma unroll
+#pragma unroll(sizeof(K) == 8 ? 2 : ITEMS)
                 for (int i = 0; i < ITEMS; i++) {
                     const uint32_t j = c0 + jb + i * GKC_WAVE;
-                    if (j < n) { key[i] = src[j]; or_diff(olo, ohi, key[i], ref); }
+                    if (j < n) {
+                        if constexpr (sizeof(K) != 8) {
+                            key[i] = src[j];
+                            or_diff(olo, ohi, key[i], ref);
+                        } else {
+                            const K xk = src[j];          // not kept: the table holds the keys
+                            or_diff(olo, ohi, xk, ref);
+                            const unsigned long long x = xk;
+                            uint32_t h = (uint32_t)((x * 0x9E3779B97F4A7C15ull) >> (64 - LOG_CAP));
+                            // bounded probe: a full table (possible only past
+                            // HASH_LIMIT) fails the range instead of looping
+                            // (a failed range stops inserting: its probes would only lengthen)
+                            bool done = *(volatile uint32_t*)&s_hfail != 0;
+                            for (uint32_t pr = 0; pr < CAP && !done; pr++) {
+                                const unsigned long long old =
+                                    atomicCAS(reinterpret_cast<unsigned long long*>(&s_keys[h]), (unsigned long long)EMPTY, x);
+                                if (old == EMPTY || old == x) {
+                                    atomicAdd(&s_cnt[h], 1u);
+                                    if (old == EMPTY && atomicAdd(&s_occ, 1u) + 1 > HASH_LIMIT) s_hfail = 1;
+                                    done = true;
+                                }
+                                h = (h + 1) & (CAP - 1);
+                            }
+                            if (!done) s_hfail = 1;
+                        }
+                    }
                 }
             }
         }
@@ -856,31 +889,20 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
         uint32_t cnt[ITEMS];
         // 64-bit keys in LDS: equal keys are counted first in an LDS hash table,
         // so only the distinct keys are sorted (with their counts)
-        const bool hashed = sizeof(K) == 8 && !heavy;
+        // (the table and s_hfail are complete: the range's reads and inserts
+        // precede the barrier above)
+        const bool hashed = sizeof(K) == 8 && s_hfail == 0;
+        if (sizeof(K) == 8 && !hashed && !heavy) {     // sorted whole in LDS: its keys in registers
+#pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                const uint32_t j = jb + i * GKC_WAVE;
+                if (j < n) key[i] = src[j];
+            }
+        }
         bool ranked = false;                   // staged already by rank of comparison
         uint32_t nsort = n;                    // entries sorted in LDS
         if constexpr (sizeof(K) == 8) {
             if (hashed) {
-                constexpr uint64_t EMPTY = ~0ull;  // never canonical: its reverse complement (0) is smaller
-                constexpr int LOG_CAP = __builtin_ctz(CAP);
-                for (uint32_t i = tid; i < CAP; i += BC_NT) { s_keys[i] = EMPTY; s_cnt[i] = 0; }
-                __syncthreads();
-#pragma unroll
-                for (int i = 0; i < ITEMS; i++) {
-                    if (wbase + i * GKC_WAVE >= n) continue;
-                    const uint32_t j = jb + i * GKC_WAVE;
-                    if (j < n) {
-                        const unsigned long long x = key[i];
-                        uint32_t h = (uint32_t)((x * 0x9E3779B97F4A7C15ull) >> (64 - LOG_CAP));
-                        for (;;) {     // n <= CAP slots: a free or matching slot exists
-                            const unsigned long long old =
-                                atomicCAS(reinterpret_cast<unsigned long long*>(&s_keys[h]), (unsigned long long)EMPTY, x);
-                            if (old == EMPTY || old == x) { atomicAdd(&s_cnt[h], 1u); break; }
-                            h = (h + 1) & (CAP - 1);
-                        }
-                    }
-                }
-                __syncthreads();
                 // compact the occupied slots (slot i*BC_NT + tid of thread tid)
                 uint32_t occ = 0;
 #pragma unroll
@@ -905,42 +927,71 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
                     if (key[i] != EMPTY) { s_keys[pos] = key[i]; s_cnt[pos] = cnt[i]; pos++; }
                 nsort = dtot;
                 __syncthreads();
-                if (nsort <= RANK_MAX) {
-                    // few distinct keys: rank each kept key among the kept keys by
-                    // comparison (its place in key order) and stage it, no sort
+                {
+                    // rank each kept distinct key among the kept ones (its place
+                    // in key order) and stage it, without a full sort: bucket the
+                    // entries by their 8 highest varying bits (one counting
+                    // scatter in LDS), then compare within the bucket only
+                    const int bsh = nbits > RADIX_BITS ? nbits - RADIX_BITS : 0;
+                    uint32_t* b_n = s_whist[0];      // entries per bucket
+                    uint32_t* b_k = s_whist[1];      // kept entries per bucket
+                    uint32_t* b_s = s_whist[2];      // bucket start (entries)
+                    uint32_t* b_ks = s_whist[3];     // bucket start (kept entries)
+                    if (tid < RADIX) { b_n[tid] = 0; b_k[tid] = 0; }
+                    __syncthreads();
+                    uint32_t bp[ITEMS];              // bucket << 16 | place in bucket
+#pragma unroll
+                    for (int i = 0; i < ITEMS; i++) {
+                        const uint32_t e = jb + i * GKC_WAVE;
+                        bp[i] = 0xFFFFFFFFu;
+                        if (e < nsort) {
+                            key[i] = s_keys[e]; cnt[i] = s_cnt[e];
+                            const uint32_t b = KeyTraits<K>::digit(key[i], bsh);
+                            bp[i] = (b << 16) | atomicAdd(&b_n[b], 1u);
+                            if (cnt[i] >= to.min_cnt && (to.max_cnt == 0 || cnt[i] <= to.max_cnt)) atomicAdd(&b_k[b], 1u);
+                        }
+                    }
+                    __syncthreads();
+                    uint32_t all_n, all_k;
+                    const uint32_t ex_n = digit_scan(tid < RADIX ? b_n[tid] : 0u, tid, lane, wv, s_red, &all_n);
+                    __syncthreads();
+                    const uint32_t ex_k = digit_scan(tid < RADIX ? b_k[tid] : 0u, tid, lane, wv, s_red, &all_k);
+                    if (tid < RADIX) { b_s[tid] = ex_n; b_ks[tid] = ex_k; }
+                    __syncthreads();
+#pragma unroll
+                    for (int i = 0; i < ITEMS; i++) {
+                        if (bp[i] != 0xFFFFFFFFu) {
+                            const uint32_t p2 = b_s[bp[i] >> 16] + (bp[i] & 0xFFFFu);
+                            s_keys[p2] = key[i]; s_cnt[p2] = cnt[i];
+                        }
+                    }
+                    __syncthreads();
                     K* tk = alt + sa.key_start[seg] + rg.x;
                     uint32_t* tc = reinterpret_cast<uint32_t*>(keys + sa.key_start[seg] + rg.x);
-                    uint32_t kept = 0;
-                    for (uint32_t e = tid; e < nsort; e += BC_NT) {
-                        const uint64_t x = s_keys[e];
-                        const uint32_t c = s_cnt[e];
+#pragma unroll
+                    for (int i = 0; i < ITEMS; i++) {
+                        if (bp[i] == 0xFFFFFFFFu) continue;
+                        const uint64_t x = key[i];
+                        const uint32_t c = cnt[i];
                         if (c >= to.min_cnt && (to.max_cnt == 0 || c <= to.max_cnt)) {
-                            uint32_t r = 0;
-                            for (uint32_t q = 0; q < nsort; q++) {
+                            const uint32_t b = bp[i] >> 16;
+                            const uint32_t q0 = b_s[b], q1 = q0 + b_n[b];
+                            uint32_t r = b_ks[b];
+                            for (uint32_t q = q0; q < q1; q++) {
                                 const uint32_t cq = s_cnt[q];
                                 r += (s_keys[q] < x && cq >= to.min_cnt && (to.max_cnt == 0 || cq <= to.max_cnt)) ? 1u : 0u;
                             }
                             tk[r] = x; tc[r] = c;
-                            kept++;
                         }
                     }
-#pragma unroll
-                    for (int off = 1; off < GKC_WAVE; off <<= 1) kept += __shfl_xor(kept, off);
-                    if (lane == 0 && kept) atomicAdd(&s_out, kept);
-                    if (tid == 0) s_nd = nsort;
+                    if (tid == 0) { s_nd = nsort; s_out = all_k; }
                     ranked = true;
                     __syncthreads();
-                } else {
-#pragma unroll
-                    for (int i = 0; i < ITEMS; i++) {
-                        const uint32_t j = jb + i * GKC_WAVE;
-                        if (j < nsort) { key[i] = s_keys[j]; cnt[i] = s_cnt[j]; }
-                    }
                 }
             }
         }
         if (!ranked) {
-        if (!heavy) {
+        if (!heavy || hashed) {
             // 2a. LSD radix sort of the range (or of its distinct keys) in LDS
             //     (entries in registers, j = jb + 64 i)
             for (int ps = 0; ps < npass; ps++) {
@@ -1042,7 +1093,7 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
         }
         // 3. run-length count of the sorted range, CAP keys at a time (one
         //    chunk when the range was sorted in LDS), emitted in key order
-        for (uint32_t c0 = 0; c0 == 0 || c0 < n; c0 += CAP) {
+        for (uint32_t c0 = 0; c0 == 0 || (!hashed && c0 < n); c0 += CAP) {
             const uint32_t nv = hashed ? nsort : (n > c0 ? min(CAP, n - c0) : 0);
             uint32_t keepmask = 0;
             if (hashed) {
