@@ -1441,28 +1441,38 @@ template <typename K>
 __global__ __launch_bounds__(256) void k_wavecopy(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys,
                                                   const K* __restrict__ alt, const uint2* __restrict__ ranges,
                                                   HeavyList hl, TableOut to) {
+    // a wave takes 64 consecutive windows: lane l reads window t0 + l's
+    // offsets (no chain of dependent loads per window), then the wave copies
+    // the windows' entries one window after the other
     const uint32_t lane = __lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / GKC_WAVE;
     const uint32_t nw = gridDim.x * blockDim.x / GKC_WAVE;
     const uint32_t total = plan->tot_rl_tiles, S = plan->n_segs;
-    uint32_t seg = 0;
-    for (uint32_t t = wave; t < total; t += nw) {
-        while (seg + 1 < S && sa.rl_tile_start[seg + 1] <= t) seg++;      // t only grows
-        const uint32_t last = sa.rl_tile_start[seg + 1] - 1;
-        const uint32_t off = hl.kept[t];
-        const uint32_t end = t < last ? hl.kept[t + 1] : to.n_entries[seg];
-        const uint32_t m = end - off;
-        if (!m) continue;
-        const uint32_t a = ranges[t].x;
-        const K* sk = alt + sa.key_start[seg] + a;
-        const uint32_t* sc = reinterpret_cast<const uint32_t*>(keys + sa.key_start[seg] + a);
-        const uint32_t cap = to.table_cap[seg];
-        K* okey = reinterpret_cast<K*>(to.keys) + to.table_start[seg];
-        uint32_t* ocnt = to.counts + to.table_start[seg];
-        for (uint32_t e = lane; e < m; e += GKC_WAVE) {
-            const uint32_t idx = off + e;
-            if (idx < cap) { okey[idx] = sk[e]; ocnt[idx] = sc[e]; }
-            else atomicOr(to.overflow, 1u);
+    for (uint32_t t0 = wave * GKC_WAVE; t0 < total; t0 += nw * GKC_WAVE) {
+        const uint32_t t = t0 + lane;
+        uint32_t off = 0, m = 0, a = 0, seg = 0;
+        if (t < total) {
+            seg = find_seg(sa.rl_tile_start, S, t);
+            const uint32_t last = sa.rl_tile_start[seg + 1] - 1;
+            off = hl.kept[t];
+            m = (t < last ? hl.kept[t + 1] : to.n_entries[seg]) - off;
+            if (m) a = ranges[t].x;
+        }
+        uint64_t todo = __ballot(m != 0);
+        while (todo) {
+            const uint32_t b = (uint32_t)__builtin_ctzll(todo);
+            todo &= todo - 1;
+            const uint32_t sg = __shfl(seg, b), o = __shfl(off, b), mb = __shfl(m, b), ab = __shfl(a, b);
+            const K* sk = alt + sa.key_start[sg] + ab;
+            const uint32_t* sc = reinterpret_cast<const uint32_t*>(keys + sa.key_start[sg] + ab);
+            const uint32_t cap = to.table_cap[sg];
+            K* okey = reinterpret_cast<K*>(to.keys) + to.table_start[sg];
+            uint32_t* ocnt = to.counts + to.table_start[sg];
+            for (uint32_t e = lane; e < mb; e += GKC_WAVE) {
+                const uint32_t idx = o + e;
+                if (idx < cap) { okey[idx] = sk[e]; ocnt[idx] = sc[e]; }
+                else atomicOr(to.overflow, 1u);
+            }
         }
     }
 }
