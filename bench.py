@@ -123,13 +123,15 @@ def main() -> int:
     ctx = capi.Counter(w.k, w.p, w.n_parts, hpc=w.hpc, filter_mode=w.filter_mode, part_mode=w.part_mode,
                        min_count=w.min_count, max_count=w.max_count)
     forms = args.forms.split(",")
-    tkey = autotune.bucket_key(w.k, rs.bases / max(rs.n, 1), N / w.n_parts)
+    tkey = autotune.bucket_key(w.k, rs.bases / max(rs.n, 1), N / w.n_parts, w.n_parts)
     if args.tune:
         run_step(ctx, capi.BATCHED, capi.BATCHED)
         res = autotune.tune_sort(ctx, log=_log)
+        resp = autotune.tune_bin_passes(ctx, log=_log)
         cache = autotune.load_cache()
         cache[tkey] = {"sort_items": res["sort_items"], "sort_blocks_per_cu": res["sort_blocks_per_cu"],
-                       "ms_pass": res["ms_pass"], "workload": w.name}
+                       "ms_pass": res["ms_pass"], "bin_passes": resp["bin_passes"], "ms_count": resp["ms_count"],
+                       "workload": w.name}
         autotune.save_cache(cache)
     tuning = autotune.apply(ctx, tkey)
     _log(f"tuning {tkey}: {tuning or 'defaults'}")

@@ -2,11 +2,14 @@
 
 Picks the onesweep tile size (keys per thread) and the number of persistent
 workgroups per CU of the sort passes by timing every candidate on the data
-already in a counter's bins (the batched form, instrumented pass timing), and
-caches the winner keyed on (key width, read-length bucket, partition-size
-bucket) in gkc_amd/tune_cache.json.  Every candidate produces the same table
-(bit-exactness across variants is checked in tests/test_autotune.py); only
-speed differs.
+already in a counter's bins (the batched form, instrumented pass timing),
+then the number of onesweep passes before the binned count (2 or 3 for
+64-bit keys: fewer passes leave larger bins, which pays when the bins stay
+small enough for the wave-level counts), and caches the winners keyed on
+(key width, read-length bucket, partition-size bucket, partitions) in
+gkc_amd/tune_cache.json.  Every candidate produces the same table
+(bit-exactness across variants is checked in tests/test_autotune.py and
+tests/test_gpu_binned.py); only speed differs.
 """
 from __future__ import annotations
 
@@ -21,11 +24,12 @@ CANDIDATES_64 = [(i, b) for i in (8, 12, 16, 20, 24) for b in (0, 2)]
 CANDIDATES_128 = [(i, b) for i in (4, 8, 12) for b in (0, 2)]
 
 
-def bucket_key(k: int, mean_read_len: float, keys_per_partition: float) -> str:
+def bucket_key(k: int, mean_read_len: float, keys_per_partition: float, n_parts: int | None = None) -> str:
     kw = "u64" if k <= 32 else "u128"
     rl = int(round(math.log2(max(mean_read_len, 1.0))))
     ps = int(round(math.log2(max(keys_per_partition, 1.0))))
-    return f"{kw}/rl{rl}/ps{ps}"
+    key = f"{kw}/rl{rl}/ps{ps}"
+    return key if n_parts is None else f"{key}/P{n_parts}"
 
 
 def load_cache() -> dict:
@@ -46,17 +50,18 @@ def lookup(key: str) -> dict | None:
 
 def nearest(key: str) -> dict | None:
     """Exact bucket, else the entry with the same key width and the closest
-    partition-size bucket."""
+    partition-size bucket (a differing partition count or read-length
+    bucket counting as one step further)."""
     cache = load_cache()
     if key in cache:
         return cache[key]
-    kw, _, ps = key.split("/")
+    f = key.split("/")
     best, dist = None, None
     for k2, v in cache.items():
-        kw2, _, ps2 = k2.split("/")
-        if kw2 != kw:
+        f2 = k2.split("/")
+        if f2[0] != f[0]:
             continue
-        d = abs(int(ps2[2:]) - int(ps[2:]))
+        d = 2 * abs(int(f2[2][2:]) - int(f[2][2:])) + (f2[1] != f[1]) + ((f2[3:] or None) != (f[3:] or None))
         if dist is None or d < dist:
             best, dist = v, d
     return best
@@ -65,7 +70,7 @@ def nearest(key: str) -> dict | None:
 def apply(counter: "capi.Counter", key: str) -> dict | None:
     t = nearest(key)
     if t:
-        counter.set_tuning(t["sort_items"], t["sort_blocks_per_cu"])
+        counter.set_tuning(t["sort_items"], t["sort_blocks_per_cu"], bin_passes=t.get("bin_passes", 0))
     return t
 
 
@@ -94,4 +99,28 @@ def tune_sort(counter: "capi.Counter", reps: int = 2, log=None) -> dict:
             os.environ["GKC_NO_GRAPH"] = prev
     win = min(sweep, key=lambda r: r["ms_pass"])
     counter.set_tuning(win["sort_items"], win["sort_blocks_per_cu"])
+    return {**win, "sweep": sweep}
+
+
+def tune_bin_passes(counter: "capi.Counter", reps: int = 3, log=None) -> dict:
+    """Time whole batched counts (graph replay) of the counter's current bins
+    with 2 and 3 onesweep passes before the binned count (64-bit keys; 128-bit
+    keys keep the default); returns the best {bin_passes, ms_count} and the
+    sweep.  The sort tuning already set is kept."""
+    t0 = counter.get_tuning()
+    if counter.k > 32:
+        return {"bin_passes": 0, "ms_count": None, "sweep": []}
+    sweep = []
+    for npass in (3, 2):
+        counter.set_tuning(t0["sort_items"], t0["sort_blocks_per_cu"], t0["single_list"], bin_passes=npass)
+        best = None
+        for _ in range(reps):
+            counter.count(capi.BATCHED)
+            ms = counter.stats()["ms_count"]
+            best = ms if best is None else min(best, ms)
+        sweep.append({"bin_passes": npass, "ms_count": best})
+        if log:
+            log(f"tune bin_passes={npass}: {best:.3f} ms per count")
+    win = min(sweep, key=lambda r: r["ms_count"])
+    counter.set_tuning(t0["sort_items"], t0["sort_blocks_per_cu"], t0["single_list"], bin_passes=win["bin_passes"])
     return {**win, "sweep": sweep}

@@ -170,18 +170,27 @@ constexpr uint32_t WC_BATCH = 16;   // windows claimed per atomic
 // Heavy-range list of a binned count: tiles whose range exceeds the
 // capacity of the counting kernel, counted ahead by k_bincount.
 struct HeavyList {
-    uint32_t* tiles;          // [*count] tile ids
+    uint32_t* tiles;          // [*count] tile ids of the windows k_bincount counts
     uint32_t* count;          // appended by k_binranges
     uint32_t* next;           // claim counter of k_bincount
     uint2* res;               // [tile] {kept entries, distinct keys} of each heavy tile
     uint32_t* kept;           // [tile] kept entries of each window, then their offsets (k_tilescan)
+    uint32_t* mid;            // [*mid_count] tile ids of the windows k_midcount counts, or null
+    uint32_t* mid_count;      // appended by k_binranges
+    uint32_t* mid_next;       // claim counter of k_midcount
 };
+// Medium heavy windows (64-bit keys, WC_CAP < n <= MID_MAX): one wave each
+// with an LDS hash table of MID_SLOTS slots (at most 75 % full).
+constexpr uint32_t MID_SLOTS = 2048;
+constexpr uint32_t MID_MAX = MID_SLOTS / 4 * 3;
 
 template <typename K>
 __global__ void k_binranges(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys, uint2* __restrict__ ranges,
                             int tile_keys, int bsh, int bin_bits, uint32_t heavy_cap, HeavyList hl);
 template <typename K>
 __global__ void k_bincount(Plan* __restrict__ plan, SegArrays sa, K* __restrict__ keys, K* __restrict__ alt,
+                           const uint2* __restrict__ ranges, TableOut to, HeavyList hl);
+__global__ void k_midcount(Plan* __restrict__ plan, SegArrays sa, uint64_t* __restrict__ keys, uint64_t* __restrict__ alt,
                            const uint2* __restrict__ ranges, TableOut to, HeavyList hl);
 template <typename K>
 __global__ void k_wavecount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, K* __restrict__ keys,

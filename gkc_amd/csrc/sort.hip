@@ -715,7 +715,10 @@ __global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegA
         if (act) {
             const uint32_t n = a < hi ? b - a : 0u;
             ranges[t] = make_uint2(a, n);
-            if (hl.tiles && n > heavy_cap) hl.tiles[atomicAdd(hl.count, 1u)] = t;
+            if (hl.tiles && n > heavy_cap) {
+                if (hl.mid && n <= MID_MAX) hl.mid[atomicAdd(hl.mid_count, 1u)] = t;
+                else hl.tiles[atomicAdd(hl.count, 1u)] = t;
+            }
         }
     }
 }
@@ -1242,6 +1245,78 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
         }   // !ranked
         if (tid == 0) hl.res[tile] = make_uint2(s_out, s_nd);
         __syncthreads();
+    }
+}
+
+// Medium heavy windows, 64-bit keys (WC_CAP < n <= MID_MAX): one wave per
+// window, no workgroup barriers.  The wave's LDS hash table has MID_SLOTS
+// slots, at most 75 % full, so every insert finds its slot; the kept
+// distinct keys are compacted to the front, ranked among themselves by
+// comparison (their place in key order) and staged as k_bincount stages them
+// (keys at alt[a + r], counts as u32 over the range's own keys, read
+// already); {kept, distinct} to hl.res[tile].
+__global__ __launch_bounds__(GKC_WAVE) void k_midcount(Plan* __restrict__ plan, SegArrays sa, uint64_t* __restrict__ keys,
+                                                       uint64_t* __restrict__ alt, const uint2* __restrict__ ranges,
+                                                       TableOut to, HeavyList hl) {
+    __shared__ uint64_t s_key[MID_SLOTS];
+    __shared__ uint32_t s_cnt[MID_SLOTS];
+    constexpr uint64_t EMPTY = ~0ull;  // never canonical: its reverse complement (0) is smaller
+    constexpr int LOG = __builtin_ctz(MID_SLOTS);
+    static_assert((MID_SLOTS & (MID_SLOTS - 1)) == 0 && MID_MAX <= MID_SLOTS, "table size");
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = lanemask_lt();
+    const uint32_t nmid = *hl.mid_count;
+    for (;;) {
+        uint32_t q = 0;
+        if (lane == 0) q = atomicAdd(hl.mid_next, 1u);
+        q = __shfl(q, 0);
+        if (q >= nmid) break;
+        const uint32_t tile = hl.mid[q];
+        const uint2 rg = ranges[tile];
+        const uint32_t n = rg.y;                 // WC_CAP < n <= MID_MAX
+        const uint32_t seg = find_seg(sa.rl_tile_start, plan->n_segs, tile);
+        uint64_t* kp = keys + sa.key_start[seg] + rg.x;
+        for (uint32_t i = lane; i < MID_SLOTS; i += GKC_WAVE) { s_key[i] = EMPTY; s_cnt[i] = 0; }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // other lanes' slots cleared
+        for (uint32_t j = lane; j < n; j += GKC_WAVE) {
+            const unsigned long long x = kp[j];
+            uint32_t h = (uint32_t)((x * 0x9E3779B97F4A7C15ull) >> (64 - LOG));
+            for (;;) {     // n <= MID_MAX < MID_SLOTS keys: a free or matching slot exists
+                const unsigned long long old = atomicCAS(reinterpret_cast<unsigned long long*>(&s_key[h]), EMPTY, x);
+                if (old == EMPTY || old == x) { atomicAdd(&s_cnt[h], 1u); break; }
+                h = (h + 1) & (MID_SLOTS - 1);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // every lane's inserts
+        // compact the kept distinct keys to the front (slot order)
+        uint32_t m = 0, d = 0;
+        for (uint32_t i0 = 0; i0 < MID_SLOTS; i0 += GKC_WAVE) {
+            const uint32_t slot = i0 + lane;
+            const uint64_t key = s_key[slot];
+            const uint32_t c = s_cnt[slot];
+            const bool occ = key != EMPTY;
+            const bool keep = occ && c >= to.min_cnt && (to.max_cnt == 0 || c <= to.max_cnt);
+            d += (uint32_t)__popcll(__ballot(occ));
+            const uint64_t kb = __ballot(keep);
+            if (keep) {
+                const uint32_t p = m + (uint32_t)__popcll(kb & lt);
+                s_key[p] = key; s_cnt[p] = c;      // p <= this slot's index: later slots are not disturbed
+            }
+            m += (uint32_t)__popcll(kb);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // compacted entries of every lane
+        uint64_t* tk = alt + sa.key_start[seg] + rg.x;
+        uint32_t* tc = reinterpret_cast<uint32_t*>(kp);
+        for (uint32_t e0 = 0; e0 < m; e0 += GKC_WAVE) {
+            const uint32_t e = e0 + lane;
+            uint64_t key = 0;
+            if (e < m) key = s_key[e];
+            uint32_t r = 0;
+            for (uint32_t qq = 0; qq < m; qq++) r += s_key[qq] < key ? 1u : 0u;
+            if (e < m) { tk[r] = key; tc[r] = s_cnt[e]; }
+        }
+        if (lane == 0) hl.res[tile] = make_uint2(m, d);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // this window's reads before the next clear
     }
 }
 

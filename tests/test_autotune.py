@@ -12,14 +12,32 @@ from tests.helpers import assert_tables_equal
 
 def test_bucket_keys():
     assert autotune.bucket_key(31, 150, 4.7e6) == "u64/rl7/ps22"
+    assert autotune.bucket_key(31, 150, 4.7e6, 256) == "u64/rl7/ps22/P256"
     assert autotune.bucket_key(63, 250, 3e6) == "u128/rl8/ps22"
     assert autotune.bucket_key(25, 10000, 2.1e6).startswith("u64/rl13/")
+
+
+def test_nearest_prefers_exact_then_partition_size(tmp_path, monkeypatch):
+    import json
+    cache = {"u64/rl7/ps22/P256": {"sort_items": 20, "sort_blocks_per_cu": 0, "bin_passes": 2},
+             "u64/rl7/ps22/P1024": {"sort_items": 20, "sort_blocks_per_cu": 0, "bin_passes": 3},
+             "u64/rl7/ps18/P64": {"sort_items": 12, "sort_blocks_per_cu": 0, "bin_passes": 2},
+             "u128/rl8/ps22/P1024": {"sort_items": 8, "sort_blocks_per_cu": 0}}
+    f = tmp_path / "cache.json"
+    f.write_text(json.dumps(cache))
+    monkeypatch.setattr(autotune, "CACHE", str(f))
+    assert autotune.nearest("u64/rl7/ps22/P1024")["bin_passes"] == 3
+    assert autotune.nearest("u64/rl7/ps22/P512")["sort_items"] == 20      # same size bucket, other P
+    assert autotune.nearest("u64/rl7/ps18/P128")["sort_items"] == 12      # closest size bucket
+    assert autotune.nearest("u128/rl7/ps20/P16")["sort_items"] == 8       # key width first
+    assert autotune.nearest("u64/rl7/ps22") is not None                   # keys without P still resolve
 
 
 def test_cache_entries_are_valid_variants():
     for key, v in autotune.load_cache().items():
         cands = autotune.CANDIDATES_64 if key.startswith("u64") else autotune.CANDIDATES_128
         assert (v["sort_items"], v["sort_blocks_per_cu"]) in cands, key
+        assert v.get("bin_passes", 0) in ((0, 2, 3) if key.startswith("u64") else (0,)), key
 
 
 @pytest.mark.gpu
