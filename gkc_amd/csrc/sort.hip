@@ -638,87 +638,62 @@ __device__ __forceinline__ void or_diff(uint64_t& lo, uint64_t& hi, const u128k&
     lo |= a.lo ^ b.lo; hi |= a.hi ^ b.hi;
 }
 
+// First index in [j, end) whose bin (key bits [bsh, bsh + bin_bits)) is not
+// pb, or end; kp[j - 1] is in bin pb.  The keys are sorted by bin, so bin pb
+// is one run: a short linear scan (most runs end within a few keys), then an
+// exponential and a binary search.
+template <typename K>
+__device__ __forceinline__ uint32_t bin_run_end(const K* kp, uint32_t j, uint32_t end, uint32_t pb, int bsh, int bin_bits) {
+    constexpr uint32_t SCAN = 16;
+    const uint32_t lin = min(end, j + SCAN);
+    for (; j < lin; j++)
+        if (bits_of(kp[j], bsh, bin_bits) != pb) return j;
+    if (j >= end) return end;
+    uint32_t lo = j - 1, step = SCAN;        // kp[lo] is in bin pb
+    uint32_t hi = end;                       // first index known past the run (or end)
+    for (;;) {
+        const uint32_t x = lo + step;
+        if (x >= end) break;
+        if (bits_of(kp[x], bsh, bin_bits) != pb) { hi = x; break; }
+        lo = x;
+        step *= 2;
+    }
+    while (hi - lo > 1) {                    // kp[lo] in pb; hi past the run or end
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (bits_of(kp[mid], bsh, bin_bits) != pb) hi = mid; else lo = mid;
+    }
+    return hi;
+}
+
 // Window ranges: ranges[t] = {a, n} (n = 0 when no bin starts in the
-// window); bins are key bits [bsh, bsh + bin_bits).  Each lane takes one
-// window and scans its two boundaries key by key (bins are small: most scans
-// end within a key or two); a boundary not found within SCAN keys is then
-// found by the whole wave, 64 keys a step.  Windows whose range holds more
-// than heavy_cap keys are appended to the heavy list (when it is given).
+// window); bins are key bits [bsh, bsh + bin_bits).  One lane per window:
+// a = the first bin start in [lo, hi), the range ends at the first bin start
+// at or after hi.  Windows whose range holds more than heavy_cap keys are
+// appended to a heavy list (when one is given): 64-bit keys of up to MID_MAX
+// keys to k_midcount's, the others to k_bincount's.
 template <typename K>
 __global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys,
                                                    uint2* __restrict__ ranges, int tile_keys, int bsh, int bin_bits,
                                                    uint32_t heavy_cap, HeavyList hl) {
-    constexpr uint32_t SCAN = 32;
     const uint32_t TILE = (uint32_t)tile_keys;
-    const uint32_t lane = __lane_id();
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / GKC_WAVE;
-    const uint32_t nw = gridDim.x * blockDim.x / GKC_WAVE;
     const uint32_t total = plan->tot_rl_tiles, S = plan->n_segs;
-    // wave w takes windows [w*64, w*64 + 64), [w*64 + 64*nw, ...), ...
-    for (uint32_t t0 = wave * GKC_WAVE; t0 < total; t0 += nw * GKC_WAVE) {
-        const uint32_t t = t0 + lane;
-        const bool act = t < total;
-        uint32_t seg = 0, nseg = 0, lo = 0, hi = 0;
-        const K* kp = keys;
-        if (act) {
-            seg = find_seg(sa.rl_tile_start, S, t);
-            kp = keys + sa.key_start[seg];
-            nseg = sa.seg_keys[seg];
-            lo = (t - sa.rl_tile_start[seg]) * TILE;
-            hi = min(lo + TILE, nseg);
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += nthreads) {
+        const uint32_t seg = find_seg(sa.rl_tile_start, S, t);
+        const K* kp = keys + sa.key_start[seg];
+        const uint32_t nseg = sa.seg_keys[seg];
+        const uint32_t lo = (t - sa.rl_tile_start[seg]) * TILE;
+        const uint32_t hi = min(lo + TILE, nseg);
+        const uint32_t a = lo == 0 ? 0u : bin_run_end(kp, lo, hi, bits_of(kp[lo - 1], bsh, bin_bits), bsh, bin_bits);
+        uint32_t n = 0;
+        if (a < hi) {
+            const uint32_t b = hi >= nseg ? nseg : bin_run_end(kp, hi, nseg, bits_of(kp[hi - 1], bsh, bin_bits), bsh, bin_bits);
+            n = b - a;
         }
-        // a: first bin start in [lo, hi) (hi when none); b: first bin start >= hi
-        uint32_t a = lo, b = hi;
-        bool a_done = !act || lo == 0, b_done = !act || hi >= nseg;
-        if (!a_done) {
-            const uint32_t pb = bits_of(kp[lo - 1], bsh, bin_bits);
-            for (uint32_t s2 = 0; s2 < SCAN && !a_done; s2++, a++)
-                if (a >= hi || bits_of(kp[a], bsh, bin_bits) != pb) a_done = true;
-            if (a_done) a--;             // undo the loop's last increment
-        }
-        if (a_done && act && a < hi && !b_done) {
-            const uint32_t pb = bits_of(kp[hi - 1], bsh, bin_bits);
-            for (uint32_t s2 = 0; s2 < SCAN && !b_done; s2++, b++)
-                if (b >= nseg || bits_of(kp[b], bsh, bin_bits) != pb) b_done = true;
-            if (b_done) b--;
-        }
-        // boundaries still open: the wave scans them one window at a time
-        uint64_t open = __ballot(act && (!a_done || (a < hi && !b_done)));
-        while (open) {
-            const uint32_t L = (uint32_t)__builtin_ctzll(open);
-            open &= open - 1;
-            const K* kq = reinterpret_cast<const K*>(__shfl(reinterpret_cast<uintptr_t>(kp), L));
-            const uint32_t qhi = __shfl(hi, L), qnseg = __shfl(nseg, L), qlo = __shfl(lo, L);
-            uint32_t qa = __shfl(a, L);
-            const bool qa_done = __shfl((uint32_t)a_done, L) != 0;
-            if (!qa_done) {              // continue the a-scan from qa
-                const uint32_t pb = bits_of(kq[qlo - 1], bsh, bin_bits);
-                for (;; qa += GKC_WAVE) {
-                    const uint32_t j = qa + lane;
-                    const bool stop = j >= qhi || bits_of(kq[j], bsh, bin_bits) != pb;
-                    const uint64_t m = __ballot(stop);
-                    if (m) { qa += (uint32_t)__builtin_ctzll(m); break; }
-                }
-            }
-            uint32_t qb = qa_done ? __shfl(b, L) : qhi;
-            if (qa < qhi && qhi < qnseg && !(qa_done && __shfl((uint32_t)b_done, L) != 0)) {
-                const uint32_t pb = bits_of(kq[qhi - 1], bsh, bin_bits);
-                for (;; qb += GKC_WAVE) {
-                    const uint32_t j = qb + lane;
-                    const bool stop = j >= qnseg || bits_of(kq[j], bsh, bin_bits) != pb;
-                    const uint64_t m = __ballot(stop);
-                    if (m) { qb += (uint32_t)__builtin_ctzll(m); break; }
-                }
-            }
-            if (lane == L) { a = qa; b = qb; }
-        }
-        if (act) {
-            const uint32_t n = a < hi ? b - a : 0u;
-            ranges[t] = make_uint2(a, n);
-            if (hl.tiles && n > heavy_cap) {
-                if (hl.mid && n <= MID_MAX) hl.mid[atomicAdd(hl.mid_count, 1u)] = t;
-                else hl.tiles[atomicAdd(hl.count, 1u)] = t;
-            }
+        ranges[t] = make_uint2(a, n);
+        if (hl.tiles && n > heavy_cap) {
+            if (hl.mid && n <= MID_MAX) hl.mid[atomicAdd(hl.mid_count, 1u)] = t;
+            else hl.tiles[atomicAdd(hl.count, 1u)] = t;
         }
     }
 }
