@@ -297,23 +297,25 @@ __global__ __launch_bounds__(256) void k_hist_scan(Plan* __restrict__ plan, uint
     }
 }
 
-#ifndef ONESWEEP_WAVES_PER_EU
-#define ONESWEEP_WAVES_PER_EU 3
-#endif
+// register target of the onesweep pass: waves per SIMD (64-bit keys, no
+// prefetch; 128-bit keys, next tile prefetched in registers)
+template <typename K> constexpr int onesweep_waves() { return sizeof(K) == 16 ? 3 : 4; }
 // One onesweep pass over all segments.  Tile = SORT_NT * ITEMS keys of one
 // segment; tiles are claimed in order from the work lists, so every tile's
 // predecessors are claimed by running workgroups (forward progress of the
-// look-back).  Per tile:
+// look-back).  Per tile (claimed, then its keys loaded at once: a tile
+// claimed ahead of time would publish its aggregate later and hold up its
+// successors' look-back, which costs more than the load latency the other
+// workgroups on the CU hide):
 //   1. digits and the tile's digit histogram (LDS atomics, no returns)
 //   2. publish the aggregate at once, so successors can look back early
-//   3. prefetch the keys of the next claimed tile (in flight during 4-6)
 //   4. stable rank within each wave in (item, lane) order: lanes with equal
 //      digits are matched with RADIX_BITS ballots, the running per-wave digit
 //      counters live in LDS
 //   5. decoupled look-back per digit across the segment's earlier tiles
 //   6. reorder the tile in LDS by digit and write each digit's run out
 template <typename K, int ITEMS>
-__global__ __launch_bounds__(SORT_NT, ONESWEEP_WAVES_PER_EU) void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ in,
+__global__ __launch_bounds__(SORT_NT, onesweep_waves<K>()) void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ in,
                                                       K* __restrict__ out, uint32_t* __restrict__ lookback, int pass) {
     constexpr int TILE = SORT_NT * ITEMS;
     constexpr int NWAVE = SORT_NT / GKC_WAVE;
@@ -331,139 +333,252 @@ __global__ __launch_bounds__(SORT_NT, ONESWEEP_WAVES_PER_EU) void k_onesweep(Pla
     uint32_t list = blockIdx.x % wl.n_lists;
     const uint32_t wbase = wv * GKC_WAVE * ITEMS;   // wave wv owns tile keys [wbase, wbase + 64*ITEMS)
 
-    // prologue: first tile
-    if (threadIdx.x == 0) {
-        uint32_t sg = 0, ix = 0;
-        s_ok = claim<0>(wl, ctr, list, sg, ix);
-        s_seg = sg; s_idx = ix;
-    }
-    __syncthreads();
-    if (!s_ok) return;
-    uint32_t seg = s_seg, ti = s_idx;
-    uint32_t nseg = sa.seg_keys[seg];
-    uint64_t kbase = sa.key_start[seg];
-    uint32_t tn = min((uint32_t)TILE, nseg - ti * TILE);
-    K key[ITEMS];
-#pragma unroll
-    for (int i = 0; i < ITEMS; i++) {
-        const uint32_t j = wbase + i * GKC_WAVE + lane;
-        if (j < tn) key[i] = in[kbase + (uint64_t)ti * TILE + j];
-    }
-    for (;;) {
-        // claim the next tile while this one is being processed
-        __syncthreads();
+    if constexpr (sizeof(K) == 16) {
+        // 128-bit keys: the next tile's keys are prefetched into registers
+        // during this one (their tiles are large enough that it pays)
+        // prologue: first tile
         if (threadIdx.x == 0) {
             uint32_t sg = 0, ix = 0;
             s_ok = claim<0>(wl, ctr, list, sg, ix);
             s_seg = sg; s_idx = ix;
         }
-        s_hist[d] = 0;
-        for (int i = threadIdx.x; i < NWAVE * RADIX; i += SORT_NT) (&s_whist[0][0])[i] = 0;
         __syncthreads();
-        const bool have_next = s_ok;
-        const uint32_t nxt_seg = s_seg, nxt_ti = s_idx;
-        const uint32_t tile = sa.sort_tile_start[seg] + ti;      // look-back slot
-        // 1. digits + tile histogram
-        uint32_t dig[ITEMS];          // digit in bits [8:0] (RADIX = invalid), rank in [31:9]
-#pragma unroll
+        if (!s_ok) return;
+        uint32_t seg = s_seg, ti = s_idx;
+        uint32_t nseg = sa.seg_keys[seg];
+        uint64_t kbase = sa.key_start[seg];
+        uint32_t tn = min((uint32_t)TILE, nseg - ti * TILE);
+        K key[ITEMS];
+    #pragma unroll
         for (int i = 0; i < ITEMS; i++) {
             const uint32_t j = wbase + i * GKC_WAVE + lane;
-            dig[i] = (j < tn) ? KeyTraits<K>::digit(key[i], sh) : RADIX;
-            if (dig[i] < RADIX) atomicAdd(&s_hist[dig[i]], 1u);
+            if (j < tn) key[i] = in[kbase + (uint64_t)ti * TILE + j];
         }
-        __syncthreads();
-        // 2. aggregate out
-        const uint32_t total = s_hist[d];
-        uint32_t* lb = lookback + (size_t)tile * RADIX;
-        st_agent(&lb[d], (ti == 0 ? LB_INC : LB_AGG) | total);
-        // 3. prefetch the next tile
-        K nkey[ITEMS];
-        uint32_t n_nseg = 0, n_tn = 0;
-        uint64_t n_kbase = 0;
-        if (have_next) {
-            n_nseg = sa.seg_keys[nxt_seg];
-            n_kbase = sa.key_start[nxt_seg];
-            n_tn = min((uint32_t)TILE, n_nseg - nxt_ti * TILE);
-#pragma unroll
+        for (;;) {
+            // claim the next tile while this one is being processed
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t sg = 0, ix = 0;
+                s_ok = claim<0>(wl, ctr, list, sg, ix);
+                s_seg = sg; s_idx = ix;
+            }
+            s_hist[d] = 0;
+            for (int i = threadIdx.x; i < NWAVE * RADIX; i += SORT_NT) (&s_whist[0][0])[i] = 0;
+            __syncthreads();
+            const bool have_next = s_ok;
+            const uint32_t nxt_seg = s_seg, nxt_ti = s_idx;
+            const uint32_t tile = sa.sort_tile_start[seg] + ti;      // look-back slot
+            // 1. digits + tile histogram
+            uint32_t dig[ITEMS];          // digit in bits [8:0] (RADIX = invalid), rank in [31:9]
+    #pragma unroll
             for (int i = 0; i < ITEMS; i++) {
                 const uint32_t j = wbase + i * GKC_WAVE + lane;
-                if (j < n_tn) nkey[i] = in[n_kbase + (uint64_t)nxt_ti * TILE + j];
+                dig[i] = (j < tn) ? KeyTraits<K>::digit(key[i], sh) : RADIX;
+                if (dig[i] < RADIX) atomicAdd(&s_hist[dig[i]], 1u);
+            }
+            __syncthreads();
+            // 2. aggregate out
+            const uint32_t total = s_hist[d];
+            uint32_t* lb = lookback + (size_t)tile * RADIX;
+            st_agent(&lb[d], (ti == 0 ? LB_INC : LB_AGG) | total);
+            // 3. prefetch the next tile
+            K nkey[ITEMS];
+            uint32_t n_nseg = 0, n_tn = 0;
+            uint64_t n_kbase = 0;
+            if (have_next) {
+                n_nseg = sa.seg_keys[nxt_seg];
+                n_kbase = sa.key_start[nxt_seg];
+                n_tn = min((uint32_t)TILE, n_nseg - nxt_ti * TILE);
+    #pragma unroll
+                for (int i = 0; i < ITEMS; i++) {
+                    const uint32_t j = wbase + i * GKC_WAVE + lane;
+                    if (j < n_tn) nkey[i] = in[n_kbase + (uint64_t)nxt_ti * TILE + j];
+                }
+            }
+            // 4. stable rank within the wave
+            const uint64_t lt = lanemask_lt();
+    #pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                const uint32_t dd = dig[i];
+                // peers (lanes with this lane's digit): the complement of the OR
+                // over the digit bits of ballot(bit) ^ own bit, in 32-bit halves
+                const uint64_t v = __ballot(dd < RADIX);
+                uint32_t mlo = ~(uint32_t)v, mhi = ~(uint32_t)(v >> 32);
+    #pragma unroll
+                for (int b = 0; b < RADIX_BITS; b++) {
+                    const uint32_t bm = (uint32_t)((int32_t)(dd << (31 - b)) >> 31);     // 0 or ~0
+                    const uint64_t bal = __ballot(bm != 0u);
+                    mlo |= (uint32_t)bal ^ bm;
+                    mhi |= (uint32_t)(bal >> 32) ^ bm;
+                }
+                if (dd < RADIX) {
+                    const uint64_t peers = ~(((uint64_t)mhi << 32) | mlo);
+                    const uint32_t before = s_whist[wv][dd];
+                    const uint64_t lower = peers & lt;
+                    dig[i] = dd | ((before + (uint32_t)__popcll(lower)) << 9);
+                    if (lower == 0) s_whist[wv][dd] = before + (uint32_t)__popcll(peers);
+                }
+            }
+            // tile-local digit starts: exclusive scan over the 256 digits, one
+            // wave-level shuffle scan per 64 digits plus the 4 wave sums
+            uint32_t inc = total;
+    #pragma unroll
+            for (int off = 1; off < GKC_WAVE; off <<= 1) {
+                const uint32_t y = __shfl_up(inc, off);
+                if ((int)lane >= off) inc += y;
+            }
+            if (lane == GKC_WAVE - 1) s_wsum[wv] = inc;
+            __syncthreads();                                         // ranking done, wave sums visible
+            uint32_t wsum = 0;
+    #pragma unroll
+            for (int w = 0; w < NWAVE; w++) wsum += (w < (int)wv) ? s_wsum[w] : 0u;
+            s_dstart[d] = wsum + inc - total;
+            {   // per digit: exclusive prefix over the waves' running counters
+                uint32_t acc = 0;
+    #pragma unroll
+                for (int w = 0; w < NWAVE; w++) { const uint32_t c = s_whist[w][d]; s_whist[w][d] = acc; acc += c; }
+            }
+            __syncthreads();
+            // 5. decoupled look-back, one digit per thread
+            uint32_t excl = 0;
+            if (ti > 0) {
+                int64_t j = (int64_t)tile - 1;
+                for (;;) {
+                    uint32_t v = ld_agent(&lookback[(size_t)j * RADIX + d]);
+                    while ((v & ~LB_VAL) == 0) { __builtin_amdgcn_s_sleep(1); v = ld_agent(&lookback[(size_t)j * RADIX + d]); }
+                    excl += v & LB_VAL;
+                    if (v & LB_INC) break;
+                    j--;
+                }
+                st_agent(&lb[d], LB_INC | (excl + total));
+            }
+            const uint32_t* hs = sa.hist + ((size_t)seg * MAX_PASS + pass) * RADIX;
+            s_gbase[d] = kbase + hs[d] + excl - s_dstart[d];
+            // 6. reorder in LDS by digit, write each digit's run out
+    #pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                const uint32_t dd = dig[i] & 511u;
+                if (dd < RADIX) s_keys[s_dstart[dd] + s_whist[wv][dd] + (dig[i] >> 9)] = key[i];
+            }
+            __syncthreads();
+            for (uint32_t j = threadIdx.x; j < tn; j += SORT_NT) {
+                const K x = s_keys[j];
+                out[s_gbase[KeyTraits<K>::digit(x, sh)] + j] = x;
+            }
+            if (!have_next) break;
+            seg = nxt_seg; ti = nxt_ti; nseg = n_nseg; kbase = n_kbase; tn = n_tn;
+    #pragma unroll
+            for (int i = 0; i < ITEMS; i++) key[i] = nkey[i];
+        }
+    } else {
+        K key[ITEMS];
+        uint32_t seg = 0, ti = 0, nseg = 0, tn = 0;
+        uint64_t kbase = 0;
+        for (;;) {
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t sg = 0, ix = 0;
+                s_ok = claim<0>(wl, ctr, list, sg, ix);
+                s_seg = sg; s_idx = ix;
+            }
+            s_hist[d] = 0;
+            for (int i = threadIdx.x; i < NWAVE * RADIX; i += SORT_NT) (&s_whist[0][0])[i] = 0;
+            __syncthreads();
+            if (!s_ok) break;
+            seg = s_seg; ti = s_idx;
+            nseg = sa.seg_keys[seg];
+            kbase = sa.key_start[seg];
+            tn = min((uint32_t)TILE, nseg - ti * TILE);
+    #pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                const uint32_t j = wbase + i * GKC_WAVE + lane;
+                if (j < tn) key[i] = in[kbase + (uint64_t)ti * TILE + j];
+            }
+            const uint32_t tile = sa.sort_tile_start[seg] + ti;      // look-back slot
+            // 1. digits + tile histogram
+            uint32_t dig[ITEMS];          // digit in bits [8:0] (RADIX = invalid), rank in [31:9]
+    #pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                const uint32_t j = wbase + i * GKC_WAVE + lane;
+                dig[i] = (j < tn) ? KeyTraits<K>::digit(key[i], sh) : RADIX;
+                if (dig[i] < RADIX) atomicAdd(&s_hist[dig[i]], 1u);
+            }
+            __syncthreads();
+            // 2. aggregate out
+            const uint32_t total = s_hist[d];
+            uint32_t* lb = lookback + (size_t)tile * RADIX;
+            st_agent(&lb[d], (ti == 0 ? LB_INC : LB_AGG) | total);
+            // 4. stable rank within the wave
+            const uint64_t lt = lanemask_lt();
+    #pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                const uint32_t dd = dig[i];
+                // peers (lanes with this lane's digit): the complement of the OR
+                // over the digit bits of ballot(bit) ^ own bit, in 32-bit halves
+                const uint64_t v = __ballot(dd < RADIX);
+                uint32_t mlo = ~(uint32_t)v, mhi = ~(uint32_t)(v >> 32);
+    #pragma unroll
+                for (int b = 0; b < RADIX_BITS; b++) {
+                    const uint32_t bm = (uint32_t)((int32_t)(dd << (31 - b)) >> 31);     // 0 or ~0
+                    const uint64_t bal = __ballot(bm != 0u);
+                    mlo |= (uint32_t)bal ^ bm;
+                    mhi |= (uint32_t)(bal >> 32) ^ bm;
+                }
+                if (dd < RADIX) {
+                    const uint64_t peers = ~(((uint64_t)mhi << 32) | mlo);
+                    const uint32_t before = s_whist[wv][dd];
+                    const uint64_t lower = peers & lt;
+                    dig[i] = dd | ((before + (uint32_t)__popcll(lower)) << 9);
+                    if (lower == 0) s_whist[wv][dd] = before + (uint32_t)__popcll(peers);
+                }
+            }
+            // tile-local digit starts: exclusive scan over the 256 digits, one
+            // wave-level shuffle scan per 64 digits plus the 4 wave sums
+            uint32_t inc = total;
+    #pragma unroll
+            for (int off = 1; off < GKC_WAVE; off <<= 1) {
+                const uint32_t y = __shfl_up(inc, off);
+                if ((int)lane >= off) inc += y;
+            }
+            if (lane == GKC_WAVE - 1) s_wsum[wv] = inc;
+            __syncthreads();                                         // ranking done, wave sums visible
+            uint32_t wsum = 0;
+    #pragma unroll
+            for (int w = 0; w < NWAVE; w++) wsum += (w < (int)wv) ? s_wsum[w] : 0u;
+            s_dstart[d] = wsum + inc - total;
+            {   // per digit: exclusive prefix over the waves' running counters
+                uint32_t acc = 0;
+    #pragma unroll
+                for (int w = 0; w < NWAVE; w++) { const uint32_t c = s_whist[w][d]; s_whist[w][d] = acc; acc += c; }
+            }
+            __syncthreads();
+            // 5. decoupled look-back, one digit per thread
+            uint32_t excl = 0;
+            if (ti > 0) {
+                int64_t j = (int64_t)tile - 1;
+                for (;;) {
+                    uint32_t v = ld_agent(&lookback[(size_t)j * RADIX + d]);
+                    while ((v & ~LB_VAL) == 0) { __builtin_amdgcn_s_sleep(1); v = ld_agent(&lookback[(size_t)j * RADIX + d]); }
+                    excl += v & LB_VAL;
+                    if (v & LB_INC) break;
+                    j--;
+                }
+                st_agent(&lb[d], LB_INC | (excl + total));
+            }
+            const uint32_t* hs = sa.hist + ((size_t)seg * MAX_PASS + pass) * RADIX;
+            s_gbase[d] = kbase + hs[d] + excl - s_dstart[d];
+            // 6. reorder in LDS by digit, write each digit's run out
+    #pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                const uint32_t dd = dig[i] & 511u;
+                if (dd < RADIX) s_keys[s_dstart[dd] + s_whist[wv][dd] + (dig[i] >> 9)] = key[i];
+            }
+            __syncthreads();
+            for (uint32_t j = threadIdx.x; j < tn; j += SORT_NT) {
+                const K x = s_keys[j];
+                out[s_gbase[KeyTraits<K>::digit(x, sh)] + j] = x;
             }
         }
-        // 4. stable rank within the wave
-        const uint64_t lt = lanemask_lt();
-#pragma unroll
-        for (int i = 0; i < ITEMS; i++) {
-            const uint32_t dd = dig[i];
-            // peers (lanes with this lane's digit): the complement of the OR
-            // over the digit bits of ballot(bit) ^ own bit, in 32-bit halves
-            const uint64_t v = __ballot(dd < RADIX);
-            uint32_t mlo = ~(uint32_t)v, mhi = ~(uint32_t)(v >> 32);
-#pragma unroll
-            for (int b = 0; b < RADIX_BITS; b++) {
-                const uint32_t bm = (uint32_t)((int32_t)(dd << (31 - b)) >> 31);     // 0 or ~0
-                const uint64_t bal = __ballot(bm != 0u);
-                mlo |= (uint32_t)bal ^ bm;
-                mhi |= (uint32_t)(bal >> 32) ^ bm;
-            }
-            if (dd < RADIX) {
-                const uint64_t peers = ~(((uint64_t)mhi << 32) | mlo);
-                const uint32_t before = s_whist[wv][dd];
-                const uint64_t lower = peers & lt;
-                dig[i] = dd | ((before + (uint32_t)__popcll(lower)) << 9);
-                if (lower == 0) s_whist[wv][dd] = before + (uint32_t)__popcll(peers);
-            }
-        }
-        // tile-local digit starts: exclusive scan over the 256 digits, one
-        // wave-level shuffle scan per 64 digits plus the 4 wave sums
-        uint32_t inc = total;
-#pragma unroll
-        for (int off = 1; off < GKC_WAVE; off <<= 1) {
-            const uint32_t y = __shfl_up(inc, off);
-            if ((int)lane >= off) inc += y;
-        }
-        if (lane == GKC_WAVE - 1) s_wsum[wv] = inc;
-        __syncthreads();                                         // ranking done, wave sums visible
-        uint32_t wsum = 0;
-#pragma unroll
-        for (int w = 0; w < NWAVE; w++) wsum += (w < (int)wv) ? s_wsum[w] : 0u;
-        s_dstart[d] = wsum + inc - total;
-        {   // per digit: exclusive prefix over the waves' running counters
-            uint32_t acc = 0;
-#pragma unroll
-            for (int w = 0; w < NWAVE; w++) { const uint32_t c = s_whist[w][d]; s_whist[w][d] = acc; acc += c; }
-        }
-        __syncthreads();
-        // 5. decoupled look-back, one digit per thread
-        uint32_t excl = 0;
-        if (ti > 0) {
-            int64_t j = (int64_t)tile - 1;
-            for (;;) {
-                uint32_t v = ld_agent(&lookback[(size_t)j * RADIX + d]);
-                while ((v & ~LB_VAL) == 0) { __builtin_amdgcn_s_sleep(1); v = ld_agent(&lookback[(size_t)j * RADIX + d]); }
-                excl += v & LB_VAL;
-                if (v & LB_INC) break;
-                j--;
-            }
-            st_agent(&lb[d], LB_INC | (excl + total));
-        }
-        const uint32_t* hs = sa.hist + ((size_t)seg * MAX_PASS + pass) * RADIX;
-        s_gbase[d] = kbase + hs[d] + excl - s_dstart[d];
-        // 6. reorder in LDS by digit, write each digit's run out
-#pragma unroll
-        for (int i = 0; i < ITEMS; i++) {
-            const uint32_t dd = dig[i] & 511u;
-            if (dd < RADIX) s_keys[s_dstart[dd] + s_whist[wv][dd] + (dig[i] >> 9)] = key[i];
-        }
-        __syncthreads();
-        for (uint32_t j = threadIdx.x; j < tn; j += SORT_NT) {
-            const K x = s_keys[j];
-            out[s_gbase[KeyTraits<K>::digit(x, sh)] + j] = x;
-        }
-        if (!have_next) break;
-        seg = nxt_seg; ti = nxt_ti; nseg = n_nseg; kbase = n_kbase; tn = n_tn;
-#pragma unroll
-        for (int i = 0; i < ITEMS; i++) key[i] = nkey[i];
     }
 }
 
@@ -1223,6 +1338,24 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
     }
 }
 
+// Number of the first m keys of a 16-byte aligned LDS array below x (two
+// 64-bit keys per 16-byte read).
+__device__ __forceinline__ uint32_t count_below(const uint64_t* a, uint32_t m, uint64_t x) {
+    const ulonglong2* a2 = reinterpret_cast<const ulonglong2*>(a);
+    uint32_t r = 0, q = 0;
+    for (; q + 2 <= m; q += 2) {
+        const ulonglong2 v = a2[q >> 1];
+        r += (v.x < x ? 1u : 0u) + (v.y < x ? 1u : 0u);
+    }
+    if (q < m) r += a[q] < x ? 1u : 0u;
+    return r;
+}
+__device__ __forceinline__ uint32_t count_below(const u128k* a, uint32_t m, const u128k& x) {
+    uint32_t r = 0;
+    for (uint32_t q = 0; q < m; q++) r += less128(a[q], x) ? 1u : 0u;
+    return r;
+}
+
 // Medium heavy windows, 64-bit keys (WC_CAP < n <= MID_MAX): one wave per
 // window, no workgroup barriers.  The wave's LDS hash table has MID_SLOTS
 // slots, at most 75 % full, so every insert finds its slot; the kept
@@ -1233,7 +1366,7 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
 __global__ __launch_bounds__(GKC_WAVE) void k_midcount(Plan* __restrict__ plan, SegArrays sa, uint64_t* __restrict__ keys,
                                                        uint64_t* __restrict__ alt, const uint2* __restrict__ ranges,
                                                        TableOut to, HeavyList hl) {
-    __shared__ uint64_t s_key[MID_SLOTS];
+    __shared__ __align__(16) uint64_t s_key[MID_SLOTS];
     __shared__ uint32_t s_cnt[MID_SLOTS];
     constexpr uint64_t EMPTY = ~0ull;  // never canonical: its reverse complement (0) is smaller
     constexpr int LOG = __builtin_ctz(MID_SLOTS);
@@ -1286,8 +1419,7 @@ __global__ __launch_bounds__(GKC_WAVE) void k_midcount(Plan* __restrict__ plan, 
             const uint32_t e = e0 + lane;
             uint64_t key = 0;
             if (e < m) key = s_key[e];
-            uint32_t r = 0;
-            for (uint32_t qq = 0; qq < m; qq++) r += s_key[qq] < key ? 1u : 0u;
+            const uint32_t r = count_below(s_key, m, key);
             if (e < m) { tk[r] = key; tc[r] = s_cnt[e]; }
         }
         if (lane == 0) hl.res[tile] = make_uint2(m, d);
@@ -1341,8 +1473,6 @@ template <> struct WaveTable<u128k> {
         }
     }
 };
-__device__ __forceinline__ bool key_less(uint64_t a, uint64_t b) { return a < b; }
-__device__ __forceinline__ bool key_less(const u128k& a, const u128k& b) { return less128(a, b); }
 
 // Wave-level binned count (see WC_TILE): every wave claims batches of
 // windows and, per window range [a, a + n):
@@ -1361,7 +1491,7 @@ __global__ __launch_bounds__(WC_NT) void k_wavecount(Plan* __restrict__ plan, Se
     constexpr int NW = WC_NT / GKC_WAVE;
     constexpr int IT = WC_CAP / GKC_WAVE;
     static_assert((WC_CAP & (WC_CAP - 1)) == 0, "table size");
-    __shared__ WaveTable<K> s_tab[NW];
+    __shared__ __align__(16) WaveTable<K> s_tab[NW];
     const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE;
     WaveTable<K>& T = s_tab[wv];
     uint32_t list = (blockIdx.x * NW + wv) % wl.n_lists;
@@ -1446,8 +1576,7 @@ __global__ __launch_bounds__(WC_NT) void k_wavecount(Plan* __restrict__ plan, Se
                 const uint32_t e = i * GKC_WAVE + lane;
                 K key{};
                 if (e < m) key = T.key[e];
-                uint32_t r = 0;
-                for (uint32_t q = 0; q < m; q++) r += key_less(T.key[q], key) ? 1u : 0u;
+                const uint32_t r = count_below(T.key, m, key);
                 if (e < m) { sk[r] = key; sc[r] = T.cnt[e]; }
             }
         }
