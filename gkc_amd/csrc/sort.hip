@@ -793,16 +793,31 @@ __global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegA
     const uint32_t TILE = (uint32_t)tile_keys;
     const uint32_t total = plan->tot_rl_tiles, S = plan->n_segs;
     const uint32_t nthreads = gridDim.x * blockDim.x;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += nthreads) {
-        const uint32_t seg = find_seg(sa.rl_tile_start, S, t);
-        const K* kp = keys + sa.key_start[seg];
-        const uint32_t nseg = sa.seg_keys[seg];
-        const uint32_t lo = (t - sa.rl_tile_start[seg]) * TILE;
-        const uint32_t hi = min(lo + TILE, nseg);
-        const uint32_t a = lo == 0 ? 0u : bin_run_end(kp, lo, hi, bits_of(kp[lo - 1], bsh, bin_bits), bsh, bin_bits);
+    const uint32_t lane = __lane_id();
+    // the waves' lanes take consecutive windows, so the range end of window t
+    // (the first bin start at or after its end) is usually the next lane's a
+    for (uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x - lane; t0 < total; t0 += nthreads) {
+        const uint32_t t = t0 + lane;
+        const bool act = t < total;
+        uint32_t seg = 0, lo = 0, hi = 0, nseg = 0, a = 0;
+        const K* kp = keys;
+        if (act) {
+            seg = find_seg(sa.rl_tile_start, S, t);
+            kp = keys + sa.key_start[seg];
+            nseg = sa.seg_keys[seg];
+            lo = (t - sa.rl_tile_start[seg]) * TILE;
+            hi = min(lo + TILE, nseg);
+            a = lo == 0 ? 0u : bin_run_end(kp, lo, hi, bits_of(kp[lo - 1], bsh, bin_bits), bsh, bin_bits);
+        }
+        // the next window's a, if it is this segment's and holds a bin start
+        const uint32_t a_next = __shfl_down(a, 1), seg_next = __shfl_down(seg, 1), hi_next = __shfl_down(hi, 1);
+        const bool next_ok = lane + 1 < GKC_WAVE && t + 1 < total && seg_next == seg && a_next < hi_next;
+        if (!act) continue;
         uint32_t n = 0;
         if (a < hi) {
-            const uint32_t b = hi >= nseg ? nseg : bin_run_end(kp, hi, nseg, bits_of(kp[hi - 1], bsh, bin_bits), bsh, bin_bits);
+            const uint32_t b = hi >= nseg ? nseg
+                             : next_ok ? a_next
+                                       : bin_run_end(kp, hi, nseg, bits_of(kp[hi - 1], bsh, bin_bits), bsh, bin_bits);
             n = b - a;
         }
         ranges[t] = make_uint2(a, n);
