@@ -1204,7 +1204,8 @@ static K* enqueue_sort_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs, uint6
 // sort left in `sorted`.  d_toverflow[0]: table capacity exceeded;
 // d_toverflow[1], [2]: count and claim counter of the heavy windows
 // k_bincount counts ahead into staging (in the LSD form: ranges too large
-// for the run-length kernel), [3], [4]: the same for k_midcount.
+// for the run-length kernel), [3] [4] and [5] [6]: the same for the two
+// k_midcount lists.
 // Wave-level count: `prep` (the first pass over the sorted keys of a
 // gkc_count) counts every window into staging and scans the windows' kept
 // counts into table offsets; an emit pass then only copies staging to the
@@ -1223,15 +1224,19 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs
     K* alt = sorted == c->keysA.as<K>() ? c->keysB.as<K>() : c->keysA.as<K>();
     uint2* ranges = c->d_ranges.as<uint2>();
     // d_toverflow: [0] table overflow, [1] [2] k_bincount's list count and claim counter, [3] [4]
-    // k_midcount's (64-bit keys; the second half of d_hlist holds its list)
+    // and [5] [6] the two k_midcount lists' (64-bit keys; the second and third thirds of d_hlist
+    // hold those lists)
     uint32_t* ctrs = c->d_toverflow.as<uint32_t>();
-    HeavyList hl{c->d_hlist.as<uint32_t>(), ctrs + 1, ctrs + 2, c->d_hres.as<uint2>(), c->d_wkept.as<uint32_t>(),
-                 sizeof(K) == 8 ? c->d_hlist.as<uint32_t>() + rl_tiles_cap : nullptr, ctrs + 3, ctrs + 4};
+    uint32_t* hlist = c->d_hlist.as<uint32_t>();
+    const bool mid = sizeof(K) == 8;
+    HeavyList hl{hlist, ctrs + 1, ctrs + 2, c->d_hres.as<uint2>(), c->d_wkept.as<uint32_t>(),
+                 {mid ? hlist + rl_tiles_cap : nullptr, mid ? hlist + 2 * rl_tiles_cap : nullptr},
+                 {ctrs + 3, ctrs + 5}, {ctrs + 4, ctrs + 6}};
     if (bs.wave) {
         {
             if (prep) {
                 zero(c, c->d_nent.p, P); zero(c, c->d_ndist.p, P);
-                zero(c, ctrs + 1, 4);
+                zero(c, ctrs + 1, 6);
                 zero(c, &plan->ctr[ctr_rl()], NLIST);
                 const int g_r = occupancy_blocks((const void*)k_binranges<K>, 256, c->n_cu);
                 hipLaunchKernelGGL(k_binranges<K>, dim3(g_r), dim3(256), 0, c->st, plan, sa, (const K*)sorted, ranges,
@@ -1240,8 +1245,11 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs
                 hipLaunchKernelGGL(k_bincount<K>, dim3(g_h), dim3(BC_NT), 0, c->st, plan, sa, sorted, alt,
                                    (const uint2*)ranges, to, hl);
                 if constexpr (sizeof(K) == 8) {
-                    const int g_m = occupancy_blocks((const void*)k_midcount, GKC_WAVE, c->n_cu);
-                    hipLaunchKernelGGL(k_midcount, dim3(g_m), dim3(GKC_WAVE), 0, c->st, plan, sa, sorted, alt,
+                    const int g_m0 = occupancy_blocks((const void*)k_midcount<0>, GKC_WAVE, c->n_cu);
+                    hipLaunchKernelGGL(k_midcount<0>, dim3(g_m0), dim3(GKC_WAVE), 0, c->st, plan, sa, sorted, alt,
+                                       (const uint2*)ranges, to, hl);
+                    const int g_m1 = occupancy_blocks((const void*)k_midcount<1>, GKC_WAVE, c->n_cu);
+                    hipLaunchKernelGGL(k_midcount<1>, dim3(g_m1), dim3(GKC_WAVE), 0, c->st, plan, sa, sorted, alt,
                                        (const uint2*)ranges, to, hl);
                 }
                 const int g_w = occupancy_blocks((const void*)k_wavecount<K>, WC_NT, c->n_cu);
@@ -1317,7 +1325,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     sa.hist = c->d_hist.as<uint32_t>();
     K* sorted = (bs.npass % 2 == 0) ? c->keysA.as<K>() : c->keysB.as<K>();
     c->d_ranges.ensure(rl_tiles_cap * sizeof(uint2));
-    c->d_hlist.ensure(2 * rl_tiles_cap * sizeof(uint32_t));
+    c->d_hlist.ensure(3 * rl_tiles_cap * sizeof(uint32_t));
     c->d_hres.ensure(rl_tiles_cap * sizeof(uint2));
     c->d_wkept.ensure(rl_tiles_cap * sizeof(uint32_t));
     set_lists(c, P, c->h_kmer_cur.data(), c->h_hdr_cur.data(), c->tune.single_list == 1 ? 1 : NLIST,
@@ -1331,7 +1339,8 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     (void)occupancy_blocks((const void*)k_binranges<K>, 256, c->n_cu);
     (void)occupancy_blocks((const void*)k_bincount<K>, BC_NT, c->n_cu);
     (void)occupancy_blocks((const void*)k_wavecount<K>, WC_NT, c->n_cu);
-    (void)occupancy_blocks((const void*)k_midcount, GKC_WAVE, c->n_cu);
+    (void)occupancy_blocks((const void*)k_midcount<0>, GKC_WAVE, c->n_cu);
+    (void)occupancy_blocks((const void*)k_midcount<1>, GKC_WAVE, c->n_cu);
     (void)occupancy_blocks((const void*)k_wavecopy<K>, 256, c->n_cu);
     HIPX(hipEventRecord(c->ev[0], c->st));
     std::vector<const void*> key = {c->keysA.p, c->keysB.p, c->lookback.p, c->words.p, c->hdrs.p, c->d_ranges.p,
@@ -1357,14 +1366,14 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     HIPX(hipEventRecord(c->ev[4], c->st));
     std::vector<uint32_t>& caps = c->h_caps;
     caps.assign(P, 0);
-    uint32_t flags_dev[4] = {0, 0, 0, 0};
+    uint32_t flags_dev[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (presized) {
         caps = c->presize_caps;
     } else {
         if (timed) enqueue_runlength_all<K>(c, sa, bs, sorted, rl_tiles_cap, 0, true);
         else HIPX(hipGraphLaunch(c->g_cnt, c->st));
         HIPX(hipMemcpyAsync(caps.data(), c->d_nent.p, P * 4, hipMemcpyDeviceToHost, c->st));
-        HIPX(hipMemcpyAsync(flags_dev, c->d_toverflow.p, 16, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipMemcpyAsync(flags_dev, c->d_toverflow.p, 32, hipMemcpyDeviceToHost, c->st));
         HIPX(hipStreamSynchronize(c->st));
         c->stats.host_syncs++;
     }
@@ -1401,10 +1410,10 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     std::vector<uint32_t> ndist(P);
     HIPX(hipMemcpyAsync(nent.data(), c->d_nent.p, P * 4, hipMemcpyDeviceToHost, c->st));
     HIPX(hipMemcpyAsync(ndist.data(), c->d_ndist.p, P * 4, hipMemcpyDeviceToHost, c->st));
-    HIPX(hipMemcpyAsync(flags_dev, c->d_toverflow.p, 16, hipMemcpyDeviceToHost, c->st));
+    HIPX(hipMemcpyAsync(flags_dev, c->d_toverflow.p, 32, hipMemcpyDeviceToHost, c->st));
     HIPX(hipStreamSynchronize(c->st));
     c->stats.host_syncs++;
-    c->stats.heavy_ranges = flags_dev[1] + flags_dev[3];
+    c->stats.heavy_ranges = flags_dev[1] + flags_dev[3] + flags_dev[5];
     if (flags_dev[0]) {
         // under-prediction caught on device: count-then-allocate emission with
         // the exact per-partition entry counts the emit pass just measured
@@ -1418,7 +1427,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
         HIPX(hipMemcpyAsync(c->d_tcap.p, caps.data(), P * 4, hipMemcpyHostToDevice, c->st));
         emit_pass();
         HIPX(hipMemcpyAsync(nent.data(), c->d_nent.p, P * 4, hipMemcpyDeviceToHost, c->st));
-        HIPX(hipMemcpyAsync(flags_dev, c->d_toverflow.p, 16, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipMemcpyAsync(flags_dev, c->d_toverflow.p, 32, hipMemcpyDeviceToHost, c->st));
         HIPX(hipStreamSynchronize(c->st));
         c->stats.host_syncs++;
         if (flags_dev[0]) throw ArgError{"internal: table overflow after exact sizing", GKC_ESTATE};

@@ -175,14 +175,16 @@ struct HeavyList {
     uint32_t* next;           // claim counter of k_bincount
     uint2* res;               // [tile] {kept entries, distinct keys} of each heavy tile
     uint32_t* kept;           // [tile] kept entries of each window, then their offsets (k_tilescan)
-    uint32_t* mid;            // [*mid_count] tile ids of the windows k_midcount counts, or null
-    uint32_t* mid_count;      // appended by k_binranges
-    uint32_t* mid_next;       // claim counter of k_midcount
+    uint32_t* mid[2];         // [*mid_count[i]] tile ids of the windows k_midcount<., i> counts, or null
+    uint32_t* mid_count[2];   // appended by k_binranges
+    uint32_t* mid_next[2];    // claim counters of k_midcount
 };
 // Medium heavy windows (64-bit keys, WC_CAP < n <= MID_MAX): one wave each
-// with an LDS hash table of MID_SLOTS slots (at most 75 % full).
-constexpr uint32_t MID_SLOTS = 2048;
-constexpr uint32_t MID_MAX = MID_SLOTS / 4 * 3;
+// with an LDS hash table at most 75 % full: list 0 up to mid_max(0) keys in
+// mid_slots(0) slots, list 1 the larger ones in mid_slots(1).
+__host__ __device__ constexpr uint32_t mid_slots(int list) { return list == 0 ? 1024u : 2048u; }
+__host__ __device__ constexpr uint32_t mid_max(int list) { return mid_slots(list) / 4 * 3; }
+constexpr uint32_t MID_MAX = mid_max(1);
 
 template <typename K>
 __global__ void k_binranges(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys, uint2* __restrict__ ranges,
@@ -190,6 +192,7 @@ __global__ void k_binranges(Plan* __restrict__ plan, SegArrays sa, const K* __re
 template <typename K>
 __global__ void k_bincount(Plan* __restrict__ plan, SegArrays sa, K* __restrict__ keys, K* __restrict__ alt,
                            const uint2* __restrict__ ranges, TableOut to, HeavyList hl);
+template <int L>
 __global__ void k_midcount(Plan* __restrict__ plan, SegArrays sa, uint64_t* __restrict__ keys, uint64_t* __restrict__ alt,
                            const uint2* __restrict__ ranges, TableOut to, HeavyList hl);
 template <typename K>

@@ -822,7 +822,10 @@ __global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegA
         }
         ranges[t] = make_uint2(a, n);
         if (hl.tiles && n > heavy_cap) {
-            if (hl.mid && n <= MID_MAX) hl.mid[atomicAdd(hl.mid_count, 1u)] = t;
+            if (hl.mid[0] && n <= MID_MAX) {
+                const int l = n <= mid_max(0) ? 0 : 1;
+                hl.mid[l][atomicAdd(hl.mid_count[l], 1u)] = t;
+            }
             else hl.tiles[atomicAdd(hl.count, 1u)] = t;
         }
     }
@@ -1371,32 +1374,35 @@ __device__ __forceinline__ uint32_t count_below(const u128k* a, uint32_t m, cons
     return r;
 }
 
-// Medium heavy windows, 64-bit keys (WC_CAP < n <= MID_MAX): one wave per
-// window, no workgroup barriers.  The wave's LDS hash table has MID_SLOTS
-// slots, at most 75 % full, so every insert finds its slot; the kept
+// Medium heavy windows, 64-bit keys (list L: up to mid_max(L) keys, more
+// than WC_CAP or mid_max(0)): one wave per window, no workgroup barriers.
+// The wave's LDS hash table has mid_slots(L) slots, at most 75 % full, so
+// every insert finds its slot; the kept
 // distinct keys are compacted to the front, ranked among themselves by
 // comparison (their place in key order) and staged as k_bincount stages them
 // (keys at alt[a + r], counts as u32 over the range's own keys, read
 // already); {kept, distinct} to hl.res[tile].
+template <int L>
 __global__ __launch_bounds__(GKC_WAVE) void k_midcount(Plan* __restrict__ plan, SegArrays sa, uint64_t* __restrict__ keys,
                                                        uint64_t* __restrict__ alt, const uint2* __restrict__ ranges,
                                                        TableOut to, HeavyList hl) {
+    constexpr uint32_t MID_SLOTS = mid_slots(L);
     __shared__ __align__(16) uint64_t s_key[MID_SLOTS];
     __shared__ uint32_t s_cnt[MID_SLOTS];
     constexpr uint64_t EMPTY = ~0ull;  // never canonical: its reverse complement (0) is smaller
     constexpr int LOG = __builtin_ctz(MID_SLOTS);
-    static_assert((MID_SLOTS & (MID_SLOTS - 1)) == 0 && MID_MAX <= MID_SLOTS, "table size");
+    static_assert((MID_SLOTS & (MID_SLOTS - 1)) == 0 && mid_max(L) < MID_SLOTS, "table size");
     const uint32_t lane = __lane_id();
     const uint64_t lt = lanemask_lt();
-    const uint32_t nmid = *hl.mid_count;
+    const uint32_t nmid = *hl.mid_count[L];
     for (;;) {
         uint32_t q = 0;
-        if (lane == 0) q = atomicAdd(hl.mid_next, 1u);
+        if (lane == 0) q = atomicAdd(hl.mid_next[L], 1u);
         q = __shfl(q, 0);
         if (q >= nmid) break;
-        const uint32_t tile = hl.mid[q];
+        const uint32_t tile = hl.mid[L][q];
         const uint2 rg = ranges[tile];
-        const uint32_t n = rg.y;                 // WC_CAP < n <= MID_MAX
+        const uint32_t n = rg.y;                 // n <= mid_max(L)
         const uint32_t seg = find_seg(sa.rl_tile_start, plan->n_segs, tile);
         uint64_t* kp = keys + sa.key_start[seg] + rg.x;
         for (uint32_t i = lane; i < MID_SLOTS; i += GKC_WAVE) { s_key[i] = EMPTY; s_cnt[i] = 0; }
@@ -1404,7 +1410,7 @@ __global__ __launch_bounds__(GKC_WAVE) void k_midcount(Plan* __restrict__ plan, 
         for (uint32_t j = lane; j < n; j += GKC_WAVE) {
             const unsigned long long x = kp[j];
             uint32_t h = (uint32_t)((x * 0x9E3779B97F4A7C15ull) >> (64 - LOG));
-            for (;;) {     // n <= MID_MAX < MID_SLOTS keys: a free or matching slot exists
+            for (;;) {     // n <= mid_max(L) < MID_SLOTS keys: a free or matching slot exists
                 const unsigned long long old = atomicCAS(reinterpret_cast<unsigned long long*>(&s_key[h]), EMPTY, x);
                 if (old == EMPTY || old == x) { atomicAdd(&s_cnt[h], 1u); break; }
                 h = (h + 1) & (MID_SLOTS - 1);
@@ -1441,6 +1447,8 @@ __global__ __launch_bounds__(GKC_WAVE) void k_midcount(Plan* __restrict__ plan, 
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // this window's reads before the next clear
     }
 }
+template __global__ void k_midcount<0>(Plan*, SegArrays, uint64_t*, uint64_t*, const uint2*, TableOut, HeavyList);
+template __global__ void k_midcount<1>(Plan*, SegArrays, uint64_t*, uint64_t*, const uint2*, TableOut, HeavyList);
 
 // One wave's LDS hash table of a window (k_wavecount).  64-bit keys: the key
 // word is the slot's state (EMPTY = all ones, never a canonical k-mer: its

@@ -174,13 +174,14 @@ def test_binned_presized_fallback(gpu):
 
 
 @pytest.mark.parametrize("n_distinct,copies,minc", [
-    (400, 3, 1), (600, 2, 2), (1000, 1, 1), (1536, 1, 1),      # 512 < n <= 1536: one wave (k_midcount)
+    (600, 1, 1), (768, 1, 1), (500, 2, 2),                     # 512 < n <= 768: one wave, 1024-slot table
+    (769, 1, 1), (400, 3, 1), (600, 2, 2), (1000, 1, 1), (1536, 1, 1),   # <= 1536: one wave, 2048 slots
     (1537, 1, 1), (1500, 2, 1), (1500, 3, 3), (3000, 2, 1), (3000, 2, 2),   # larger: a workgroup (k_bincount)
 ])
 def test_heavy_window_in_lds(gpu, n_distinct, copies, minc):
     """One 24-bit bin of n > 512 keys (n = the sum over the distinct keys of
     1 + r % copies): a heavy window counted in LDS, by one wave with its own
-    hash table up to 1536 keys, else by a workgroup (hash count, distinct
+    hash table up to 1536 keys (a 1024-slot table up to 768), else by a workgroup (hash count, distinct
     keys bucketed by their highest varying bits and ranked within the
     bucket)."""
     rng = np.random.default_rng(n_distinct + copies)
