@@ -58,21 +58,18 @@ __device__ __forceinline__ uint32_t part_of(uint64_t mm, uint32_t P, int mode) {
 }
 
 // Reverse complement of a k-mer held in the low 2k bits (k <= 32).
+// (bit-reversing the word reverses the order of the 2-bit codes and swaps
+// the two bits of each; the pair swap restores them)
+__device__ __forceinline__ uint64_t rev2_64(uint64_t y) {   // reverse the order of the 2-bit groups
+    y = __builtin_bitreverse64(y);
+    return ((y >> 1) & 0x5555555555555555ULL) | ((y & 0x5555555555555555ULL) << 1);
+}
 __device__ __forceinline__ uint64_t revcomp64(uint64_t x, int k) {
-    uint64_t y = ~x;
-    y = ((y >> 2) & 0x3333333333333333ULL) | ((y & 0x3333333333333333ULL) << 2);
-    y = ((y >> 4) & 0x0F0F0F0F0F0F0F0FULL) | ((y & 0x0F0F0F0F0F0F0F0FULL) << 4);
-    y = __builtin_bswap64(y);
-    return y >> (64 - 2 * k);
+    return rev2_64(~x) >> (64 - 2 * k);
 }
 __device__ __forceinline__ uint64_t canonical64(uint64_t x, int k) {
     uint64_t r = revcomp64(x, k);
     return r < x ? r : x;
-}
-__device__ __forceinline__ uint64_t rev2_64(uint64_t y) {   // reverse 2-bit groups
-    y = ((y >> 2) & 0x3333333333333333ULL) | ((y & 0x3333333333333333ULL) << 2);
-    y = ((y >> 4) & 0x0F0F0F0F0F0F0F0FULL) | ((y & 0x0F0F0F0F0F0F0F0FULL) << 4);
-    return __builtin_bswap64(y);
 }
 // 128-bit: value = hi:lo, k <= 64, low 2k bits used.
 __device__ __forceinline__ u128k revcomp128(u128k x, int k) {
