@@ -913,8 +913,9 @@ static K* sort_segments(gkc_ctx* c, SegArrays sa, const uint64_t* hdr_base, cons
     zero(c, sa.hist, (uint64_t)n_segs * MAX_PASS * RADIX);
     HIPX(hipEventRecord(c->ev[2], c->st));
     const uint64_t items = n_hdrs / EX_HDRS + n_segs;
-    int g_ex = (int)std::min<uint64_t>(items, occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu));
-    hipLaunchKernelGGL(k_extract<K>, dim3(std::max(g_ex, 1)), dim3(SORT_NT), 0, c->st, plan, sa, c->wl, c->hdrs.as<SkmHdr>(), hdr_base,
+    const size_t ex_lds = (size_t)c->npass * RADIX * sizeof(uint32_t);     // digit histograms
+    int g_ex = (int)std::min<uint64_t>(items, occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu, ex_lds));
+    hipLaunchKernelGGL(k_extract<K>, dim3(std::max(g_ex, 1)), dim3(SORT_NT), ex_lds, c->st, plan, sa, c->wl, c->hdrs.as<SkmHdr>(), hdr_base,
                        c->words.as<uint32_t>(), word_base, c->keysA.as<K>(), c->prm.k);
     HIPX(hipGetLastError());
     hipLaunchKernelGGL(k_hist_scan, dim3(std::max<uint32_t>(1, (n_segs * MAX_PASS + 3) / 4)), dim3(256), 0, c->st, plan, sa.hist);
@@ -1179,8 +1180,9 @@ static K* enqueue_sort_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs, uint6
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, c->st, plan, sa, TILE, bs.rl_tile);
     zero(c, sa.hist, (uint64_t)P * MAX_PASS * RADIX);
     if (timed) HIPX(hipEventRecord(c->ev[2], c->st));
-    const int g_ex = occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu);
-    hipLaunchKernelGGL(k_extract<K>, dim3(g_ex), dim3(SORT_NT), 0, c->st, plan, sa, c->wl, c->hdrs.as<SkmHdr>(),
+    const size_t ex_lds = (size_t)bs.npass * RADIX * sizeof(uint32_t);     // digit histograms
+    const int g_ex = occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu, ex_lds);
+    hipLaunchKernelGGL(k_extract<K>, dim3(g_ex), dim3(SORT_NT), ex_lds, c->st, plan, sa, c->wl, c->hdrs.as<SkmHdr>(),
                        c->d_hdr_base.as<uint64_t>(), c->words.as<uint32_t>(), c->d_word_base.as<uint64_t>(),
                        c->keysA.as<K>(), c->prm.k);
     hipLaunchKernelGGL(k_hist_scan, dim3((P * MAX_PASS + 3) / 4), dim3(256), 0, c->st, plan, sa.hist);
@@ -1333,7 +1335,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
 
     // table capacities: predicted (presized) or a count pass (count-then-allocate)
     const bool presized = (flags & GKC_PRESIZED) && c->presize_caps.size() == P;
-    (void)occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu);
+    (void)occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu, (size_t)bs.npass * RADIX * sizeof(uint32_t));
     (void)sort_grid<K>(c);
     (void)occupancy_blocks((const void*)k_runlength<K>, SORT_NT, c->n_cu);
     (void)occupancy_blocks((const void*)k_binranges<K>, 256, c->n_cu);

@@ -197,32 +197,39 @@ __global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, Se
                                                      const uint32_t* __restrict__ words,
                                                      const uint64_t* __restrict__ word_base, K* __restrict__ keys, int k) {
     constexpr int NWAVE = SORT_NT / GKC_WAVE;
-    __shared__ uint32_t s_hist[MAX_PASS * RADIX];
+    extern __shared__ uint32_t s_hist[];                // [n_pass * RADIX] (dynamic LDS)
     __shared__ uint32_t s_pre[NWAVE][GKC_WAVE + 1];     // inclusive k-mer prefix of the wave's headers
     __shared__ uint32_t s_koff[NWAVE][GKC_WAVE], s_woff[NWAVE][GKC_WAVE];
+    __shared__ uint32_t s_flag[NWAVE][EX_U * GKC_WAVE];  // 1 + header starting at each position of a block
     __shared__ uint32_t s_seg, s_idx, s_ok;
     const int npass = plan->n_pass;
     const int sh0 = plan->sh0;
     const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE;
     uint32_t list = blockIdx.x % wl.n_lists;
+    // the next item is claimed, and its headers loaded, while this one is
+    // extracted and its histograms are flushed
+    auto load_hdr = [&](uint32_t sg, uint32_t ix) {
+        const uint32_t h = ix * EX_HDRS + threadIdx.x;
+        SkmHdr hd{0u, 0u, 0u, 0u};
+        if (h < sa.seg_hdrs[sg]) hd = hdrs[hdr_base[sg] + h];
+        return hd;
+    };
+    if (threadIdx.x == 0) {
+        uint32_t sg = 0, ix = 0;
+        s_ok = claim<2>(wl, &plan->ctr[ctr_extract()], list, sg, ix);
+        s_seg = sg; s_idx = ix;
+    }
+    for (int i = threadIdx.x; i < npass * RADIX; i += SORT_NT) s_hist[i] = 0;
+    __syncthreads();
+    if (!s_ok) return;
+    uint32_t seg = s_seg;
+    SkmHdr hd = load_hdr(seg, s_idx);
     for (;;) {
-        if (threadIdx.x == 0) {
-            uint32_t sg = 0, ix = 0;
-            s_ok = claim<2>(wl, &plan->ctr[ctr_extract()], list, sg, ix);
-            s_seg = sg; s_idx = ix;
-        }
-        for (int i = threadIdx.x; i < npass * RADIX; i += SORT_NT) s_hist[i] = 0;
-        __syncthreads();
-        if (!s_ok) break;
-        const uint32_t seg = s_seg;
-        const uint32_t h = s_idx * EX_HDRS + threadIdx.x;
-        uint32_t n = 0;
-        if (h < sa.seg_hdrs[seg]) {
-            const SkmHdr hd = hdrs[hdr_base[seg] + h];
-            n = hd.n_kmers;
-            s_koff[wv][lane] = hd.kmer_off;
-            s_woff[wv][lane] = hd.word_off;
-        }
+        uint32_t c_ok = 0, c_sg = 0, c_ix = 0;
+        if (threadIdx.x == 0) c_ok = claim<2>(wl, &plan->ctr[ctr_extract()], list, c_sg, c_ix) ? 1u : 0u;
+        const uint32_t n = hd.n_kmers;
+        s_koff[wv][lane] = hd.kmer_off;
+        s_woff[wv][lane] = hd.word_off;
         uint32_t inc = n;
 #pragma unroll
         for (int off = 1; off < GKC_WAVE; off <<= 1) {
@@ -235,40 +242,54 @@ __global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, Se
         __syncthreads();
         const uint32_t* wseg = words + word_base[seg];
         K* kseg = keys + sa.key_start[seg];
-        // EX_U k-mers per lane at a time, in phases (header search, bases,
-        // store), so their memory latencies overlap
-        for (uint32_t q0 = lane; q0 < T; q0 += EX_U * GKC_WAVE) {
+        // blocks of EX_U * 64 consecutive k-mers, k-mer qb + 64u + lane on
+        // lane `lane`: each non-empty header flags its start in LDS; a k-mer's
+        // header is the last flag at or before it (a ballot and a lane
+        // shuffle), or the header carried from the previous positions
+        const uint32_t pre = inc - n;            // this lane's header's first k-mer
+        uint32_t cur = 0;                        // header of the position before the current 64
+        for (uint32_t qb = 0; qb < T; qb += EX_U * GKC_WAVE) {
+#pragma unroll
+            for (int u = 0; u < EX_U; u++) s_flag[wv][u * GKC_WAVE + lane] = 0;
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // other lanes' clears
+            if (n && pre >= qb && pre < qb + EX_U * GKC_WAVE) s_flag[wv][pre - qb] = lane + 1;
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // other lanes' flags
             uint32_t hj[EX_U], ho[EX_U];
 #pragma unroll
             for (int u = 0; u < EX_U; u++) {
-                const uint32_t q = min(q0 + u * GKC_WAVE, T - 1);
-                // header of k-mer q: largest j with pre[j] <= q
-                uint32_t lo = 0, hi = GKC_WAVE;
-#pragma unroll
-                for (int it = 0; it < 6; it++) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (s_pre[wv][mid] <= q) lo = mid; else hi = mid;
-                }
-                hj[u] = lo;
-                ho[u] = q - s_pre[wv][lo];
+                const uint32_t f = s_flag[wv][u * GKC_WAVE + lane];
+                const uint64_t M = __ballot(f != 0);
+                const uint64_t le = M & (lanemask_lt() | (1ull << lane));
+                const uint32_t fs = __shfl(f, le ? 63 - __builtin_clzll(le) : 0);
+                hj[u] = le ? fs - 1 : cur;
+                if (M) cur = __shfl(f, 63 - __builtin_clzll(M)) - 1;
+                const uint32_t q = min(qb + u * GKC_WAVE + lane, T - 1);   // past T: a valid load, not stored
+                ho[u] = q - s_pre[wv][hj[u]];
             }
             K c[EX_U];
 #pragma unroll
             for (int u = 0; u < EX_U; u++) c[u] = canon_of(kmer_at<K>(wseg + s_woff[wv][hj[u]], ho[u], k), k);
 #pragma unroll
             for (int u = 0; u < EX_U; u++) {
-                if (q0 + u * GKC_WAVE < T) {
+                if (qb + u * GKC_WAVE + lane < T) {
                     kseg[s_koff[wv][hj[u]] + ho[u]] = c[u];
                     for (int ps = 0; ps < npass; ps++)
                         atomicAdd(&s_hist[ps * RADIX + KeyTraits<K>::digit(c[u], sh0 + ps * RADIX_BITS)], 1u);
                 }
             }
         }
-        __syncthreads();
+        if (threadIdx.x == 0) { s_ok = c_ok; s_seg = c_sg; s_idx = c_ix; }
+        __syncthreads();                                 // the item's k-mers are out; the next item is known
+        const bool more = s_ok;
+        const uint32_t nseg = s_seg;
+        if (more) hd = load_hdr(nseg, s_idx);
         uint32_t* gh = sa.hist + (size_t)seg * MAX_PASS * RADIX;
         for (int i = threadIdx.x; i < npass * RADIX; i += SORT_NT)
             if (s_hist[i]) atomicAdd(&gh[i], s_hist[i]);
-        __syncthreads();
+        __syncthreads();                                 // flushed before the histograms are cleared
+        for (int i = threadIdx.x; i < npass * RADIX; i += SORT_NT) s_hist[i] = 0;
+        if (!more) break;
+        seg = nseg;
     }
 }
 
