@@ -776,11 +776,11 @@ __device__ __forceinline__ void or_diff(uint64_t& lo, uint64_t& hi, const u128k&
 
 // First index in [j, end) whose bin (key bits [bsh, bsh + bin_bits)) is not
 // pb, or end; kp[j - 1] is in bin pb.  The keys are sorted by bin, so bin pb
-// is one run: a short linear scan (most runs end within a few keys), then an
+// is one run: a short linear scan (a 3-pass bin ends within a few keys), then an
 // exponential and a binary search.
 template <typename K>
 __device__ __forceinline__ uint32_t bin_run_end(const K* kp, uint32_t j, uint32_t end, uint32_t pb, int bsh, int bin_bits) {
-    constexpr uint32_t SCAN = 16;
+    constexpr uint32_t SCAN = 4;
     const uint32_t lin = min(end, j + SCAN);
     for (; j < lin; j++)
         if (bits_of(kp[j], bsh, bin_bits) != pb) return j;
