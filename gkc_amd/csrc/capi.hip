@@ -1227,18 +1227,18 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs
     uint2* ranges = c->d_ranges.as<uint2>();
     // d_toverflow: [0] table overflow, [1] [2] k_bincount's list count and claim counter, [3] [4]
     // and [5] [6] the two k_midcount lists' (64-bit keys; the second and third thirds of d_hlist
-    // hold those lists)
+    // hold those lists), [7] medium windows k_wavecount took
     uint32_t* ctrs = c->d_toverflow.as<uint32_t>();
     uint32_t* hlist = c->d_hlist.as<uint32_t>();
     const bool mid = sizeof(K) == 8;
     HeavyList hl{hlist, ctrs + 1, ctrs + 2, c->d_hres.as<uint2>(), c->d_wkept.as<uint32_t>(),
                  {mid ? hlist + rl_tiles_cap : nullptr, mid ? hlist + 2 * rl_tiles_cap : nullptr},
-                 {ctrs + 3, ctrs + 5}, {ctrs + 4, ctrs + 6}};
+                 {ctrs + 3, ctrs + 5}, {ctrs + 4, ctrs + 6}, mid ? ctrs + 7 : nullptr};
     if (bs.wave) {
         {
             if (prep) {
                 zero(c, c->d_nent.p, P); zero(c, c->d_ndist.p, P);
-                zero(c, ctrs + 1, 6);
+                zero(c, ctrs + 1, 7);
                 zero(c, &plan->ctr[ctr_rl()], NLIST);
                 const int g_r = occupancy_blocks((const void*)k_binranges<K>, 256, c->n_cu);
                 hipLaunchKernelGGL(k_binranges<K>, dim3(g_r), dim3(256), 0, c->st, plan, sa, (const K*)sorted, ranges,
@@ -1246,7 +1246,10 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs
                 const int g_h = occupancy_blocks((const void*)k_bincount<K>, BC_NT, c->n_cu);
                 hipLaunchKernelGGL(k_bincount<K>, dim3(g_h), dim3(BC_NT), 0, c->st, plan, sa, sorted, alt,
                                    (const uint2*)ranges, to, hl);
-                if constexpr (sizeof(K) == 8) {
+                const int g_w = occupancy_blocks((const void*)k_wavecount<K>, WC_NT, c->n_cu);
+                hipLaunchKernelGGL(k_wavecount<K>, dim3(g_w), dim3(WC_NT), 0, c->st, plan, sa, c->wl, sorted, alt,
+                                   (const uint2*)ranges, hl, to);
+                if constexpr (sizeof(K) == 8) {     // the medium windows k_wavecount handed on
                     const int g_m0 = occupancy_blocks((const void*)k_midcount<0>, GKC_WAVE, c->n_cu);
                     hipLaunchKernelGGL(k_midcount<0>, dim3(g_m0), dim3(GKC_WAVE), 0, c->st, plan, sa, sorted, alt,
                                        (const uint2*)ranges, to, hl);
@@ -1254,9 +1257,6 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs
                     hipLaunchKernelGGL(k_midcount<1>, dim3(g_m1), dim3(GKC_WAVE), 0, c->st, plan, sa, sorted, alt,
                                        (const uint2*)ranges, to, hl);
                 }
-                const int g_w = occupancy_blocks((const void*)k_wavecount<K>, WC_NT, c->n_cu);
-                hipLaunchKernelGGL(k_wavecount<K>, dim3(g_w), dim3(WC_NT), 0, c->st, plan, sa, c->wl, sorted, alt,
-                                   (const uint2*)ranges, hl, to);
                 hipLaunchKernelGGL(k_tilescan, dim3(std::min<uint32_t>(P, 4 * c->n_cu)), dim3(1024), 0, c->st, plan, sa,
                                    hl.kept);
             }
@@ -1415,7 +1415,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     HIPX(hipMemcpyAsync(flags_dev, c->d_toverflow.p, 32, hipMemcpyDeviceToHost, c->st));
     HIPX(hipStreamSynchronize(c->st));
     c->stats.host_syncs++;
-    c->stats.heavy_ranges = flags_dev[1] + flags_dev[3] + flags_dev[5];
+    c->stats.heavy_ranges = flags_dev[1] + flags_dev[7];
     if (flags_dev[0]) {
         // under-prediction caught on device: count-then-allocate emission with
         // the exact per-partition entry counts the emit pass just measured

@@ -161,11 +161,16 @@ __global__ void k_runlength(Plan* __restrict__ plan, SegArrays sa, WorkLists wl,
 // staging, with no workgroup barriers and no wait on other windows; a scan of
 // the windows' kept counts gives their table offsets and a copy writes the
 // table.  Ranges of more than WC_CAP keys are counted beforehand by
-// k_bincount into the same staging.
+// k_bincount into the same staging; 64-bit ranges of up to MID_MAX keys are
+// tried in the wave's table (64 keys at a time while at most WC_FILL slots
+// are taken, so a free slot always remains) and handed to k_midcount when
+// they hold more distinct keys.
 constexpr int WC_NT = 256;
 constexpr int WC_TILE = 256;
 constexpr int WC_CAP = 512;
+constexpr uint32_t WC_FILL = 384;
 constexpr uint32_t WC_BATCH = 16;   // windows claimed per atomic
+static_assert(WC_FILL + 64 < (uint32_t)WC_CAP, "a free slot remains after any 64 inserts");
 
 // Heavy-range list of a binned count: tiles whose range exceeds the
 // capacity of the counting kernel, counted ahead by k_bincount.
@@ -176,12 +181,14 @@ struct HeavyList {
     uint2* res;               // [tile] {kept entries, distinct keys} of each heavy tile
     uint32_t* kept;           // [tile] kept entries of each window, then their offsets (k_tilescan)
     uint32_t* mid[2];         // [*mid_count[i]] tile ids of the windows k_midcount<., i> counts, or null
-    uint32_t* mid_count[2];   // appended by k_binranges
+    uint32_t* mid_count[2];   // appended by k_wavecount
     uint32_t* mid_next[2];    // claim counters of k_midcount
+    uint32_t* mid_seen;       // windows of WC_CAP < n <= MID_MAX keys k_wavecount took (statistics)
 };
-// Medium heavy windows (64-bit keys, WC_CAP < n <= MID_MAX): one wave each
-// with an LDS hash table at most 75 % full: list 0 up to mid_max(0) keys in
-// mid_slots(0) slots, list 1 the larger ones in mid_slots(1).
+// Medium heavy windows (64-bit keys, WC_CAP < n <= MID_MAX) with more than
+// WC_FILL distinct keys: one wave each with an LDS hash table at most 75 %
+// full: list 0 up to mid_max(0) keys in mid_slots(0) slots, list 1 the larger
+// ones in mid_slots(1).
 __host__ __device__ constexpr uint32_t mid_slots(int list) { return list == 0 ? 1024u : 2048u; }
 __host__ __device__ constexpr uint32_t mid_max(int list) { return mid_slots(list) / 4 * 3; }
 constexpr uint32_t MID_MAX = mid_max(1);

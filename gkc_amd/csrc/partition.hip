@@ -275,7 +275,7 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
 
         // 4. validity and boundaries (chunk positions fit 32 bits)
         uint32_t bmask = 0;
-        uint32_t wend[P1_V];
+        uint32_t wend[P1_V] = {};
         {
             const uint32_t g0 = t0 + tid * P1_V;
             uint32_t r = 0, rbeg = 0, rnext = 0, rend = 0;
@@ -301,8 +301,47 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
             for (int s = 0; s < P1_V; s++) cmp[s] = mm[s];
 #pragma unroll
             for (int s = 0; s < P1_V; s++) mm[s] = cmp[s] >> CSH;      // minimizer values
+            // Positions in at most two reads (read r, then r + 1 from rnext on),
+            // no per-position outputs: the valid and read-start positions as bit
+            // ranges per read.  A position is valid when it starts a k-mer inside
+            // its read (rbeg <= g, g + k <= rend).
+            const uint32_t nv = tid * P1_V < (int)tlen ? min((uint32_t)P1_V, tlen - tid * P1_V) : 0u;
+            bool slow = nv != 0;
+            if (!WIN && !prm.mm_out && !prm.sketch && ro_lds && nv) {
+                const uint32_t gend = g0 + nv;
+                auto vbits = [&](uint32_t rb, uint32_t rn, uint32_t re) -> uint32_t {
+                    const uint32_t lo = max(g0, rb);
+                    if (re + 1 < (uint32_t)k + lo) return 0u;
+                    const uint32_t hi = min(min(gend, rn), re + 1 - (uint32_t)k);
+                    return hi <= lo ? 0u : ((1u << (hi - g0)) - 1u) & ~((1u << (lo - g0)) - 1u);
+                };
+                uint32_t split = P1_V, rnext2 = 0, rend2 = 0;
+                bool two = true;
+                if (rnext < gend) {             // r + 1 <= the tile's last read: its end is staged
+                    split = rnext - g0;
+                    rnext2 = s_ro[r - r0 + 2];
+                    two = rnext2 >= gend;
+                    rend2 = cd.lens ? rnext + (uint32_t)cd.lens[r + 1] : rnext2;
+                }
+                if (two) {
+                    slow = false;
+                    const uint32_t va = vbits(rbeg, rnext, rend);
+                    const uint32_t vb = split < P1_V ? vbits(rnext, rnext2, rend2) : 0u;
+                    uint32_t chg = 0;
 #pragma unroll
-            for (int s = 0; s < P1_V; s++) {
+                    for (int s = 0; s < P1_V; s++) { chg |= (mm[s] != prev ? 1u : 0u) << s; prev = mm[s]; }
+                    uint32_t fm = g0 == t0 ? 1u : 0u;
+                    if (rbeg >= g0 && rbeg < gend) fm |= 1u << (rbeg - g0);
+                    if (split < P1_V) fm |= 1u << split;
+                    bmask = (va | vb) & (fm | chg);
+                    const uint32_t wa = va ? min(rend + 1 - (uint32_t)k - t0, tlen) : 0u;
+                    const uint32_t wb = vb ? min(rend2 + 1 - (uint32_t)k - t0, tlen) : 0u;
+#pragma unroll
+                    for (int s = 0; s < P1_V; s++) wend[s] = (uint32_t)s < split ? wa : wb;
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < P1_V && slow; s++) {
                 const uint32_t g = g0 + s;
                 wend[s] = 0;
                 if (tid * P1_V + s < (int)tlen) {
@@ -354,15 +393,19 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
 #pragma unroll
             for (int w2 = 0; w2 < P1_NT / GKC_WAVE; w2++) { const uint32_t x = s_scan[w2]; o += (uint32_t)w2 < wv ? x : 0u; tot += x; }
             if (tid == 0) s_ns = tot;
+            auto emit = [&](auto part) {
 #pragma unroll
-            for (int s = 0; s < P1_V; s++) {
-                if (bmask & (1u << s)) {
-                    s_rpos[o] = (uint16_t)((slot << 11) | (tid * P1_V + s));
-                    s_rn[o] = (uint16_t)wend[s];
-                    s_rpart[o] = (uint16_t)part_mm<MM>(mm[s], P, prm.pmode);
-                    o++;
+                for (int s = 0; s < P1_V; s++) {
+                    if (bmask & (1u << s)) {
+                        s_rpos[o] = (uint16_t)((slot << 11) | (tid * P1_V + s));
+                        s_rn[o] = (uint16_t)wend[s];
+                        s_rpart[o] = (uint16_t)part(mm[s]);
+                        o++;
+                    }
                 }
-            }
+            };
+            if ((P & (P - 1)) == 0 && prm.pmode != 1) emit([&](MM v) { return (uint32_t)v & (P - 1); });
+            else emit([&](MM v) { return part_mm<MM>(v, P, prm.pmode); });
         }
         __syncthreads();
         // 6. k-mers of each new record: up to the next superkmer's start or its read's last window

@@ -805,8 +805,8 @@ __device__ __forceinline__ uint32_t bin_run_end(const K* kp, uint32_t j, uint32_
 // window); bins are key bits [bsh, bsh + bin_bits).  One lane per window:
 // a = the first bin start in [lo, hi), the range ends at the first bin start
 // at or after hi.  Windows whose range holds more than heavy_cap keys are
-// appended to a heavy list (when one is given): 64-bit keys of up to MID_MAX
-// keys to k_midcount's, the others to k_bincount's.
+// appended to k_bincount's list (when one is given), except 64-bit ranges of
+// up to MID_MAX keys, which k_wavecount tries first.
 template <typename K>
 __global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys,
                                                    uint2* __restrict__ ranges, int tile_keys, int bsh, int bin_bits,
@@ -842,13 +842,7 @@ __global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegA
             n = b - a;
         }
         ranges[t] = make_uint2(a, n);
-        if (hl.tiles && n > heavy_cap) {
-            if (hl.mid[0] && n <= MID_MAX) {
-                const int l = n <= mid_max(0) ? 0 : 1;
-                hl.mid[l][atomicAdd(hl.mid_count[l], 1u)] = t;
-            }
-            else hl.tiles[atomicAdd(hl.count, 1u)] = t;
-        }
+        if (hl.tiles && n > heavy_cap && !(hl.mid[0] && n <= MID_MAX)) hl.tiles[atomicAdd(hl.count, 1u)] = t;
     }
 }
 
@@ -1395,14 +1389,15 @@ __device__ __forceinline__ uint32_t count_below(const u128k* a, uint32_t m, cons
     return r;
 }
 
-// Medium heavy windows, 64-bit keys (list L: up to mid_max(L) keys, more
-// than WC_CAP or mid_max(0)): one wave per window, no workgroup barriers.
-// The wave's LDS hash table has mid_slots(L) slots, at most 75 % full, so
-// every insert finds its slot; the kept
-// distinct keys are compacted to the front, ranked among themselves by
-// comparison (their place in key order) and staged as k_bincount stages them
-// (keys at alt[a + r], counts as u32 over the range's own keys, read
-// already); {kept, distinct} to hl.res[tile].
+// Medium heavy windows, 64-bit keys, that k_wavecount handed on (more than
+// WC_FILL distinct keys; list L: up to mid_max(L) keys, more than WC_CAP or
+// mid_max(0)): one wave per window, no workgroup barriers.  The wave's LDS
+// hash table has mid_slots(L) slots, at most 75 % full, so every insert finds
+// its slot; the kept distinct keys are compacted to the front, ranked among
+// themselves by comparison (their place in key order) and staged as
+// k_bincount stages them (keys at alt[a + r], counts as u32 over the range's
+// own keys, read already); the kept count to hl.kept[tile], both counts added
+// to the segment's totals.
 template <int L>
 __global__ __launch_bounds__(GKC_WAVE) void k_midcount(Plan* __restrict__ plan, SegArrays sa, uint64_t* __restrict__ keys,
                                                        uint64_t* __restrict__ alt, const uint2* __restrict__ ranges,
@@ -1464,7 +1459,11 @@ __global__ __launch_bounds__(GKC_WAVE) void k_midcount(Plan* __restrict__ plan, 
             const uint32_t r = count_below(s_key, m, key);
             if (e < m) { tk[r] = key; tc[r] = s_cnt[e]; }
         }
-        if (lane == 0) hl.res[tile] = make_uint2(m, d);
+        if (lane == 0) {
+            hl.kept[tile] = m;
+            atomicAdd(&to.n_distinct[seg], d);
+            if (m) atomicAdd(&to.n_entries[seg], m);
+        }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // this window's reads before the next clear
     }
 }
@@ -1485,12 +1484,13 @@ template <> struct WaveTable<uint64_t> {
     static constexpr uint64_t EMPTY = ~0ull;
     __device__ void clear(uint32_t i) { key[i] = EMPTY; cnt[i] = 0; }
     __device__ bool used(uint32_t i) const { return key[i] != EMPTY; }
-    __device__ void insert(uint64_t x) {
+    // true when x took a free slot (a key not seen before)
+    __device__ bool insert(uint64_t x) {
         uint32_t h = (uint32_t)(((unsigned long long)x * 0x9E3779B97F4A7C15ull) >> (64 - __builtin_ctz(WC_CAP)));
-        for (;;) {     // at most WC_CAP keys: a free or matching slot exists
+        for (;;) {     // fewer than WC_CAP distinct keys: a free or matching slot exists
             const unsigned long long old = atomicCAS(reinterpret_cast<unsigned long long*>(&key[h]),
                                                      (unsigned long long)EMPTY, (unsigned long long)x);
-            if (old == EMPTY || old == x) { atomicAdd(&cnt[h], 1u); return; }
+            if (old == EMPTY || old == x) { atomicAdd(&cnt[h], 1u); return old == EMPTY; }
             h = (h + 1) & (WC_CAP - 1);
         }
     }
@@ -1501,7 +1501,7 @@ template <> struct WaveTable<u128k> {
     uint32_t tag[WC_CAP];
     __device__ void clear(uint32_t i) { tag[i] = 0; cnt[i] = 0; }
     __device__ bool used(uint32_t i) const { return tag[i] != 0; }
-    __device__ void insert(const u128k& x) {
+    __device__ bool insert(const u128k& x) {
         const unsigned long long hv = ((unsigned long long)x.lo ^ ((unsigned long long)x.hi * 0xC2B2AE3D27D4EB4Full)) *
                                       0x9E3779B97F4A7C15ull;
         uint32_t h = (uint32_t)(hv >> (64 - __builtin_ctz(WC_CAP)));
@@ -1512,7 +1512,7 @@ template <> struct WaveTable<u128k> {
             // the winners' key stores before any lane's compare: keeps the
             // compiler from reading key[h] ahead of a store it does not make
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-            if ((old == 0u || old == fp) && key[h] == x) { atomicAdd(&cnt[h], 1u); return; }
+            if ((old == 0u || old == fp) && key[h] == x) { atomicAdd(&cnt[h], 1u); return old == 0u; }
             h = (h + 1) & (WC_CAP - 1);
         }
     }
@@ -1543,6 +1543,8 @@ __global__ __launch_bounds__(WC_NT) void k_wavecount(Plan* __restrict__ plan, Se
     uint32_t t = 0, t1 = 0;                                   // claimed batch [t, t1) of list `list`
     uint32_t oi = 0;                                          // order index of t's segment in the list
     const uint32_t* uoff = wl.w + 33 + wl.n_segs + 1 * wl.n_segs;   // window offsets by order index
+    constexpr bool MIDS = sizeof(K) == 8;                     // 64-bit keys: medium windows tried here
+    uint32_t n_mid = 0;                                       // such windows taken (lane 0's count is used)
     for (;;) {
         uint32_t seg = 0, ti = 0, ok = 1;
         if (t >= t1) {
@@ -1578,7 +1580,9 @@ __global__ __launch_bounds__(WC_NT) void k_wavecount(Plan* __restrict__ plan, Se
         K* sk = alt + sa.key_start[seg] + rg.x;
         uint32_t* sc = reinterpret_cast<uint32_t*>(kp);
         uint32_t d = 0, m = 0;
-        if (n > (uint32_t)WC_CAP) {
+        bool handed = false;                     // more than WC_FILL distinct keys: k_midcount counts it
+        const bool mid = MIDS && hl.mid[0] && n > (uint32_t)WC_CAP && n <= MID_MAX;
+        if (n > (uint32_t)WC_CAP && !mid) {
             const uint2 hr = hl.res[tile];
             m = hr.x; d = hr.y;
         } else if (n) {
@@ -1591,10 +1595,39 @@ __global__ __launch_bounds__(WC_NT) void k_wavecount(Plan* __restrict__ plan, Se
                 T.clear(j);
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // other lanes' slots cleared
+            if (!mid) {
 #pragma unroll
-            for (int i = 0; i < IT; i++) {
-                if (i * GKC_WAVE < (int)n && i * GKC_WAVE + lane < n) T.insert(x[i]);
+                for (int i = 0; i < IT; i++) {
+                    if (i * GKC_WAVE < (int)n && i * GKC_WAVE + lane < n) T.insert(x[i]);
+                }
+            } else if constexpr (MIDS) {
+                // WC_CAP-key chunks, 64 keys at a time while at most WC_FILL slots are taken
+                n_mid++;
+                uint32_t taken = 0;
+                for (uint32_t c0 = 0; c0 < n && !handed; c0 += WC_CAP) {
+                    if (c0) {
+#pragma unroll
+                        for (int i = 0; i < IT; i++) {
+                            const uint32_t j = c0 + i * GKC_WAVE + lane;
+                            if (c0 + i * GKC_WAVE < n && j < n) x[i] = kp[j];
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < IT; i++) {
+                        if (c0 + i * GKC_WAVE >= n) break;
+                        if (taken > WC_FILL) { handed = true; break; }
+                        bool fresh = false;
+                        if (c0 + i * GKC_WAVE + lane < n) fresh = T.insert(x[i]);
+                        taken += (uint32_t)__popcll(__ballot(fresh));
+                    }
+                }
+                if (handed && lane == 0) {
+                    const int l = n <= mid_max(0) ? 0 : 1;
+                    hl.mid[l][atomicAdd(hl.mid_count[l], 1u)] = tile;
+                }
             }
+        }
+        if (n && !handed && !(n > (uint32_t)WC_CAP && !mid)) {
             // compact the kept distinct keys to the front of the table
 #pragma unroll
             for (int i = 0; i < IT; i++) {
@@ -1624,9 +1657,10 @@ __global__ __launch_bounds__(WC_NT) void k_wavecount(Plan* __restrict__ plan, Se
                 if (e < m) { sk[r] = key; sc[r] = T.cnt[e]; }
             }
         }
-        if (lane == 0) hl.kept[tile] = m;
+        if (lane == 0 && !handed) hl.kept[tile] = m;
         acc_d += d; acc_m += m;
     }
+    if (MIDS && lane == 0 && n_mid && hl.mid_seen) atomicAdd(hl.mid_seen, n_mid);
 }
 
 // In-place exclusive scan of the kept counts of each segment's windows (one

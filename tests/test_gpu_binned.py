@@ -174,16 +174,19 @@ def test_binned_presized_fallback(gpu):
 
 
 @pytest.mark.parametrize("n_distinct,copies,minc", [
+    (200, 6, 1), (120, 12, 3), (300, 8, 2), (384, 4, 1), (385, 4, 1),   # <= 384 distinct: the wave table
     (600, 1, 1), (768, 1, 1), (500, 2, 2),                     # 512 < n <= 768: one wave, 1024-slot table
     (769, 1, 1), (400, 3, 1), (600, 2, 2), (1000, 1, 1), (1536, 1, 1),   # <= 1536: one wave, 2048 slots
     (1537, 1, 1), (1500, 2, 1), (1500, 3, 3), (3000, 2, 1), (3000, 2, 2),   # larger: a workgroup (k_bincount)
 ])
 def test_heavy_window_in_lds(gpu, n_distinct, copies, minc):
     """One 24-bit bin of n > 512 keys (n = the sum over the distinct keys of
-    1 + r % copies): a heavy window counted in LDS, by one wave with its own
-    hash table up to 1536 keys (a 1024-slot table up to 768), else by a workgroup (hash count, distinct
-    keys bucketed by their highest varying bits and ranked within the
-    bucket)."""
+    1 + r % copies): a heavy window counted in LDS.  Up to 1536 keys the
+    wave-level count tries it in its 512-slot table, 64 keys at a time, and
+    hands it on once more than 384 slots are taken, to one wave with a larger
+    table (1024 slots up to 768 keys, else 2048); larger windows go to a
+    workgroup (hash count, distinct keys bucketed by their highest varying
+    bits and ranked within the bucket)."""
     rng = np.random.default_rng(n_distinct + copies)
     body = rng.choice(list(b"ACGT"), (n_distinct, 11)).astype(np.uint8)
     seqs = []
