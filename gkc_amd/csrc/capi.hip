@@ -798,7 +798,7 @@ static void ensure_count_buffers(gkc_ctx* c, uint64_t max_keys, uint32_t n_segs)
     c->d_tcap.ensure(n_segs * 4 + 4);
     c->d_nent.ensure(n_segs * 4 + 4);
     c->d_ndist.ensure(n_segs * 4 + 4);
-    c->d_toverflow.ensure(32);
+    c->d_toverflow.ensure(64);
 }
 
 static void zero(gkc_ctx* c, void* p, uint64_t n32) {
@@ -1227,25 +1227,27 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs
     uint2* ranges = c->d_ranges.as<uint2>();
     // d_toverflow: [0] table overflow, [1] [2] k_bincount's list count and claim counter, [3] [4]
     // and [5] [6] the two k_midcount lists' (64-bit keys; the second and third thirds of d_hlist
-    // hold those lists), [7] medium windows k_wavecount took
+    // hold those lists), [7] medium windows k_wavecount took, [8] [9] the second k_bincount
+    // pass's list count and claim counter (its list: the fourth quarter of d_hlist)
     uint32_t* ctrs = c->d_toverflow.as<uint32_t>();
     uint32_t* hlist = c->d_hlist.as<uint32_t>();
     const bool mid = sizeof(K) == 8;
     HeavyList hl{hlist, ctrs + 1, ctrs + 2, c->d_hres.as<uint2>(), c->d_wkept.as<uint32_t>(),
                  {mid ? hlist + rl_tiles_cap : nullptr, mid ? hlist + 2 * rl_tiles_cap : nullptr},
-                 {ctrs + 3, ctrs + 5}, {ctrs + 4, ctrs + 6}, mid ? ctrs + 7 : nullptr};
+                 {ctrs + 3, ctrs + 5}, {ctrs + 4, ctrs + 6}, mid ? ctrs + 7 : nullptr,
+                 mid ? hlist + 3 * rl_tiles_cap : nullptr, ctrs + 8, ctrs + 9};
     if (bs.wave) {
         {
             if (prep) {
                 zero(c, c->d_nent.p, P); zero(c, c->d_ndist.p, P);
-                zero(c, ctrs + 1, 7);
+                zero(c, ctrs + 1, 9);
                 zero(c, &plan->ctr[ctr_rl()], NLIST);
                 const int g_r = occupancy_blocks((const void*)k_binranges<K>, 256, c->n_cu);
                 hipLaunchKernelGGL(k_binranges<K>, dim3(g_r), dim3(256), 0, c->st, plan, sa, (const K*)sorted, ranges,
                                    bs.rl_tile, (int)bs.sh0, bs.bin_bits, (uint32_t)WC_CAP, hl);
                 const int g_h = occupancy_blocks((const void*)k_bincount<K>, BC_NT, c->n_cu);
                 hipLaunchKernelGGL(k_bincount<K>, dim3(g_h), dim3(BC_NT), 0, c->st, plan, sa, sorted, alt,
-                                   (const uint2*)ranges, to, hl);
+                                   (const uint2*)ranges, to, hl, 0);
                 const int g_w = occupancy_blocks((const void*)k_wavecount<K>, WC_NT, c->n_cu);
                 hipLaunchKernelGGL(k_wavecount<K>, dim3(g_w), dim3(WC_NT), 0, c->st, plan, sa, c->wl, sorted, alt,
                                    (const uint2*)ranges, hl, to);
@@ -1256,6 +1258,8 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs
                     const int g_m1 = occupancy_blocks((const void*)k_midcount<1>, GKC_WAVE, c->n_cu);
                     hipLaunchKernelGGL(k_midcount<1>, dim3(g_m1), dim3(GKC_WAVE), 0, c->st, plan, sa, sorted, alt,
                                        (const uint2*)ranges, to, hl);
+                    hipLaunchKernelGGL(k_bincount<K>, dim3(g_h), dim3(BC_NT), 0, c->st, plan, sa, sorted, alt,
+                                       (const uint2*)ranges, to, hl, 1);
                 }
                 hipLaunchKernelGGL(k_tilescan, dim3(std::min<uint32_t>(P, 4 * c->n_cu)), dim3(1024), 0, c->st, plan, sa,
                                    hl.kept);
@@ -1327,7 +1331,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     sa.hist = c->d_hist.as<uint32_t>();
     K* sorted = (bs.npass % 2 == 0) ? c->keysA.as<K>() : c->keysB.as<K>();
     c->d_ranges.ensure(rl_tiles_cap * sizeof(uint2));
-    c->d_hlist.ensure(3 * rl_tiles_cap * sizeof(uint32_t));
+    c->d_hlist.ensure(4 * rl_tiles_cap * sizeof(uint32_t));
     c->d_hres.ensure(rl_tiles_cap * sizeof(uint2));
     c->d_wkept.ensure(rl_tiles_cap * sizeof(uint32_t));
     set_lists(c, P, c->h_kmer_cur.data(), c->h_hdr_cur.data(), c->tune.single_list == 1 ? 1 : NLIST,

@@ -161,14 +161,16 @@ __global__ void k_runlength(Plan* __restrict__ plan, SegArrays sa, WorkLists wl,
 // staging, with no workgroup barriers and no wait on other windows; a scan of
 // the windows' kept counts gives their table offsets and a copy writes the
 // table.  Ranges of more than WC_CAP keys are counted beforehand by
-// k_bincount into the same staging; 64-bit ranges of up to MID_MAX keys are
-// tried in the wave's table (64 keys at a time while at most WC_FILL slots
-// are taken, so a free slot always remains) and handed to k_midcount when
-// they hold more distinct keys.
+// k_bincount into the same staging; 64-bit ranges of up to WC_TRY_MAX keys
+// are tried in the wave's table (64 keys at a time while at most WC_FILL
+// slots are taken, so a free slot always remains) and handed on when they
+// hold more distinct keys: to k_midcount up to MID_MAX keys, else to a second
+// k_bincount pass.
 constexpr int WC_NT = 256;
 constexpr int WC_TILE = 256;
 constexpr int WC_CAP = 512;
 constexpr uint32_t WC_FILL = 384;
+constexpr uint32_t WC_TRY_MAX = 8192;
 constexpr uint32_t WC_BATCH = 16;   // windows claimed per atomic
 static_assert(WC_FILL + 64 < (uint32_t)WC_CAP, "a free slot remains after any 64 inserts");
 
@@ -183,7 +185,10 @@ struct HeavyList {
     uint32_t* mid[2];         // [*mid_count[i]] tile ids of the windows k_midcount<., i> counts, or null
     uint32_t* mid_count[2];   // appended by k_wavecount
     uint32_t* mid_next[2];    // claim counters of k_midcount
-    uint32_t* mid_seen;       // windows of WC_CAP < n <= MID_MAX keys k_wavecount took (statistics)
+    uint32_t* mid_seen;       // windows of WC_CAP < n <= WC_TRY_MAX keys k_wavecount took (statistics)
+    uint32_t* late;           // [*late_count] windows of more than MID_MAX keys k_wavecount handed on, or null
+    uint32_t* late_count;     // appended by k_wavecount
+    uint32_t* late_next;      // claim counter of the second k_bincount pass
 };
 // Medium heavy windows (64-bit keys, WC_CAP < n <= MID_MAX) with more than
 // WC_FILL distinct keys: one wave each with an LDS hash table at most 75 %
@@ -198,7 +203,7 @@ __global__ void k_binranges(Plan* __restrict__ plan, SegArrays sa, const K* __re
                             int tile_keys, int bsh, int bin_bits, uint32_t heavy_cap, HeavyList hl);
 template <typename K>
 __global__ void k_bincount(Plan* __restrict__ plan, SegArrays sa, K* __restrict__ keys, K* __restrict__ alt,
-                           const uint2* __restrict__ ranges, TableOut to, HeavyList hl);
+                           const uint2* __restrict__ ranges, TableOut to, HeavyList hl, int late);
 template <int L>
 __global__ void k_midcount(Plan* __restrict__ plan, SegArrays sa, uint64_t* __restrict__ keys, uint64_t* __restrict__ alt,
                            const uint2* __restrict__ ranges, TableOut to, HeavyList hl);

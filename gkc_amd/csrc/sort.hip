@@ -806,7 +806,7 @@ __device__ __forceinline__ uint32_t bin_run_end(const K* kp, uint32_t j, uint32_
 // a = the first bin start in [lo, hi), the range ends at the first bin start
 // at or after hi.  Windows whose range holds more than heavy_cap keys are
 // appended to k_bincount's list (when one is given), except 64-bit ranges of
-// up to MID_MAX keys, which k_wavecount tries first.
+// up to WC_TRY_MAX keys, which k_wavecount tries first.
 template <typename K>
 __global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys,
                                                    uint2* __restrict__ ranges, int tile_keys, int bsh, int bin_bits,
@@ -842,7 +842,7 @@ __global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegA
             n = b - a;
         }
         ranges[t] = make_uint2(a, n);
-        if (hl.tiles && n > heavy_cap && !(hl.mid[0] && n <= MID_MAX)) hl.tiles[atomicAdd(hl.count, 1u)] = t;
+        if (hl.tiles && n > heavy_cap && !(hl.mid[0] && n <= WC_TRY_MAX)) hl.tiles[atomicAdd(hl.count, 1u)] = t;
     }
 }
 
@@ -898,12 +898,14 @@ __device__ __forceinline__ uint32_t digit_scan(uint32_t v, uint32_t tid, uint32_
 // Counts the windows of the heavy list, one workgroup per window at a time.
 // The kept entries of a range at a go to staging, keys at alt[a + i] and
 // counts as u32 over the range's own keys (they trail the reads: entry i
-// never overtakes key 2i), and {kept, distinct} to hl.res[tile].
+// never overtakes key 2i), and {kept, distinct} to hl.res[tile] (for
+// k_wavecount).  late: the windows k_wavecount handed on (after it), whose
+// kept count goes to hl.kept[tile] and both counts to the segment's totals.
 
 template <typename K>
 __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, SegArrays sa, K* __restrict__ keys,
                                                     K* __restrict__ alt, const uint2* __restrict__ ranges,
-                                                    TableOut to, HeavyList hl) {
+                                                    TableOut to, HeavyList hl, int late) {
     constexpr int ITEMS = BinTraits<K>::ITEMS;
     constexpr uint32_t CAP = bc_cap<K>();
     constexpr int NWAVE = BC_NT / GKC_WAVE;
@@ -929,10 +931,10 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
     for (;;) {
         if (tid == 0) {
             uint32_t sg = 0, ix = 0;
-            const uint32_t q = atomicAdd(hl.next, 1u);
-            const bool ok = q < *hl.count;
+            const uint32_t q = atomicAdd(late ? hl.late_next : hl.next, 1u);
+            const bool ok = q < *(late ? hl.late_count : hl.count);
             if (ok) {
-                const uint32_t t = hl.tiles[q];
+                const uint32_t t = (late ? hl.late : hl.tiles)[q];
                 sg = find_seg(sa.rl_tile_start, plan->n_segs, t);
                 ix = t - sa.rl_tile_start[sg];
             }
@@ -1366,7 +1368,14 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
             if (tid == 0) s_out += ctot;
         }
         }   // !ranked
-        if (tid == 0) hl.res[tile] = make_uint2(s_out, s_nd);
+        if (tid == 0) {
+            if (!late) hl.res[tile] = make_uint2(s_out, s_nd);
+            else {
+                hl.kept[tile] = s_out;
+                atomicAdd(&to.n_distinct[seg], s_nd);
+                if (s_out) atomicAdd(&to.n_entries[seg], s_out);
+            }
+        }
         __syncthreads();
     }
 }
@@ -1581,7 +1590,7 @@ __global__ __launch_bounds__(WC_NT) void k_wavecount(Plan* __restrict__ plan, Se
         uint32_t* sc = reinterpret_cast<uint32_t*>(kp);
         uint32_t d = 0, m = 0;
         bool handed = false;                     // more than WC_FILL distinct keys: k_midcount counts it
-        const bool mid = MIDS && hl.mid[0] && n > (uint32_t)WC_CAP && n <= MID_MAX;
+        const bool mid = MIDS && hl.mid[0] && n > (uint32_t)WC_CAP && n <= WC_TRY_MAX;
         if (n > (uint32_t)WC_CAP && !mid) {
             const uint2 hr = hl.res[tile];
             m = hr.x; d = hr.y;
@@ -1622,8 +1631,12 @@ __global__ __launch_bounds__(WC_NT) void k_wavecount(Plan* __restrict__ plan, Se
                     }
                 }
                 if (handed && lane == 0) {
-                    const int l = n <= mid_max(0) ? 0 : 1;
-                    hl.mid[l][atomicAdd(hl.mid_count[l], 1u)] = tile;
+                    if (n <= MID_MAX) {
+                        const int l = n <= mid_max(0) ? 0 : 1;
+                        hl.mid[l][atomicAdd(hl.mid_count[l], 1u)] = tile;
+                    } else {
+                        hl.late[atomicAdd(hl.late_count, 1u)] = tile;
+                    }
                 }
             }
         }
@@ -1736,8 +1749,8 @@ __global__ __launch_bounds__(256) void k_wavecopy(Plan* __restrict__ plan, SegAr
 
 template __global__ void k_binranges<uint64_t>(Plan*, SegArrays, const uint64_t*, uint2*, int, int, int, uint32_t, HeavyList);
 template __global__ void k_binranges<u128k>(Plan*, SegArrays, const u128k*, uint2*, int, int, int, uint32_t, HeavyList);
-template __global__ void k_bincount<uint64_t>(Plan*, SegArrays, uint64_t*, uint64_t*, const uint2*, TableOut, HeavyList);
-template __global__ void k_bincount<u128k>(Plan*, SegArrays, u128k*, u128k*, const uint2*, TableOut, HeavyList);
+template __global__ void k_bincount<uint64_t>(Plan*, SegArrays, uint64_t*, uint64_t*, const uint2*, TableOut, HeavyList, int);
+template __global__ void k_bincount<u128k>(Plan*, SegArrays, u128k*, u128k*, const uint2*, TableOut, HeavyList, int);
 template __global__ void k_wavecount<uint64_t>(Plan*, SegArrays, WorkLists, uint64_t*, uint64_t*, const uint2*, HeavyList, TableOut);
 template __global__ void k_wavecount<u128k>(Plan*, SegArrays, WorkLists, u128k*, u128k*, const uint2*, HeavyList, TableOut);
 template __global__ void k_wavecopy<uint64_t>(Plan*, SegArrays, const uint64_t*, const uint64_t*, const uint2*, HeavyList, TableOut);

@@ -178,15 +178,17 @@ def test_binned_presized_fallback(gpu):
     (600, 1, 1), (768, 1, 1), (500, 2, 2),                     # 512 < n <= 768: one wave, 1024-slot table
     (769, 1, 1), (400, 3, 1), (600, 2, 2), (1000, 1, 1), (1536, 1, 1),   # <= 1536: one wave, 2048 slots
     (1537, 1, 1), (1500, 2, 1), (1500, 3, 3), (3000, 2, 1), (3000, 2, 2),   # larger: a workgroup (k_bincount)
+    (300, 20, 1), (380, 40, 5), (3000, 4, 2), (500, 40, 1),    # up to 8192 keys tried in the wave table
 ])
 def test_heavy_window_in_lds(gpu, n_distinct, copies, minc):
     """One 24-bit bin of n > 512 keys (n = the sum over the distinct keys of
     1 + r % copies): a heavy window counted in LDS.  Up to 1536 keys the
     wave-level count tries it in its 512-slot table, 64 keys at a time, and
     hands it on once more than 384 slots are taken, to one wave with a larger
-    table (1024 slots up to 768 keys, else 2048); larger windows go to a
-    workgroup (hash count, distinct keys bucketed by their highest varying
-    bits and ranked within the bucket)."""
+    table (1024 slots up to 768 keys, else 2048); up to 8192 keys the same
+    try, then a workgroup (hash count, distinct keys bucketed by their highest
+    varying bits and ranked within the bucket), which counts larger windows
+    from the start."""
     rng = np.random.default_rng(n_distinct + copies)
     body = rng.choice(list(b"ACGT"), (n_distinct, 11)).astype(np.uint8)
     seqs = []
