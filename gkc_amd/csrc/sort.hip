@@ -1493,14 +1493,16 @@ template <> struct WaveTable<uint64_t> {
     static constexpr uint64_t EMPTY = ~0ull;
     __device__ void clear(uint32_t i) { key[i] = EMPTY; cnt[i] = 0; }
     __device__ bool used(uint32_t i) const { return key[i] != EMPTY; }
-    // true when x took a free slot (a key not seen before)
-    __device__ bool insert(uint64_t x) {
-        uint32_t h = (uint32_t)(((unsigned long long)x * 0x9E3779B97F4A7C15ull) >> (64 - __builtin_ctz(WC_CAP)));
-        for (;;) {     // fewer than WC_CAP distinct keys: a free or matching slot exists
+    // true when x took a free slot (a key not seen before); the table is its
+    // first 2^lg slots
+    __device__ bool insert(uint64_t x, int lg = __builtin_ctz(WC_CAP)) {
+        const uint32_t mask = (1u << lg) - 1u;
+        uint32_t h = (uint32_t)(((unsigned long long)x * 0x9E3779B97F4A7C15ull) >> (64 - lg));
+        for (;;) {     // fewer distinct keys than slots: a free or matching slot exists
             const unsigned long long old = atomicCAS(reinterpret_cast<unsigned long long*>(&key[h]),
                                                      (unsigned long long)EMPTY, (unsigned long long)x);
             if (old == EMPTY || old == x) { atomicAdd(&cnt[h], 1u); return old == EMPTY; }
-            h = (h + 1) & (WC_CAP - 1);
+            h = (h + 1) & mask;
         }
     }
 };
@@ -1538,7 +1540,7 @@ template <> struct WaveTable<u128k> {
 // distinct and kept totals are accumulated.  No window waits for another:
 // k_tilescan turns the kept counts into table offsets, k_wavecopy writes.
 template <typename K>
-__global__ __launch_bounds__(WC_NT) void k_wavecount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl,
+__global__ __launch_bounds__(WC_NT, 6) void k_wavecount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl,
                                                      K* __restrict__ keys, K* __restrict__ alt,
                                                      const uint2* __restrict__ ranges, HeavyList hl, TableOut to) {
     constexpr int NW = WC_NT / GKC_WAVE;
@@ -1554,6 +1556,7 @@ __global__ __launch_bounds__(WC_NT) void k_wavecount(Plan* __restrict__ plan, Se
     const uint32_t* uoff = wl.w + 33 + wl.n_segs + 1 * wl.n_segs;   // window offsets by order index
     constexpr bool MIDS = sizeof(K) == 8;                     // 64-bit keys: medium windows tried here
     uint32_t n_mid = 0;                                       // such windows taken (lane 0's count is used)
+    uint32_t no_small = 0;                                    // windows left to count in the full table at once
     for (;;) {
         uint32_t seg = 0, ti = 0, ok = 1;
         if (t >= t1) {
@@ -1589,6 +1592,7 @@ __global__ __launch_bounds__(WC_NT) void k_wavecount(Plan* __restrict__ plan, Se
         K* sk = alt + sa.key_start[seg] + rg.x;
         uint32_t* sc = reinterpret_cast<uint32_t*>(kp);
         uint32_t d = 0, m = 0;
+        uint32_t slots = WC_CAP;                 // table slots in use
         bool handed = false;                     // more than WC_FILL distinct keys: k_midcount counts it
         const bool mid = MIDS && hl.mid[0] && n > (uint32_t)WC_CAP && n <= WC_TRY_MAX;
         if (n > (uint32_t)WC_CAP && !mid) {
@@ -1597,14 +1601,41 @@ __global__ __launch_bounds__(WC_NT) void k_wavecount(Plan* __restrict__ plan, Se
         } else if (n) {
             // hash-count the range in the wave's table
             K x[IT];
+            const bool small = MIDS && !mid && no_small == 0;
 #pragma unroll
             for (int i = 0; i < IT; i++) {
                 const uint32_t j = i * GKC_WAVE + lane;
                 if (i * GKC_WAVE < (int)n && j < n) x[i] = kp[j];
-                T.clear(j);
+                if (!small || i * GKC_WAVE < (int)WC_SMALL) T.clear(j);
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // other lanes' slots cleared
-            if (!mid) {
+            if (small) {
+                if constexpr (MIDS) {
+                    uint32_t taken = 0;
+                    bool over = false;
+#pragma unroll
+                    for (int i = 0; i < IT; i++) {
+                        if (i * GKC_WAVE >= (int)n) break;
+                        if (taken > WC_SMALL_FILL) { over = true; break; }
+                        bool fresh = false;
+                        if (i * GKC_WAVE + lane < n) fresh = T.insert(x[i], __builtin_ctz(WC_SMALL));
+                        taken += (uint32_t)__popcll(__ballot(fresh));
+                    }
+                    if (over) {                 // count again in the full table
+                        no_small = 16;
+#pragma unroll
+                        for (int i = 0; i < IT; i++) T.clear(i * GKC_WAVE + lane);
+                        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+#pragma unroll
+                        for (int i = 0; i < IT; i++) {
+                            if (i * GKC_WAVE < (int)n && i * GKC_WAVE + lane < n) T.insert(x[i]);
+                        }
+                    } else {
+                        slots = WC_SMALL;
+                    }
+                }
+            } else if (!mid) {
+                if (no_small) no_small--;
 #pragma unroll
                 for (int i = 0; i < IT; i++) {
                     if (i * GKC_WAVE < (int)n && i * GKC_WAVE + lane < n) T.insert(x[i]);
@@ -1644,6 +1675,7 @@ __global__ __launch_bounds__(WC_NT) void k_wavecount(Plan* __restrict__ plan, Se
             // compact the kept distinct keys to the front of the table
 #pragma unroll
             for (int i = 0; i < IT; i++) {
+                if (i * GKC_WAVE >= (int)slots) break;
                 const uint32_t slot = i * GKC_WAVE + lane;
                 const bool occ = T.used(slot);
                 K key{};
