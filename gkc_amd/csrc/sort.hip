@@ -728,6 +728,7 @@ __global__ __launch_bounds__(SORT_NT) void k_runlength(Plan* __restrict__ plan, 
             atomicAdd(&to.n_entries[seg], tile_total);
         }
         __syncthreads();
+        bool over = false;                      // an entry past the capacity: one flag per wave
         if (to.emit && keep) {
             const uint32_t cap = to.table_cap[seg];
             uint32_t idx = (uint32_t)s_excl + s_scan[threadIdx.x] - nk;
@@ -737,11 +738,13 @@ __global__ __launch_bounds__(SORT_NT) void k_runlength(Plan* __restrict__ plan, 
             for (int i = 0; i < RL_ITEMS; i++) {
                 if (keep & (1u << i)) {
                     if (idx < cap) { tk[idx] = x[i]; tc[idx] = cnt[i]; }
-                    else atomicOr(to.overflow, 1u);
+                    else over = true;
                     idx++;
                 }
             }
         }
+        const uint64_t ob = __ballot(over);
+        if (ob && __lane_id() == (uint32_t)__builtin_ctzll(ob)) atomicOr(to.overflow, 1u);
         __syncthreads();
     }
 }
@@ -1770,10 +1773,10 @@ __global__ __launch_bounds__(256) void k_wavecopy(Plan* __restrict__ plan, SegAr
             const uint32_t cap = to.table_cap[sg];
             K* okey = reinterpret_cast<K*>(to.keys) + to.table_start[sg];
             uint32_t* ocnt = to.counts + to.table_start[sg];
+            if (lane == 0 && o + mb > cap) atomicOr(to.overflow, 1u);     // one flag per window
             for (uint32_t e = lane; e < mb; e += GKC_WAVE) {
                 const uint32_t idx = o + e;
                 if (idx < cap) { okey[idx] = sk[e]; ocnt[idx] = sc[e]; }
-                else atomicOr(to.overflow, 1u);
             }
         }
     }
