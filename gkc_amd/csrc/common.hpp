@@ -91,6 +91,12 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// Orders the LDS accesses of one wave's lanes around this point (LDS work
+// of a wave runs in issue order, so only the compiler must not move accesses
+// across it).  A workgroup-scope fence would also wait for the wave's
+// outstanding global loads and stores.
+__device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
+
 // Device-scope (all XCDs) relaxed atomics used by decoupled look-back.
 __device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -155,22 +155,29 @@ __global__ void k_zero32(uint32_t* p, uint64_t n) {
 template <typename K>
 __device__ __forceinline__ K kmer_at(const uint32_t* wds, uint32_t o, int k);
 
+// Every word a k-mer may reach is loaded, whether or not this one reaches it
+// (the word buffers end with 8 words of padding), and the shifts are
+// branch-free (exact for sh = 0 too), so no load waits on another and the
+// loads of several k-mers are in flight together.
 template <>
 __device__ __forceinline__ uint64_t kmer_at<uint64_t>(const uint32_t* wds, uint32_t o, int k) {
-    const uint32_t wi = o >> 4, sh = (o & 15) * 2;
-    const uint64_t a = ((uint64_t)wds[wi] << 32) | wds[wi + 1];
-    uint64_t x = sh ? (a << sh) | ((uint64_t)wds[wi + 2] >> (32 - sh)) : a;
+    const uint32_t wi = o >> 4, sh = (o & 15) * 2;        // sh <= 30
+    const uint32_t w0 = wds[wi], w1 = wds[wi + 1], w2 = wds[wi + 2];
+    const uint64_t a = ((uint64_t)w0 << 32) | w1;
+    const uint64_t x = (a << sh) | ((uint64_t)w2 >> (32 - sh));
     return x >> (64 - 2 * k);
 }
 template <>
 __device__ __forceinline__ u128k kmer_at<u128k>(const uint32_t* wds, uint32_t o, int k) {
-    const uint32_t wi = o >> 4, sh = (o & 15) * 2;
-    uint64_t h = ((uint64_t)wds[wi] << 32) | wds[wi + 1];
-    uint64_t l = ((uint64_t)wds[wi + 2] << 32) | wds[wi + 3];
-    if (sh) {
-        const uint64_t n = (uint64_t)wds[wi + 4] << 32;
-        h = (h << sh) | (l >> (64 - sh));
-        l = (l << sh) | (n >> (64 - sh));
+    const uint32_t wi = o >> 4, sh = (o & 15) * 2;        // sh <= 30
+    const uint32_t w0 = wds[wi], w1 = wds[wi + 1], w2 = wds[wi + 2], w3 = wds[wi + 3], w4 = wds[wi + 4];
+    uint64_t h = ((uint64_t)w0 << 32) | w1;
+    uint64_t l = ((uint64_t)w2 << 32) | w3;
+    {
+        const uint64_t n = (uint64_t)w4 << 32;
+        const uint32_t rs = 63 - sh;                       // (v >> 1) >> (63 - sh) == v >> (64 - sh), 0 for sh = 0
+        h = (h << sh) | ((l >> 1) >> rs);
+        l = (l << sh) | ((n >> 1) >> rs);
     }
     // top 2k bits of h:l -> value
     const int s = 128 - 2 * k;
@@ -251,9 +258,9 @@ __global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, Se
         for (uint32_t qb = 0; qb < T; qb += EX_U * GKC_WAVE) {
 #pragma unroll
             for (int u = 0; u < EX_U; u++) s_flag[wv][u * GKC_WAVE + lane] = 0;
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // other lanes' clears
+            wave_lds_fence();    // other lanes' clears
             if (n && pre >= qb && pre < qb + EX_U * GKC_WAVE) s_flag[wv][pre - qb] = lane + 1;
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // other lanes' flags
+            wave_lds_fence();    // other lanes' flags
             uint32_t hj[EX_U], ho[EX_U];
 #pragma unroll
             for (int u = 0; u < EX_U; u++) {
@@ -1434,7 +1441,7 @@ __global__ __launch_bounds__(GKC_WAVE) void k_midcount(Plan* __restrict__ plan, 
         const uint32_t seg = find_seg(sa.rl_tile_start, plan->n_segs, tile);
         uint64_t* kp = keys + sa.key_start[seg] + rg.x;
         for (uint32_t i = lane; i < MID_SLOTS; i += GKC_WAVE) { s_key[i] = EMPTY; s_cnt[i] = 0; }
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // other lanes' slots cleared
+        wave_lds_fence();    // other lanes' slots cleared
         for (uint32_t j = lane; j < n; j += GKC_WAVE) {
             const unsigned long long x = kp[j];
             uint32_t h = (uint32_t)((x * 0x9E3779B97F4A7C15ull) >> (64 - LOG));
@@ -1444,7 +1451,7 @@ __global__ __launch_bounds__(GKC_WAVE) void k_midcount(Plan* __restrict__ plan, 
                 h = (h + 1) & (MID_SLOTS - 1);
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // every lane's inserts
+        wave_lds_fence();    // every lane's inserts
         // compact the kept distinct keys to the front (slot order)
         uint32_t m = 0, d = 0;
         for (uint32_t i0 = 0; i0 < MID_SLOTS; i0 += GKC_WAVE) {
@@ -1461,7 +1468,7 @@ __global__ __launch_bounds__(GKC_WAVE) void k_midcount(Plan* __restrict__ plan, 
             }
             m += (uint32_t)__popcll(kb);
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // compacted entries of every lane
+        wave_lds_fence();    // compacted entries of every lane
         uint64_t* tk = alt + sa.key_start[seg] + rg.x;
         uint32_t* tc = reinterpret_cast<uint32_t*>(kp);
         for (uint32_t e0 = 0; e0 < m; e0 += GKC_WAVE) {
@@ -1476,7 +1483,7 @@ __global__ __launch_bounds__(GKC_WAVE) void k_midcount(Plan* __restrict__ plan, 
             atomicAdd(&to.n_distinct[seg], d);
             if (m) atomicAdd(&to.n_entries[seg], m);
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // this window's reads before the next clear
+        wave_lds_fence();    // this window's reads before the next clear
     }
 }
 template __global__ void k_midcount<0>(Plan*, SegArrays, uint64_t*, uint64_t*, const uint2*, TableOut, HeavyList);
@@ -1525,7 +1532,7 @@ template <> struct WaveTable<u128k> {
             if (old == 0u) key[h] = x;
             // the winners' key stores before any lane's compare: keeps the
             // compiler from reading key[h] ahead of a store it does not make
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+            wave_lds_fence();
             if ((old == 0u || old == fp) && key[h] == x) { atomicAdd(&cnt[h], 1u); return old == 0u; }
             h = (h + 1) & (WC_CAP - 1);
         }
@@ -1611,7 +1618,7 @@ __global__ __launch_bounds__(WC_NT, 6) void k_wavecount(Plan* __restrict__ plan,
                 if (i * GKC_WAVE < (int)n && j < n) x[i] = kp[j];
                 if (!small || i * GKC_WAVE < (int)WC_SMALL) T.clear(j);
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // other lanes' slots cleared
+            wave_lds_fence();    // other lanes' slots cleared
             if (small) {
                 if constexpr (MIDS) {
                     uint32_t taken = 0;
@@ -1628,7 +1635,7 @@ __global__ __launch_bounds__(WC_NT, 6) void k_wavecount(Plan* __restrict__ plan,
                         no_small = 16;
 #pragma unroll
                         for (int i = 0; i < IT; i++) T.clear(i * GKC_WAVE + lane);
-                        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+                        wave_lds_fence();
 #pragma unroll
                         for (int i = 0; i < IT; i++) {
                             if (i * GKC_WAVE < (int)n && i * GKC_WAVE + lane < n) T.insert(x[i]);
@@ -1693,7 +1700,7 @@ __global__ __launch_bounds__(WC_NT, 6) void k_wavecount(Plan* __restrict__ plan,
                 }
                 m += (uint32_t)__popcll(kb);
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");    // compacted entries of every lane
+            wave_lds_fence();    // compacted entries of every lane
             // rank of each kept key among the kept keys = its place in key order; stage
 #pragma unroll
             for (int i = 0; i < IT; i++) {
