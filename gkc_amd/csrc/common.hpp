@@ -22,6 +22,14 @@ struct ChunkDesc {
     uint32_t lead;            // positions [0, lead) precede read 0 (16-byte alignment of `reads`)
 };
 
+// Global-memory view of a pointer that the compiler cannot place (one read
+// from memory rather than passed as a kernel argument): its loads become
+// global loads, which complete in order, instead of flat loads, which make
+// every wait cover all outstanding memory operations.
+#define GKC_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const GKC_GLOBAL T* as_global(const T* p) { return (const GKC_GLOBAL T*)p; }
+
 // Superkmer header in a partition bin (16 bytes).
 struct SkmHdr {
     uint32_t word_off;   // partition-local offset of its first 2-bit word

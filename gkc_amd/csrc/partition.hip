@@ -15,7 +15,7 @@
 namespace gkc {
 
 // Largest r with offs[r] <= pos (absolute), i.e. the read containing pos.
-__device__ __forceinline__ uint32_t find_read(const uint64_t* offs, uint32_t n, uint64_t pos) {
+__device__ __forceinline__ uint32_t find_read(const GKC_GLOBAL uint64_t* offs, uint32_t n, uint64_t pos) {
     uint32_t lo = 0, hi = n;   // invariant: offs[lo] <= pos < offs[hi] (offs[n] > pos)
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) >> 1;
@@ -126,6 +126,7 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
     __shared__ uint32_t s_scan[P1_NT / GKC_WAVE];
     __shared__ uint32_t s_ro[P1_RO];              // offsets (chunk positions) of reads r0 .. r0 + n_ro - 1
     __shared__ uint32_t s_ns, s_ovf;
+    __shared__ uint32_t s_cr[2];                    // reads holding the current tile's window starts: first, last
     // per partition: flush-local [47:32] headers, [31:16] k-mers, [15:0] words;
     // then the flush's global bases [63:32] words, [31:0] headers
     extern __shared__ unsigned long long s_loc[];
@@ -141,37 +142,64 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
     uint32_t tile = blockIdx.x;
     if (tile >= cd.n_tiles) return;
 
+    const GKC_GLOBAL uint8_t* g_reads = as_global(cd.reads);
+    const GKC_GLOBAL uint64_t* g_offs = as_global(cd.offs);
+    const GKC_GLOBAL uint32_t* g_offs32 = as_global(reinterpret_cast<const uint32_t*>(cd.offs));   // low words at 2r
+    const GKC_GLOBAL int32_t* g_lens = as_global(cd.lens);
+    const GKC_GLOBAL uint32_t* g_tr0 = as_global(cd.tile_r0);
+    const uint32_t base32 = (uint32_t)cd.base;      // chunk positions: low words of offsets minus this
+
+    // The next tile's inputs load while this one is processed.  No loaded
+    // value is computed with at load time (a use would wait for the load at
+    // once): read offsets stay raw until staged, and the read holding the
+    // tile after next is looked up at a clamped index, its end case applied
+    // when it is used.
     // reads holding tile t's window starts: a .. b (b holds the next tile's first position)
-    auto tile_reads = [&](uint32_t t, uint32_t& a, uint32_t& b) {
-        uint32_t tv = t;
+    auto tile_reads_raw = [&](uint32_t t, uint32_t& a, uint32_t& braw) {
+        uint32_t tv = min(t, cd.n_tiles - 1);
         asm volatile("" : "+v"(tv));                // per-lane loads, off the scalar cache path
-        a = cd.tile_r0[tv];
-        b = tv + 1 < cd.n_tiles ? cd.tile_r0[tv + 1] : cd.n_reads - 1;
+        a = g_tr0[tv];
+        braw = g_tr0[min(tv + 1, cd.n_tiles - 1)];
     };
+    auto tile_b = [&](uint32_t t, uint32_t braw) { return t + 1 < cd.n_tiles ? braw : cd.n_reads - 1; };
     uint4 pf = make_uint4(0u, 0u, 0u, 0u);          // this thread's 16 bases of the tile
-    uint32_t pro0 = 0, pro1 = 0;                    // read offsets tid, tid + P1_NT of the tile
+    uint32_t pro0 = 0, pro1 = 0;                    // low words of read offsets tid, tid + P1_NT of the tile
+    // (offsets first: the range they need is worked out before any load of this call is in flight)
     auto fetch = [&](uint32_t t, uint32_t q0, uint32_t q1) {
         const uint32_t tt0 = t * P1_T;
         const uint32_t bl = min((uint32_t)(P1_T + MAXK + 32), cd.n_bases - tt0);
-        if (tid < P1_NCW && (uint32_t)tid * 16 + 16 <= bl) pf = reinterpret_cast<const uint4*>(cd.reads + tt0)[tid];
         const uint32_t n = q1 - q0 + 2;
         if (n <= P1_RO) {
-            if ((uint32_t)tid < n) pro0 = (uint32_t)(cd.offs[q0 + tid] - cd.base);
-            if ((uint32_t)tid + P1_NT < n) pro1 = (uint32_t)(cd.offs[q0 + tid + P1_NT] - cd.base);
+            if ((uint32_t)tid < n) pro0 = g_offs32[2 * (q0 + tid)];
+            if ((uint32_t)tid + P1_NT < n) pro1 = g_offs32[2 * (q0 + tid + P1_NT)];
+        }
+        if (tid < P1_NCW && (uint32_t)tid * 16 + 16 <= bl) {
+            typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+            const u32x4 v = reinterpret_cast<const GKC_GLOBAL u32x4*>(g_reads + tt0)[tid];
+            pf = make_uint4(v.x, v.y, v.z, v.w);
         }
     };
-    uint32_t c_r0, c_r1, n_r0 = 0, n_r1 = 0;
-    tile_reads(tile, c_r0, c_r1);
-    fetch(tile, c_r0, c_r1);
-    if (tile + G < cd.n_tiles) tile_reads(tile + G, n_r0, n_r1);
+    // the current tile's read range passes from one iteration to the next in
+    // LDS (a register copy of the value just loaded for the tile after next
+    // would wait for that load)
+    uint32_t n_r0 = 0, n_r1 = 0;                    // the tile after this one; n_r1 raw (tile_b applies the end case)
+    {
+        uint32_t a, braw;
+        tile_reads_raw(tile, a, braw);
+        const uint32_t b = tile_b(tile, braw);
+        fetch(tile, a, b);
+        if (tid == 0) { s_cr[0] = a; s_cr[1] = b; }
+    }
+    tile_reads_raw(tile + G, n_r0, n_r1);
     for (uint32_t i = tid; i < P; i += P1_NT) s_loc[i] = 0;
     uint32_t acc = 0, slot = 0;                     // records and tiles held
+    __syncthreads();
 
     for (;;) {
         const uint32_t t0 = tile * P1_T;
         const uint32_t tlen = min((uint32_t)P1_T, cd.n_bases - t0);      // window starts owned
         const uint32_t blen = min((uint32_t)(P1_T + MAXK + 32), cd.n_bases - t0);  // bases staged
-        const uint32_t r0 = c_r0, r1 = c_r1;
+        const uint32_t r0 = s_cr[0], r1 = s_cr[1];
         const uint32_t n_ro = r1 - r0 + 2;                                 // offsets r0 .. r1 + 1
         const bool ro_lds = n_ro <= P1_RO;
         uint32_t* const sc = s_code[slot];
@@ -183,20 +211,21 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
             if (b + 16 <= blen) wd = pack16(pf);
             else
                 for (uint32_t t = 0; t < 16; t++)
-                    if (b + t < blen) wd |= base_code(cd.reads[t0 + b + t]) << (30 - 2 * t);
+                    if (b + t < blen) wd |= base_code(g_reads[t0 + b + t]) << (30 - 2 * t);
             sc[tid] = wd;
         }
         if (ro_lds) {
-            if ((uint32_t)tid < n_ro) s_ro[tid] = pro0;
-            if ((uint32_t)tid + P1_NT < n_ro) s_ro[tid + P1_NT] = pro1;
+            if ((uint32_t)tid < n_ro) s_ro[tid] = pro0 - base32;
+            if ((uint32_t)tid + P1_NT < n_ro) s_ro[tid + P1_NT] = pro1 - base32;
         }
         __syncthreads();
         const uint32_t nt = tile + G;
         const bool more = nt < cd.n_tiles;
         if (more) {                                 // the next tile's inputs load during this one
-            fetch(nt, n_r0, n_r1);
-            c_r0 = n_r0; c_r1 = n_r1;
-            if (nt + G < cd.n_tiles) tile_reads(nt + G, n_r0, n_r1);
+            const uint32_t nb = tile_b(nt, n_r1);
+            fetch(nt, n_r0, nb);
+            if (tid == 0) { s_cr[0] = n_r0; s_cr[1] = nb; }     // read after this tile's later barriers
+            tile_reads_raw(nt + G, n_r0, n_r1);
         }
 
         // 2. candidate value per p-mer start j: min over the passing forward and
@@ -289,11 +318,11 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
                     }
                     r = r0 + lo;
                 } else {
-                    r = find_read(cd.offs, cd.n_reads, cd.base + gs);
+                    r = find_read(g_offs, cd.n_reads, cd.base + gs);
                 }
-                rbeg = ro_lds ? s_ro[r - r0] : (uint32_t)(cd.offs[r] - cd.base);
-                rnext = ro_lds ? s_ro[r - r0 + 1] : (uint32_t)(cd.offs[r + 1] - cd.base);
-                rend = cd.lens ? rbeg + (uint32_t)cd.lens[r] : rnext;
+                rbeg = ro_lds ? s_ro[r - r0] : (uint32_t)(g_offs[r] - cd.base);
+                rnext = ro_lds ? s_ro[r - r0 + 1] : (uint32_t)(g_offs[r + 1] - cd.base);
+                rend = cd.lens ? rbeg + (uint32_t)g_lens[r] : rnext;
             }
             MM prev = tid > 0 ? s_last[tid - 1] : (MM)0;
             MM cmp[P1_V];
@@ -321,7 +350,7 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
                     split = rnext - g0;
                     rnext2 = s_ro[r - r0 + 2];
                     two = rnext2 >= gend;
-                    rend2 = cd.lens ? rnext + (uint32_t)cd.lens[r + 1] : rnext2;
+                    rend2 = cd.lens ? rnext + (uint32_t)g_lens[r + 1] : rnext2;
                 }
                 if (two) {
                     slow = false;
@@ -348,8 +377,8 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
                     while (g >= rnext) {            // advance (skips empty reads)
                         r++;
                         rbeg = rnext;
-                        rnext = ro_lds ? s_ro[r - r0 + 1] : (uint32_t)(cd.offs[r + 1] - cd.base);
-                        rend = cd.lens ? rbeg + (uint32_t)cd.lens[r] : rnext;
+                        rnext = ro_lds ? s_ro[r - r0 + 1] : (uint32_t)(g_offs[r + 1] - cd.base);
+                        rend = cd.lens ? rbeg + (uint32_t)g_lens[r] : rnext;
                     }
                     const bool valid = g >= rbeg && g < rend && rend - g >= (uint32_t)k;
                     if (valid) {
