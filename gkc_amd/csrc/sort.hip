@@ -1754,36 +1754,64 @@ __global__ __launch_bounds__(256) void k_wavecopy(Plan* __restrict__ plan, SegAr
                                                   const K* __restrict__ alt, const uint2* __restrict__ ranges,
                                                   HeavyList hl, TableOut to) {
     // a wave takes 64 consecutive windows: lane l reads window t0 + l's
-    // offsets (no chain of dependent loads per window), then the wave copies
-    // the windows' entries one window after the other
+    // offsets and its segment's bases (no chain of dependent loads per
+    // window), then the wave copies the windows' entries U windows at a time,
+    // the loads of all U in flight before their stores
+    constexpr int U = 4;
     const uint32_t lane = __lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / GKC_WAVE;
     const uint32_t nw = gridDim.x * blockDim.x / GKC_WAVE;
     const uint32_t total = plan->tot_rl_tiles, S = plan->n_segs;
     for (uint32_t t0 = wave * GKC_WAVE; t0 < total; t0 += nw * GKC_WAVE) {
         const uint32_t t = t0 + lane;
-        uint32_t off = 0, m = 0, a = 0, seg = 0;
+        uint32_t off = 0, m = 0, a = 0, cap = 0;
+        uint64_t kst = 0, tst = 0;
         if (t < total) {
-            seg = find_seg(sa.rl_tile_start, S, t);
+            const uint32_t seg = find_seg(sa.rl_tile_start, S, t);
             const uint32_t last = sa.rl_tile_start[seg + 1] - 1;
             off = hl.kept[t];
             m = (t < last ? hl.kept[t + 1] : to.n_entries[seg]) - off;
-            if (m) a = ranges[t].x;
+            if (m) { a = ranges[t].x; kst = sa.key_start[seg]; tst = to.table_start[seg]; cap = to.table_cap[seg]; }
         }
+        const uint64_t over = __ballot(m && off + m > cap);
+        if (over && lane == (uint32_t)__builtin_ctzll(over)) atomicOr(to.overflow, 1u);   // one flag per wave
         uint64_t todo = __ballot(m != 0);
         while (todo) {
-            const uint32_t b = (uint32_t)__builtin_ctzll(todo);
-            todo &= todo - 1;
-            const uint32_t sg = __shfl(seg, b), o = __shfl(off, b), mb = __shfl(m, b), ab = __shfl(a, b);
-            const K* sk = alt + sa.key_start[sg] + ab;
-            const uint32_t* sc = reinterpret_cast<const uint32_t*>(keys + sa.key_start[sg] + ab);
-            const uint32_t cap = to.table_cap[sg];
-            K* okey = reinterpret_cast<K*>(to.keys) + to.table_start[sg];
-            uint32_t* ocnt = to.counts + to.table_start[sg];
-            if (lane == 0 && o + mb > cap) atomicOr(to.overflow, 1u);     // one flag per window
-            for (uint32_t e = lane; e < mb; e += GKC_WAVE) {
-                const uint32_t idx = o + e;
-                if (idx < cap) { okey[idx] = sk[e]; ocnt[idx] = sc[e]; }
+            uint32_t mb[U], o[U], cp[U];
+            uint64_t src[U], dst[U];
+            K kv[U];
+            uint32_t cv[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const uint32_t b = todo ? (uint32_t)__builtin_ctzll(todo) : 0u;
+                const bool valid = todo != 0;
+                todo &= todo - 1;
+                mb[u] = valid ? __shfl(m, b) : 0u;
+                o[u] = __shfl(off, b);
+                cp[u] = __shfl(cap, b);
+                src[u] = __shfl(kst, b) + __shfl(a, b);
+                dst[u] = __shfl(tst, b);
+                if (lane < mb[u] && o[u] + lane < cp[u]) {
+                    kv[u] = alt[src[u] + lane];
+                    cv[u] = reinterpret_cast<const uint32_t*>(keys + src[u])[lane];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (lane < mb[u] && o[u] + lane < cp[u]) {
+                    reinterpret_cast<K*>(to.keys)[dst[u] + o[u] + lane] = kv[u];
+                    to.counts[dst[u] + o[u] + lane] = cv[u];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {             // windows of more than 64 entries: the rest
+                for (uint32_t e = lane + GKC_WAVE; e < mb[u]; e += GKC_WAVE) {
+                    const uint32_t idx = o[u] + e;
+                    if (idx < cp[u]) {
+                        reinterpret_cast<K*>(to.keys)[dst[u] + idx] = alt[src[u] + e];
+                        to.counts[dst[u] + idx] = reinterpret_cast<const uint32_t*>(keys + src[u])[e];
+                    }
+                }
             }
         }
     }
