@@ -170,9 +170,9 @@ constexpr int WC_NT = 256;
 constexpr int WC_TILE = 256;
 constexpr int WC_CAP = 512;
 constexpr uint32_t WC_FILL = 384;
-// 64-bit windows of up to WC_CAP keys are first counted in the first
-// WC_SMALL slots of the table (cleared and compacted in half the time), and
-// again in all WC_CAP slots once more than WC_SMALL_FILL of them are taken.
+// Windows of up to WC_CAP keys are first counted in the first WC_SMALL
+// slots of the table (cleared and compacted in half the time), and again in
+// all WC_CAP slots once more than WC_SMALL_FILL of them are taken.
 constexpr uint32_t WC_SMALL = 256;
 constexpr uint32_t WC_SMALL_FILL = 191;
 static_assert(WC_SMALL_FILL + 64 < WC_SMALL, "a free slot remains after any 64 inserts");

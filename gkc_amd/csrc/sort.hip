@@ -1522,10 +1522,11 @@ template <> struct WaveTable<u128k> {
     uint32_t tag[WC_CAP];
     __device__ void clear(uint32_t i) { tag[i] = 0; cnt[i] = 0; }
     __device__ bool used(uint32_t i) const { return tag[i] != 0; }
-    __device__ bool insert(const u128k& x) {
+    __device__ bool insert(const u128k& x, int lg = __builtin_ctz(WC_CAP)) {
+        const uint32_t mask = (1u << lg) - 1u;
         const unsigned long long hv = ((unsigned long long)x.lo ^ ((unsigned long long)x.hi * 0xC2B2AE3D27D4EB4Full)) *
                                       0x9E3779B97F4A7C15ull;
-        uint32_t h = (uint32_t)(hv >> (64 - __builtin_ctz(WC_CAP)));
+        uint32_t h = (uint32_t)(hv >> (64 - lg));
         const uint32_t fp = (uint32_t)hv | 1u;
         for (;;) {
             const uint32_t old = atomicCAS(&tag[h], 0u, fp);
@@ -1534,7 +1535,7 @@ template <> struct WaveTable<u128k> {
             // compiler from reading key[h] ahead of a store it does not make
             wave_lds_fence();
             if ((old == 0u || old == fp) && key[h] == x) { atomicAdd(&cnt[h], 1u); return old == 0u; }
-            h = (h + 1) & (WC_CAP - 1);
+            h = (h + 1) & mask;
         }
     }
 };
@@ -1549,8 +1550,10 @@ template <> struct WaveTable<u128k> {
 // and records the window's kept count in hl.kept[tile].  The per-segment
 // distinct and kept totals are accumulated.  No window waits for another:
 // k_tilescan turns the kept counts into table offsets, k_wavecopy writes.
+// waves per SIMD the LDS tables allow (64-bit keys: 6 KB per wave; 128-bit: 12 KB)
+template <typename K> constexpr int wavecount_waves() { return sizeof(K) == 16 ? 3 : 6; }
 template <typename K>
-__global__ __launch_bounds__(WC_NT, 6) void k_wavecount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl,
+__global__ __launch_bounds__(WC_NT, wavecount_waves<K>()) void k_wavecount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl,
                                                      K* __restrict__ keys, K* __restrict__ alt,
                                                      const uint2* __restrict__ ranges, HeavyList hl, TableOut to) {
     constexpr int NW = WC_NT / GKC_WAVE;
@@ -1611,7 +1614,7 @@ __global__ __launch_bounds__(WC_NT, 6) void k_wavecount(Plan* __restrict__ plan,
         } else if (n) {
             // hash-count the range in the wave's table
             K x[IT];
-            const bool small = MIDS && !mid && no_small == 0;
+            const bool small = !mid && no_small == 0;
 #pragma unroll
             for (int i = 0; i < IT; i++) {
                 const uint32_t j = i * GKC_WAVE + lane;
@@ -1620,29 +1623,27 @@ __global__ __launch_bounds__(WC_NT, 6) void k_wavecount(Plan* __restrict__ plan,
             }
             wave_lds_fence();    // other lanes' slots cleared
             if (small) {
-                if constexpr (MIDS) {
-                    uint32_t taken = 0;
-                    bool over = false;
+                uint32_t taken = 0;
+                bool over = false;
+#pragma unroll
+                for (int i = 0; i < IT; i++) {
+                    if (i * GKC_WAVE >= (int)n) break;
+                    if (taken > WC_SMALL_FILL) { over = true; break; }
+                    bool fresh = false;
+                    if (i * GKC_WAVE + lane < n) fresh = T.insert(x[i], __builtin_ctz(WC_SMALL));
+                    taken += (uint32_t)__popcll(__ballot(fresh));
+                }
+                if (over) {                     // count again in the full table
+                    no_small = 16;
+#pragma unroll
+                    for (int i = 0; i < IT; i++) T.clear(i * GKC_WAVE + lane);
+                    wave_lds_fence();
 #pragma unroll
                     for (int i = 0; i < IT; i++) {
-                        if (i * GKC_WAVE >= (int)n) break;
-                        if (taken > WC_SMALL_FILL) { over = true; break; }
-                        bool fresh = false;
-                        if (i * GKC_WAVE + lane < n) fresh = T.insert(x[i], __builtin_ctz(WC_SMALL));
-                        taken += (uint32_t)__popcll(__ballot(fresh));
+                        if (i * GKC_WAVE < (int)n && i * GKC_WAVE + lane < n) T.insert(x[i]);
                     }
-                    if (over) {                 // count again in the full table
-                        no_small = 16;
-#pragma unroll
-                        for (int i = 0; i < IT; i++) T.clear(i * GKC_WAVE + lane);
-                        wave_lds_fence();
-#pragma unroll
-                        for (int i = 0; i < IT; i++) {
-                            if (i * GKC_WAVE < (int)n && i * GKC_WAVE + lane < n) T.insert(x[i]);
-                        }
-                    } else {
-                        slots = WC_SMALL;
-                    }
+                } else {
+                    slots = WC_SMALL;
                 }
             } else if (!mid) {
                 if (no_small) no_small--;

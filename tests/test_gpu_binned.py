@@ -202,3 +202,20 @@ def test_heavy_window_in_lds(gpu, n_distinct, copies, minc):
     gt, st = _count(reads, offs, 31, 11, 1, mode=0, min_count=minc)
     assert st["binned"] == 1 and st["heavy_ranges"] >= 1
     assert_tables_equal(gt, ot, 1, 31)
+
+
+@pytest.mark.parametrize("npss,minc", [(3, 1), (2, 1), (3, 2)])
+def test_wave_table_full_after_small(gpu, npss, minc):
+    """Random reads (almost every k-mer distinct, a few repeated): windows of
+    up to 512 keys hold more than 191 distinct keys, so the wave-level count
+    gives up its 256-slot table and counts them again in all 512 slots; both
+    paths meet in one table (mixed with windows that fit the small table)."""
+    rng = np.random.default_rng(23 + npss)
+    seqs = [bytes(rng.choice(list(b"ACGT"), 150).tolist()) for _ in range(12000)]
+    seqs += seqs[:3000]                                   # some k-mers twice
+    reads, offs = _reads(seqs)
+    ot = O.count(reads, offs, 31, 11, 2, min_count=minc)
+    _say("wave table full after small", npss, minc)
+    gt, st = _count(reads, offs, 31, 11, 2, mode=0, min_count=minc, bin_passes=npss)
+    assert st["binned"] == 1 and st["n_pass"] == npss
+    assert_tables_equal(gt, ot, 2, 31)
