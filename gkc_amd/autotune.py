@@ -3,9 +3,9 @@
 Picks the onesweep tile size (keys per thread) and the number of persistent
 workgroups per CU of the sort passes by timing every candidate on the data
 already in a counter's bins (the batched form, instrumented pass timing),
-then the number of onesweep passes before the binned count (2 or 3 for
-64-bit keys: fewer passes leave larger bins, which pays when the bins stay
-small enough for the wave-level counts), and caches the winners keyed on
+then the number of onesweep passes before the binned count (2 or 3: fewer
+passes leave larger bins, which pays when the bins stay small enough for
+the wave-level counts), and caches the winners keyed on
 (key width, read-length bucket, partition-size bucket, partitions) in
 gkc_amd/tune_cache.json.  Every candidate produces the same table
 (bit-exactness across variants is checked in tests/test_autotune.py and
@@ -104,12 +104,10 @@ def tune_sort(counter: "capi.Counter", reps: int = 2, log=None) -> dict:
 
 def tune_bin_passes(counter: "capi.Counter", reps: int = 3, log=None) -> dict:
     """Time whole batched counts (graph replay) of the counter's current bins
-    with 2 and 3 onesweep passes before the binned count (64-bit keys; 128-bit
-    keys keep the default); returns the best {bin_passes, ms_count} and the
-    sweep.  The sort tuning already set is kept."""
+    with 2 and 3 onesweep passes before the binned count; returns the best
+    {bin_passes, ms_count} and the sweep.  The sort tuning already set is
+    kept."""
     t0 = counter.get_tuning()
-    if counter.k > 32:
-        return {"bin_passes": 0, "ms_count": None, "sweep": []}
     sweep = []
     for npass in (3, 2):
         counter.set_tuning(t0["sort_items"], t0["sort_blocks_per_cu"], t0["single_list"], bin_passes=npass)

@@ -37,7 +37,7 @@ def test_cache_entries_are_valid_variants():
     for key, v in autotune.load_cache().items():
         cands = autotune.CANDIDATES_64 if key.startswith("u64") else autotune.CANDIDATES_128
         assert (v["sort_items"], v["sort_blocks_per_cu"]) in cands, key
-        assert v.get("bin_passes", 0) in ((0, 2, 3) if key.startswith("u64") else (0,)), key
+        assert v.get("bin_passes", 0) in (0, 2, 3), key
 
 
 @pytest.mark.gpu
