@@ -70,7 +70,8 @@ def nearest(key: str) -> dict | None:
 def apply(counter: "capi.Counter", key: str) -> dict | None:
     t = nearest(key)
     if t:
-        counter.set_tuning(t["sort_items"], t["sort_blocks_per_cu"], bin_passes=t.get("bin_passes", 0))
+        counter.set_tuning(t["sort_items"], t["sort_blocks_per_cu"], bin_passes=t.get("bin_passes", 0),
+                           count_window=t.get("count_window", 0))
     return t
 
 
@@ -102,23 +103,31 @@ def tune_sort(counter: "capi.Counter", reps: int = 2, log=None) -> dict:
     return {**win, "sweep": sweep}
 
 
+COUNT_WINDOWS = (256, 384)
+
+
 def tune_bin_passes(counter: "capi.Counter", reps: int = 3, log=None) -> dict:
     """Time whole batched counts (graph replay) of the counter's current bins
-    with 2 and 3 onesweep passes before the binned count; returns the best
-    {bin_passes, ms_count} and the sweep.  The sort tuning already set is
+    with 2 and 3 onesweep passes before the binned count and windows of 256
+    and 384 keys (larger windows spread each window's fixed work over more
+    keys, but hold more distinct keys); returns the best {bin_passes,
+    count_window, ms_count} and the sweep.  The sort tuning already set is
     kept."""
     t0 = counter.get_tuning()
     sweep = []
     for npass in (3, 2):
-        counter.set_tuning(t0["sort_items"], t0["sort_blocks_per_cu"], t0["single_list"], bin_passes=npass)
-        best = None
-        for _ in range(reps):
-            counter.count(capi.BATCHED)
-            ms = counter.stats()["ms_count"]
-            best = ms if best is None else min(best, ms)
-        sweep.append({"bin_passes": npass, "ms_count": best})
-        if log:
-            log(f"tune bin_passes={npass}: {best:.3f} ms per count")
+        for cw in COUNT_WINDOWS:
+            counter.set_tuning(t0["sort_items"], t0["sort_blocks_per_cu"], t0["single_list"], bin_passes=npass,
+                               count_window=cw)
+            best = None
+            for _ in range(reps):
+                counter.count(capi.BATCHED)
+                ms = counter.stats()["ms_count"]
+                best = ms if best is None else min(best, ms)
+            sweep.append({"bin_passes": npass, "count_window": cw, "ms_count": best})
+            if log:
+                log(f"tune bin_passes={npass} count_window={cw}: {best:.3f} ms per count")
     win = min(sweep, key=lambda r: r["ms_count"])
-    counter.set_tuning(t0["sort_items"], t0["sort_blocks_per_cu"], t0["single_list"], bin_passes=win["bin_passes"])
+    counter.set_tuning(t0["sort_items"], t0["sort_blocks_per_cu"], t0["single_list"], bin_passes=win["bin_passes"],
+                       count_window=win["count_window"])
     return {**win, "sweep": sweep}

@@ -749,6 +749,8 @@ int gkc_set_tuning(gkc_ctx* c, const gkc_tuning* t) {
         if (!t) throw ArgError{"null tuning", GKC_EINVAL};
         if (t->count_mode < 0 || t->count_mode > 1) throw ArgError{"count_mode must be 0 or 1", GKC_EINVAL};
         if (t->bin_passes < 0 || t->bin_passes > 3) throw ArgError{"bin_passes must be 0..3", GKC_EINVAL};
+        if (t->count_window != 0 && (t->count_window < 128 || t->count_window > 1024 || t->count_window % 64))
+            throw ArgError{"count_window must be 0 or a multiple of 64 in 128..1024", GKC_EINVAL};
         c->tune = *t;
         return 0;
     });
@@ -1165,7 +1167,8 @@ static BatchShape batch_shape(const gkc_ctx* c) {
     const int kb = 2 * c->prm.k;
     if (c->tune.count_mode == 1 || c->prm.k < 8) return BatchShape{false, false, c->npass, 0u, RL_TILE, 0};
     const int np = std::min(c->tune.bin_passes > 0 ? c->tune.bin_passes : 3, (kb + 7) / 8);
-    return BatchShape{true, true, np, (uint32_t)std::max(0, kb - 8 * np), WC_TILE, 8 * np};
+    const int win = c->tune.count_window > 0 ? c->tune.count_window : WC_TILE;
+    return BatchShape{true, true, np, (uint32_t)std::max(0, kb - 8 * np), win, 8 * np};
 }
 
 // Enqueue extraction + the onesweep passes over every partition at once.
@@ -1354,7 +1357,8 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
                                     (const void*)(uintptr_t)tiles_cap, (const void*)(uintptr_t)bs.npass,
                                     (const void*)(uintptr_t)sort_items<K>(c), (const void*)(uintptr_t)sort_grid<K>(c),
                                     c->d_wl.p, (const void*)(uintptr_t)c->wl.n_lists,
-                                    (const void*)(uintptr_t)(bs.binned ? 1 : 0), (const void*)(uintptr_t)(bs.wave ? 1 : 0)};
+                                    (const void*)(uintptr_t)(bs.binned ? 1 : 0), (const void*)(uintptr_t)(bs.wave ? 1 : 0),
+                                    (const void*)(uintptr_t)bs.rl_tile};
     if (timed) {
         enqueue_sort_all<K>(c, sa, bs, tiles_cap, true);
     } else {

@@ -81,6 +81,8 @@ typedef struct gkc_tuning {
     int32_t count_mode;          /* batched form: 0 binned (onesweep passes on the top key bits, then
                                     per-window hash count of whole bins; default for k >= 8), 1 full LSD sort */
     int32_t bin_passes;          /* binned form: 8-bit onesweep passes before the count, 1..3 (0: automatic) */
+    int32_t count_window;        /* binned form: keys per count window (the bins starting in it), a multiple
+                                    of 64 in 128..1024 (0: 256); the same table for every value */
 } gkc_tuning;
 
 const char *gkc_last_error(void);

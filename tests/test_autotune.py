@@ -38,6 +38,7 @@ def test_cache_entries_are_valid_variants():
         cands = autotune.CANDIDATES_64 if key.startswith("u64") else autotune.CANDIDATES_128
         assert (v["sort_items"], v["sort_blocks_per_cu"]) in cands, key
         assert v.get("bin_passes", 0) in (0, 2, 3), key
+        assert v.get("count_window", 0) in (0,) + autotune.COUNT_WINDOWS, key
 
 
 @pytest.mark.gpu

@@ -23,10 +23,11 @@ def _say(*a):
     print("[gpu-case]", *a, flush=True)
 
 
-def _count(reads, offs, k, p, P, mode, hpc=False, min_count=1, max_count=0, batches=2, caps=None, bin_passes=0):
+def _count(reads, offs, k, p, P, mode, hpc=False, min_count=1, max_count=0, batches=2, caps=None, bin_passes=0,
+           count_window=0):
     n = len(offs) - 1
     with capi.Counter(k, p, P, hpc=hpc, min_count=min_count, max_count=max_count) as c:
-        c.set_tuning(count_mode=mode, bin_passes=bin_passes)
+        c.set_tuning(count_mode=mode, bin_passes=bin_passes, count_window=count_window)
         bounds = np.linspace(0, n, batches + 1).astype(int)
         for a, b in zip(bounds[:-1], bounds[1:]):
             if b > a:
@@ -202,6 +203,20 @@ def test_heavy_window_in_lds(gpu, n_distinct, copies, minc):
     gt, st = _count(reads, offs, 31, 11, 1, mode=0, min_count=minc)
     assert st["binned"] == 1 and st["heavy_ranges"] >= 1
     assert_tables_equal(gt, ot, 1, 31)
+
+
+@pytest.mark.parametrize("k,p,P,cw", [(31, 11, 4, 128), (31, 11, 4, 384), (31, 11, 4, 1024), (45, 15, 4, 384)])
+def test_count_window(gpu, k, p, P, cw):
+    """Every window size of the binned count gives the oracle's table (larger
+    windows: more ranges of over 512 keys, counted 64 keys at a time)."""
+    w = W.workload(1, scale=0.03)
+    rs = W.generate(w)
+    ot = O.count(rs.reads, rs.offs, k, p, P)
+    for npss in (2, 3):
+        _say("count window", cw, npss, k)
+        gt, st = _count(rs.reads, rs.offs, k, p, P, mode=0, bin_passes=npss, count_window=cw)
+        assert st["binned"] == 1 and st["n_pass"] == npss
+        assert_tables_equal(gt, ot, P, k)
 
 
 @pytest.mark.parametrize("npss,minc", [(3, 1), (2, 1), (3, 2)])
