@@ -94,6 +94,7 @@ SIGNATURES = {
     "gkc_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "gkc_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "gkc_part_kmers": (C.c_int, [C.c_void_p, _u64p]),
+    "gkc_parse_reads": (C.c_int64, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
 _bound = None
@@ -263,6 +264,15 @@ class Counter:
                                       max(cap, 1), C.byref(ne), C.byref(nd)))
         n = ne.value
         return lo[:n], (None if hi is None else hi[:n]), cnt[:n], nd.value
+
+
+def parse_reads(text_ptr: int, n_bytes: int, bases_ptr: int, offs_ptr: int, fmt: int = 0, stream: int = 0) -> int:
+    """FASTQ / single-line FASTA text in device memory to the read CSR (device
+    buffers: bases of n_bytes bytes, offs of n_bytes // 2 + 2 entries), the
+    reference's ReadLoader record rules (read_loader.hpp:44-196) on the GPU;
+    returns the number of reads.  fmt: 0 from the first byte, 1 FASTA, 2 FASTQ."""
+    return _check(lib().gkc_parse_reads(C.c_void_p(text_ptr), n_bytes, fmt, C.c_void_p(bases_ptr),
+                                        C.c_void_p(offs_ptr), C.c_void_p(stream)))
 
 
 def pack_superkmers(seqs: list[bytes]) -> tuple[np.ndarray, np.ndarray]:

@@ -43,6 +43,12 @@ struct HipError {
     } while (0)
 struct ArgError { std::string msg; int code; };
 
+namespace gkc {
+// ingest.hip: text -> read CSR on the device (0, or a GKC_E* code with err set)
+int parse_reads(const char* text, uint64_t n, int format, char* bases, uint64_t* offs, hipStream_t st,
+                int64_t* n_reads, std::string& err);
+}
+
 namespace {
 
 struct DevBuf {
@@ -358,6 +364,15 @@ int gkc_device_select(int device) {
     return 0;
 }
 int gkc_version(void) { return 10000; }
+
+int64_t gkc_parse_reads(const char* text, uint64_t n_bytes, int format, char* bases, uint64_t* offs, void* stream) {
+    if (!offs || (n_bytes && (!text || !bases))) { g_err = "null buffer"; return GKC_EINVAL; }
+    int64_t n = 0;
+    std::string err;
+    const int rc = gkc::parse_reads(text, n_bytes, format, bases, offs, static_cast<hipStream_t>(stream), &n, err);
+    if (rc) { g_err = err; return rc; }
+    return n;
+}
 
 gkc_ctx* gkc_create(int device, const gkc_params* prm) {
     if (!prm) { g_err = "null params"; return nullptr; }

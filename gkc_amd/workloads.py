@@ -240,3 +240,41 @@ def read_fasta(path: str) -> ReadSet:
     lib.gkc_read_fasta(path.encode(), reads.ctypes.data_as(ctypes.c_char_p),
                        offs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(tot))
     return ReadSet(reads[:tot.value], offs)
+
+
+def reads_text(reads: np.ndarray, offs: np.ndarray, fmt: str = "fastq", crlf: bool = False,
+               final_newline: bool = True) -> np.ndarray:
+    """FASTQ ('@r' header, the sequence, '+', a quality line of printable
+    bytes derived from the bases, which include '@' and '+') or single-line
+    FASTA ('>r' header, the sequence) text of a read CSR, as uint8."""
+    reads = np.ascontiguousarray(reads, np.uint8)
+    offs = np.ascontiguousarray(offs, np.uint64).astype(np.int64)
+    lens = np.diff(offs)
+    nl = b"\r\n" if crlf else b"\n"
+    hdr = (b"@r" if fmt == "fastq" else b">r") + nl
+    sep = b"+" + nl
+    fq = fmt == "fastq"
+    rec = len(hdr) + lens + len(nl) + ((len(sep) + lens + len(nl)) if fq else 0)
+    start = np.zeros(len(lens), np.int64)
+    np.cumsum(rec[:-1], out=start[1:])
+    total = int(rec.sum())
+    out = np.empty(total, np.uint8)
+    ar = np.arange(int(offs[-1] - offs[0]), dtype=np.int64)
+    base0 = offs[:-1] - offs[0]
+
+    def put(pos, bs):
+        for j, ch in enumerate(bs):
+            out[pos + j] = ch
+    put(start, hdr)
+    s0 = start + len(hdr)
+    out[np.repeat(s0 - base0, lens) + ar] = reads[offs[0]:offs[-1]]
+    put(s0 + lens, nl)
+    if fq:
+        p = s0 + lens + len(nl)
+        put(p, sep)
+        q0 = p + len(sep)
+        out[np.repeat(q0 - base0, lens) + ar] = (reads[offs[0]:offs[-1]] % 40) + 33
+        put(q0 + lens, nl)
+    if not final_newline and total:
+        out = out[:total - len(nl)]
+    return out

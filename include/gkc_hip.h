@@ -110,6 +110,20 @@ int gkc_superkmer_arrays(gkc_ctx *ctx, const char *reads, const uint64_t *offs, 
                          uint32_t *minimizers, int32_t *skm_offs, char *reads_out,
                          int32_t *read_len, int32_t *hpc_orig_pos);
 
+/* ---- on-device read ingest (replaces the host ReadLoader's record parsing,
+ * read_loader.hpp:44-196: _load_fastq / _load_fasta) ----
+ * FASTQ text (4-line records: '@' header, sequence, '+', quality) or
+ * single-line FASTA text ('>' header, one sequence line) in device memory to
+ * the read CSR gkc_partition takes, on the device: the reads' bases back to
+ * back in `bases` and n + 1 absolute offsets in `offs` (offs[0] = 0).  The read
+ * is the sequence line without a trailing '\r'; the last line may lack its
+ * newline.  format: 0 from the first byte, 1 FASTA, 2 FASTQ.  Capacities:
+ * bases n_bytes bytes, offs n_bytes / 2 + 2 entries.  Runs on `stream` (a
+ * hipStream_t, NULL for the default stream) and returns when done: the
+ * number of reads, or a negative GKC_E* code (GKC_EINVAL for text that is not
+ * whole records of the format). */
+int64_t gkc_parse_reads(const char *text, uint64_t n_bytes, int format, char *bases, uint64_t *offs, void *stream);
+
 /* ---- device-resident path (what GenSuperkmerGPU + SKMStore::add_skms do,
  * superkmers.hpp:167-186, without leaving the GPU) ----
  * Encode, minimizer, superkmer and scatter one batch of reads into the

@@ -159,3 +159,49 @@ def brute_force_table(reads: np.ndarray, offs: np.ndarray, k: int, hpc_on: bool 
                 v = (v << 2) | _CODE[ch]
             table[v] = table.get(v, 0) + 1
     return table
+
+
+class ParseError(ValueError):
+    pass
+
+
+def parse_reads(text, fmt: int = 0):
+    """Reads of FASTQ / single-line FASTA text as a CSR (bases uint8, offsets
+    uint64), restating the reference's record parsing:
+    read_loader.hpp:44-121 (_load_fastq: 4-line records, '@' header,
+    sequence, '+', quality; the read is the sequence line) and
+    read_loader.hpp:123-196 (_load_fasta: '>' header and one sequence line;
+    the last line may lack its newline); a '\\r' ending a sequence line is
+    dropped (:103, :177).  Text that is not whole records of the format raises
+    ParseError.  fmt: 0 from the first byte, 1 FASTA, 2 FASTQ."""
+    a = np.frombuffer(bytes(text), np.uint8) if not isinstance(text, np.ndarray) else np.ascontiguousarray(text, np.uint8)
+    n = len(a)
+    if n == 0:
+        return np.zeros(0, np.uint8), np.zeros(1, np.uint64)
+    if fmt == 0:
+        fmt = 1 if a[0] == ord(">") else 2 if a[0] == ord("@") else 0
+        if fmt == 0:
+            raise ParseError("text starts with neither '>' nor '@'")
+    L = 4 if fmt == 2 else 2
+    nl = np.flatnonzero(a == 10).astype(np.int64)
+    starts = np.concatenate([[0], nl + 1])
+    ends = np.concatenate([nl, [n]])
+    if a[-1] == 10:                      # no line after the final newline
+        starts, ends = starts[:-1], ends[:-1]
+    if len(starts) % L:
+        raise ParseError("text does not hold whole records")
+    hs, he = starts[0::L], ends[0::L]
+    ok = (hs < he) & (a[np.minimum(hs, n - 1)] == (ord("@") if fmt == 2 else ord(">")))
+    if fmt == 2:
+        ps, pe = starts[2::L], ends[2::L]
+        ok &= (ps < pe) & (a[np.minimum(ps, n - 1)] == ord("+"))
+    if not ok.all():
+        raise ParseError("malformed record")
+    ss, se = starts[1::L], ends[1::L].copy()
+    cr = (se > ss) & (a[np.maximum(se - 1, 0)] == 13)
+    se -= cr
+    lens = se - ss
+    offs = np.zeros(len(lens) + 1, np.uint64)
+    np.cumsum(lens, out=offs[1:])
+    idx = np.repeat(ss - offs[:-1].astype(np.int64), lens) + np.arange(int(offs[-1]), dtype=np.int64)
+    return a[idx], offs
