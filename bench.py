@@ -68,6 +68,48 @@ def learned_report(predictor, ctx, part_distinct, part_kmers) -> dict:
             "margin": predictor.margin, "trained_on": predictor.info.get("pairs")}
 
 
+def ingest_report(rs, max_reads, with_cpu, np, torch, capi, W) -> dict:
+    """On-device FASTQ ingest (gkc_parse_reads) of the first reads of the
+    workload: the text resident in HBM, the call timed by wall clock (it
+    returns when its work is done; scratch allocation included), its output
+    checked against the reads.  Roofline: text read once, bases and offsets
+    written once, over the call's time."""
+    nr = min(max_reads, rs.n)
+    t = W.reads_text(rs.reads, rs.offs[:nr + 1], "fastq")
+    n = len(t)
+    nb = int(rs.offs[nr] - rs.offs[0])
+    text = torch.from_numpy(t).cuda()
+    bases = torch.empty(n, dtype=torch.uint8, device="cuda")
+    offs = torch.empty(n // 2 + 2, dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+    args = (text.data_ptr(), n, bases.data_ptr(), offs.data_ptr())
+    got = capi.parse_reads(*args)
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        got = capi.parse_reads(*args)
+        times.append(time.perf_counter() - t0)
+    ms = sorted(times)[len(times) // 2] * 1e3
+    want = torch.from_numpy(np.ascontiguousarray(rs.reads[int(rs.offs[0]):int(rs.offs[nr])])).cuda()
+    ok = got == nr and int(offs[nr].item()) == nb and bool(torch.equal(bases[:nb], want))
+    alg = n + nb + 8 * (nr + 1)
+    rep = {"format": "fastq", "text_bytes": n, "reads": nr, "bases": nb, "ms": ms, "text_GB_per_s": n / ms / 1e6,
+           "reads_per_s": nr / ms * 1e3, "output_equal": ok,
+           "roofline": {"bound": "hbm", "achieved": alg / ms / 1e6, "peak": 8000.0, "unit": "GB/s",
+                        "frac": alg / ms / 1e6 / 8000.0, "algorithmic_bytes": alg}}
+    if with_cpu:
+        from oracle import oracle as O
+        ns = min(200_000, nr)
+        ts = W.reads_text(rs.reads, rs.offs[:ns + 1], "fastq")
+        t0 = time.perf_counter()
+        O.parse_reads(ts)
+        dt = time.perf_counter() - t0
+        rep["cpu_baseline"] = {"value": len(ts) / dt / 1e9, "unit": "text GB/s", "cores": 1, "kind": "port",
+                               "sample": f"FASTQ text of the first {ns} reads ({len(ts)} bytes), oracle parse"}
+    del text, bases, offs, want
+    return rep
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +121,8 @@ def main() -> int:
     ap.add_argument("--cpu-reads", type=int, default=1_500_000, help="reads in the CPU baseline sample")
     ap.add_argument("--forms", default="plain,batched")
     ap.add_argument("--tune", action="store_true", help="run the autotuner on this workload and cache the result")
+    ap.add_argument("--no-ingest", action="store_true", help="skip the on-device FASTQ ingest measurement")
+    ap.add_argument("--ingest-reads", type=int, default=2_000_000, help="reads in the ingest sample")
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with a 1 GiB device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -254,6 +298,8 @@ def main() -> int:
                                           f"oracle phase1 {ot.times[0]:.2f}s + phase2 {ot.times[1]:.2f}s"}
         line["bit_exact_sample"] = exact
     ctx.close()
+    if not args.no_ingest:
+        line["ingest"] = ingest_report(rs, args.ingest_reads, not args.no_cpu, np, torch, capi, W)
     if args.calib_copy:
         a = torch.ones(1 << 28, dtype=torch.int32, device=dev)
         b = torch.empty_like(a)
