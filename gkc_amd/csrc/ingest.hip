@@ -13,9 +13,9 @@
 // reported as malformed rather than guessed at.
 //
 // Steps, every one over all the text at once: newlines counted per 4 KiB
-// block, the counts scanned, the newline positions written in order; per
-// record its header / separator checked and its sequence line located; the
-// sequence lengths scanned into offsets; the sequence bytes gathered.
+// block and the counts scanned; each newline ranked and placed in its record
+// (sequence start and end, header / separator checks); the sequence lengths
+// scanned into offsets; the sequence bytes gathered by output chunks.
 #include <algorithm>
 #include <string>
 
@@ -35,12 +35,13 @@ struct TextView {
     __device__ uint8_t at(uint64_t q) const { return base[lead + q]; }
 };
 
-// Newline mask of the view's bytes [a, a + 16): bit i for byte a + i, only
-// for bytes of the text (lead <= a + i < lead + n).
-__device__ __forceinline__ uint32_t nl_mask16(const TextView& tv, uint64_t a) {
+// The view's bytes [a, a + 16) as four little-endian words: loaded whole
+// where they end inside the text, byte by byte (0 past the text's end) at the
+// text's end, not at all (0) where they hold no text byte.
+__device__ __forceinline__ void load16(const TextView& tv, uint64_t a, uint32_t w[4]) {
     const uint64_t lo = tv.lead, hi = tv.lead + tv.n;
-    if (a + 16 <= lo || a >= hi) return 0u;
-    uint32_t w[4];
+    w[0] = w[1] = w[2] = w[3] = 0u;
+    if (a + 16 <= lo || a >= hi) return;
     if (a + 16 <= hi) {
         const uint4 v = *reinterpret_cast<const uint4*>(tv.base + a);
         w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
@@ -56,6 +57,13 @@ __device__ __forceinline__ uint32_t nl_mask16(const TextView& tv, uint64_t a) {
             w[j] = x;
         }
     }
+}
+
+// Newline mask of the view's bytes [a, a + 16) held in w: bit i for byte
+// a + i, only for bytes of the text (lead <= a + i < lead + n).
+__device__ __forceinline__ uint32_t nl_bits(const TextView& tv, uint64_t a, const uint32_t w[4]) {
+    const uint64_t lo = tv.lead, hi = tv.lead + tv.n;
+    if (a + 16 <= lo || a >= hi) return 0u;
     uint32_t m = 0;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -69,6 +77,12 @@ __device__ __forceinline__ uint32_t nl_mask16(const TextView& tv, uint64_t a) {
     if (a < lo) m &= ~0u << (uint32_t)(lo - a);
     if (a + 16 > hi) m &= (1u << (uint32_t)(hi - a)) - 1u;
     return m;
+}
+
+__device__ __forceinline__ uint32_t nl_mask16(const TextView& tv, uint64_t a) {
+    uint32_t w[4];
+    load16(tv, a, w);
+    return nl_bits(tv, a, w);
 }
 
 // Block sum of one value per thread (IN_NT threads).
@@ -89,9 +103,15 @@ __device__ __forceinline__ uint64_t block_excl_scan(uint64_t v, uint64_t* s_w, u
     return before + inc - v;
 }
 
-// Newlines per block.
-__global__ __launch_bounds__(IN_NT) void k_nl_count(TextView tv, uint64_t nblk, uint64_t* __restrict__ cnt) {
+// Newlines per block.  Block 0 also records the text's first and last bytes
+// in misc[0] and clears the malformed-record flag misc[1].
+__global__ __launch_bounds__(IN_NT) void k_nl_count(TextView tv, uint64_t nblk, uint64_t* __restrict__ cnt,
+                                                    uint64_t* __restrict__ misc) {
     __shared__ uint64_t s_w[IN_NT / GKC_WAVE];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        misc[0] = (uint64_t)tv.at(0) | (uint64_t)tv.at(tv.n - 1) << 8;
+        misc[1] = 0;
+    }
     for (uint64_t b = blockIdx.x; b < nblk; b += gridDim.x) {
         const uint32_t m = nl_mask16(tv, b * IN_BLK + 16 * threadIdx.x);
         uint64_t tot;
@@ -101,61 +121,81 @@ __global__ __launch_bounds__(IN_NT) void k_nl_count(TextView tv, uint64_t nblk, 
     }
 }
 
-// Newline positions (text positions, ascending); incl[b] = newlines of blocks 0..b.
-__global__ __launch_bounds__(IN_NT) void k_nl_write(TextView tv, uint64_t nblk, const uint64_t* __restrict__ incl,
-                                                    uint64_t* __restrict__ nl) {
+// Every newline, from its rank g among the newlines (incl[b] = newlines of
+// blocks 0..b), placed in its record: with L lines per record (FASTQ 4,
+// FASTA 2), newline g ends line g of record r = g / L.
+//   g % L == 0      header end: the sequence starts after it     -> src[r]
+//   g % L == 1      sequence end, less a '\r' before it         -> end[r]
+//                   (FASTQ: the next line must start with '+')
+//   g % L == L - 1  record end: the next record's header must start with
+//                   the format's marker
+// A last FASTA line without newline is the last record's sequence; its end,
+// tail_end, comes from the host (~0 when there is none).  The first header is
+// checked on the host.
+__global__ __launch_bounds__(IN_NT) void k_lines(TextView tv, uint64_t nblk, const uint64_t* __restrict__ incl,
+                                                 int fastq, uint64_t nrec, uint64_t tail_end,
+                                                 uint64_t* __restrict__ src, uint64_t* __restrict__ end,
+                                                 uint32_t* __restrict__ bad) {
     __shared__ uint64_t s_w[IN_NT / GKC_WAVE];
+    const uint32_t lg = fastq ? 2 : 1, lm = (1u << lg) - 1;
+    const uint8_t marker = fastq ? '@' : '>';
+    const uint32_t lane = __lane_id();
+    uint32_t w[4];                                              // this block's bytes, loaded one block ahead
+    if (blockIdx.x < nblk) load16(tv, (uint64_t)blockIdx.x * IN_BLK + 16 * threadIdx.x, w);
     for (uint64_t b = blockIdx.x; b < nblk; b += gridDim.x) {
         const uint64_t a = b * IN_BLK + 16 * threadIdx.x;
-        uint32_t m = nl_mask16(tv, a);
+        const uint64_t inc_b = incl[b];
+        // the bytes just before and after this thread's 16: the neighbouring
+        // lanes' words, or memory at the wave's edges (loaded here, where they
+        // are text, so that the newline loop below holds no load; before the
+        // next block's bytes, so that waiting for them leaves those in flight)
+        uint32_t prev = __shfl_up(w[3], 1) >> 24, next = __shfl_down(w[0], 1) & 0xFFu;
+        if (lane == 0 && a > tv.lead && a <= tv.lead + tv.n) prev = tv.base[a - 1];
+        if (lane == GKC_WAVE - 1 && a + 16 < tv.lead + tv.n) next = tv.base[a + 16];
+        uint32_t nw[4] = {0u, 0u, 0u, 0u};
+        if (b + gridDim.x < nblk) load16(tv, (b + gridDim.x) * IN_BLK + 16 * threadIdx.x, nw);
+        uint32_t m = nl_bits(tv, a, w);
+        auto byte_at = [&](int i) -> uint32_t {                 // view byte a + i, -1 <= i <= 16
+            if (i < 0) return prev;
+            if (i > 15) return next;
+            return (w[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+        };
         uint64_t tot;
         const uint64_t before = block_excl_scan((uint64_t)__popc(m), s_w, &tot);
-        uint64_t o = incl[b] - tot + before;
-        while (m) {
-            const uint32_t i = (uint32_t)__builtin_ctz(m);
-            m &= m - 1;
-            nl[o++] = a + i - tv.lead;
+        uint64_t g = inc_b - tot + before;
+        bool wrong = false;
+        for (; m; m &= m - 1, g++) {
+            const int i = __builtin_ctz(m);
+            const uint64_t p = a + (uint32_t)i - tv.lead;                      // text position of the newline
+            const uint32_t j = (uint32_t)g & lm;
+            const uint64_t r = g >> lg;
+            if (j == 0) {
+                src[r] = p + 1;
+                if (r == nrec - 1 && tail_end != ~0ull) end[r] = tail_end;
+            }
+            if (j == 1) {
+                end[r] = byte_at(i - 1) == '\r' ? p - 1 : p;                    // p >= 1: a header line precedes
+                if (fastq) wrong |= byte_at(i + 1) != '+';
+            }
+            if (j == lm && r + 1 < nrec) wrong |= byte_at(i + 1) != marker;
         }
+        if (wrong) atomicOr(bad, 1u);
+#pragma unroll
+        for (int k = 0; k < 4; k++) w[k] = nw[k];
         __syncthreads();
     }
 }
 
-// Per record: the header (and, FASTQ, the separator) checked, the sequence
-// line's start and length (without a trailing '\r').  Line i of the text is
-// [start(i), end(i)): start(0) = 0, start(i) = nl[i - 1] + 1, end(i) = nl[i]
-// or the text's end for a last line without newline.
-__global__ __launch_bounds__(IN_NT) void k_records(TextView tv, const uint64_t* __restrict__ nl, uint64_t n_nl,
-                                                   uint64_t nrec, int fastq, uint64_t* __restrict__ src,
-                                                   uint64_t* __restrict__ len, uint32_t* __restrict__ bad) {
-    const uint64_t L = fastq ? 4 : 2;
-    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrec; r += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t h = L * r;
-        auto lstart = [&](uint64_t i) { return i == 0 ? 0ull : nl[i - 1] + 1; };
-        auto lend = [&](uint64_t i) { return i < n_nl ? nl[i] : tv.n; };
-        const uint64_t hs = lstart(h);
-        bool ok = hs < lend(h) && tv.at(hs) == (fastq ? '@' : '>');
-        const uint64_t ss = lstart(h + 1);
-        uint64_t se = lend(h + 1);
-        if (se > ss && tv.at(se - 1) == '\r') se--;
-        if (fastq) {
-            const uint64_t ps = lstart(h + 2);
-            ok = ok && ps < lend(h + 2) && tv.at(ps) == '+';
-        }
-        if (!ok) atomicOr(bad, 1u);
-        src[r] = ss;
-        len[r] = se - ss;
-    }
-}
-
-// In-place inclusive scan of x[0, n) (u64): each 1024-element group scanned
-// by a workgroup with its total to gsum[g]; gsum scanned by one workgroup;
-// the groups' exclusive offsets added.
+// In-place inclusive scan of x[0, n) - sub[0, n) (u64; sub may be null): each
+// 1024-element group scanned by a workgroup with its total to gsum[g]; gsum
+// scanned by one workgroup; the groups' exclusive offsets added.
 constexpr int SC_NT = 1024;
-__global__ __launch_bounds__(SC_NT) void k_scan_groups(uint64_t* __restrict__ x, uint64_t n, uint64_t* __restrict__ gsum) {
+__global__ __launch_bounds__(SC_NT) void k_scan_groups(uint64_t* __restrict__ x, const uint64_t* __restrict__ sub,
+                                                       uint64_t n, uint64_t* __restrict__ gsum) {
     __shared__ uint64_t s_w[SC_NT / GKC_WAVE];
     const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE;
     const uint64_t i = (uint64_t)blockIdx.x * SC_NT + threadIdx.x;
-    const uint64_t v = i < n ? x[i] : 0ull;
+    const uint64_t v = i < n ? x[i] - (sub ? sub[i] : 0ull) : 0ull;
     uint64_t inc = v;
 #pragma unroll
     for (int off = 1; off < GKC_WAVE; off <<= 1) {
@@ -198,16 +238,142 @@ __global__ __launch_bounds__(SC_NT) void k_scan_add(uint64_t* __restrict__ x, ui
     if (i < n) x[i] += gsum[blockIdx.x];
 }
 
-// The reads' bases back to back: one wave per read (grid-stride).
-__global__ __launch_bounds__(IN_NT) void k_gather(TextView tv, const uint64_t* __restrict__ src,
-                                                  const uint64_t* __restrict__ offs, uint64_t nrec,
-                                                  uint8_t* __restrict__ bases) {
-    const uint32_t lane = __lane_id();
-    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / GKC_WAVE;
-    const uint64_t nw = (uint64_t)gridDim.x * blockDim.x / GKC_WAVE;
-    for (uint64_t r = wave; r < nrec; r += nw) {
-        const uint64_t s = src[r], o = offs[r], l = offs[r + 1] - o;
-        for (uint64_t i = lane; i < l; i += GKC_WAVE) bases[o + i] = tv.at(s + i);
+// The bases output viewed from the 4-byte boundary at or below it: output
+// byte q is base[lead + q].
+struct OutView {
+    uint8_t* base;
+    uint64_t lead;
+};
+
+constexpr int GA_NT = 256;
+constexpr int GA_U = 8;                                 // 64-dword steps per pass: 2 KiB of loads in flight per wave
+constexpr uint64_t GA_CHUNK = 4 * GKC_WAVE * GA_U * 4;  // output-view bytes per chunk (8 KiB, 4 passes)
+
+// Sequence bytes to the output, by output chunks: each wave takes 8 KiB
+// chunks of the output view (chunk edges on 4-byte boundaries) and writes
+// them as aligned dwords.  Record r's output byte q comes from text byte
+// q + src[r] - offs[r].  The wave keeps a window of 64 records (their ends
+// and that difference) in LDS, starting at the record that holds the
+// chunk's first byte (a 64-way search over offs); each lane finds its
+// dword's record in the window by stepping forward from the record that
+// holds the pass's first byte, loads the two aligned text dwords that hold
+// the dword's 4 bytes and shifts them together; a dword where one record
+// ends and the next begins takes two such pairs and a byte mask.  A dword at
+// a pass's edge or with bytes of three or more records is written byte by
+// byte, its loads issued before its stores.
+// A pass stops at the window's last record end; the window then moves on by
+// 64 records (empty records take window slots but no bytes).
+__global__ __launch_bounds__(GA_NT) void k_gather(TextView tv, const uint64_t* __restrict__ src,
+                                                  const uint64_t* __restrict__ offs, uint64_t nrec, OutView ov) {
+    __shared__ uint64_t s_e[GA_NT / GKC_WAVE][GKC_WAVE], s_d[GA_NT / GKC_WAVE][GKC_WAVE];
+    const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE;
+    uint64_t* se = s_e[wv];
+    uint64_t* sd = s_d[wv];
+    const GKC_GLOBAL uint32_t* tw = as_global(reinterpret_cast<const uint32_t*>(tv.base));
+    uint32_t* ow = reinterpret_cast<uint32_t*>(ov.base);
+    const uint64_t T = offs[nrec];
+    const uint64_t vend = ov.lead + T;                          // output view end
+    const uint64_t nchunk = (vend + GA_CHUNK - 1) / GA_CHUNK;
+    const uint64_t safe = tv.lead >> 2;                          // a dword holding text byte 0
+    for (uint64_t c = (uint64_t)blockIdx.x * (GA_NT / GKC_WAVE) + wv; c < nchunk;
+         c += (uint64_t)gridDim.x * (GA_NT / GKC_WAVE)) {
+        uint64_t q = std::max<uint64_t>(c * GA_CHUNK, ov.lead) - ov.lead;      // output cursor
+        const uint64_t qc = std::min<uint64_t>((c + 1) * GA_CHUNK, vend) - ov.lead;
+        if (q >= qc) continue;                                  // no bytes (every read empty)
+        // the record holding byte q: offs[lo] <= q < offs[lo + 1]
+        uint64_t lo = 0, hi = nrec;
+        while (hi - lo > 1) {
+            const uint64_t step = (hi - lo + GKC_WAVE - 1) / GKC_WAVE, pos = lo + lane * step;
+            const uint64_t b = __ballot(pos < hi && offs[pos] <= q);            // lane 0 always
+            lo += (uint64_t)(63 - __clzll(b)) * step;
+            hi = std::min(hi, lo + step);
+        }
+        uint64_t wr = lo - GKC_WAVE, E = q, e = 0;
+        while (q < qc) {
+            if (q >= E) {                                        // move the window on
+                wr += GKC_WAVE;
+                const uint64_t r = wr + lane;
+                e = r < nrec ? offs[r + 1] : ~0ull;
+                se[lane] = e;
+                sd[lane] = r < nrec ? src[r] - offs[r] : 0ull;
+                E = __shfl(e, GKC_WAVE - 1);
+                wave_lds_fence();
+                continue;
+            }
+            const uint64_t D0 = (ov.lead + q) >> 2;
+            const uint64_t qe = std::min(std::min(qc, E), 4 * (D0 + GKC_WAVE * GA_U) - ov.lead);
+            const uint64_t va0 = ov.lead + q, vb0 = ov.lead + qe;           // view bytes of this pass
+            // window slot of the record holding byte q; each lane's slot only
+            // moves forward as its dwords do
+            uint32_t idx = (uint32_t)__popcll(__ballot(e <= q));
+            // per step: the aligned text dwords [a, a + 1] of the bytes from the
+            // dword's first record and [b, b + 1] of those from the next, and in
+            // `how` the two shifts (bits 0-7, 8-15), how many leading bytes come
+            // from the first record (16-18; 4 = all), the form (20-21: 0 nothing,
+            // 1 aligned dwords, 2 byte by byte) and the first record's slot (24-29)
+            uint32_t wa0[GA_U], wa1[GA_U], wb0[GA_U], wb1[GA_U], how[GA_U];
+#pragma unroll
+            for (int u = 0; u < GA_U; u++) {
+                const uint64_t D = D0 + GKC_WAVE * u + lane;
+                const uint64_t va = std::max(4 * D, va0), vb = std::min(4 * D + 4, vb0);
+                uint64_t A = safe, B = safe;
+                uint32_t h = 0;
+                if (va < vb) {
+                    const uint64_t pa = va - ov.lead;
+                    while (se[idx] <= pa) idx++;
+                    h = 2u << 20 | idx << 24;
+                    if (va == 4 * D && vb == 4 * D + 4) {
+                        const uint64_t ea = se[idx];
+                        const uint64_t Va = tv.lead + pa + sd[idx];
+                        if (ea >= pa + 4) {                          // one record
+                            A = B = Va >> 2;
+                            h = 1u << 20 | 4u << 16 | (uint32_t)(Va & 3) * 8 | (uint32_t)(Va & 3) * 8 << 8;
+                        } else if (se[idx + 1] >= pa + 4) {          // two: a record ends inside the dword
+                            const uint64_t Vb = tv.lead + pa + sd[idx + 1];
+                            A = Va >> 2;
+                            B = Vb >> 2;
+                            h = 1u << 20 | (uint32_t)(ea - pa) << 16 | (uint32_t)(Va & 3) * 8 | (uint32_t)(Vb & 3) * 8 << 8;
+                        }
+                    }
+                }
+                how[u] = h;
+                const uint32_t sa = h & 0xFF, sb = (h >> 8) & 0xFF;
+                wa0[u] = tw[A];
+                wa1[u] = tw[sa ? A + 1 : A];
+                wb0[u] = tw[B];
+                wb1[u] = tw[sb ? B + 1 : B];
+            }
+#pragma unroll
+            for (int u = 0; u < GA_U; u++) {
+                const uint64_t D = D0 + GKC_WAVE * u + lane;
+                const uint32_t h = how[u], form = (h >> 20) & 3;
+                if (form == 1) {
+                    const uint32_t sa = h & 0xFF, sb = (h >> 8) & 0xFF, na = (h >> 16) & 7;
+                    const uint32_t xa = sa ? (wa0[u] >> sa) | (wa1[u] << (32 - sa)) : wa0[u];
+                    const uint32_t xb = sb ? (wb0[u] >> sb) | (wb1[u] << (32 - sb)) : wb0[u];
+                    const uint32_t m = na >= 4 ? ~0u : (1u << (8 * na)) - 1u;
+                    ow[D] = (xa & m) | (xb & ~m);
+                } else if (form == 2) {                              // pass edge, or 3+ records in the dword
+                    const uint64_t va = std::max(4 * D, va0), vb = std::min(4 * D + 4, vb0);
+                    uint32_t k = (h >> 24) & 63;
+                    uint8_t bt[4];
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        const uint64_t v = 4 * D + t;
+                        bt[t] = 0;
+                        if (v >= va && v < vb) {
+                            const uint64_t p = v - ov.lead;
+                            while (se[k] <= p) k++;
+                            bt[t] = tv.at(p + sd[k]);
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < 4; t++)
+                        if (4 * D + t >= va && 4 * D + t < vb) ov.base[4 * D + t] = bt[t];
+                }
+            }
+            q = qe;
+        }
     }
 }
 
@@ -223,76 +389,81 @@ int hip_fail(hipError_t e, const char* where, std::string& err) {
 }
 #define IN_HIP(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return hip_fail(e_, #x, err); } while (0)
 
-int scan_inplace(uint64_t* x, uint64_t n, hipStream_t st, std::string& err) {
-    if (n == 0) return 0;
-    const uint64_t ng = (n + SC_NT - 1) / SC_NT;
-    Scratch g;
-    IN_HIP(hipMalloc(&g.p, ng * 8));
-    uint64_t* gs = static_cast<uint64_t*>(g.p);
-    hipLaunchKernelGGL(k_scan_groups, dim3((uint32_t)ng), dim3(SC_NT), 0, st, x, n, gs);
+uint64_t scan_groups(uint64_t n) { return (n + SC_NT - 1) / SC_NT; }
+
+// Inclusive scan of x - sub over n elements, with scan_groups(n) words of
+// group sums at gs; queued on st.
+void scan_inplace(uint64_t* x, const uint64_t* sub, uint64_t n, uint64_t* gs, hipStream_t st) {
+    if (n == 0) return;
+    const uint64_t ng = scan_groups(n);
+    hipLaunchKernelGGL(k_scan_groups, dim3((uint32_t)ng), dim3(SC_NT), 0, st, x, sub, n, gs);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SC_NT), 0, st, gs, ng);
     hipLaunchKernelGGL(k_scan_add, dim3((uint32_t)ng), dim3(SC_NT), 0, st, x, n, (const uint64_t*)gs);
-    IN_HIP(hipGetLastError());
-    IN_HIP(hipStreamSynchronize(st));                // before the group sums are freed
-    return 0;
 }
 }  // namespace
 
+// Two device round trips: after the newline count (the host needs the record
+// count and the text's first and last bytes) and at the end (the malformed
+// flag).
 int parse_reads(const char* text, uint64_t n, int format, char* bases, uint64_t* offs, hipStream_t st,
                 int64_t* n_reads, std::string& err) {
     *n_reads = 0;
+    if (format < 0 || format > 2) { err = "format must be 0 (from the first byte), 1 (FASTA) or 2 (FASTQ)"; return GKC_EINVAL; }
     if (n == 0) {
         IN_HIP(hipMemsetAsync(offs, 0, 8, st));
         IN_HIP(hipStreamSynchronize(st));
         return 0;
     }
-    if (format < 0 || format > 2) { err = "format must be 0 (from the first byte), 1 (FASTA) or 2 (FASTQ)"; return GKC_EINVAL; }
-    uint8_t ends[2];
-    IN_HIP(hipMemcpyAsync(&ends[0], text, 1, hipMemcpyDeviceToHost, st));
-    IN_HIP(hipMemcpyAsync(&ends[1], text + n - 1, 1, hipMemcpyDeviceToHost, st));
-    IN_HIP(hipStreamSynchronize(st));
-    if (format == 0) format = ends[0] == '>' ? 1 : ends[0] == '@' ? 2 : 0;
-    if (format == 0) { err = "text starts with neither '>' (FASTA) nor '@' (FASTQ)"; return GKC_EINVAL; }
-    const int fastq = format == 2;
-
     TextView tv;
     tv.lead = reinterpret_cast<uintptr_t>(text) & 15u;
     tv.base = reinterpret_cast<const uint8_t*>(text) - tv.lead;
     tv.n = n;
     const uint64_t nblk = (tv.lead + n + IN_BLK - 1) / IN_BLK;
     const uint32_t grid = (uint32_t)std::min<uint64_t>(nblk, 65536);
-    Scratch sb;
-    IN_HIP(hipMalloc(&sb.p, nblk * 8));
-    uint64_t* blk = static_cast<uint64_t*>(sb.p);
-    hipLaunchKernelGGL(k_nl_count, dim3(grid), dim3(IN_NT), 0, st, tv, nblk, blk);
-    IN_HIP(hipGetLastError());
-    if (int rc = scan_inplace(blk, nblk, st, err)) return rc;
-    uint64_t n_nl = 0;
-    IN_HIP(hipMemcpyAsync(&n_nl, blk + nblk - 1, 8, hipMemcpyDeviceToHost, st));
-    IN_HIP(hipStreamSynchronize(st));
 
-    const uint64_t n_lines = n_nl + (ends[1] != '\n' ? 1 : 0);
+    // blk[nblk] | misc[2] | group sums
+    Scratch sa;
+    IN_HIP(hipMalloc(&sa.p, (nblk + 2 + scan_groups(nblk)) * 8));
+    uint64_t* blk = static_cast<uint64_t*>(sa.p);
+    uint64_t* misc = blk + nblk;
+    hipLaunchKernelGGL(k_nl_count, dim3(grid), dim3(IN_NT), 0, st, tv, nblk, blk, misc);
+    scan_inplace(blk, nullptr, nblk, misc + 2, st);
+    IN_HIP(hipGetLastError());
+    uint64_t h[2];
+    IN_HIP(hipMemcpyAsync(h, blk + nblk - 1, 16, hipMemcpyDeviceToHost, st));
+    IN_HIP(hipStreamSynchronize(st));
+    const uint64_t n_nl = h[0];
+    const uint8_t first = (uint8_t)h[1], last = (uint8_t)(h[1] >> 8);
+
+    if (format == 0) format = first == '>' ? 1 : first == '@' ? 2 : 0;
+    if (format == 0) { err = "text starts with neither '>' (FASTA) nor '@' (FASTQ)"; return GKC_EINVAL; }
+    const int fastq = format == 2;
+    if (first != (fastq ? '@' : '>')) {
+        err = fastq ? "malformed FASTQ record (header without '@' or separator without '+')" : "malformed FASTA record (header without '>')";
+        return GKC_EINVAL;
+    }
+    const uint64_t n_lines = n_nl + (last != '\n' ? 1 : 0);
     const uint64_t L = fastq ? 4 : 2;
     if (n_lines % L) { err = fastq ? "FASTQ text does not hold whole 4-line records" : "FASTA text does not hold whole 2-line records"; return GKC_EINVAL; }
     const uint64_t nrec = n_lines / L;
+    const uint64_t tail_end = (!fastq && last != '\n') ? n - (last == '\r' ? 1 : 0) : ~0ull;
 
-    Scratch snl, ssrc, sbad;
-    IN_HIP(hipMalloc(&snl.p, std::max<uint64_t>(n_nl, 1) * 8));
-    IN_HIP(hipMalloc(&ssrc.p, std::max<uint64_t>(nrec, 1) * 8));
-    IN_HIP(hipMalloc(&sbad.p, 4));
-    uint64_t* nl = static_cast<uint64_t*>(snl.p);
-    uint64_t* src = static_cast<uint64_t*>(ssrc.p);
-    uint32_t* bad = static_cast<uint32_t*>(sbad.p);
-    IN_HIP(hipMemsetAsync(bad, 0, 4, st));
+    // src[nrec] | group sums
+    Scratch sb;
+    IN_HIP(hipMalloc(&sb.p, (nrec + scan_groups(nrec)) * 8));
+    uint64_t* src = static_cast<uint64_t*>(sb.p);
+    uint32_t* bad = reinterpret_cast<uint32_t*>(misc + 1);
     IN_HIP(hipMemsetAsync(offs, 0, 8, st));
-    hipLaunchKernelGGL(k_nl_write, dim3(grid), dim3(IN_NT), 0, st, tv, nblk, (const uint64_t*)blk, nl);
-    const uint32_t rgrid = (uint32_t)std::min<uint64_t>((nrec + IN_NT - 1) / IN_NT + 1, 65536);
-    hipLaunchKernelGGL(k_records, dim3(rgrid), dim3(IN_NT), 0, st, tv, (const uint64_t*)nl, n_nl, nrec, fastq, src,
-                       offs + 1, bad);
-    IN_HIP(hipGetLastError());
-    if (int rc = scan_inplace(offs + 1, nrec, st, err)) return rc;
-    hipLaunchKernelGGL(k_gather, dim3(rgrid), dim3(IN_NT), 0, st, tv, (const uint64_t*)src, (const uint64_t*)offs, nrec,
-                       reinterpret_cast<uint8_t*>(bases));
+    hipLaunchKernelGGL(k_lines, dim3(grid), dim3(IN_NT), 0, st, tv, nblk, (const uint64_t*)blk, fastq, nrec, tail_end,
+                       src, offs + 1, bad);
+    scan_inplace(offs + 1, src, nrec, src + nrec, st);
+    OutView ov;
+    ov.lead = reinterpret_cast<uintptr_t>(bases) & 3u;
+    ov.base = reinterpret_cast<uint8_t*>(bases) - ov.lead;
+    // the output is at most n bytes: chunks of GA_CHUNK view bytes, 4 per workgroup
+    const uint64_t max_chunks = (ov.lead + n + GA_CHUNK - 1) / GA_CHUNK;
+    const uint32_t ggrid = (uint32_t)std::min<uint64_t>((max_chunks + 3) / 4, 2048);
+    hipLaunchKernelGGL(k_gather, dim3(ggrid), dim3(GA_NT), 0, st, tv, (const uint64_t*)src, (const uint64_t*)offs, nrec, ov);
     IN_HIP(hipGetLastError());
     uint32_t hbad = 0;
     IN_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, st));

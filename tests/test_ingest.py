@@ -24,6 +24,7 @@ FIXTURES = [
     (b">s1\nACGT\n>s2\nGG", [b"ACGT", b"GG"]),                               # last line without newline
     (b">s1\r\nAC\r\n>s2\r\n\r\n", [b"AC", b""]),
     (b">x\nacgtN\n", [b"acgtN"]),                                            # bytes kept as they are
+    (b">a\n\n>b\n\n", [b"", b""]),                                          # no bases at all
 ]
 MALFORMED = [
     b"@r\nAC\n+\n",              # 3 lines
@@ -74,20 +75,48 @@ def test_oracle_round_trip(fmt, crlf, final):
     assert _naive(small) == [bytes(sr[int(so[i]):int(so[i + 1])]) for i in range(40)]
 
 
+def _ragged(seed: int, n: int, fmt: str, crlf: bool, final: bool):
+    """Reads of mixed lengths: empty (with a run of 150 in a row), 1-3 bases,
+    short, ~150 and 5-20 kb; the last read non-empty (an empty last FASTQ
+    quality line without newline is not a line)."""
+    rng = np.random.default_rng(seed)
+    kind = rng.integers(0, 5, n)
+    lens = np.select([kind == 0, kind == 1, kind == 2, kind == 3],
+                     [0, rng.integers(1, 4, n), rng.integers(100, 200, n), rng.integers(5000, 20000, n)],
+                     rng.integers(4, 40, n))
+    lens[n // 3:n // 3 + 150] = 0
+    lens[-1] = max(int(lens[-1]), 1)
+    offs = np.zeros(n + 1, np.uint64)
+    offs[1:] = np.cumsum(lens)
+    reads = np.frombuffer(b"ACGTN", np.uint8)[rng.integers(0, 5, int(offs[-1]))]
+    return W.reads_text(reads, offs, fmt, crlf, final), reads, offs
+
+
+@pytest.mark.parametrize("fmt,crlf,final", [("fastq", False, False), ("fasta", True, True)])
+def test_oracle_ragged(fmt, crlf, final):
+    t, reads, offs = _ragged(7, 400, fmt, crlf, final)
+    r, o = O.parse_reads(t)
+    assert np.array_equal(o, offs) and np.array_equal(r, reads)
+
+
 # ---------------------------------------------------------------- GPU path
 
-def _gpu_parse(text: np.ndarray, fmt: int = 0, shift: int = 0):
+def _gpu_parse(text: np.ndarray, fmt: int = 0, shift: int = 0, out_shift: int = 0):
     import torch
     from gkc_amd import capi
     n = len(text)
     buf = torch.zeros(n + shift + 16, dtype=torch.uint8, device="cuda")
     if n:
         buf[shift:shift + n] = torch.from_numpy(np.ascontiguousarray(text)).cuda()
-    bases = torch.empty(max(n, 1), dtype=torch.uint8, device="cuda")
+    guard = 64
+    bases = torch.full((max(n, 1) + out_shift + guard,), 0xEE, dtype=torch.uint8, device="cuda")
     offs = torch.empty(n // 2 + 2, dtype=torch.int64, device="cuda")
-    nr = capi.parse_reads(buf.data_ptr() + shift, n, bases.data_ptr(), offs.data_ptr(), fmt)
+    nr = capi.parse_reads(buf.data_ptr() + shift, n, bases.data_ptr() + out_shift, offs.data_ptr(), fmt)
     o = offs[:nr + 1].cpu().numpy().view(np.uint64)
-    return bases[:int(o[-1])].cpu().numpy(), o
+    b = bases.cpu().numpy()
+    T = int(o[-1])
+    assert (b[:out_shift] == 0xEE).all() and (b[out_shift + T:] == 0xEE).all()     # nothing written outside
+    return b[out_shift:out_shift + T], o
 
 
 def _say(*a):
@@ -126,6 +155,19 @@ def test_gpu_matches_oracle(gpu, fmt, crlf, final, shift):
     r, o = _gpu_parse(t, shift=shift)
     orr, oo = O.parse_reads(t)
     assert np.array_equal(o, oo) and np.array_equal(r, orr)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fmt,crlf,final,shift,out_shift", [("fastq", False, False, 0, 0), ("fastq", True, True, 7, 1),
+                                                            ("fasta", False, True, 2, 2), ("fasta", True, False, 13, 3)])
+def test_gpu_ragged(gpu, fmt, crlf, final, shift, out_shift):
+    """Empty runs longer than the gather's 64-record window, reads of 1-3
+    bases (dwords split between records) and reads longer than its 8 KiB
+    output chunk, at every output alignment."""
+    t, reads, offs = _ragged(11 + out_shift, 500, fmt, crlf, final)
+    _say("ingest ragged", fmt, crlf, final, shift, out_shift, len(t))
+    r, o = _gpu_parse(t, shift=shift, out_shift=out_shift)
+    assert np.array_equal(o, offs) and np.array_equal(r, reads)
 
 
 @pytest.mark.gpu
