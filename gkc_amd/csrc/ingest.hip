@@ -69,9 +69,10 @@ __device__ __forceinline__ uint32_t nl_bits(const TextView& tv, uint64_t a, cons
     for (int j = 0; j < 4; j++) {
         const uint32_t x = w[j] ^ 0x0A0A0A0Au;                 // newline bytes -> 0
         uint32_t t = (x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;          // exact zero-byte test: bit 7 of each byte
-        t = ~(t | x | 0x7F7F7F7Fu);
-#pragma unroll
-        for (int b = 0; b < 4; b++) m |= ((t >> (8 * b + 7)) & 1u) << (4 * j + b);
+        t = ~(t | x | 0x7F7F7F7Fu) >> 7;                       // bits 0, 8, 16, 24
+        t |= t >> 7;                                           // bits 0, 1 | 9, 16 | 17, 24
+        t |= t >> 14;                                          // bits 0, 1, 2, 3 (and above)
+        m |= (t & 15u) << (4 * j);
     }
     // bytes outside the text
     if (a < lo) m &= ~0u << (uint32_t)(lo - a);
@@ -140,6 +141,7 @@ __global__ __launch_bounds__(IN_NT) void k_lines(TextView tv, uint64_t nblk, con
     const uint32_t lg = fastq ? 2 : 1, lm = (1u << lg) - 1;
     const uint8_t marker = fastq ? '@' : '>';
     const uint32_t lane = __lane_id();
+    if (blockIdx.x == 0 && threadIdx.x == 0 && tail_end != ~0ull) end[nrec - 1] = tail_end;
     uint32_t w[4];                                              // this block's bytes, loaded one block ahead
     if (blockIdx.x < nblk) load16(tv, (uint64_t)blockIdx.x * IN_BLK + 16 * threadIdx.x, w);
     for (uint64_t b = blockIdx.x; b < nblk; b += gridDim.x) {
@@ -163,21 +165,16 @@ __global__ __launch_bounds__(IN_NT) void k_lines(TextView tv, uint64_t nblk, con
         uint64_t tot;
         const uint64_t before = block_excl_scan((uint64_t)__popc(m), s_w, &tot);
         uint64_t g = inc_b - tot + before;
+        const uint64_t p0 = a - tv.lead;                                     // text position of byte a
         bool wrong = false;
         for (; m; m &= m - 1, g++) {
             const int i = __builtin_ctz(m);
-            const uint64_t p = a + (uint32_t)i - tv.lead;                      // text position of the newline
-            const uint32_t j = (uint32_t)g & lm;
+            const uint64_t p = p0 + (uint32_t)i;                               // the newline's text position
+            const uint32_t j = (uint32_t)g & lm, after = byte_at(i + 1);
             const uint64_t r = g >> lg;
-            if (j == 0) {
-                src[r] = p + 1;
-                if (r == nrec - 1 && tail_end != ~0ull) end[r] = tail_end;
-            }
-            if (j == 1) {
-                end[r] = byte_at(i - 1) == '\r' ? p - 1 : p;                    // p >= 1: a header line precedes
-                if (fastq) wrong |= byte_at(i + 1) != '+';
-            }
-            if (j == lm && r + 1 < nrec) wrong |= byte_at(i + 1) != marker;
+            if (j <= 1)                  // j == 1: p >= 1, a header line precedes
+                (j == 0 ? src : end)[r] = j == 0 ? p + 1 : p - (byte_at(i - 1) == '\r' ? 1 : 0);
+            wrong |= (fastq && j == 1 && after != '+') || (j == lm && r + 1 < nrec && after != marker);
         }
         if (wrong) atomicOr(bad, 1u);
 #pragma unroll
