@@ -110,6 +110,43 @@ def ingest_report(rs, max_reads, with_cpu, np, torch, capi, W) -> dict:
     return rep
 
 
+def table_file_report(ctx, directory, np) -> dict:
+    """gkc_table_write of the step's table (the last count) to a file in
+    `directory`, deleted afterwards: wall clock of the call (device-to-host
+    copies through pinned staging overlapping the file writes), bytes, and the
+    file read back: header and partition offsets equal to the device table's,
+    the first partition's entries equal to gkc_table_copy's."""
+    import shutil
+    import tempfile
+    from gkc_amd import table_file as TF
+    po, pd, pk = ctx.table_info()
+    kw = 1 if ctx.k <= 32 else 2
+    need = 64 + 8 * (3 * len(pd) + 1) + (8 * kw + 4) * int(po[-1])
+    free = shutil.disk_usage(directory).free
+    if free < 2 * need:
+        return {"skipped": f"{directory}: {free} bytes free, the file needs {need}"}
+    t0 = time.perf_counter()
+    ctx.write_table(os.devnull)                 # the same call with no file system behind it
+    dt_null = time.perf_counter() - t0
+    fd, path = tempfile.mkstemp(suffix=".gkct", dir=directory)
+    os.close(fd)
+    try:
+        t0 = time.perf_counter()
+        n = ctx.write_table(path)
+        dt = time.perf_counter() - t0
+        t = TF.read_table(path)
+        lo, hi, cnt, _, _, _ = ctx.table(0)
+        m = len(cnt)
+        ok = (n == need and bool(np.array_equal(t.part_off, po)) and bool(np.array_equal(t.part_kmers, pk))
+              and bool(np.array_equal(t.keys[:m, 0], lo)) and bool(np.array_equal(t.counts[:m], cnt))
+              and (hi is None or bool(np.array_equal(t.keys[:m, 1], hi))))
+        del t
+        return {"dir": directory, "bytes": n, "entries": int(po[-1]), "ms": dt * 1e3, "GB_per_s": n / dt / 1e9,
+                "to_devnull_ms": dt_null * 1e3, "to_devnull_GB_per_s": n / dt_null / 1e9, "read_back_equal": ok}
+    finally:
+        os.unlink(path)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +160,8 @@ def main() -> int:
     ap.add_argument("--tune", action="store_true", help="run the autotuner on this workload and cache the result")
     ap.add_argument("--no-ingest", action="store_true", help="skip the on-device FASTQ ingest measurement")
     ap.add_argument("--ingest-reads", type=int, default=2_000_000, help="reads in the ingest sample")
+    ap.add_argument("--no-table-file", action="store_true", help="skip the table file write measurement")
+    ap.add_argument("--table-dir", default=None, help="directory of the table file measurement (default: temp dir)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with a 1 GiB device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -297,6 +336,9 @@ def main() -> int:
                                 "sample": f"first {nr} reads of the same workload ({n_sub} k-mers), "
                                           f"oracle phase1 {ot.times[0]:.2f}s + phase2 {ot.times[1]:.2f}s"}
         line["bit_exact_sample"] = exact
+    if not args.no_table_file:
+        import tempfile
+        line["table_file"] = table_file_report(ctx, args.table_dir or tempfile.gettempdir(), np)
     ctx.close()
     if not args.no_ingest:
         line["ingest"] = ingest_report(rs, args.ingest_reads, not args.no_cpu, np, torch, capi, W)

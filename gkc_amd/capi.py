@@ -12,6 +12,7 @@ exits, gkc_cuda.hpp:4-9).
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -69,6 +70,7 @@ SIGNATURES = {
     "gkc_count": (C.c_int, [C.c_void_p, C.c_uint32]),
     "gkc_table_info": (C.c_int, [C.c_void_p, _u64p, _u64p, _u64p]),
     "gkc_table_copy": (C.c_int64, [C.c_void_p, C.c_int32, _u64p, _u64p, _u32p, C.c_uint64]),
+    "gkc_table_write": (C.c_int64, [C.c_void_p, C.c_char_p]),
     "gkc_table_starts": (C.c_int, [C.c_void_p, _u64p]),
     "gkc_table_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(_u32p), _u64p]),
     "gkc_count_packed": (C.c_int, [C.c_void_p, _u32p, C.c_uint64, _u32p, C.c_uint64, _u64p, _u64p, _u32p, C.c_uint64,
@@ -186,6 +188,11 @@ class Counter:
         got = _check(lib().gkc_table_copy(self.h, part, _p(lo, _u64p), _p(hi, _u64p) if hi is not None else None,
                                           _p(cnt, _u32p), max(n, 1)))
         return lo[:got], (None if hi is None else hi[:got]), cnt[:got], po, pd, pk
+
+    def write_table(self, path) -> int:
+        """The table to a file (gkc_table_write); returns the bytes written.
+        Read it back with gkc_amd.table_file.read_table."""
+        return _check(lib().gkc_table_write(self.h, os.fsencode(path)))
 
     def stats(self) -> dict:
         s = Stats()

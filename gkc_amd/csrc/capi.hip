@@ -25,6 +25,9 @@
 #include <string>
 #include <vector>
 #include <cmath>
+#include <cerrno>
+#include <fcntl.h>
+#include <unistd.h>
 
 #include "../../include/gkc_hip.h"
 #include "kernels.hpp"
@@ -718,6 +721,104 @@ int64_t gkc_table_copy(gkc_ctx* c, int32_t part, uint64_t* key_lo, uint64_t* key
         if (c->keyw == 16)
             for (uint64_t i = 0; i < n; i++) { key_lo[i] = tmp[i].lo; if (key_hi) key_hi[i] = tmp[i].hi; }
         ret = (int64_t)n;
+        return 0;
+    });
+    return rc < 0 ? rc : ret;
+}
+
+namespace {
+struct PinnedBuf {
+    void* p = nullptr;
+    ~PinnedBuf() { if (p) (void)hipHostFree(p); }
+};
+struct FileFd {
+    int fd = -1;
+    ~FileFd() { if (fd >= 0) ::close(fd); }
+};
+struct Events2 {
+    hipEvent_t e[2]{};
+    ~Events2() { for (auto& x : e) if (x) (void)hipEventDestroy(x); }
+};
+void pwrite_all(int fd, const void* b, size_t n, uint64_t off, const char* path) {
+    const char* q = static_cast<const char*>(b);
+    while (n) {
+        const ssize_t w = ::pwrite(fd, q, n, (off_t)off);
+        if (w < 0 && errno == EINTR) continue;
+        if (w <= 0) throw ArgError{std::string("writing ") + path + " failed: " + std::strerror(w < 0 ? errno : EIO), GKC_EIO};
+        q += w;
+        n -= (size_t)w;
+        off += (uint64_t)w;
+    }
+}
+constexpr size_t TF_STAGE = 64u << 20;   // bytes per pinned staging buffer
+}  // namespace
+
+int64_t gkc_table_write(gkc_ctx* c, const char* path) {
+    int64_t ret = 0;
+    int rc = guarded(c, [&] {
+        if (!c->counted) throw ArgError{"gkc_count has not run", GKC_ESTATE};
+        if (!path) throw ArgError{"null path", GKC_EINVAL};
+        const uint32_t P = c->prm.n_parts, kw = (uint32_t)c->keyw / 8;
+        const uint64_t N = c->part_off[P];
+        uint64_t nk = 0;
+        for (uint32_t q = 0; q < P; q++) nk += c->part_kmers[q];
+        unsigned char hdr[64] = {};
+        std::memcpy(hdr, "GKCTAB1", 8);
+        const uint32_t h32[9] = {1u, (uint32_t)c->prm.k, (uint32_t)c->prm.p, (uint32_t)c->prm.hpc, P, kw,
+                                 c->prm.min_count, c->prm.max_count, 0u};
+        std::memcpy(hdr + 8, h32, 32);
+        std::memcpy(hdr + 40, &N, 8);
+        std::memcpy(hdr + 48, &nk, 8);
+        const uint64_t o_info = 64, o_keys = o_info + 8 * (3 * (uint64_t)P + 1), o_cnts = o_keys + N * 8 * kw;
+        FileFd f;
+        f.fd = ::open(path, O_WRONLY | O_CREAT | O_TRUNC, 0644);
+        if (f.fd < 0) throw ArgError{std::string("cannot create ") + path + ": " + std::strerror(errno), GKC_EIO};
+        pwrite_all(f.fd, hdr, 64, 0, path);
+        pwrite_all(f.fd, c->part_off.data(), 8 * ((size_t)P + 1), o_info, path);
+        pwrite_all(f.fd, c->part_distinct.data(), 8 * (size_t)P, o_info + 8 * ((uint64_t)P + 1), path);
+        pwrite_all(f.fd, c->part_kmers.data(), 8 * (size_t)P, o_info + 8 * (2 * (uint64_t)P + 1), path);
+        // chunks of the physically contiguous partition runs, at most `per` entries each
+        struct Chunk { uint64_t src, dst, m; };
+        std::vector<Chunk> ch;
+        const uint64_t per = TF_STAGE / (8 * kw + 4);
+        for (uint32_t q = 0; q < P;) {
+            uint32_t e = q + 1;
+            while (e < P && c->part_start[e] == c->part_start[q] + (c->part_off[e] - c->part_off[q])) e++;
+            const uint64_t m = c->part_off[e] - c->part_off[q];
+            for (uint64_t i = 0; i < m; i += per) ch.push_back({c->part_start[q] + i, c->part_off[q] + i, std::min(per, m - i)});
+            q = e;
+        }
+        PinnedBuf stage[2];
+        Events2 ev;
+        struct Drain {                                 // no copy into the staging buffers outlives them
+            hipStream_t s;
+            ~Drain() { (void)hipStreamSynchronize(s); }
+        } drain{c->st};
+        for (int b = 0; b < 2 && b < (int)ch.size(); b++) {
+            HIPX(hipHostMalloc(&stage[b].p, TF_STAGE, hipHostMallocDefault));
+            HIPX(hipEventCreateWithFlags(&ev.e[b], hipEventDisableTiming));
+        }
+        const char* tk = c->tkeys.as<char>();
+        const uint32_t* tc = c->tcnts.as<uint32_t>();
+        auto issue = [&](size_t j) {
+            const int b = (int)(j & 1);
+            char* h = static_cast<char*>(stage[b].p);
+            HIPX(hipMemcpyAsync(h, tk + ch[j].src * 8 * kw, ch[j].m * 8 * kw, hipMemcpyDeviceToHost, c->st));
+            HIPX(hipMemcpyAsync(h + ch[j].m * 8 * kw, tc + ch[j].src, ch[j].m * 4, hipMemcpyDeviceToHost, c->st));
+            HIPX(hipEventRecord(ev.e[b], c->st));
+        };
+        if (!ch.empty()) issue(0);
+        for (size_t j = 0; j < ch.size(); j++) {
+            if (j + 1 < ch.size()) issue(j + 1);       // its buffer's previous chunk (j - 1) is written
+            const int b = (int)(j & 1);
+            HIPX(hipEventSynchronize(ev.e[b]));
+            const char* h = static_cast<const char*>(stage[b].p);
+            pwrite_all(f.fd, h, ch[j].m * 8 * kw, o_keys + ch[j].dst * 8 * kw, path);
+            pwrite_all(f.fd, h + ch[j].m * 8 * kw, ch[j].m * 4, o_cnts + ch[j].dst * 4, path);
+        }
+        if (::close(f.fd) != 0) { f.fd = -1; throw ArgError{std::string("closing ") + path + " failed: " + std::strerror(errno), GKC_EIO}; }
+        f.fd = -1;
+        ret = (int64_t)(o_cnts + 4 * N);
         return 0;
     });
     return rc < 0 ? rc : ret;

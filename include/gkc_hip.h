@@ -27,7 +27,8 @@ enum {
     GKC_EHIP = -2,       /* HIP runtime error */
     GKC_ENOMEM = -3,     /* device allocation failed */
     GKC_ESTATE = -4,     /* call out of sequence */
-    GKC_ECAP = -5        /* output buffer too small */
+    GKC_ECAP = -5,       /* output buffer too small */
+    GKC_EIO = -6         /* file could not be created or written */
 };
 
 /* Flags of gkc_partition / gkc_count. */
@@ -160,6 +161,20 @@ int64_t gkc_table_copy(gkc_ctx *ctx, int32_t part, uint64_t *key_lo, uint64_t *k
 /* Physical start of each partition's entries in the device table (a presized
  * count leaves unused capacity between partitions; part_off is logical). */
 int gkc_table_starts(gkc_ctx *ctx, uint64_t *starts);
+/* ---- table file (no counterpart in the reference: kmc_counting_GPU computes
+ * its table and drops it, kmer_counting.cu:187-190) ----
+ * The last gkc_count's table to a new file at `path` (truncated if it exists),
+ * streamed from the device through two pinned staging buffers: the copy of
+ * one chunk runs while the previous one is written.  Little-endian layout:
+ *   64-byte header: char magic[8] = "GKCTAB1\0"; u32 version = 1, k, p, hpc,
+ *     n_parts, key_words (1: k <= 32, 2: k > 32), min_count, max_count;
+ *     u64 n_entries, n_kmers (k-mer occurrences counted), reserved = 0
+ *   u64 part_off[n_parts + 1], part_distinct[n_parts], part_kmers[n_parts]
+ *   u64 keys[n_entries][key_words] (key_words 2: low then high 64 bits),
+ *     partition by partition, ascending within each
+ *   u32 counts[n_entries]
+ * Returns the bytes written, or a negative GKC_E* code (GKC_EIO: the file). */
+int64_t gkc_table_write(gkc_ctx *ctx, const char *path);
 /* Device pointers of the table (valid until the next gkc_count/gkc_reset). */
 int gkc_table_device(gkc_ctx *ctx, void **keys, uint32_t **counts, uint64_t *n_entries);
 
