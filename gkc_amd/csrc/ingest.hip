@@ -138,24 +138,27 @@ __global__ __launch_bounds__(IN_NT) void k_lines(TextView tv, uint64_t nblk, con
                                                  uint64_t* __restrict__ src, uint64_t* __restrict__ end,
                                                  uint32_t* __restrict__ bad) {
     __shared__ uint64_t s_w[IN_NT / GKC_WAVE];
+    __shared__ uint32_t s_nl[IN_BLK];                           // at most one newline per byte
     const uint32_t lg = fastq ? 2 : 1, lm = (1u << lg) - 1;
     const uint8_t marker = fastq ? '@' : '>';
-    const uint32_t lane = __lane_id();
     if (blockIdx.x == 0 && threadIdx.x == 0 && tail_end != ~0ull) end[nrec - 1] = tail_end;
-    uint32_t w[4];                                              // this block's bytes, loaded one block ahead
-    if (blockIdx.x < nblk) load16(tv, (uint64_t)blockIdx.x * IN_BLK + 16 * threadIdx.x, w);
+    // a thread's 16 bytes and the bytes just before and after them (from
+    // addresses clamped into the text: they are used only where they are
+    // text), loaded one block ahead, so that nothing loaded is waited for
+    // before the end of the iteration
+    const uint64_t lo = tv.lead, hi = tv.lead + tv.n - 1;
+    auto load_chunk = [&](uint64_t a, uint32_t w[4], uint32_t& pb, uint32_t& nb) {
+        load16(tv, a, w);
+        pb = tv.base[std::min(std::max(a, lo + 1) - 1, hi)];
+        nb = tv.base[std::min(std::max(a + 16, lo), hi)];
+    };
+    uint32_t w[4] = {0u, 0u, 0u, 0u}, prev = 0, next = 0;
+    if (blockIdx.x < nblk) load_chunk((uint64_t)blockIdx.x * IN_BLK + 16 * threadIdx.x, w, prev, next);
     for (uint64_t b = blockIdx.x; b < nblk; b += gridDim.x) {
         const uint64_t a = b * IN_BLK + 16 * threadIdx.x;
         const uint64_t inc_b = incl[b];
-        // the bytes just before and after this thread's 16: the neighbouring
-        // lanes' words, or memory at the wave's edges (loaded here, where they
-        // are text, so that the newline loop below holds no load; before the
-        // next block's bytes, so that waiting for them leaves those in flight)
-        uint32_t prev = __shfl_up(w[3], 1) >> 24, next = __shfl_down(w[0], 1) & 0xFFu;
-        if (lane == 0 && a > tv.lead && a <= tv.lead + tv.n) prev = tv.base[a - 1];
-        if (lane == GKC_WAVE - 1 && a + 16 < tv.lead + tv.n) next = tv.base[a + 16];
-        uint32_t nw[4] = {0u, 0u, 0u, 0u};
-        if (b + gridDim.x < nblk) load16(tv, (b + gridDim.x) * IN_BLK + 16 * threadIdx.x, nw);
+        uint32_t nw[4] = {0u, 0u, 0u, 0u}, nprev = 0, nnext = 0;
+        if (b + gridDim.x < nblk) load_chunk((b + gridDim.x) * IN_BLK + 16 * threadIdx.x, nw, nprev, nnext);
         uint32_t m = nl_bits(tv, a, w);
         auto byte_at = [&](int i) -> uint32_t {                 // view byte a + i, -1 <= i <= 16
             if (i < 0) return prev;
@@ -163,22 +166,31 @@ __global__ __launch_bounds__(IN_NT) void k_lines(TextView tv, uint64_t nblk, con
             return (w[i >> 2] >> (8 * (i & 3))) & 0xFFu;
         };
         uint64_t tot;
-        const uint64_t before = block_excl_scan((uint64_t)__popc(m), s_w, &tot);
-        uint64_t g = inc_b - tot + before;
-        const uint64_t p0 = a - tv.lead;                                     // text position of byte a
-        bool wrong = false;
-        for (; m; m &= m - 1, g++) {
+        uint32_t o = (uint32_t)block_excl_scan((uint64_t)__popc(m), s_w, &tot);
+        // the block's newlines in order in LDS: offset in the block (12 bits),
+        // the bytes before (bits 12-19) and after (20-27) it
+        for (; m; m &= m - 1) {
             const int i = __builtin_ctz(m);
-            const uint64_t p = p0 + (uint32_t)i;                               // the newline's text position
-            const uint32_t j = (uint32_t)g & lm, after = byte_at(i + 1);
+            s_nl[o++] = (16 * threadIdx.x + (uint32_t)i) | byte_at(i - 1) << 12 | byte_at(i + 1) << 20;
+        }
+        __syncthreads();
+        // then placed one per thread, newline rank g = the block's first + t
+        const uint64_t g0 = inc_b - tot, base = b * IN_BLK - tv.lead;
+        bool wrong = false;
+        for (uint32_t t = threadIdx.x; t < (uint32_t)tot; t += IN_NT) {
+            const uint32_t e = s_nl[t], after = e >> 20, before = (e >> 12) & 0xFFu;
+            const uint64_t g = g0 + t, p = base + (e & 0xFFFu);                 // the newline's text position
+            const uint32_t j = (uint32_t)g & lm;
             const uint64_t r = g >> lg;
             if (j <= 1)                  // j == 1: p >= 1, a header line precedes
-                (j == 0 ? src : end)[r] = j == 0 ? p + 1 : p - (byte_at(i - 1) == '\r' ? 1 : 0);
+                (j == 0 ? src : end)[r] = j == 0 ? p + 1 : p - (before == '\r' ? 1 : 0);
             wrong |= (fastq && j == 1 && after != '+') || (j == lm && r + 1 < nrec && after != marker);
         }
         if (wrong) atomicOr(bad, 1u);
 #pragma unroll
         for (int k = 0; k < 4; k++) w[k] = nw[k];
+        prev = nprev;
+        next = nnext;
         __syncthreads();
     }
 }
