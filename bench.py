@@ -167,8 +167,8 @@ def main() -> int:
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
-                                   "(partition ownership + one all-to-all of superkmer bins) but not run",
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding (partition ownership + one "
+                                   "all-to-all of superkmer bins) is designed but not run",
                           "n_gpus": args.gpus}))
         return 2
 
