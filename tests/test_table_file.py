@@ -109,3 +109,28 @@ def test_device_file_errors(gpu, tmp_path):
         with pytest.raises(capi.GkcError, match="cannot create"):
             c.write_table(tmp_path / "no_such_dir" / "x.gkct")
         assert c.write_table(tmp_path / "x.gkct") > 0    # the context still works
+
+
+@pytest.mark.gpu
+def test_device_file_many_chunks(gpu, tmp_path):
+    """A presized table of tens of millions of entries: several 64 MiB
+    staging chunks (both buffers used in turn, runs split across chunks)."""
+    from gkc_amd import capi
+    w = W.workload(2, scale=0.15)
+    rs = W.generate(w)
+    print("[gpu-case] table file, many chunks", rs.n, flush=True)
+    path = tmp_path / "big.gkct"
+    with capi.Counter(w.k, w.p, w.n_parts) as c:
+        c.partition(rs.reads, rs.offs, flags=capi.BATCHED)
+        c.count(capi.BATCHED | capi.PRESIZED)
+        nbytes = c.write_table(path)
+        lo, hi, cnt, po, pd, pk = c.table()
+    assert len(cnt) * 12 > 3 * (64 << 20)
+    twin = tmp_path / "host.gkct"
+    assert TF.write_table(twin, w.k, w.p, 0, 1, 0, po, pd, pk, lo, hi, cnt) == nbytes
+    with open(path, "rb") as a, open(twin, "rb") as b:
+        while True:
+            x, y = a.read(1 << 24), b.read(1 << 24)
+            assert x == y
+            if not x:
+                break
