@@ -17,6 +17,7 @@
 // (sequence start and end, header / separator checks); the sequence lengths
 // scanned into offsets; the sequence bytes gathered by output chunks.
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "gkc_hip.h"
@@ -387,10 +388,15 @@ __global__ __launch_bounds__(GA_NT) void k_gather(TextView tv, const uint64_t* _
 }
 
 namespace {
-struct Scratch {
-    void* p = nullptr;
-    ~Scratch() { if (p) (void)hipFree(p); }
+// Scratch kept per device between calls (grown when an input needs more;
+// held until the process ends); a call holds its device's lock throughout.
+struct DevScratch {
+    std::mutex mu;
+    void* p[2] = {nullptr, nullptr};
+    size_t bytes[2] = {0, 0};
 };
+constexpr int MAX_DEV = 64;
+DevScratch g_scratch[MAX_DEV];
 int hip_fail(hipError_t e, const char* where, std::string& err) {
     (void)hipGetLastError();
     err = std::string("HIP error ") + hipGetErrorString(e) + " in " + where;
@@ -431,9 +437,30 @@ int parse_reads(const char* text, uint64_t n, int format, char* bases, uint64_t*
     const uint32_t grid = (uint32_t)std::min<uint64_t>(nblk, 65536);
 
     // blk[nblk] | misc[2] | group sums
-    Scratch sa;
-    IN_HIP(hipMalloc(&sa.p, (nblk + 2 + scan_groups(nblk)) * 8));
-    uint64_t* blk = static_cast<uint64_t*>(sa.p);
+    int dev = 0;
+    IN_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEV) { err = "device id out of range"; return GKC_EINVAL; }
+    DevScratch& ds = g_scratch[dev];
+    std::lock_guard<std::mutex> lk(ds.mu);
+    struct Drain {                               // no work on the scratch outlives the lock
+        hipStream_t s;
+        ~Drain() { (void)hipStreamSynchronize(s); }
+    } drain{st};
+    auto grow = [&](int i, size_t need) -> hipError_t {
+        if (need <= ds.bytes[i]) return hipSuccess;
+        if (ds.p[i]) {
+            const hipError_t e = hipFree(ds.p[i]);
+            ds.p[i] = nullptr;
+            ds.bytes[i] = 0;
+            if (e != hipSuccess) return e;
+        }
+        const hipError_t e = hipMalloc(&ds.p[i], need);
+        if (e == hipSuccess) ds.bytes[i] = need;
+        else ds.p[i] = nullptr;
+        return e;
+    };
+    IN_HIP(grow(0, (nblk + 2 + scan_groups(nblk)) * 8));
+    uint64_t* blk = static_cast<uint64_t*>(ds.p[0]);
     uint64_t* misc = blk + nblk;
     hipLaunchKernelGGL(k_nl_count, dim3(grid), dim3(IN_NT), 0, st, tv, nblk, blk, misc);
     scan_inplace(blk, nullptr, nblk, misc + 2, st);
@@ -458,9 +485,8 @@ int parse_reads(const char* text, uint64_t n, int format, char* bases, uint64_t*
     const uint64_t tail_end = (!fastq && last != '\n') ? n - (last == '\r' ? 1 : 0) : ~0ull;
 
     // src[nrec] | group sums
-    Scratch sb;
-    IN_HIP(hipMalloc(&sb.p, (nrec + scan_groups(nrec)) * 8));
-    uint64_t* src = static_cast<uint64_t*>(sb.p);
+    IN_HIP(grow(1, (nrec + scan_groups(nrec)) * 8));
+    uint64_t* src = static_cast<uint64_t*>(ds.p[1]);
     uint32_t* bad = reinterpret_cast<uint32_t*>(misc + 1);
     IN_HIP(hipMemsetAsync(offs, 0, 8, st));
     hipLaunchKernelGGL(k_lines, dim3(grid), dim3(IN_NT), 0, st, tv, nblk, (const uint64_t*)blk, fastq, nrec, tail_end,

@@ -120,7 +120,9 @@ int gkc_superkmer_arrays(gkc_ctx *ctx, const char *reads, const uint64_t *offs, 
  * is the sequence line without a trailing '\r'; the last line may lack its
  * newline.  format: 0 from the first byte, 1 FASTA, 2 FASTQ.  Capacities:
  * bases n_bytes bytes, offs n_bytes / 2 + 2 entries.  Runs on `stream` (a
- * hipStream_t, NULL for the default stream) and returns when done: the
+ * hipStream_t, NULL for the default stream); the library keeps its scratch
+ * for the device between calls (grown as inputs need, held until the process
+ * ends) and serialises calls on one device.  Returns when done: the
  * number of reads, or a negative GKC_E* code (GKC_EINVAL for text that is not
  * whole records of the format). */
 int64_t gkc_parse_reads(const char *text, uint64_t n_bytes, int format, char *bases, uint64_t *offs, void *stream);
