@@ -1157,7 +1157,7 @@ static void count_impl(gkc_ctx* c, uint32_t flags) {
         tot_hdrs += c->h_hdr_cur[q];
         tot_words += c->h_word_cur[q];
     }
-    c->stats = gkc_stats{};
+    c->stats = gkc_stats{c->stats.ms_partition};   // the count's figures; the last partition's time is kept
     c->stats.n_kmers = tot_keys; c->stats.n_superkmers = tot_hdrs; c->stats.n_words = tot_words;
     c->stats.n_pass = c->npass;
     HIPX(hipEventRecord(c->ev[0], c->st));
@@ -1436,7 +1436,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
         c->part_kmers[q] = c->h_kmer_cur[q];
         tot_keys += c->h_kmer_cur[q]; tot_hdrs += c->h_hdr_cur[q]; tot_words += c->h_word_cur[q];
     }
-    c->stats = gkc_stats{};
+    c->stats = gkc_stats{c->stats.ms_partition};   // the count's figures; the last partition's time is kept
     c->stats.n_kmers = tot_keys; c->stats.n_superkmers = tot_hdrs; c->stats.n_words = tot_words;
     c->stats.n_pass = bs.npass;
     c->stats.binned = bs.binned ? 1 : 0;
