@@ -97,6 +97,7 @@ SIGNATURES = {
     "gkc_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "gkc_part_kmers": (C.c_int, [C.c_void_p, _u64p]),
     "gkc_parse_reads": (C.c_int64, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gkc_parse_reads_noalloc": (C.c_int64, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
 _bound = None
@@ -280,6 +281,19 @@ def parse_reads(text_ptr: int, n_bytes: int, bases_ptr: int, offs_ptr: int, fmt:
     returns the number of reads.  fmt: 0 from the first byte, 1 FASTA, 2 FASTQ."""
     return _check(lib().gkc_parse_reads(C.c_void_p(text_ptr), n_bytes, fmt, C.c_void_p(bases_ptr),
                                         C.c_void_p(offs_ptr), C.c_void_p(stream)))
+
+
+# bytes of gkc_parse_reads_noalloc's fixed device scratch (GKC_PARSE_NOALLOC_SCRATCH_BYTES)
+PARSE_NOALLOC_SCRATCH_BYTES = 32 << 20
+
+
+def parse_reads_noalloc(text_ptr: int, n_bytes: int, bases_ptr: int, offs_ptr: int, fmt: int = 0,
+                        stream: int = 0) -> int:
+    """parse_reads with no device allocation: the library's fixed per-GPU
+    scratch array instead of allocated scratch; raises GkcError (GKC_ECAP)
+    for a text too large for it."""
+    return _check(lib().gkc_parse_reads_noalloc(C.c_void_p(text_ptr), n_bytes, fmt, C.c_void_p(bases_ptr),
+                                                C.c_void_p(offs_ptr), C.c_void_p(stream)))
 
 
 def pack_superkmers(seqs: list[bytes]) -> tuple[np.ndarray, np.ndarray]:

@@ -50,6 +50,8 @@ namespace gkc {
 // ingest.hip: text -> read CSR on the device (0, or a GKC_E* code with err set)
 int parse_reads(const char* text, uint64_t n, int format, char* bases, uint64_t* offs, hipStream_t st,
                 int64_t* n_reads, std::string& err);
+int parse_reads_noalloc(const char* text, uint64_t n, int format, char* bases, uint64_t* offs, hipStream_t st,
+                int64_t* n_reads, std::string& err);
 }
 
 namespace {
@@ -373,6 +375,15 @@ int64_t gkc_parse_reads(const char* text, uint64_t n_bytes, int format, char* ba
     int64_t n = 0;
     std::string err;
     const int rc = gkc::parse_reads(text, n_bytes, format, bases, offs, static_cast<hipStream_t>(stream), &n, err);
+    if (rc) { g_err = err; return rc; }
+    return n;
+}
+
+int64_t gkc_parse_reads_noalloc(const char* text, uint64_t n_bytes, int format, char* bases, uint64_t* offs, void* stream) {
+    if (!offs || (n_bytes && (!text || !bases))) { g_err = "null buffer"; return GKC_EINVAL; }
+    int64_t n = 0;
+    std::string err;
+    const int rc = gkc::parse_reads_noalloc(text, n_bytes, format, bases, offs, static_cast<hipStream_t>(stream), &n, err);
     if (rc) { g_err = err; return rc; }
     return n;
 }

@@ -26,6 +26,8 @@
 namespace gkc {
 
 constexpr int IN_NT = 256;                     // threads per block
+// u64 words of gkc_parse_reads_noalloc's fixed scratch (GKC_PARSE_NOALLOC_SCRATCH_BYTES)
+constexpr uint64_t NOALLOC_SCRATCH_WORDS = GKC_PARSE_NOALLOC_SCRATCH_BYTES / 8;
 constexpr uint64_t IN_BLK = IN_NT * 16;        // bytes of the aligned view per block (16 per thread)
 
 // The text viewed from the 16-byte boundary at or below it: text byte q is
@@ -388,6 +390,7 @@ __global__ __launch_bounds__(GA_NT) void k_gather(TextView tv, const uint64_t* _
 }
 
 namespace {
+constexpr int MAX_DEV = 64;
 // Scratch kept per device between calls (grown when an input needs more;
 // held until the process ends); a call holds its device's lock throughout.
 struct DevScratch {
@@ -395,8 +398,19 @@ struct DevScratch {
     void* p[2] = {nullptr, nullptr};
     size_t bytes[2] = {0, 0};
 };
-constexpr int MAX_DEV = 64;
 DevScratch g_scratch[MAX_DEV];
+
+// The fixed scratch of gkc_parse_reads_noalloc: one array per GPU in the
+// library's device image.  Host code never passes the symbol itself (a
+// host-side stand-in); it takes the device address from hipGetSymbolAddress,
+// once per device, and passes that.
+__device__ __align__(256) uint64_t scratch[NOALLOC_SCRATCH_WORDS];
+struct FixedScratch {
+    std::mutex mu;
+    uint64_t* ptr = nullptr;       // device address of `scratch` on this device
+};
+FixedScratch g_fixed[MAX_DEV];
+
 int hip_fail(hipError_t e, const char* where, std::string& err) {
     (void)hipGetLastError();
     err = std::string("HIP error ") + hipGetErrorString(e) + " in " + where;
@@ -415,20 +429,21 @@ void scan_inplace(uint64_t* x, const uint64_t* sub, uint64_t n, uint64_t* gs, hi
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SC_NT), 0, st, gs, ng);
     hipLaunchKernelGGL(k_scan_add, dim3((uint32_t)ng), dim3(SC_NT), 0, st, x, n, (const uint64_t*)gs);
 }
-}  // namespace
 
-// Two device round trips: after the newline count (the host needs the record
-// count and the text's first and last bytes) and at the end (the malformed
-// flag).
-int parse_reads(const char* text, uint64_t n, int format, char* bases, uint64_t* offs, hipStream_t st,
-                int64_t* n_reads, std::string& err) {
-    *n_reads = 0;
-    if (format < 0 || format > 2) { err = "format must be 0 (from the first byte), 1 (FASTA) or 2 (FASTQ)"; return GKC_EINVAL; }
-    if (n == 0) {
-        IN_HIP(hipMemsetAsync(offs, 0, 8, st));
-        IN_HIP(hipStreamSynchronize(st));
-        return 0;
-    }
+// Words of the two scratch regions: block counts (+ first/last bytes, the
+// malformed flag and the scan's group sums), then record starts (+ group sums).
+uint64_t words_blocks(uint64_t nblk) { return nblk + 2 + scan_groups(nblk); }
+uint64_t words_records(uint64_t nrec) { return nrec + scan_groups(nrec); }
+
+// The parse proper.  region(i, words, &p) hands out scratch region i (0:
+// block counts, 1: record starts) of at least `words` u64 words, or fails
+// with a GKC_E* code and err set.  Two device round trips: after the newline
+// count (the host needs the record count and the text's first and last
+// bytes) and at the end (the malformed flag).  The kernels write every
+// scratch word they read, except the malformed flag, which k_nl_count clears.
+template <typename Region>
+int parse_core(const char* text, uint64_t n, int format, char* bases, uint64_t* offs, hipStream_t st,
+               int64_t* n_reads, std::string& err, Region&& region) {
     TextView tv;
     tv.lead = reinterpret_cast<uintptr_t>(text) & 15u;
     tv.base = reinterpret_cast<const uint8_t*>(text) - tv.lead;
@@ -437,30 +452,8 @@ int parse_reads(const char* text, uint64_t n, int format, char* bases, uint64_t*
     const uint32_t grid = (uint32_t)std::min<uint64_t>(nblk, 65536);
 
     // blk[nblk] | misc[2] | group sums
-    int dev = 0;
-    IN_HIP(hipGetDevice(&dev));
-    if (dev < 0 || dev >= MAX_DEV) { err = "device id out of range"; return GKC_EINVAL; }
-    DevScratch& ds = g_scratch[dev];
-    std::lock_guard<std::mutex> lk(ds.mu);
-    struct Drain {                               // no work on the scratch outlives the lock
-        hipStream_t s;
-        ~Drain() { (void)hipStreamSynchronize(s); }
-    } drain{st};
-    auto grow = [&](int i, size_t need) -> hipError_t {
-        if (need <= ds.bytes[i]) return hipSuccess;
-        if (ds.p[i]) {
-            const hipError_t e = hipFree(ds.p[i]);
-            ds.p[i] = nullptr;
-            ds.bytes[i] = 0;
-            if (e != hipSuccess) return e;
-        }
-        const hipError_t e = hipMalloc(&ds.p[i], need);
-        if (e == hipSuccess) ds.bytes[i] = need;
-        else ds.p[i] = nullptr;
-        return e;
-    };
-    IN_HIP(grow(0, (nblk + 2 + scan_groups(nblk)) * 8));
-    uint64_t* blk = static_cast<uint64_t*>(ds.p[0]);
+    uint64_t* blk = nullptr;
+    if (int rc = region(0, words_blocks(nblk), &blk)) return rc;
     uint64_t* misc = blk + nblk;
     hipLaunchKernelGGL(k_nl_count, dim3(grid), dim3(IN_NT), 0, st, tv, nblk, blk, misc);
     scan_inplace(blk, nullptr, nblk, misc + 2, st);
@@ -485,8 +478,8 @@ int parse_reads(const char* text, uint64_t n, int format, char* bases, uint64_t*
     const uint64_t tail_end = (!fastq && last != '\n') ? n - (last == '\r' ? 1 : 0) : ~0ull;
 
     // src[nrec] | group sums
-    IN_HIP(grow(1, (nrec + scan_groups(nrec)) * 8));
-    uint64_t* src = static_cast<uint64_t*>(ds.p[1]);
+    uint64_t* src = nullptr;
+    if (int rc = region(1, words_records(nrec), &src)) return rc;
     uint32_t* bad = reinterpret_cast<uint32_t*>(misc + 1);
     IN_HIP(hipMemsetAsync(offs, 0, 8, st));
     hipLaunchKernelGGL(k_lines, dim3(grid), dim3(IN_NT), 0, st, tv, nblk, (const uint64_t*)blk, fastq, nrec, tail_end,
@@ -506,6 +499,92 @@ int parse_reads(const char* text, uint64_t n, int format, char* bases, uint64_t*
     if (hbad) { err = fastq ? "malformed FASTQ record (header without '@' or separator without '+')" : "malformed FASTA record (header without '>')"; return GKC_EINVAL; }
     *n_reads = (int64_t)nrec;
     return 0;
+}
+
+// Checks shared by both entry points; 1 when the text is empty (done).
+int parse_precheck(uint64_t n, int format, uint64_t* offs, hipStream_t st, std::string& err, int* rc) {
+    *rc = 0;
+    if (format < 0 || format > 2) { err = "format must be 0 (from the first byte), 1 (FASTA) or 2 (FASTQ)"; *rc = GKC_EINVAL; return 1; }
+    if (n == 0) {
+        hipError_t e = hipMemsetAsync(offs, 0, 8, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) *rc = hip_fail(e, "empty text", err);
+        return 1;
+    }
+    return 0;
+}
+
+int current_device(std::string& err, int* dev) {
+    IN_HIP(hipGetDevice(dev));
+    if (*dev < 0 || *dev >= MAX_DEV) { err = "device id out of range"; return GKC_EINVAL; }
+    return 0;
+}
+}  // namespace
+
+int parse_reads(const char* text, uint64_t n, int format, char* bases, uint64_t* offs, hipStream_t st,
+                int64_t* n_reads, std::string& err) {
+    *n_reads = 0;
+    int rc = 0;
+    if (parse_precheck(n, format, offs, st, err, &rc)) return rc;
+    int dev = 0;
+    if ((rc = current_device(err, &dev))) return rc;
+    DevScratch& ds = g_scratch[dev];
+    std::lock_guard<std::mutex> lk(ds.mu);
+    struct Drain {                               // no work on the scratch outlives the lock
+        hipStream_t s;
+        ~Drain() { (void)hipStreamSynchronize(s); }
+    } drain{st};
+    auto region = [&](int i, uint64_t words, uint64_t** p) -> int {
+        const size_t need = words * 8;
+        if (need > ds.bytes[i]) {
+            if (ds.p[i]) {
+                const hipError_t e = hipFree(ds.p[i]);
+                ds.p[i] = nullptr;
+                ds.bytes[i] = 0;
+                if (e != hipSuccess) return hip_fail(e, "hipFree(scratch)", err);
+            }
+            const hipError_t e = hipMalloc(&ds.p[i], need);
+            if (e != hipSuccess) { ds.p[i] = nullptr; return hip_fail(e, "hipMalloc(scratch)", err); }
+            ds.bytes[i] = need;
+        }
+        *p = static_cast<uint64_t*>(ds.p[i]);
+        return 0;
+    };
+    return parse_core(text, n, format, bases, offs, st, n_reads, err, region);
+}
+
+int parse_reads_noalloc(const char* text, uint64_t n, int format, char* bases, uint64_t* offs, hipStream_t st,
+                        int64_t* n_reads, std::string& err) {
+    *n_reads = 0;
+    int rc = 0;
+    if (parse_precheck(n, format, offs, st, err, &rc)) return rc;
+    int dev = 0;
+    if ((rc = current_device(err, &dev))) return rc;
+    FixedScratch& fs = g_fixed[dev];
+    std::lock_guard<std::mutex> lk(fs.mu);
+    if (!fs.ptr) {                               // first call on this device (it is current)
+        void* p = nullptr;
+        IN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        fs.ptr = static_cast<uint64_t*>(p);
+    }
+    struct Drain {
+        hipStream_t s;
+        ~Drain() { (void)hipStreamSynchronize(s); }
+    } drain{st};
+    // region 0 from the array's start, region 1 after it (256-byte aligned)
+    uint64_t used = 0;
+    auto region = [&](int i, uint64_t words, uint64_t** p) -> int {
+        const uint64_t at = i == 0 ? 0 : (used + 31) / 32 * 32;
+        if (at + words > NOALLOC_SCRATCH_WORDS) {
+            err = "text too large for gkc_parse_reads_noalloc's fixed scratch: it needs " + std::to_string((at + words) * 8) +
+                  " bytes, the scratch holds " + std::to_string(NOALLOC_SCRATCH_WORDS * 8) + " (use gkc_parse_reads)";
+            return GKC_ECAP;
+        }
+        used = at + words;
+        *p = fs.ptr + at;
+        return 0;
+    };
+    return parse_core(text, n, format, bases, offs, st, n_reads, err, region);
 }
 
 }  // namespace gkc

@@ -127,6 +127,23 @@ int gkc_superkmer_arrays(gkc_ctx *ctx, const char *reads, const uint64_t *offs, 
  * whole records of the format). */
 int64_t gkc_parse_reads(const char *text, uint64_t n_bytes, int format, char *bases, uint64_t *offs, void *stream);
 
+/* ---- gkc_parse_reads with no device allocation ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (set-up of stream-captured work, tight serving loops, hosts whose
+ * allocator forbids hipMalloc): the same arguments, output, errors and
+ * serialisation per device as gkc_parse_reads, but the call allocates
+ * nothing.  Its scratch is a fixed array of GKC_PARSE_NOALLOC_SCRATCH_BYTES
+ * in the library's device image, one per GPU, held until the process ends;
+ * every call on a GPU reuses that GPU's array (calls on one device are
+ * serialised by the library: overlapping calls would race on it).  A text
+ * needs 8 * (blocks + records) bytes of it and a little more, blocks =
+ * n_bytes / 4096: about 16 MB of FASTQ with 10 kb reads up to about 1 GB
+ * of FASTQ with 150-base reads (3.9 M records) fit; a larger text is
+ * rejected with GKC_ECAP before any of its records is written. */
+#define GKC_PARSE_NOALLOC_SCRATCH_BYTES (32ull << 20)
+int64_t gkc_parse_reads_noalloc(const char *text, uint64_t n_bytes, int format, char *bases, uint64_t *offs,
+                                void *stream);
+
 /* ---- device-resident path (what GenSuperkmerGPU + SKMStore::add_skms do,
  * superkmers.hpp:167-186, without leaving the GPU) ----
  * Encode, minimizer, superkmer and scatter one batch of reads into the
