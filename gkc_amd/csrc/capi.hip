@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -469,6 +470,19 @@ int gkc_superkmer_arrays(gkc_ctx* c, const char* reads, const uint64_t* offs, in
         if (n < 0 || !offs || (n > 0 && !reads)) throw ArgError{"bad batch", GKC_EINVAL};
         if (c->prm.p > 16) throw ArgError{"the reference's u32 minimizer arrays need p <= 16", GKC_EINVAL};
         if (n == 0) return 0;
+        {   // the offset lists [0, i1, .., len-k+1, 0] cannot skip windows: reads must be A/C/G/T only
+            static const auto acgt = [] {
+                std::array<uint8_t, 256> t{};
+                for (uint8_t ch : {'A', 'C', 'G', 'T', 'a', 'c', 'g', 't'}) t[ch] = 1;
+                return t;
+            }();
+            const uint8_t* b = reinterpret_cast<const uint8_t*>(reads);
+            const uint64_t nb = offs[n] - offs[0];
+            uint8_t all = 1;
+            for (uint64_t i = 0; i < nb; i++) all &= acgt[b[i]];
+            if (!all) throw ArgError{"read bytes other than A/C/G/T: split the reads at them first (the offset "
+                                     "lists cannot skip windows)", GKC_EINVAL};
+        }
         ChunkDesc cd;
         const uint64_t B = stage_batch(c, reads, offs, n, 0, cd);
         c->d_mm.ensure((B + 1) * 4);

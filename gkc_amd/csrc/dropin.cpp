@@ -42,6 +42,35 @@ gkc_ctx* thread_ctx(int device, int k, int p, bool hpc, unsigned min_cnt, unsign
     return c;
 }
 
+inline bool is_acgt(unsigned char ch) {
+    switch (ch | 0x20) {
+        case 'a': case 'c': case 'g': case 't': return true;
+        default: return false;
+    }
+}
+
+// The reads of one sub-batch cut at every byte other than A/C/G/T: the runs
+// of at least k such bytes, back to back, with offsets from 0 (a run shorter
+// than k holds no k-mer, before or after HPC).
+void split_reads(const char* reads, const T_CSR_cap* offs, T_read_cnt n, int k, std::vector<char>& out,
+                 std::vector<T_CSR_cap>& out_offs) {
+    out.clear();
+    out_offs.assign(1, 0);
+    for (T_read_cnt r = 0; r < n; r++) {
+        const char* s = reads + (offs[r] - offs[0]);
+        const T_CSR_cap L = offs[r + 1] - offs[r];
+        for (T_CSR_cap a = 0; a < L;) {
+            T_CSR_cap e = a;
+            while (e < L && is_acgt((unsigned char)s[e])) e++;
+            if (e - a >= (T_CSR_cap)k) {
+                out.insert(out.end(), s + a, s + e);
+                out_offs.push_back(out.size());
+            }
+            a = e + 1;
+        }
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- PinnedCSR
@@ -110,10 +139,27 @@ void GenSuperkmerGPU(PinnedCSR& pinned_reads, int K_kmer, int P_minimizer, bool 
     for (T_read_cnt cur = 0; cur < pinned_reads.n_reads; cur += per) {
         const T_read_cnt end = std::min(cur + per, pinned_reads.n_reads);
         T_CSR_cap* offs = pinned_reads.reads_offs + cur;
-        const T_CSR_cap bytes = offs[end - cur] - offs[0];
+        T_CSR_cap bytes = offs[end - cur] - offs[0];
         char* reads = pinned_reads.reads_CSR + offs[0];
+        T_read_cnt cnt = end - cur;
+        // reads holding bytes other than A/C/G/T are handed on cut at them (KMC's
+        // rule; the reference codes such bytes 255 in its minimizers and as T in
+        // CRead, gkc_cuda.cu:25-58, superkmers.hpp:67): the callback then sees the
+        // fragments of >= K_kmer bases as its reads, in a buffer this call owns
+        std::vector<char> frag;
+        std::vector<T_CSR_cap> frag_offs;
+        bool clean = true;
+        for (T_CSR_cap i = 0; i < bytes && clean; i++) clean = is_acgt((unsigned char)reads[i]);
+        if (!clean) {
+            split_reads(reads, offs, cnt, K_kmer, frag, frag_offs);
+            frag.push_back(0);
+            reads = frag.data();
+            offs = frag_offs.data();
+            cnt = (T_read_cnt)(frag_offs.size() - 1);
+            bytes = frag_offs.back();
+        }
         T_h_data hd{};
-        hd.reads_cnt = end - cur;
+        hd.reads_cnt = cnt;
         hd.reads = reads;                 // overwritten in place by the (HPC) bases
         hd.reads_offs = offs;
         hd.minimizers = new T_minimizer[bytes + 1];
@@ -136,6 +182,17 @@ void GenSuperkmerGPU(PinnedCSR& pinned_reads, int K_kmer, int P_minimizer, bool 
 // ---------------------------------------------------------------- SKMStore
 
 void SKMStore::add_skm(const char* beg, uint32_t len) {
+    // a superkmer holding bytes other than A/C/G/T is stored as its runs of
+    // A/C/G/T bytes (no k-mer holds such a byte)
+    for (uint32_t a = 0; a < len;) {
+        uint32_t e = a;
+        while (e < len && is_acgt((unsigned char)beg[e])) e++;
+        if (e > a) add_run(beg + a, e - a);
+        a = e + 1;
+    }
+}
+
+void SKMStore::add_run(const char* beg, uint32_t len) {
     const uint32_t nw = (len + 15) / 16;
     uint32_t tmp[64];
     std::vector<uint32_t> big;

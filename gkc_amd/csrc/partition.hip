@@ -57,6 +57,39 @@ __device__ __forceinline__ uint32_t pack4(uint32_t v) {
 __device__ __forceinline__ uint32_t pack16(const uint4 v) {
     return (pack4(v.x) << 24) | (pack4(v.y) << 16) | (pack4(v.z) << 8) | pack4(v.w);
 }
+// Bit i set when byte i of v is not one of A C G T a c g t: the byte is
+// compared with the lowercase letter of its own 2-bit code (v_perm picks
+// "acgt"[code] per byte), so only those eight bytes give zero.
+__device__ __forceinline__ uint32_t bad4(uint32_t v) {
+    const uint32_t e = ((v >> 2) ^ (v >> 1)) & 0x03030303u;
+    const uint32_t want = __builtin_amdgcn_perm(0u, 0x74676361u, e);    // 't' 'g' 'c' 'a'
+    const uint32_t x = (v | 0x20202020u) ^ want;                          // zero byte <=> ACGT
+    const uint32_t nz = (((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
+    return ((nz >> 7) * 0x01020408u) >> 24;                               // byte i's flag to bit i
+}
+__device__ __forceinline__ uint32_t bad16(const uint4 v) {
+    return bad4(v.x) | (bad4(v.y) << 4) | (bad4(v.z) << 8) | (bad4(v.w) << 12);
+}
+// Non-ACGT flags of tile bases [lo, hi) (hi - lo <= 80): nonzero when any.
+__device__ __forceinline__ uint32_t bad_any(const uint16_t* sb, uint32_t lo, uint32_t hi) {
+    uint32_t acc = 0;
+    for (uint32_t q = lo >> 4; q < (hi + 15) >> 4; q++) {
+        uint32_t m = sb[q];
+        if (q == lo >> 4) m &= 0xFFFFu << (lo & 15);
+        if (q == (hi - 1) >> 4) m &= 0xFFFFu >> (15 - ((hi - 1) & 15));
+        acc |= m;
+    }
+    return acc;
+}
+// First tile base >= lo flagged non-ACGT, or `end` when none before it.
+__device__ __forceinline__ uint32_t bad_next(const uint16_t* sb, uint32_t lo, uint32_t end) {
+    for (uint32_t q = lo >> 4; q * 16 < end; q++) {
+        uint32_t m = sb[q];
+        if (q == lo >> 4) m &= 0xFFFFu << (lo & 15);
+        if (m) return min(end, q * 16 + (uint32_t)__builtin_ctz(m));
+    }
+    return end;
+}
 // Codes of tile bases [j, j + n), 1 <= n <= 16, in the low 2n bits, base j highest.
 __device__ __forceinline__ uint32_t tile_codes(const uint32_t* sc, uint32_t j, uint32_t n) {
     const uint64_t x = ((uint64_t)sc[j >> 4] << 32) | sc[(j >> 4) + 1];
@@ -118,6 +151,7 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
     constexpr int NQ = P1_RCAP / P1_NT;             // records per thread in a flush
     __shared__ int8_t s_str[WIN ? P1_T + MAXK : 1];
     __shared__ uint32_t s_code[P1_MT][P1_NCW];      // staged tiles, one slot each
+    __shared__ uint16_t s_bad[P1_NCW];              // the current tile's non-ACGT bytes, a bit per base
     __shared__ MM s_v[P1_T + MAXK];
     __shared__ MM s_last[P1_NT];
     // superkmer records: slot << 11 | tile-local start; k-mers (window end
@@ -205,20 +239,29 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
         uint32_t* const sc = s_code[slot];
 
         // 1. stage: packed codes of bases [t0, t0 + blen) (code 0 past blen), read offsets
+        uint32_t badw = 0;                          // non-ACGT bytes of this thread's 16 (0 past blen)
         if (tid < P1_NCW) {
             const uint32_t b = tid * 16;
             uint32_t wd = 0;
-            if (b + 16 <= blen) wd = pack16(pf);
-            else
+            if (b + 16 <= blen) {
+                wd = pack16(pf);
+                badw = bad16(pf);
+            } else {
                 for (uint32_t t = 0; t < 16; t++)
-                    if (b + t < blen) wd |= base_code(g_reads[t0 + b + t]) << (30 - 2 * t);
+                    if (b + t < blen) {
+                        const uint32_t ch = g_reads[t0 + b + t];
+                        wd |= base_code(ch) << (30 - 2 * t);
+                        badw |= (bad4(ch | 0x41414100u) & 1u) << t;     // byte 0 tested, others 'A'
+                    }
+            }
             sc[tid] = wd;
+            s_bad[tid] = (uint16_t)badw;
         }
         if (ro_lds) {
             if ((uint32_t)tid < n_ro) s_ro[tid] = pro0 - base32;
             if ((uint32_t)tid + P1_NT < n_ro) s_ro[tid + P1_NT] = pro1 - base32;
         }
-        __syncthreads();
+        const bool tile_bad = __syncthreads_or(badw != 0);    // any non-ACGT byte staged: reads split there
         const uint32_t nt = tile + G;
         const bool more = nt < cd.n_tiles;
         if (more) {                                 // the next tile's inputs load during this one
@@ -336,7 +379,7 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
             // its read (rbeg <= g, g + k <= rend).
             const uint32_t nv = tid * P1_V < (int)tlen ? min((uint32_t)P1_V, tlen - tid * P1_V) : 0u;
             bool slow = nv != 0;
-            if (!WIN && !prm.mm_out && !prm.sketch && ro_lds && nv) {
+            if (!WIN && !prm.mm_out && !prm.sketch && ro_lds && nv && !tile_bad) {
                 const uint32_t gend = g0 + nv;
                 auto vbits = [&](uint32_t rb, uint32_t rn, uint32_t re) -> uint32_t {
                     const uint32_t lo = max(g0, rb);
@@ -380,11 +423,18 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
                         rnext = ro_lds ? s_ro[r - r0 + 1] : (uint32_t)(g_offs[r + 1] - cd.base);
                         rend = cd.lens ? rbeg + (uint32_t)g_lens[r] : rnext;
                     }
-                    const bool valid = g >= rbeg && g < rend && rend - g >= (uint32_t)k;
+                    const uint32_t gl = g - t0;     // tile-local
+                    const bool valid = g >= rbeg && g < rend && rend - g >= (uint32_t)k &&
+                                       !(tile_bad && bad_any(s_bad, gl, gl + (uint32_t)k));
                     if (valid) {
-                        const bool first = (g == rbeg) || (g == t0);
+                        const bool after_bad = tile_bad && g > rbeg && gl > 0 && bad_any(s_bad, gl - 1, gl);
+                        const bool first = (g == rbeg) || (g == t0) || after_bad;
                         if (first || mm[s] != prev) bmask |= 1u << s;
-                        wend[s] = min(rend + 1 - (uint32_t)k - t0, tlen);     // tile-local end of windows
+                        // tile-local end of windows: the read's last window, or the window
+                        // before the next non-ACGT base
+                        uint32_t we = min(rend + 1 - (uint32_t)k - t0, tlen);
+                        if (tile_bad) we = min(we, bad_next(s_bad, gl + (uint32_t)k, we + (uint32_t)k - 1) + 1 - (uint32_t)k);
+                        wend[s] = we;
                         if (prm.mm_out) prm.mm_out[g] = (uint32_t)mm[s];
                         if constexpr (WIN) {
                             if (prm.win_mm) {

@@ -20,7 +20,11 @@
  *                 extraction), :134 (ascending sort), :148-191 (run-length,
  *                 clamp to MAX_KMER_CNT, intended [min,max] filter)
  * Reads shorter than k are skipped (the reference reads out of bounds for
- * them).  k <= 64 is supported with unsigned __int128 keys, p <= 32 with
+ * them).  A byte other than A C G T (either case) splits its read, after
+ * HPC: only windows of k such bases are k-mers (the reference codes such
+ * bytes 255 in the minimizer and as T in CRead, gkc_cuda.cu:25-58,
+ * superkmers.hpp:67; this build counts no k-mer that holds one, as KMC
+ * does).  k <= 64 is supported with unsigned __int128 keys, p <= 32 with
  * 64-bit minimizer values; k <= 32 / p <= 16 is the reference's own range.
  * The per-window (minimizer, position, strand) output has no reference
  * implementation (only declared fields, gkc_cuda.cu:159-160); its rule is
@@ -118,9 +122,17 @@ static uint64_t window_mm(const uint8_t *s, int k, int p, int fmode,
     return best;
 }
 
+/* Length of the run of A/C/G/T bytes starting at s[0] (at most n). */
+static int64_t acgt_run(const uint8_t *s, int64_t n) {
+    int64_t i = 0;
+    while (i < n && BASEMAP[s[i]] != 255) i++;
+    return i;
+}
+
 /* Per-window minimizers for a batch (GPU_GenMinimizer).  lens[r] is the
  * (possibly HPC) length; reads of batch-local read r start at offs[r]-offs[0].
- * mm[start + i] for i in [0, len-k]; pos/strand optional. */
+ * mm[start + i] for every window i of the read whose k bases are A/C/G/T;
+ * the other entries are not written.  pos/strand optional. */
 void or_window_minimizers(const uint8_t *reads, const uint64_t *offs, const int32_t *lens,
                           int64_t n, int k, int p, int fmode,
                           uint64_t *mm, uint8_t *pos, int8_t *strand) {
@@ -129,9 +141,13 @@ void or_window_minimizers(const uint8_t *reads, const uint64_t *offs, const int3
     for (int64_t r = 0; r < n; r++) {
         uint64_t s = offs[r] - base;
         int64_t L = lens ? lens[r] : (int64_t)(offs[r + 1] - offs[r]);
-        for (int64_t i = 0; i + k <= L; i++)
-            mm[s + i] = window_mm(reads + s + i, k, p, fmode,
-                                  pos ? pos + s + i : NULL, strand ? strand + s + i : NULL);
+        for (int64_t a = 0; a < L;) {                   /* ACGT fragments [a, a + f) */
+            const int64_t f = acgt_run(reads + s + a, L - a);
+            for (int64_t i = a; i + k <= a + f; i++)
+                mm[s + i] = window_mm(reads + s + i, k, p, fmode,
+                                      pos ? pos + s + i : NULL, strand ? strand + s + i : NULL);
+            a += f + 1;
+        }
     }
 }
 
@@ -252,17 +268,22 @@ static void *phase1_worker(void *arg) {
     for (int64_t r = j->r0; r < j->r1; r++) {
         uint64_t s = j->offs[r] - base;
         int64_t L = j->lens ? j->lens[r] : (int64_t)(j->offs[r + 1] - j->offs[r]);
-        if (L < j->k) continue;
-        int64_t W = L - j->k + 1;
-        for (int64_t i = 0; i < W; i++) mm[i] = window_mm(j->reads + s + i, j->k, j->p, j->fmode, NULL, NULL);
-        int64_t beg = 0;
-        for (int64_t i = 1; i <= W; i++) {
-            if (i == W || mm[i] != mm[i - 1]) {
-                /* superkmer over windows [beg, i): bases [beg, i + k - 1) */
-                uint32_t part = or_partition(mm[beg], j->P, j->pmode);
-                skm_ref ref = {s + (uint64_t)beg, (uint32_t)(i - beg + j->k - 1)};
-                vec_push(&j->bins[part], ref);
-                beg = i;
+        for (int64_t a = 0; a < L;) {                   /* each ACGT fragment as a read */
+            const int64_t f = acgt_run(j->reads + s + a, L - a);
+            const uint64_t fs = s + (uint64_t)a;
+            a += f + 1;
+            if (f < j->k) continue;
+            int64_t W = f - j->k + 1;
+            for (int64_t i = 0; i < W; i++) mm[i] = window_mm(j->reads + fs + i, j->k, j->p, j->fmode, NULL, NULL);
+            int64_t beg = 0;
+            for (int64_t i = 1; i <= W; i++) {
+                if (i == W || mm[i] != mm[i - 1]) {
+                    /* superkmer over windows [beg, i): bases [beg, i + k - 1) */
+                    uint32_t part = or_partition(mm[beg], j->P, j->pmode);
+                    skm_ref ref = {fs + (uint64_t)beg, (uint32_t)(i - beg + j->k - 1)};
+                    vec_push(&j->bins[part], ref);
+                    beg = i;
+                }
             }
         }
     }

@@ -144,7 +144,8 @@ def brute_force_table(reads: np.ndarray, offs: np.ndarray, k: int, hpc_on: bool 
     """Canonical k-mer -> count with a dict over the read strings: the
     canonical form is taken as the lexicographically smaller of the k-mer
     string and its reverse complement string (A<C<G<T), an independent
-    statement of min(x, rc(x)) on 2-bit values."""
+    statement of min(x, rc(x)) on 2-bit values.  A k-mer holding a byte
+    other than A/C/G/T (either case) is not counted."""
     table: dict[int, int] = {}
     for r in range(len(offs) - 1):
         s = bytes(reads[int(offs[r] - offs[0]):int(offs[r + 1] - offs[0])])
@@ -152,6 +153,8 @@ def brute_force_table(reads: np.ndarray, offs: np.ndarray, k: int, hpc_on: bool 
             s = bytes([s[0]] + [s[i] for i in range(1, len(s)) if s[i] != s[i - 1]])
         for i in range(len(s) - k + 1):
             fw = s[i:i + k].upper()
+            if fw.strip(b"ACGT"):
+                continue
             rc = fw.translate(_COMP)[::-1]
             c = min(fw, rc)
             v = 0

@@ -137,6 +137,82 @@ def test_count_filter():
     assert int(t.part_distinct.sum()) == len(full)
 
 
+def _with_other_bytes(rs, rng, every=97):
+    """The workload's reads with bytes other than A/C/G/T written in: single
+    N, runs of N, lowercase n and IUPAC / gap characters."""
+    reads = rs.reads.copy()
+    junk = np.frombuffer(b"NNNNnRYKM-.*X", np.uint8)
+    at = rng.choice(len(reads), size=len(reads) // every, replace=False)
+    reads[at] = junk[rng.integers(len(junk), size=len(at))]
+    for a in rng.choice(len(reads) - 40, size=len(reads) // 3000, replace=False):
+        reads[a:a + int(rng.integers(2, 40))] = ord("N")
+    return reads
+
+
+def _split_frags(reads, offs):
+    """Every read cut into its runs of A/C/G/T bytes."""
+    import re
+    frags = []
+    for r in range(len(offs) - 1):
+        s = bytes(reads[int(offs[r]):int(offs[r + 1])])
+        frags += [f for f in re.split(rb"[^ACGTacgt]+", s) if f]
+    return frags
+
+
+def _split_at_other_bytes(reads, offs):
+    """The fragments of _split_frags as a read CSR."""
+    return _csr(_split_frags(reads, offs))
+
+
+@pytest.mark.parametrize("hpc,k,p,P", [(False, 31, 11, 64), (True, 25, 15, 512), (False, 63, 19, 64)])
+def test_non_acgt_bytes_split_reads(hpc, k, p, P):
+    """A byte other than A/C/G/T splits its read: the oracle's table equals
+    the brute force that skips k-mers holding one, and equals the oracle's
+    table of the reads cut at those bytes beforehand (after HPC, which
+    collapses runs of N like any run)."""
+    rng = np.random.default_rng(7 + k)
+    rs = W.generate(W.workload(1, scale=0.01))
+    reads = _with_other_bytes(rs, rng)
+    t = O.count(reads, rs.offs, k, p, P, hpc=hpc)
+    if hpc:
+        src, lens, _ = O.hpc(reads, rs.offs)
+        seqs = [bytes(src[int(rs.offs[r]):int(rs.offs[r]) + int(lens[r])]) for r in range(rs.n)]
+        hr, ho = _csr(seqs)
+        fr, fo = _split_at_other_bytes(hr, ho)
+    else:
+        fr, fo = _split_at_other_bytes(reads, rs.offs)
+    t2 = O.count(fr, fo, k, p, P)
+    for a in ("key_lo", "cnt", "part_off", "part_kmers"):
+        np.testing.assert_array_equal(getattr(t, a), getattr(t2, a), err_msg=a)
+    if k <= 32:
+        bf = O.brute_force_table(reads, rs.offs, k, hpc_on=hpc)
+        assert _union(t) == bf
+    assert int(t.part_kmers.sum()) < rs.kmers(k)
+
+
+def test_non_acgt_window_records():
+    """Per-window records are written for windows of k A/C/G/T bytes only;
+    those equal the records of the same window in the split reads."""
+    k, p = 25, 15
+    reads, offs = _csr([b"ACGTTGCAAGGCTTACGATCGATCGGATNCCGATTAGCATGCATCGATCGACTAGCTAGCAT", b"N" * 30,
+                        b"ACGGTACGTAGCTAGCTAGTCGATCGTAGCTAGCTAGnTCGATCGATCGA"])
+    got, gpos, gst = O.window_minimizers(reads, offs, k, p, 1)      # unwritten: strand -2
+    written = 0
+    for r in range(3):
+        s = bytes(reads[int(offs[r]):int(offs[r + 1])])
+        for i in range(len(s) - k + 1):
+            w = s[i:i + k]
+            if w.upper().strip(b"ACGT"):
+                assert gst[int(offs[r]) + i] == -2
+                continue
+            written += 1
+            wr, wo = _csr([w])
+            m1, p1, s1 = O.window_minimizers(wr, wo, k, p, 1)
+            a = int(offs[r]) + i
+            assert (got[a], gpos[a], gst[a]) == (m1[0], p1[0], s1[0])
+    assert written == sum(max(0, len(f) - k + 1) for f in _split_frags(reads, offs))
+
+
 # ---------------------------------------------------------------- invariants
 
 @pytest.mark.parametrize("k,p,fmode", [(31, 11, 1), (31, 11, 0), (25, 15, 1), (63, 19, 1), (20, 20, 1)])
@@ -175,6 +251,96 @@ def test_minimizer_sliding_window_formulation():
         w = k - p + 1
         win = np.lib.stride_tricks.sliding_window_view(v, w).min(axis=1)
         np.testing.assert_array_equal(mm[int(rs.offs[r]):int(rs.offs[r]) + len(win)], win.astype(np.uint64))
+
+
+# ---------------------------------------------------------------- per-window (minimizer, position, strand)
+
+def _window_rule(w: bytes, p: int, fmode: int):
+    """Brute force over all 2(k-p+1) candidates of one window: the smallest
+    passing value; among the candidates holding it, the largest start; at
+    that start the reverse complement (strand 1) over the forward p-mer
+    (strand 0), -1 when both hold it (f == rc); (4^p-1, 255, -1) when no
+    candidate passes."""
+    M = 4 ** p - 1
+    cands = []
+    for j in range(len(w) - p + 1):
+        sub = w[j:j + p]
+        f, r = _enc(sub), _enc(sub.translate(COMP)[::-1])
+        for v, st in ((f, 0), (r, 1)):
+            top = v >> (2 * (p - 3))
+            if fmode == 0 or top not in (0, 4, 8):
+                cands.append((v, j, st))
+    if not cands:
+        return M, 255, -1
+    v = min(c[0] for c in cands)
+    J = max(c[1] for c in cands if c[0] == v)
+    sts = {c[2] for c in cands if c[0] == v and c[1] == J}
+    return v, J, (-1 if len(sts) == 2 else sts.pop())
+
+
+def _windows(rng, k, n, alphabet):
+    out = []
+    for _ in range(n):
+        a = alphabet[rng.integers(len(alphabet))]
+        out.append(bytes(rng.choice(list(a), size=k).tolist()))
+    out += [b"A" * k, b"T" * k, (b"ACGT" * k)[:k], (b"AT" * k)[:k], (b"GC" * k)[:k], (b"AAC" * k)[:k]]
+    return out
+
+
+@pytest.mark.parametrize("k,p,fmode", [(9, 3, 1), (9, 4, 1), (10, 4, 0), (12, 5, 1), (14, 6, 0),
+                                       (25, 15, 1), (31, 11, 1), (20, 8, 1)])
+def test_window_position_strand_brute_force(k, p, fmode):
+    """The oracle's per-window records equal an independent brute force of
+    the rule over every candidate, on random windows over small alphabets
+    (many ties of value and of strand) and periodic and homopolymer ones;
+    even p gives palindromic p-mers (f == rc, strand -1)."""
+    rng = np.random.default_rng(1000 * k + p)
+    seqs = _windows(rng, k, 400, [b"ACGT", b"AC", b"AT", b"GC", b"ACG", b"AAT"])
+    reads, offs = _csr(seqs)
+    mm, pos, strand = O.window_minimizers(reads, offs, k, p, fmode)
+    seen_tie = seen_pal = False
+    for i, w in enumerate(seqs):
+        a = int(offs[i])
+        want = _window_rule(w, p, fmode)
+        assert (int(mm[a]), int(pos[a]), int(strand[a])) == want, (w, want)
+        seen_pal |= want[2] == -1 and want[1] != 255
+        seen_tie |= sum(1 for j in range(k - p + 1) if want[0] in (_enc(w[j:j + p]), _enc(w[j:j + p].translate(COMP)[::-1]))) > 1
+    assert seen_tie
+    if p % 2 == 0:
+        assert seen_pal
+
+
+@pytest.mark.parametrize("k,p", [(25, 15), (31, 11), (12, 4)])
+def test_window_record_invariants(k, p):
+    """Closed-form checks of the records on generated reads: the p-mer at
+    (pos, strand) has the reported value; for a window whose minimum sits at
+    one start only, its reverse complement reports the mirrored start
+    k-p-pos and the other strand."""
+    rs = W.generate(W.workload(1, scale=0.002))
+    mm, pos, strand = O.window_minimizers(rs.reads, rs.offs, k, p, 1)
+    checked = mirrored = 0
+    for r in range(0, rs.n, 5):
+        s = bytes(rs.reads[int(rs.offs[r]):int(rs.offs[r + 1])])
+        for i in range(0, len(s) - k + 1, 3):
+            a = int(rs.offs[r]) + i
+            w = s[i:i + k]
+            if pos[a] == 255:
+                continue
+            sub = w[pos[a]:pos[a] + p]
+            f, rc = _enc(sub), _enc(sub.translate(COMP)[::-1])
+            want = {0: f, 1: rc, -1: f}[int(strand[a])]
+            assert want == int(mm[a]) and (strand[a] != -1 or f == rc)
+            checked += 1
+            starts = [j for j in range(k - p + 1)
+                      if int(mm[a]) in (_enc(w[j:j + p]), _enc(w[j:j + p].translate(COMP)[::-1]))]
+            if len(starts) == 1:
+                rw = w.translate(COMP)[::-1]
+                r2, o2 = _csr([rw])
+                m2, p2, s2 = O.window_minimizers(r2, o2, k, p, 1)
+                assert int(m2[0]) == int(mm[a]) and int(p2[0]) == k - p - int(pos[a])
+                assert int(s2[0]) == {0: 1, 1: 0, -1: -1}[int(strand[a])]
+                mirrored += 1
+    assert checked > 1000 and mirrored > 500
 
 
 def test_union_independent_of_partitioning():
