@@ -15,6 +15,7 @@
 
 #include "types.h"
 
+GKC_DROPIN_BEGIN
 struct CUDAParams {
     int NUM_BLOCKS_PER_GRID, NUM_THREADS_PER_BLOCK;   /* reads per sub-batch = their product */
     int n_streams;                                    /* accepted; the library pipelines on its own stream */
@@ -38,6 +39,7 @@ public:
 };
 
 enum CountTask { SKMPartition, SKMPartWithPos, StoreMinimizerPos };
+GKC_DROPIN_END
 
 /* Bases per partition of one sub-batch (reference gkc_cuda.cu:350-365). */
 void CalcSKMPartSize_instream(T_read_cnt reads_cnt, T_read_len *superkmer_offs, T_CSR_cap *reads_offs,

@@ -15,12 +15,14 @@
 
 #include "types.h"
 
+GKC_DROPIN_BEGIN
 class SKMStore {
 public:
     std::atomic<size_t> tot_size{0};        /* total bases added */
     SKMStore() {}
     ~SKMStore() {}
-    /* Append one superkmer of `len` ASCII bases. */
+    /* Append one superkmer of `len` ASCII bases (a superkmer holding bytes
+     * other than A/C/G/T is stored as its runs of A/C/G/T bytes). */
     void add_skm(const char *beg, uint32_t len);
     /* Number of superkmers stored. */
     size_t size() const;
@@ -34,9 +36,11 @@ public:
     void take(std::vector<uint32_t> &words, std::vector<uint32_t> &lens);
 
 private:
+    void add_run(const char *beg, uint32_t len);
     mutable std::mutex mu_;
     std::vector<uint32_t> words_;
     std::vector<uint32_t> lens_;
 };
+GKC_DROPIN_END
 
 #endif

@@ -16,6 +16,14 @@
 #define _out_
 #endif
 
+/* The drop-in's record types live in an inline namespace: code written
+ * against these headers names them unqualified as before, but their mangled
+ * names differ from the reference's, so an object compiled against the
+ * reference's own headers (whose SKMStore and T_h_data have other layouts)
+ * fails to link against libgkc_dropin.so instead of corrupting memory. */
+#define GKC_DROPIN_BEGIN inline namespace gkc_mi355x_v1 {
+#define GKC_DROPIN_END }
+
 typedef size_t T_CSR_cap;          /* CSR offsets */
 typedef int T_CSR_count;
 typedef unsigned short T_skm_len;  /* the reference's superkmer length type */
@@ -25,13 +33,18 @@ typedef unsigned int T_minimizer;  /* minimizer values, p <= 16 */
 const T_minimizer T_MM_MAX = (T_minimizer)0xFFFFFFFFu;
 typedef unsigned char T_kvalue;
 
+GKC_DROPIN_BEGIN
 struct ReadPtr {
     const char *read;
     T_read_len len;
 };
 
+GKC_DROPIN_END
+
 typedef unsigned long long T_kmer;       /* k <= 32 */
 typedef unsigned long long T_2bitkmer;
+
+GKC_DROPIN_BEGIN
 
 /* Host arrays handed to GenSuperkmerGPU's callback once per sub-batch. */
 struct T_h_data {
@@ -44,8 +57,11 @@ struct T_h_data {
     T_read_len *superkmer_offs;   /* per read: [0, i1, .., len-k+1, 0] at the read's batch-local start */
 };
 
+GKC_DROPIN_END
+
 typedef unsigned int T_kmer_cnt;
 const T_kmer_cnt MAX_KMER_CNT = (T_kmer_cnt)0xFFFFFFFFu;
+GKC_DROPIN_BEGIN
 struct T_kmc {
     T_kmer kmer;
     T_kmer_cnt cnt;
@@ -56,5 +72,6 @@ struct T_kmc128 {
     unsigned long long hi, lo;
     T_kmer_cnt cnt;
 };
+GKC_DROPIN_END
 
 #endif
