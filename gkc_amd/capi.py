@@ -205,6 +205,12 @@ class Counter:
         _check(lib().gkc_sketch(self.h, _p(out, _f32p)))
         return out
 
+    def table_starts(self) -> np.ndarray:
+        """Physical start of each partition's entries in the device table."""
+        out = np.zeros(self.n_parts, np.uint64)
+        _check(lib().gkc_table_starts(self.h, _p(out, _u64p)))
+        return out
+
     def set_table_caps(self, caps) -> None:
         caps = np.ascontiguousarray(caps, np.uint32)
         _check(lib().gkc_set_table_caps(self.h, _p(caps, _u32p)))
@@ -241,6 +247,15 @@ class Counter:
         _check(lib().gkc_window_minimizers(self.h, reads.ctypes.data, _p(offs, _u64p), len(offs) - 1, _p(mm, _u32p),
                                            _p(pos, _u8p), st.ctypes.data_as(C.POINTER(C.c_int8))))
         return {"minimizers": mm[:B], "pos": pos[:B], "strand": st[:B]}
+
+    def windows_device(self) -> tuple[int, int, int, int]:
+        """Device addresses of the (minimizer u32, position u8, strand i8)
+        records the last partition(..., EMIT_WINDOWS) wrote, and their count
+        (batch-local base index; valid until the next such call)."""
+        mm, pos, st, n = _u32p(), _u8p(), C.POINTER(C.c_int8)(), C.c_uint64(0)
+        _check(lib().gkc_windows_device(self.h, C.byref(mm), C.byref(pos), C.byref(st), C.byref(n)))
+        addr = lambda q: C.cast(q, C.c_void_p).value or 0
+        return addr(mm), addr(pos), addr(st), n.value
 
     # -- drop-in #1 arrays
     def superkmer_arrays(self, reads, offs) -> dict:

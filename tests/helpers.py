@@ -15,3 +15,15 @@ def assert_tables_equal(gpu_table, oracle_table, n_parts: int, k: int) -> None:
     if k > 32:
         np.testing.assert_array_equal(hi, o.key_hi, err_msg="keys (high 64 bits)")
     np.testing.assert_array_equal(cnt, o.cnt, err_msg="counts")
+
+
+def device_to_host(ptr: int, nbytes: int, dtype=np.uint8) -> np.ndarray:
+    """Copy nbytes of device memory at ptr to a host array (hipMemcpy of the
+    HIP runtime the library already loaded)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    out = np.empty(nbytes, np.uint8)
+    if nbytes:
+        rc = hip.hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(ptr), ctypes.c_size_t(nbytes), 2)
+        assert rc == 0, f"hipMemcpy failed: {rc}"
+    return out.view(dtype)

@@ -75,8 +75,9 @@ def test_rejects_bad_files(tmp_path):
 @pytest.mark.parametrize("cfg,n_parts,flags_name", [(1, 16, "plain"), (4, 8, "plain"), (2, 64, "batched"),
                                                     (1, 32, "presized")])
 def test_device_file_matches(gpu, tmp_path, cfg, n_parts, flags_name):
-    """Plain and batched counts (a presized table leaves gaps between
-    partitions on the device: the file is written run by run)."""
+    """Plain and batched counts (a presized table, its caps set above the
+    exact sizes, leaves gaps between partitions on the device: the file is
+    written run by run)."""
     from gkc_amd import capi
     w = W.workload(cfg, scale=0.004)
     rs = W.generate(w)
@@ -86,9 +87,17 @@ def test_device_file_matches(gpu, tmp_path, cfg, n_parts, flags_name):
     path = tmp_path / "dev.gkct"
     with capi.Counter(w.k, w.p, n_parts) as c:
         c.partition(rs.reads, rs.offs, flags=flags & capi.BATCHED)
+        if flags_name == "presized":
+            exact = np.diff(ot.part_off)
+            c.set_table_caps((exact + exact // 4 + 3).astype(np.uint32))
         c.count(flags)
         nbytes = c.write_table(path)
         lo, hi, cnt, po, pd, pk = c.table()
+        starts = c.table_starts()
+        st = c.stats()
+    if flags_name == "presized":
+        assert st["fallbacks"] == 0
+        assert np.all(np.diff(starts.astype(np.int64)) > np.diff(po.astype(np.int64))[:-1])    # gaps between partitions
     assert path.stat().st_size == nbytes
     assert_tables_equal(TF.read_table(path).as_table(), ot, n_parts, w.k)
     twin = tmp_path / "host.gkct"
@@ -122,9 +131,14 @@ def test_device_file_many_chunks(gpu, tmp_path):
     path = tmp_path / "big.gkct"
     with capi.Counter(w.k, w.p, w.n_parts) as c:
         c.partition(rs.reads, rs.offs, flags=capi.BATCHED)
+        pk0 = c.part_kmers()
+        c.set_table_caps(np.minimum(pk0 // 3 + 64, pk0).astype(np.uint32))   # above the distinct counts here
         c.count(capi.BATCHED | capi.PRESIZED)
         nbytes = c.write_table(path)
         lo, hi, cnt, po, pd, pk = c.table()
+        starts = c.table_starts()
+        assert c.stats()["fallbacks"] == 0
+    assert np.any(starts[1:] != po[1:-1])              # the device table has gaps: written run by run
     assert len(cnt) * 12 > 3 * (64 << 20)
     twin = tmp_path / "host.gkct"
     assert TF.write_table(twin, w.k, w.p, 0, 1, 0, po, pd, pk, lo, hi, cnt) == nbytes
