@@ -96,7 +96,6 @@ struct gkc_ctx {
     DevBuf d_word_base, d_hdr_base, d_word_cap, d_hdr_cap, d_word_cur, d_hdr_cur, d_kmer_cur, d_overflow;
     DevBuf d_tmp_hdr, d_tmp_word, d_tmp_kmer;    // per-batch counts of the count pass
     DevBuf d_sketch;
-    uint64_t sketch_windows = 0;
 
     // batch staging
     DevBuf d_reads, d_offs, d_lens, d_hpc_reads, d_hpc_pos, d_chunk, d_ctr, d_mm;
@@ -116,6 +115,8 @@ struct gkc_ctx {
     // counting
     DevBuf keysA, keysB, lookback, d_plan, d_segk, d_segh, d_keystart, d_itemstart, d_tilestart, d_rlstart, d_hist;
     DevBuf d_tstart, d_tcap, d_nent, d_ndist, d_toverflow;
+    DevBuf d_tcap_total;              // total table capacity of a presized binned count (u64)
+    uint64_t h_tcap_total = 0;
     DevBuf d_ranges;                           // binned-count tile ranges {a, n}
     DevBuf d_tiler0;                           // read holding the first position of each phase-1 tile
     DevBuf d_chunk_s;                          // descriptor of a chunk's sample (first-batch presizing)
@@ -200,7 +201,6 @@ static void bins_init(gkc_ctx* c) {
     HIPX(hipMemsetAsync(c->d_hdr_cap.p, 0, P * 4, c->st));
     HIPX(hipMemsetAsync(c->d_sketch.p, 0, (size_t)P * 256 * 4, c->st));
     HIPX(hipMemsetAsync(c->d_overflow.p, 0, 16, c->st));
-    c->sketch_windows = 0;
     c->counted = false;
 }
 
@@ -550,11 +550,10 @@ int gkc_windows_device(gkc_ctx* c, uint32_t** mm, uint8_t** pos, int8_t** strand
 }
 
 // Plain phase 1 of one batch: count pass, host sync, exact reservation, scatter.
-static void partition_plain(gkc_ctx* c, const ChunkDesc& cd, uint32_t flags, bool sketch_done) {
+static void partition_plain(gkc_ctx* c, const ChunkDesc& cd, uint32_t flags) {
     const uint32_t P = c->prm.n_parts;
     zero(c, c->d_tmp_hdr.p, P); zero(c, c->d_tmp_word.p, P); zero(c, c->d_tmp_kmer.p, P);
     P1Params pc = p1params(c, 0);
-    if (!sketch_done) pc.sketch = c->d_sketch.as<uint32_t>();
     launch_superkmers(c, cd, pc, binset(c, true));
     std::vector<uint32_t> nh(P), nw(P), nk(P);
     HIPX(hipMemcpyAsync(nh.data(), c->d_tmp_hdr.p, P * 4, hipMemcpyDeviceToHost, c->st));
@@ -565,6 +564,7 @@ static void partition_plain(gkc_ctx* c, const ChunkDesc& cd, uint32_t flags, boo
         if ((uint64_t)c->h_kmer_cur[q] + nk[q] >= (1ull << 30)) throw ArgError{"partition exceeds 2^30 k-mers; use more partitions", GKC_EINVAL};
     bins_reserve(c, nw, nh);
     P1Params ps = p1params(c, 1);
+    ps.sketch = c->d_sketch.as<uint32_t>();
     if (flags & GKC_EMIT_WINDOWS) {
         if (c->prm.p > 16) throw ArgError{"per-window records hold u32 minimizers: p <= 16", GKC_EINVAL};
         const uint64_t B = cd.n_bases;
@@ -586,6 +586,36 @@ __global__ void k_copy_u32(uint32_t* __restrict__ dst, const uint32_t* __restric
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         dst[i] = a[i]; dst[n + i] = b[i]; dst[2 * n + i] = cc[i];
     }
+}
+// Dense table layout of a presized binned count, from the exact kept
+// entries per segment (counted by then): starts = exclusive scan, and each
+// segment's capacity = what is left of the allocation after its start, so
+// only the total has to fit.  One block of 1024 threads, n_segs <= 1024.
+__global__ __launch_bounds__(1024) void k_table_layout(const uint32_t* __restrict__ n_entries, uint32_t n_segs,
+                                                       const uint64_t* __restrict__ total_cap,
+                                                       uint64_t* __restrict__ table_start, uint32_t* __restrict__ table_cap) {
+    __shared__ uint64_t s_w[1024 / GKC_WAVE];
+    const uint32_t t = threadIdx.x, lane = __lane_id(), wv = t / GKC_WAVE;
+    const uint64_t v = t < n_segs ? n_entries[t] : 0u;
+    uint64_t inc = v;
+#pragma unroll
+    for (int off = 1; off < GKC_WAVE; off <<= 1) {
+        const uint64_t y = __shfl_up(inc, off);
+        if ((int)lane >= off) inc += y;
+    }
+    if (lane == GKC_WAVE - 1) s_w[wv] = inc;
+    __syncthreads();
+    uint64_t before = 0;
+    for (uint32_t w = 0; w < wv; w++) before += s_w[w];
+    const uint64_t start = before + inc - v, tot = *total_cap;
+    if (t < n_segs) {
+        table_start[t] = start;
+        table_cap[t] = start >= tot ? 0u : (tot - start > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)(tot - start));
+    }
+}
+__global__ void k_copy_words(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 __global__ void k_restore_u32(const uint32_t* __restrict__ src, uint32_t* __restrict__ a, uint32_t* __restrict__ b,
                               uint32_t* __restrict__ cc, uint32_t n) {
@@ -629,11 +659,13 @@ static bool partition_presized(gkc_ctx* c, const ChunkDesc& cd) {
         }
     }
     bins_reserve(c, nw, nh);
-    c->d_backup.ensure((size_t)3 * P * 4);
+    c->d_backup.ensure((size_t)(3 + 256) * P * 4);     // cursors, then the sketch
     std::vector<const void*> key = {c->words.p, c->hdrs.p, c->d_chunk.p, c->d_backup.p};
     auto enqueue = [&] {
         hipLaunchKernelGGL(k_copy_u32, dim3((P + 255) / 256), dim3(256), 0, c->st, c->d_backup.as<uint32_t>(),
                            c->d_word_cur.as<uint32_t>(), c->d_hdr_cur.as<uint32_t>(), c->d_kmer_cur.as<uint32_t>(), P);
+        hipLaunchKernelGGL(k_copy_words, dim3(256), dim3(256), 0, c->st, c->d_backup.as<uint32_t>() + 3 * P,
+                           (const uint32_t*)c->d_sketch.as<uint32_t>(), (uint64_t)P * 256);
         zero(c, c->d_overflow.p, 1);
         P1Params ps = p1params(c, 1);
         ps.sketch = c->d_sketch.as<uint32_t>();
@@ -660,6 +692,8 @@ static bool partition_presized(gkc_ctx* c, const ChunkDesc& cd) {
     if (ovf || too_big) {
         hipLaunchKernelGGL(k_restore_u32, dim3((P + 255) / 256), dim3(256), 0, c->st, c->d_backup.as<uint32_t>(),
                            c->d_word_cur.as<uint32_t>(), c->d_hdr_cur.as<uint32_t>(), c->d_kmer_cur.as<uint32_t>(), P);
+        hipLaunchKernelGGL(k_copy_words, dim3(256), dim3(256), 0, c->st, c->d_sketch.as<uint32_t>(),
+                           (const uint32_t*)(c->d_backup.as<uint32_t>() + 3 * P), (uint64_t)P * 256);
         zero(c, c->d_overflow.p, 1);
         HIPX(hipStreamSynchronize(c->st));
         c->p1_fallbacks++;
@@ -676,12 +710,10 @@ int gkc_partition(gkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n
         ChunkDesc cd;
         HIPX(hipEventRecord(c->ev[0], c->st));
         stage_batch(c, reads, offs, n, flags, cd);
-        bool done = false, sketched = false;
-        if ((flags & GKC_BATCHED) && !(flags & GKC_EMIT_WINDOWS)) {
-            done = partition_presized(c, cd);
-            sketched = !done;                        // a failed attempt already sketched this chunk
-        }
-        if (!done) partition_plain(c, cd, flags, sketched);
+        bool done = false;
+        if ((flags & GKC_BATCHED) && !(flags & GKC_EMIT_WINDOWS))
+            done = partition_presized(c, cd);        // a failed attempt restores the cursors and the sketch
+        if (!done) partition_plain(c, cd, flags);
         c->bases_done += cd.n_bases - cd.lead;
         HIPX(hipEventRecord(c->ev[1], c->st));
         HIPX(hipEventSynchronize(c->ev[1]));
@@ -939,6 +971,7 @@ static void ensure_count_buffers(gkc_ctx* c, uint64_t max_keys, uint32_t n_segs)
     c->d_hist.ensure((size_t)n_segs * MAX_PASS * RADIX * 4);
     c->d_tstart.ensure(n_segs * 8 + 8);
     c->d_tcap.ensure(n_segs * 4 + 4);
+    c->d_tcap_total.ensure(8);
     c->d_nent.ensure(n_segs * 4 + 4);
     c->d_ndist.ensure(n_segs * 4 + 4);
     c->d_toverflow.ensure(64);
@@ -1408,6 +1441,10 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs
                 hipLaunchKernelGGL(k_tilescan, dim3(std::min<uint32_t>(P, 4 * c->n_cu)), dim3(1024), 0, c->st, plan, sa,
                                    hl.kept);
             }
+            if (emit && prep)       // presized: a dense layout from the exact counts, only the total must fit
+                hipLaunchKernelGGL(k_table_layout, dim3(1), dim3(1024), 0, c->st, (const uint32_t*)c->d_nent.as<uint32_t>(),
+                                   P, (const uint64_t*)c->d_tcap_total.as<uint64_t>(), c->d_tstart.as<uint64_t>(),
+                                   c->d_tcap.as<uint32_t>());
             if (emit) {
                 zero(c, c->d_toverflow.p, 1);
                 const int g_c = occupancy_blocks((const void*)k_wavecopy<K>, 256, c->n_cu);
@@ -1528,14 +1565,23 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
         HIPX(hipStreamSynchronize(c->st));
         c->stats.host_syncs++;
     }
-    // table placement (physical starts) and capacity
+    // table placement (physical starts) and capacity.  Presized binned
+    // count: the starts come from the exact counts on the device (dense
+    // table), the predicted capacities only size the whole table; otherwise
+    // each partition gets its own capacity
+    const bool dense = presized && bs.wave;
     c->h_tstart.assign(P, 0);
     uint64_t tot_cap = 0;
     for (uint32_t q = 0; q < P; q++) { c->h_tstart[q] = tot_cap; tot_cap += caps[q]; }
     c->tkeys.ensure((tot_cap + 1) * sizeof(K));
     c->tcnts.ensure((tot_cap + 1) * 4);
-    HIPX(hipMemcpyAsync(c->d_tstart.p, c->h_tstart.data(), P * 8, hipMemcpyHostToDevice, c->st));
-    HIPX(hipMemcpyAsync(c->d_tcap.p, caps.data(), P * 4, hipMemcpyHostToDevice, c->st));
+    if (dense) {
+        c->h_tcap_total = tot_cap;
+        HIPX(hipMemcpyAsync(c->d_tcap_total.p, &c->h_tcap_total, 8, hipMemcpyHostToDevice, c->st));
+    } else {
+        HIPX(hipMemcpyAsync(c->d_tstart.p, c->h_tstart.data(), P * 8, hipMemcpyHostToDevice, c->st));
+        HIPX(hipMemcpyAsync(c->d_tcap.p, caps.data(), P * 4, hipMemcpyHostToDevice, c->st));
+    }
     // emit pass: replay the captured graph only if it writes the current table
     // buffers (either may have moved when the table grew).  The first pass
     // over the sorted keys of this count prepares (see enqueue_runlength_all).
@@ -1582,6 +1628,9 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
         HIPX(hipStreamSynchronize(c->st));
         c->stats.host_syncs++;
         if (flags_dev[0]) throw ArgError{"internal: table overflow after exact sizing", GKC_ESTATE};
+    } else if (dense) {
+        uint64_t a = 0;
+        for (uint32_t q = 0; q < P; q++) { c->h_tstart[q] = a; a += nent[q]; }
     }
     HIPX(hipEventRecord(c->ev[1], c->st));
     HIPX(hipEventSynchronize(c->ev[1]));

@@ -90,6 +90,11 @@ __device__ __forceinline__ uint32_t bad_next(const uint16_t* sb, uint32_t lo, ui
     }
     return end;
 }
+// Sketch sample of a superkmer: its first min(k, 16) bases (2-bit codes,
+// first base highest) hashed by a multiplicative (Fibonacci) hash, kept when
+// the top three bits are zero (one superkmer in eight, decided by its bases
+// alone, so it does not depend on the launch shape).
+__device__ __forceinline__ bool sketch_sampled(uint32_t lead) { return ((lead * 0x9E3779B1u) >> 29) == 0u; }
 // Codes of tile bases [j, j + n), 1 <= n <= 16, in the low 2n bits, base j highest.
 __device__ __forceinline__ uint32_t tile_codes(const uint32_t* sc, uint32_t j, uint32_t n) {
     const uint64_t x = ((uint64_t)sc[j >> 4] << 32) | sc[(j >> 4) + 1];
@@ -379,7 +384,7 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
             // its read (rbeg <= g, g + k <= rend).
             const uint32_t nv = tid * P1_V < (int)tlen ? min((uint32_t)P1_V, tlen - tid * P1_V) : 0u;
             bool slow = nv != 0;
-            if (!WIN && !prm.mm_out && !prm.sketch && ro_lds && nv && !tile_bad) {
+            if (!WIN && !prm.mm_out && ro_lds && nv && !tile_bad) {
                 const uint32_t gend = g0 + nv;
                 auto vbits = [&](uint32_t rb, uint32_t rn, uint32_t re) -> uint32_t {
                     const uint32_t lo = max(g0, rb);
@@ -444,10 +449,6 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
                                 prm.win_pos[g] = none ? (uint8_t)255 : (uint8_t)(js - (g - t0));
                                 prm.win_strand[g] = none ? (int8_t)-1 : s_str[js];
                             }
-                        }
-                        if (prm.sketch && ((g - rbeg) & 15u) == 0) {
-                            const uint32_t q = tile_codes(sc, g - t0, 4);
-                            atomicAdd(&prm.sketch[(size_t)part_mm<MM>(mm[s], P, prm.pmode) * 256 + q], 1u);
                         }
                     }
                 }
@@ -555,6 +556,11 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
                         h.n_kmers = n;
                         h.reserved = 0;
                         bins.hdrs[bins.hdr_base[part] + (uint32_t)gb + (uint32_t)(loc[q] >> 32)] = h;
+                        if (prm.sketch && k >= 4) {          // sampled superkmers' leading 4-mers
+                            const uint32_t lead = tile_codes(csl, a, (uint32_t)min(k, 16));
+                            if (sketch_sampled(lead))
+                                atomicAdd(&prm.sketch[(size_t)part * 256 + (lead >> (2 * (min(k, 16) - 4)))], 1u);
+                        }
                         uint32_t* dst = bins.words + bins.word_base[part] + wo;
                         for (uint32_t wi = 0; wi < nw; wi++) {
                             uint32_t x = tile_codes(csl, a + wi * 16, 16);

@@ -1,15 +1,18 @@
 """Learned partition pre-sizing.
 
 A small MLP (256 -> 128 -> 128 -> 2, ReLU) reads the per-partition 4-mer
-sketch the partition kernel takes (log1p of the 4-mer counts at every 16th
-window start, by the window's partition) and predicts the partition's log
-total k-mer count and its log distinct/total ratio.  The exact total is known
+sketch the partition kernel takes during its scatter (log1p of the counts of
+the leading 4-mers of one superkmer in eight, sampled by a hash of its first
+bases) and predicts the partition's log total k-mer count and its log
+distinct/total ratio.  The exact total is known
 once the superkmers are scattered (the bins hold it), so the distinct count
-is the exact total times the predicted ratio; that count sizes the
-(k-mer, count) table of the batched counter before counting, so the
-run-length pass writes the table directly (no count pass, one host sync);
-an under-prediction is caught on the device and falls back to
-count-then-allocate (see gkc_count(GKC_BATCHED | GKC_PRESIZED)).
+is the exact total times the predicted ratio; those counts size the
+(k-mer, count) table of the batched counter before counting, so the count
+writes the table directly (no separate count pass, one host sync).  The
+binned count lays the table out densely from the exact counts it measures on
+the device, so only the sum of the predictions has to cover the table; a
+shortfall is caught on the device and falls back to count-then-allocate (see
+gkc_count(GKC_BATCHED | GKC_PRESIZED)).
 
 Training, forward, backward and inference all run in the project's own HIP
 kernels (gkc_amd/csrc/mlp.hip): FP32 master weights trained with Adam, BF16

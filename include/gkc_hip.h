@@ -215,13 +215,24 @@ void gkc_host_free(void *p);
  * (the reference's GPUReset, gkc_cuda.cu:367-373, minus the device reset). */
 int gkc_device_select(int device);
 
-/* Per-partition 4-mer sketch taken during gkc_partition: float [P][256]
- * holding log1p(count) of the 4-mer at every 16th window start of each read,
- * by the window's partition. */
+/* Per-partition 4-mer sketch taken during gkc_partition's scatter: float
+ * [P][256] holding log1p(count) of the leading 4-mers (first base most
+ * significant) of the partition's sampled superkmers, as stored in the bins
+ * (runs of windows with one minimizer, also cut where a 2048-window tile of
+ * the batch starts).  A superkmer is sampled when the 32-bit product of its
+ * first min(k, 16) bases' 2-bit codes (first base highest) and 0x9E3779B1
+ * has its top three bits zero: about one in eight, decided by its bases
+ * alone.  k < 4 takes no sketch. */
 int gkc_sketch(gkc_ctx *ctx, float *out);
 int gkc_set_tuning(gkc_ctx *ctx, const gkc_tuning *t);
 int gkc_get_tuning(gkc_ctx *ctx, gkc_tuning *t);
-/* Table capacities used by gkc_count(GKC_PRESIZED): caps[P] entries. */
+/* Table capacities used by gkc_count(GKC_PRESIZED): caps[P] entries.  The
+ * binned count (count_mode 0) lays the table out densely from the exact
+ * per-partition counts it measures on the device, so only the sum of caps
+ * must cover the table; the full LSD count (count_mode 1) gives partition q
+ * exactly caps[q] entries (unused capacity stays between partitions).  A
+ * shortfall is caught on the device and the emission is redone with exact
+ * sizes. */
 int gkc_set_table_caps(gkc_ctx *ctx, const uint32_t *caps);
 /* Per-partition k-mer totals of the bins (known exactly after gkc_partition). */
 int gkc_part_kmers(gkc_ctx *ctx, uint64_t *out);

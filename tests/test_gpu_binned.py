@@ -167,10 +167,16 @@ def test_binned_presized_fallback(gpu):
     rs = W.generate(w)
     ot = O.count(rs.reads, rs.offs, 31, 11, 8)
     exact = np.diff(ot.part_off).astype(np.uint32)
-    for name, caps, fb in [("exact", exact, 0), ("under", np.maximum(exact // 3, 1), 1)]:
+    # the binned count lays the table out densely from its exact counts: only
+    # the sum of the caps must cover the table (caps moved between partitions
+    # do not fall back); the full LSD count gives each partition its own cap
+    misplaced = (np.roll(exact, 3) + 1).astype(np.uint32)
+    assert (misplaced < exact).any() and misplaced.sum() >= exact.sum()
+    for name, caps, mode, fb in [("exact", exact, 0, 0), ("under", np.maximum(exact // 3, 1), 0, 1),
+                                 ("misplaced", misplaced, 0, 0), ("misplaced, LSD count", misplaced, 1, 1)]:
         _say("binned presized", name)
-        gt, st = _count(rs.reads, rs.offs, 31, 11, 8, mode=0, caps=caps)
-        assert st["binned"] == 1 and st["fallbacks"] == fb and st["host_syncs"] == 1 + fb
+        gt, st = _count(rs.reads, rs.offs, 31, 11, 8, mode=mode, caps=caps)
+        assert st["binned"] == (1 - mode) and st["fallbacks"] == fb and st["host_syncs"] == 1 + fb, name
         assert_tables_equal(gt, ot, 8, 31)
 
 

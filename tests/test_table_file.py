@@ -75,9 +75,9 @@ def test_rejects_bad_files(tmp_path):
 @pytest.mark.parametrize("cfg,n_parts,flags_name", [(1, 16, "plain"), (4, 8, "plain"), (2, 64, "batched"),
                                                     (1, 32, "presized")])
 def test_device_file_matches(gpu, tmp_path, cfg, n_parts, flags_name):
-    """Plain and batched counts (a presized table, its caps set above the
-    exact sizes, leaves gaps between partitions on the device: the file is
-    written run by run)."""
+    """Plain and batched counts (a presized full-LSD count, its caps set above
+    the exact sizes, leaves gaps between partitions on the device: the file
+    is written run by run)."""
     from gkc_amd import capi
     w = W.workload(cfg, scale=0.004)
     rs = W.generate(w)
@@ -86,6 +86,8 @@ def test_device_file_matches(gpu, tmp_path, cfg, n_parts, flags_name):
     ot = O.count(rs.reads, rs.offs, w.k, w.p, n_parts)
     path = tmp_path / "dev.gkct"
     with capi.Counter(w.k, w.p, n_parts) as c:
+        if flags_name == "presized":
+            c.set_tuning(count_mode=1)       # per-partition capacities (the binned count lays out densely)
         c.partition(rs.reads, rs.offs, flags=flags & capi.BATCHED)
         if flags_name == "presized":
             exact = np.diff(ot.part_off)
@@ -130,6 +132,7 @@ def test_device_file_many_chunks(gpu, tmp_path):
     print("[gpu-case] table file, many chunks", rs.n, flush=True)
     path = tmp_path / "big.gkct"
     with capi.Counter(w.k, w.p, w.n_parts) as c:
+        c.set_tuning(count_mode=1)           # per-partition capacities: gaps between partitions
         c.partition(rs.reads, rs.offs, flags=capi.BATCHED)
         pk0 = c.part_kmers()
         c.set_table_caps(np.minimum(pk0 // 3 + 64, pk0).astype(np.uint32))   # above the distinct counts here
