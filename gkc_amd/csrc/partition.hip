@@ -70,26 +70,6 @@ __device__ __forceinline__ uint32_t bad4(uint32_t v) {
 __device__ __forceinline__ uint32_t bad16(const uint4 v) {
     return bad4(v.x) | (bad4(v.y) << 4) | (bad4(v.z) << 8) | (bad4(v.w) << 12);
 }
-// Non-ACGT flags of tile bases [lo, hi) (hi - lo <= 80): nonzero when any.
-__device__ __forceinline__ uint32_t bad_any(const uint16_t* sb, uint32_t lo, uint32_t hi) {
-    uint32_t acc = 0;
-    for (uint32_t q = lo >> 4; q < (hi + 15) >> 4; q++) {
-        uint32_t m = sb[q];
-        if (q == lo >> 4) m &= 0xFFFFu << (lo & 15);
-        if (q == (hi - 1) >> 4) m &= 0xFFFFu >> (15 - ((hi - 1) & 15));
-        acc |= m;
-    }
-    return acc;
-}
-// First tile base >= lo flagged non-ACGT, or `end` when none before it.
-__device__ __forceinline__ uint32_t bad_next(const uint16_t* sb, uint32_t lo, uint32_t end) {
-    for (uint32_t q = lo >> 4; q * 16 < end; q++) {
-        uint32_t m = sb[q];
-        if (q == lo >> 4) m &= 0xFFFFu << (lo & 15);
-        if (m) return min(end, q * 16 + (uint32_t)__builtin_ctz(m));
-    }
-    return end;
-}
 // Sketch sample of a superkmer: its first min(k, 16) bases (2-bit codes,
 // first base highest) hashed by a multiplicative (Fibonacci) hash, kept when
 // the top three bits are zero (one superkmer in eight, decided by its bases
@@ -350,6 +330,37 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
         }
         __syncthreads();
 
+        // 3b. a tile holding a non-ACGT byte: for each window start, the first
+        //     such base at or after it (in the candidate array's LDS, free now):
+        //     per staged word its first flagged base, the suffix minimum over
+        //     the words (one wave), then each position from its own word
+        uint16_t* const s_nb = reinterpret_cast<uint16_t*>(s_v);
+        static_assert(P1_T * 2 + P1_NCW * 4 <= (P1_T + MAXK) * sizeof(uint32_t), "first-bad arrays fit in s_v");
+        if (tile_bad) {
+            uint32_t* const s_wf = reinterpret_cast<uint32_t*>(s_nb + P1_T);
+            if (tid < P1_NCW) { const uint32_t m = s_bad[tid]; s_wf[tid] = m ? tid * 16 + (uint32_t)__builtin_ctz(m) : 0xFFFFu; }
+            __syncthreads();
+            if (tid < GKC_WAVE) {
+                uint32_t carry = 0xFFFFu;
+                for (int q0 = ((P1_NCW - 1) / GKC_WAVE) * GKC_WAVE; q0 >= 0; q0 -= GKC_WAVE) {
+                    const int q = q0 + tid;
+                    uint32_t v = q < P1_NCW ? s_wf[q] : 0xFFFFu;
+#pragma unroll
+                    for (int off = 1; off < GKC_WAVE; off <<= 1) v = min(v, (uint32_t)__shfl_down(v, off));
+                    v = min(v, carry);
+                    if (q < P1_NCW) s_wf[q] = v;
+                    carry = __shfl(v, 0);
+                }
+            }
+            __syncthreads();
+            for (int s = 0; s < P1_V; s++) {
+                const uint32_t g = tid * P1_V + s;      // < P1_T, so word g / 16 + 1 is staged
+                const uint32_t m = (uint32_t)s_bad[g >> 4] >> (g & 15);
+                s_nb[g] = (uint16_t)(m ? g + (uint32_t)__builtin_ctz(m) : s_wf[(g >> 4) + 1]);
+            }
+            __syncthreads();
+        }
+
         // 4. validity and boundaries (chunk positions fit 32 bits)
         uint32_t bmask = 0;
         uint32_t wend[P1_V] = {};
@@ -430,15 +441,15 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
                     }
                     const uint32_t gl = g - t0;     // tile-local
                     const bool valid = g >= rbeg && g < rend && rend - g >= (uint32_t)k &&
-                                       !(tile_bad && bad_any(s_bad, gl, gl + (uint32_t)k));
+                                       !(tile_bad && s_nb[gl] < gl + (uint32_t)k);
                     if (valid) {
-                        const bool after_bad = tile_bad && g > rbeg && gl > 0 && bad_any(s_bad, gl - 1, gl);
+                        const bool after_bad = tile_bad && g > rbeg && gl > 0 && s_nb[gl - 1] == gl - 1;
                         const bool first = (g == rbeg) || (g == t0) || after_bad;
                         if (first || mm[s] != prev) bmask |= 1u << s;
                         // tile-local end of windows: the read's last window, or the window
                         // before the next non-ACGT base
                         uint32_t we = min(rend + 1 - (uint32_t)k - t0, tlen);
-                        if (tile_bad) we = min(we, bad_next(s_bad, gl + (uint32_t)k, we + (uint32_t)k - 1) + 1 - (uint32_t)k);
+                        if (tile_bad) we = min(we, (uint32_t)s_nb[gl] + 1 - (uint32_t)k);
                         wend[s] = we;
                         if (prm.mm_out) prm.mm_out[g] = (uint32_t)mm[s];
                         if constexpr (WIN) {
