@@ -572,12 +572,19 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
                             if (sketch_sampled(lead))
                                 atomicAdd(&prm.sketch[(size_t)part * 256 + (lead >> (2 * (min(k, 16) - 4)))], 1u);
                         }
+                        // word wi = bases [a + 16 wi, a + 16 wi + 16): every word of a record
+                        // has the same bit offset in the staged codes, so each is a funnel
+                        // shift of two consecutive staged words, one of them the previous one's
                         uint32_t* dst = bins.words + bins.word_base[part] + wo;
+                        const uint32_t q0 = a >> 4, sh = 2 * (a & 15);
+                        uint32_t cur = csl[q0];
                         for (uint32_t wi = 0; wi < nw; wi++) {
-                            uint32_t x = tile_codes(csl, a + wi * 16, 16);
+                            const uint32_t nxt = csl[q0 + wi + 1];
+                            uint32_t x = sh ? __builtin_amdgcn_alignbit(cur, nxt, 32 - sh) : cur;
                             const uint32_t rem = nb - wi * 16;
                             if (rem < 16) x &= ~0u << (32 - 2 * rem);     // zero past the last base
                             dst[wi] = x;
+                            cur = nxt;
                         }
                     }
                 }
