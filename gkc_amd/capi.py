@@ -21,6 +21,7 @@ from . import _lib
 DEVICE_INPUT = 1
 BATCHED = 2
 PRESIZED = 4
+LEARNED = 16
 EMIT_WINDOWS = 32
 
 _u8p = C.POINTER(C.c_uint8)
@@ -98,6 +99,9 @@ SIGNATURES = {
     "gkc_part_kmers": (C.c_int, [C.c_void_p, _u64p]),
     "gkc_parse_reads": (C.c_int64, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gkc_parse_reads_noalloc": (C.c_int64, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gkc_stream_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "gkc_set_predictor": (C.c_int, [C.c_void_p, C.c_void_p, _f32p, _f32p, _f32p, _f32p, C.c_float]),
+    "gkc_learned_caps": (C.c_int, [C.c_void_p, _u32p]),
 }
 
 _bound = None
@@ -211,6 +215,22 @@ class Counter:
         _check(lib().gkc_table_starts(self.h, _p(out, _u64p)))
         return out
 
+    def set_predictor(self, mlp_handle, x_mean, x_std, t_mean, t_std, margin: float, keep=None) -> None:
+        """Attach a learned predictor (a gkc_mlp handle and its normalisation)
+        for count(BATCHED | PRESIZED | LEARNED); mlp_handle None detaches."""
+        if mlp_handle is None:
+            _check(lib().gkc_set_predictor(self.h, None, None, None, None, None, 1.0))
+            self._pred_keep = None
+            return
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (x_mean, x_std, t_mean, t_std)]
+        _check(lib().gkc_set_predictor(self.h, mlp_handle, *[_p(a, _f32p) for a in arrs], float(margin)))
+        self._pred_keep = keep                      # the MLP must outlive its use
+
+    def learned_caps(self) -> np.ndarray:
+        out = np.zeros(self.n_parts, np.uint32)
+        _check(lib().gkc_learned_caps(self.h, _p(out, _u32p)))
+        return out
+
     def set_table_caps(self, caps) -> None:
         caps = np.ascontiguousarray(caps, np.uint32)
         _check(lib().gkc_set_table_caps(self.h, _p(caps, _u32p)))
@@ -296,6 +316,12 @@ def parse_reads(text_ptr: int, n_bytes: int, bases_ptr: int, offs_ptr: int, fmt:
     returns the number of reads.  fmt: 0 from the first byte, 1 FASTA, 2 FASTQ."""
     return _check(lib().gkc_parse_reads(C.c_void_p(text_ptr), n_bytes, fmt, C.c_void_p(bases_ptr),
                                         C.c_void_p(offs_ptr), C.c_void_p(stream)))
+
+
+def stream_copy(dst_ptr: int, src_ptr: int, nbytes: int, stream: int = 0) -> None:
+    """The library's streaming copy (device memory, 16-byte aligned sizes and
+    addresses), asynchronous on `stream`: the practical HBM ceiling."""
+    _check(lib().gkc_stream_copy(C.c_void_p(dst_ptr), C.c_void_p(src_ptr), nbytes, C.c_void_p(stream)))
 
 
 # bytes of gkc_parse_reads_noalloc's fixed device scratch (GKC_PARSE_NOALLOC_SCRATCH_BYTES)

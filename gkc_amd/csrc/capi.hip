@@ -48,6 +48,10 @@ struct HipError {
 struct ArgError { std::string msg; int code; };
 
 namespace gkc {
+// mlp.hip: learned table capacities on the device (queued on st)
+int gkc_mlp_presize_device(gkc_mlp* c, const uint32_t* sketch, const uint32_t* total, uint32_t P, const float* norm,
+                           float* x, float* y, uint32_t* caps, uint64_t* cap_sum, hipStream_t st);
+
 // ingest.hip: text -> read CSR on the device (0, or a GKC_E* code with err set)
 int parse_reads(const char* text, uint64_t n, int format, char* bases, uint64_t* offs, hipStream_t st,
                 int64_t* n_reads, std::string& err);
@@ -135,6 +139,13 @@ struct gkc_ctx {
     const void* g_emit_p_tc = nullptr;
     std::vector<const void*> g_key;
     std::vector<uint32_t> presize_caps;
+    // learned presizing on the device (gkc_set_predictor): the MLP, its
+    // normalisation + margin, feature / output buffers, and the capacities'
+    // sum read back into pinned memory while the sort runs
+    gkc_mlp* mlp = nullptr;
+    DevBuf d_pnorm, d_px, d_py, d_lcaps;
+    uint64_t* h_lsum = nullptr;
+    hipEvent_t ev_lsum = nullptr;
     bool counted = false;
     gkc_stats stats{};
 };
@@ -266,8 +277,11 @@ static uint64_t stage_batch(gkc_ctx* c, const char* reads, const uint64_t* offs,
     const uint64_t* doffs;
     uint32_t lead = 0;
     if (flags & GKC_DEVICE_INPUT) {
-        HIPX(hipMemcpy(&o0, offs, 8, hipMemcpyDeviceToHost));
-        HIPX(hipMemcpy(&on, offs + n, 8, hipMemcpyDeviceToHost));
+        // on the context's own (non-blocking) stream: a null-stream copy would
+        // also wait for the caller's work on other streams (its next upload)
+        HIPX(hipMemcpyAsync(&o0, offs, 8, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipMemcpyAsync(&on, offs + n, 8, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipStreamSynchronize(c->st));
         // align the base pointer down to 16 bytes; positions [0, lead) belong to no read
         lead = (uint32_t)(reinterpret_cast<uintptr_t>(reads) & 15);
         dreads = reinterpret_cast<const uint8_t*>(reads) - lead;
@@ -437,6 +451,8 @@ void gkc_destroy(gkc_ctx* c) {
     if (c->g_p1) (void)hipGraphExecDestroy(c->g_p1);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : c->evp) if (e) (void)hipEventDestroy(e);
+    if (c->ev_lsum) (void)hipEventDestroy(c->ev_lsum);
+    if (c->h_lsum) (void)hipHostFree(c->h_lsum);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
 }
@@ -911,7 +927,8 @@ int gkc_sketch(gkc_ctx* c, float* out) {
     return guarded(c, [&] {
         const uint32_t P = c->prm.n_parts;
         std::vector<uint32_t> h((size_t)P * 256);
-        HIPX(hipMemcpy(h.data(), c->d_sketch.p, h.size() * 4, hipMemcpyDeviceToHost));
+        HIPX(hipMemcpyAsync(h.data(), c->d_sketch.p, h.size() * 4, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipStreamSynchronize(c->st));
         for (size_t i = 0; i < h.size(); i++) out[i] = std::log1p((float)h[i]);
         return 0;
     });
@@ -941,6 +958,38 @@ int gkc_get_tuning(gkc_ctx* c, gkc_tuning* t) {
 int gkc_set_table_caps(gkc_ctx* c, const uint32_t* caps) {
     return guarded(c, [&] {
         c->presize_caps.assign(caps, caps + c->prm.n_parts);
+        return 0;
+    });
+}
+
+int gkc_learned_caps(gkc_ctx* c, uint32_t* caps) {
+    return guarded(c, [&] {
+        if (!c->mlp || !caps) throw ArgError{"no predictor set", GKC_ESTATE};
+        HIPX(hipMemcpyAsync(caps, c->d_lcaps.p, (size_t)c->prm.n_parts * 4, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipStreamSynchronize(c->st));
+        return 0;
+    });
+}
+
+int gkc_set_predictor(gkc_ctx* c, gkc_mlp* mlp, const float* x_mean, const float* x_std, const float* t_mean,
+                      const float* t_std, float margin) {
+    return guarded(c, [&] {
+        if (!mlp) { c->mlp = nullptr; return 0; }
+        if (!x_mean || !x_std || !t_mean || !t_std || !(margin >= 1.0f))
+            throw ArgError{"predictor: normalisation arrays and a margin >= 1 are required", GKC_EINVAL};
+        if (c->prm.n_parts > 1024) throw ArgError{"predictor: at most 1024 partitions", GKC_EINVAL};
+        std::vector<float> nm(2 * 256 + 5);
+        std::copy(x_mean, x_mean + 256, nm.begin());
+        std::copy(x_std, x_std + 256, nm.begin() + 256);
+        nm[512] = t_mean[0]; nm[513] = t_mean[1]; nm[514] = t_std[0]; nm[515] = t_std[1]; nm[516] = margin;
+        c->d_pnorm.ensure(nm.size() * 4);
+        HIPX(hipMemcpyAsync(c->d_pnorm.p, nm.data(), nm.size() * 4, hipMemcpyHostToDevice, c->st));
+        const uint32_t P = c->prm.n_parts;
+        c->d_px.ensure((size_t)P * 256 * 4); c->d_py.ensure((size_t)P * 2 * 4); c->d_lcaps.ensure((size_t)P * 4);
+        if (!c->h_lsum) HIPX(hipHostMalloc((void**)&c->h_lsum, 8, hipHostMallocDefault));
+        if (!c->ev_lsum) HIPX(hipEventCreateWithFlags(&c->ev_lsum, hipEventDisableTiming));
+        HIPX(hipStreamSynchronize(c->st));
+        c->mlp = mlp;
         return 0;
     });
 }
@@ -1518,8 +1567,11 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     set_lists(c, P, c->h_kmer_cur.data(), c->h_hdr_cur.data(), c->tune.single_list == 1 ? 1 : NLIST,
               SORT_NT * sort_items<K>(c), bs.rl_tile);
 
-    // table capacities: predicted (presized) or a count pass (count-then-allocate)
-    const bool presized = (flags & GKC_PRESIZED) && c->presize_caps.size() == P;
+    // table capacities: predicted on the device (learned), given by the caller
+    // (presized) or from a count pass (count-then-allocate)
+    const bool learned = (flags & GKC_PRESIZED) && (flags & GKC_LEARNED) && c->mlp;
+    if ((flags & GKC_LEARNED) && !c->mlp) throw ArgError{"GKC_LEARNED without gkc_set_predictor", GKC_ESTATE};
+    const bool presized = learned || ((flags & GKC_PRESIZED) && c->presize_caps.size() == P);
     (void)occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu, (size_t)bs.npass * RADIX * sizeof(uint32_t));
     (void)sort_grid<K>(c);
     (void)occupancy_blocks((const void*)k_runlength<K>, SORT_NT, c->n_cu);
@@ -1530,6 +1582,14 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     (void)occupancy_blocks((const void*)k_midcount<1>, GKC_WAVE, c->n_cu);
     (void)occupancy_blocks((const void*)k_wavecopy<K>, 256, c->n_cu);
     HIPX(hipEventRecord(c->ev[0], c->st));
+    if (learned) {      // capacities on the device, ahead of the sort; only their sum comes back, while it runs
+        if (gkc_mlp_presize_device(c->mlp, c->d_sketch.as<uint32_t>(), c->d_kmer_cur.as<uint32_t>(), P,
+                                   c->d_pnorm.as<float>(), c->d_px.as<float>(), c->d_py.as<float>(),
+                                   c->d_lcaps.as<uint32_t>(), c->d_tcap_total.as<uint64_t>(), c->st) != 0)
+            throw ArgError{std::string("learned presizing: ") + gkc_mlp_last_error(), GKC_EHIP};
+        HIPX(hipMemcpyAsync(c->h_lsum, c->d_tcap_total.p, 8, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipEventRecord(c->ev_lsum, c->st));
+    }
     std::vector<const void*> key = {c->keysA.p, c->keysB.p, c->lookback.p, c->words.p, c->hdrs.p, c->d_ranges.p,
                                     c->d_hlist.p, c->d_hres.p, c->d_wkept.p,
                                     (const void*)(uintptr_t)tiles_cap, (const void*)(uintptr_t)bs.npass,
@@ -1555,7 +1615,14 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     std::vector<uint32_t>& caps = c->h_caps;
     caps.assign(P, 0);
     uint32_t flags_dev[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (presized) {
+    if (learned) {
+        HIPX(hipEventSynchronize(c->ev_lsum));      // long done: the sort graph is running behind it
+        if (!bs.wave) {                             // per-partition capacities for the run-length count
+            HIPX(hipMemcpyAsync(caps.data(), c->d_lcaps.p, P * 4, hipMemcpyDeviceToHost, c->st));
+            HIPX(hipStreamSynchronize(c->st));
+            c->stats.host_syncs++;
+        }
+    } else if (presized) {
         caps = c->presize_caps;
     } else {
         if (timed) enqueue_runlength_all<K>(c, sa, bs, sorted, rl_tiles_cap, 0, true);
@@ -1573,9 +1640,11 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     c->h_tstart.assign(P, 0);
     uint64_t tot_cap = 0;
     for (uint32_t q = 0; q < P; q++) { c->h_tstart[q] = tot_cap; tot_cap += caps[q]; }
+    if (learned && dense) tot_cap = *c->h_lsum;     // the device holds the sum (and the layout reads it there)
     c->tkeys.ensure((tot_cap + 1) * sizeof(K));
     c->tcnts.ensure((tot_cap + 1) * 4);
-    if (dense) {
+    if (learned && dense) {
+    } else if (dense) {
         c->h_tcap_total = tot_cap;
         HIPX(hipMemcpyAsync(c->d_tcap_total.p, &c->h_tcap_total, 8, hipMemcpyHostToDevice, c->st));
     } else {

@@ -185,6 +185,39 @@ __global__ __launch_bounds__(64) void k_mlp_infer_bf16(const float* __restrict__
     }
 }
 
+// Features of the device predictor: x[p][i] = (log1p(sketch[p][i]) - mean[i]) / sd[i]
+// (the normalisation the predictor was trained with), one workgroup per partition.
+__global__ __launch_bounds__(NI) void k_presize_features(const uint32_t* __restrict__ sketch, const float* __restrict__ norm,
+                                                         float* __restrict__ x) {
+    const int p = blockIdx.x, i = threadIdx.x;
+    x[(long)p * NI + i] = (log1pf((float)sketch[(long)p * NI + i]) - norm[i]) / norm[NI + i];
+}
+
+// Table capacities from the predictions: partition p holds total[p] k-mers;
+// its distinct count is total * exp(min(log ratio, 0)) with the log ratio
+// de-normalised from output 1; capacity = min(ceil(distinct * margin) + 16,
+// total).  Also their sum (one workgroup of 1024 threads, P <= 1024).
+// norm: [0, NI) mean, [NI, 2 NI) sd, then t_mean[2], t_sd[2], margin.
+__global__ __launch_bounds__(1024) void k_presize_caps(const float* __restrict__ y, const uint32_t* __restrict__ total,
+                                                       uint32_t P, const float* __restrict__ norm,
+                                                       uint32_t* __restrict__ caps, uint64_t* __restrict__ cap_sum) {
+    __shared__ unsigned long long s_sum;
+    const uint32_t p = threadIdx.x;
+    if (p == 0) s_sum = 0;
+    __syncthreads();
+    if (p < P) {
+        const float* tn = norm + 2 * NI;
+        const float lr = y[p * NO + 1] * tn[3] + tn[1];
+        const double t = (double)total[p];
+        const double d = t * (double)expf(fminf(lr, 0.f));
+        const double c = fmin(ceil(d * (double)tn[4]) + 16.0, t);
+        caps[p] = (uint32_t)c;
+        atomicAdd(&s_sum, (unsigned long long)c);
+    }
+    __syncthreads();
+    if (p == 0) *cap_sum = s_sum;
+}
+
 // Plain matrix product through the same tile routine, for checking the
 // operand layout: C[32][N] = A[32][K] * B[K][N] (bf16 in, fp32 out).
 __global__ __launch_bounds__(64) void k_bf16_matmul32(const float* __restrict__ A, const unsigned short* __restrict__ B, int K,
@@ -443,6 +476,34 @@ int gkc_mlp_forward_bf16(gkc_mlp* c, const float* x, int64_t n, float* y) {
     MLPX(hipStreamSynchronize(c->st));
     return 0;
 }
+
+}  // extern "C"
+
+// Learned table presizing on the device, queued on `st` (no host round trip):
+// features from the partition sketch into x[P][256], BF16 MFMA inference into
+// y[P][2], capacities and their sum (x, y, caps, cap_sum: the caller's device
+// buffers).  `norm` is device memory in k_presize_caps' layout.  Only the
+// MLP's weights are read (their BF16 copy is refreshed first when stale).
+namespace gkc {
+int gkc_mlp_presize_device(gkc_mlp* c, const uint32_t* sketch, const uint32_t* total, uint32_t P, const float* norm,
+                           float* x, float* y, uint32_t* caps, uint64_t* cap_sum, hipStream_t st) {
+    if (!c || P == 0 || P > 1024) return mlp_fail("presize: bad arguments", GKC_EINVAL);
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->wb_valid) {
+        hipLaunchKernelGGL(k_to_bf16, dim3((NPARAM + 255) / 256), dim3(256), 0, st, c->w.p, c->wb, NPARAM);
+        MLPX(hipGetLastError());
+        c->wb_valid = true;
+    }
+    hipLaunchKernelGGL(k_presize_features, dim3(P), dim3(NI), 0, st, sketch, norm, x);
+    hipLaunchKernelGGL(k_mlp_infer_bf16, dim3((P + 31) / 32), dim3(64), 0, st, (const float*)x, (int)P, c->wb + OFF_W1,
+                       c->w.p + OFF_B1, c->wb + OFF_W2, c->w.p + OFF_B2, c->wb + OFF_W3, c->w.p + OFF_B3, y);
+    hipLaunchKernelGGL(k_presize_caps, dim3(1), dim3(1024), 0, st, (const float*)y, total, P, norm, caps, cap_sum);
+    MLPX(hipGetLastError());
+    return 0;
+}
+}  // namespace gkc
+
+extern "C" {
 
 // C[32][N] = A[32][K] * B[K][N] through the inference kernel's MFMA tile
 // routine (inputs rounded to bf16).  K <= 256, K % 16 == 0.

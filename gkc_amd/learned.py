@@ -152,6 +152,12 @@ class Predictor:
         base = tot if total is None else np.asarray(total, np.float64)
         return tot, base * np.exp(np.minimum(logs[:, 1], 0.0))
 
+    def attach(self, counter) -> None:
+        """Run this predictor on the device inside counter's counts:
+        count(BATCHED | PRESIZED | LEARNED) then needs no host round trip
+        for its capacities (see gkc_set_predictor)."""
+        counter.set_predictor(self.mlp.h, self.x_mean, self.x_std, self.t_mean, self.t_std, self.margin, keep=self)
+
     def caps(self, counter) -> np.ndarray:
         """Table capacities for counter's current bins: the exact k-mer total
         times the predicted ratio times the margin, never above the total."""

@@ -37,6 +37,8 @@ enum {
 #define GKC_PRESIZED       4u   /* gkc_count: size tables from gkc_set_table_caps / the learned
                                    predictor instead of a count pass (overflow falls back) */
 #define GKC_SYNC_ONLY      8u   /* internal: no host sync beyond what the mode needs */
+#define GKC_LEARNED       16u   /* gkc_count with GKC_PRESIZED: capacities from the predictor set by
+                                   gkc_set_predictor, computed on the device before the sort */
 #define GKC_EMIT_WINDOWS  32u   /* gkc_partition: also write per-window (minimizer, position, strand) */
 
 /* Counting parameters (the reference's PAR fields K_kmer, P_minimizer, HPC,
@@ -53,6 +55,7 @@ typedef struct gkc_params {
 } gkc_params;
 
 typedef struct gkc_ctx gkc_ctx;
+typedef struct gkc_mlp gkc_mlp;     /* the learned predictor (see below) */
 
 /* Per-stage device times of the last calls (milliseconds, HIP events). */
 typedef struct gkc_stats {
@@ -215,6 +218,14 @@ void gkc_host_free(void *p);
  * (the reference's GPUReset, gkc_cuda.cu:367-373, minus the device reset). */
 int gkc_device_select(int device);
 
+/* ---- measurement (no reference counterpart) ----
+ * Copy `bytes` (a multiple of 16; both pointers 16-byte aligned, device
+ * memory) on `stream` (hipStream_t, NULL: default) with the library's
+ * streaming-copy kernel: the practical read + write ceiling the benchmark
+ * reports its kernels against, and a kernel of known traffic for HBM byte
+ * counters.  Asynchronous; GKC_EINVAL for misaligned or odd sizes. */
+int gkc_stream_copy(void *dst, const void *src, uint64_t bytes, void *stream);
+
 /* Per-partition 4-mer sketch taken during gkc_partition's scatter: float
  * [P][256] holding log1p(count) of the leading 4-mers (first base most
  * significant) of the partition's sampled superkmers, as stored in the bins
@@ -234,6 +245,18 @@ int gkc_get_tuning(gkc_ctx *ctx, gkc_tuning *t);
  * shortfall is caught on the device and the emission is redone with exact
  * sizes. */
 int gkc_set_table_caps(gkc_ctx *ctx, const uint32_t *caps);
+/* Learned presizing on the device: gkc_count(GKC_BATCHED | GKC_PRESIZED |
+ * GKC_LEARNED) computes each partition's capacity from the sketch with `mlp`
+ * (BF16 matrix-core inference) on the context's stream before sorting:
+ * features (log1p(sketch) - x_mean) / x_std, output 1 de-normalised as
+ * y * t_std[1] + t_mean[1] = log(distinct / total), capacity
+ * min(ceil(total * exp(min(that, 0)) * margin) + 16, total).  Only their sum
+ * is read back, while the sort runs.  The MLP must outlive its use; NULL
+ * detaches.  At most 1024 partitions. */
+int gkc_set_predictor(gkc_ctx *ctx, gkc_mlp *mlp, const float *x_mean, const float *x_std, const float *t_mean,
+                      const float *t_std, float margin);
+/* The capacities (caps[P]) the last GKC_LEARNED count computed on the device. */
+int gkc_learned_caps(gkc_ctx *ctx, uint32_t *caps);
 /* Per-partition k-mer totals of the bins (known exactly after gkc_partition). */
 int gkc_part_kmers(gkc_ctx *ctx, uint64_t *out);
 
@@ -241,7 +264,6 @@ int gkc_part_kmers(gkc_ctx *ctx, uint64_t *out);
  * reference sizes nothing ahead, kmer_counting.cu:95-191 grows host vectors)
  * MLP 256 -> 128 -> 128 -> 2 with ReLU, FP32 master weights; flat parameter
  * order W1[256][128], b1[128], W2[128][128], b2[128], W3[128][2], b3[2]. */
-typedef struct gkc_mlp gkc_mlp;
 uint64_t gkc_mlp_n_params(void);
 const char *gkc_mlp_last_error(void);
 gkc_mlp *gkc_mlp_create(int device, uint64_t seed);

@@ -63,3 +63,35 @@ def test_learned_presizing_gain(gpu, cid, scale, g):
     under = int(np.sum(caps < exact))
     print(f"[learned] caps/exact {caps.sum() / exact.sum():.3f} caps/bound {caps.sum() / total.sum():.3f} "
           f"under-predicted {under}/{len(exact)} fallbacks {got['fallbacks']}", flush=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cid,scale,mode", [(2, 0.01, 0), (5, 0.004, 0), (1, 0.2, 1)])
+def test_learned_presizing_on_device(gpu, cid, scale, mode):
+    """The predictor attached to the counter runs on the device inside the
+    count (BF16 MFMA inference from the device sketch): its capacities match
+    the host-side predictor's, the table is bit-exact, and the binned count
+    needs only its final host sync (the full-LSD count also reads the
+    capacities back)."""
+    pred = learned.Predictor.load_default()
+    if pred is None:
+        pytest.skip("no trained model committed")
+    w = W.workload(cid, scale=scale)
+    rs = W.generate(w, g=104)
+    ot = O.count(rs.reads, rs.offs, w.k, w.p, w.n_parts)
+    print("[gpu-case] learned on device", w.name, mode, flush=True)
+    with capi.Counter(w.k, w.p, w.n_parts) as c:
+        c.set_tuning(count_mode=mode)
+        c.partition(rs.reads, rs.offs, flags=capi.BATCHED)
+        host_caps = pred.caps(c).astype(np.int64)
+        pred.attach(c)
+        for rep in range(2):                       # graph replays
+            c.count(capi.BATCHED | capi.PRESIZED | capi.LEARNED)
+            st = c.stats()
+            assert_tables_equal(c.table(), ot, w.n_parts, w.k)
+        dev_caps = c.learned_caps().astype(np.int64)
+    # the same arithmetic on the device (log1pf / expf, f32) as on the host (numpy)
+    assert np.all(np.abs(dev_caps - host_caps) <= host_caps // 200 + 2)
+    assert abs(int(dev_caps.sum()) - int(host_caps.sum())) <= int(host_caps.sum()) // 1000 + 16
+    assert st["fallbacks"] == 0
+    assert st["host_syncs"] == (1 if mode == 0 else 2)
