@@ -23,6 +23,8 @@ BATCHED = 2
 PRESIZED = 4
 LEARNED = 16
 EMIT_WINDOWS = 32
+WITH_POS = 64
+GKC_ECAP = -5
 
 _u8p = C.POINTER(C.c_uint8)
 _u32p = C.POINTER(C.c_uint32)
@@ -102,6 +104,9 @@ SIGNATURES = {
     "gkc_stream_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "gkc_set_predictor": (C.c_int, [C.c_void_p, C.c_void_p, _f32p, _f32p, _f32p, _f32p, C.c_float]),
     "gkc_learned_caps": (C.c_int, [C.c_void_p, _u32p]),
+    "gkc_superkmer_positions": (C.c_int64, [C.c_void_p, C.c_int32, _u32p, _u32p, _u32p, C.c_uint64]),
+    "gkc_kmer_positions": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, _u64p, _u32p, _u64p, _u64p, C.c_uint64,
+                                     C.c_uint64, _u64p, _u64p]),
 }
 
 _bound = None
@@ -253,6 +258,32 @@ class Counter:
         out = np.zeros(self.n_parts, np.uint64)
         _check(lib().gkc_part_kmers(self.h, _p(out, _u64p)))
         return out
+
+    def superkmer_positions(self, part: int):
+        """Partition `part`'s superkmers after partition(..., WITH_POS):
+        (read, offset of the first base in its read, k-mers), reads numbered
+        from 0 at the last reset."""
+        n = _check(lib().gkc_superkmer_positions(self.h, part, None, None, None, 0))
+        rd, of, nk = (np.zeros(max(n, 1), np.uint32) for _ in range(3))
+        _check(lib().gkc_superkmer_positions(self.h, part, _p(rd, _u32p), _p(of, _u32p), _p(nk, _u32p), n))
+        return rd[:n], of[:n], nk[:n]
+
+    def kmer_positions(self, part: int, sample_mod: int = 1):
+        """After count() on WITH_POS bins (k <= 32): the partition's sampled
+        k-mers (mix64(key) % sample_mod == 0) as (keys, counts, pos_off,
+        positions); positions[pos_off[e]:pos_off[e + 1]] are k-mer e's
+        read << 32 | offset << 1 | strand, ascending."""
+        ne, npos = np.zeros(1, np.uint64), np.zeros(1, np.uint64)
+        r = lib().gkc_kmer_positions(self.h, part, sample_mod, None, None, None, None, 0, 0, _p(ne, _u64p),
+                                     _p(npos, _u64p))
+        if r != GKC_ECAP:
+            _check(r)
+        e, q = int(ne[0]), int(npos[0])
+        keys, cnts = np.zeros(max(e, 1), np.uint64), np.zeros(max(e, 1), np.uint32)
+        off, pos = np.zeros(e + 1, np.uint64), np.zeros(max(q, 1), np.uint64)
+        _check(lib().gkc_kmer_positions(self.h, part, sample_mod, _p(keys, _u64p), _p(cnts, _u32p), _p(off, _u64p),
+                                        _p(pos, _u64p), e, q, _p(ne, _u64p), _p(npos, _u64p)))
+        return keys[:e], cnts[:e], off, pos[:q]
 
     def synchronize(self) -> None:
         _check(lib().gkc_synchronize(self.h))

@@ -106,6 +106,9 @@ struct gkc_ctx {
     DevBuf d_win_mm, d_win_pos, d_win_str;     // per-window records of the last partition call
     DevBuf d_backup;                           // cursors before a presized scatter (3 x P)
     uint64_t bases_done = 0;                   // bases partitioned since the last reset
+    uint64_t reads_done = 0;                   // reads partitioned since the last reset
+    int pos_mode = 0;                          // since the last reset: 0 no batch, 1 with positions, 2 without
+    DevBuf skmpos;                             // GKC_WITH_POS: {read, offset} beside each superkmer header
     uint32_t p1_fallbacks = 0;                 // presized scatters redone since the last reset
     gkc_tuning tune{};                         // kernel variants picked by the autotuner
     DevBuf d_wl;                               // work lists of the phase-2 kernels
@@ -243,10 +246,18 @@ static void bins_reserve(gkc_ctx* c, const std::vector<uint32_t>& need_w, const 
         if (c->h_hdr_cur[q]) HIPX(hipMemcpyAsync(nh.as<SkmHdr>() + nhb[q], c->hdrs.as<SkmHdr>() + c->h_hdr_base[q],
                                                  (size_t)c->h_hdr_cur[q] * sizeof(SkmHdr), hipMemcpyDeviceToDevice, c->st));
     }
+    DevBuf np;
+    if (c->pos_mode == 1) {         // superkmer positions follow the header layout
+        np.ensure(th * sizeof(uint2) + 16);
+        for (uint32_t q = 0; q < P; q++)
+            if (c->h_hdr_cur[q]) HIPX(hipMemcpyAsync(np.as<uint2>() + nhb[q], c->skmpos.as<uint2>() + c->h_hdr_base[q],
+                                                     (size_t)c->h_hdr_cur[q] * sizeof(uint2), hipMemcpyDeviceToDevice, c->st));
+    }
     HIPX(hipMemsetAsync(static_cast<char*>(nw.p) + tw * 4, 0, 32, c->st));
     HIPX(hipStreamSynchronize(c->st));
     std::swap(c->words.p, nw.p); std::swap(c->words.bytes, nw.bytes);
     std::swap(c->hdrs.p, nh.p); std::swap(c->hdrs.bytes, nh.bytes);
+    if (c->pos_mode == 1) { std::swap(c->skmpos.p, np.p); std::swap(c->skmpos.bytes, np.bytes); }
     c->h_word_base = nwb; c->h_hdr_base = nhb; c->h_word_cap = nwc; c->h_hdr_cap = nhc;
     HIPX(hipMemcpyAsync(c->d_word_base.p, nwb.data(), P * 8, hipMemcpyHostToDevice, c->st));
     HIPX(hipMemcpyAsync(c->d_hdr_base.p, nhb.data(), P * 8, hipMemcpyHostToDevice, c->st));
@@ -266,6 +277,7 @@ static BinSet binset(gkc_ctx* c, bool counting) {
         b.word_cur = c->d_word_cur.as<uint32_t>(); b.hdr_cur = c->d_hdr_cur.as<uint32_t>(); b.kmer_cur = c->d_kmer_cur.as<uint32_t>();
     }
     b.overflow = c->d_overflow.as<uint32_t>();
+    b.pos = c->pos_mode == 1 ? c->skmpos.as<uint2>() : nullptr;
     return b;
 }
 
@@ -300,6 +312,8 @@ static uint64_t stage_batch(gkc_ctx* c, const char* reads, const uint64_t* offs,
     if (B + lead >= (1ull << 31)) throw ArgError{"a batch must hold fewer than 2^31 bases", GKC_EINVAL};
     cd.reads = dreads; cd.offs = doffs; cd.lens = nullptr; cd.base = o0 - lead;
     cd.n_reads = (uint32_t)n; cd.n_bases = (uint32_t)(B + lead); cd.n_tiles = cdiv(B + lead, P1_T); cd.lead = lead;
+    if (c->reads_done + (uint64_t)n >= (1ull << 32)) throw ArgError{"more than 2^32 reads since the last reset", GKC_EINVAL};
+    cd.read0 = (uint32_t)c->reads_done;
     if (c->prm.hpc && n > 0) {
         // compressed reads keep the input's layout, lead included: read r's runs
         // start at the same position, so the tiles index them like the input
@@ -473,6 +487,8 @@ int gkc_reset(gkc_ctx* c) {
         zero(c, c->d_word_cur.p, P); zero(c, c->d_hdr_cur.p, P); zero(c, c->d_kmer_cur.p, P);
         zero(c, c->d_sketch.p, (uint64_t)P * 256); zero(c, c->d_overflow.p, 4);
         c->bases_done = 0;
+        c->reads_done = 0;
+        c->pos_mode = 0;
         c->p1_fallbacks = 0;
         c->counted = false;
         HIPX(hipStreamSynchronize(c->st));
@@ -676,7 +692,8 @@ static bool partition_presized(gkc_ctx* c, const ChunkDesc& cd) {
     }
     bins_reserve(c, nw, nh);
     c->d_backup.ensure((size_t)(3 + 256) * P * 4);     // cursors, then the sketch
-    std::vector<const void*> key = {c->words.p, c->hdrs.p, c->d_chunk.p, c->d_backup.p};
+    std::vector<const void*> key = {c->words.p, c->hdrs.p, c->d_chunk.p, c->d_backup.p, c->skmpos.p,
+                                    (const void*)(uintptr_t)c->pos_mode};
     auto enqueue = [&] {
         hipLaunchKernelGGL(k_copy_u32, dim3((P + 255) / 256), dim3(256), 0, c->st, c->d_backup.as<uint32_t>(),
                            c->d_word_cur.as<uint32_t>(), c->d_hdr_cur.as<uint32_t>(), c->d_kmer_cur.as<uint32_t>(), P);
@@ -723,6 +740,15 @@ int gkc_partition(gkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n
     return guarded(c, [&] {
         if (n < 0 || !offs || (n > 0 && !reads)) throw ArgError{"bad batch", GKC_EINVAL};
         if (n == 0) return 0;
+        const int mode = (flags & GKC_WITH_POS) ? 1 : 2;
+        if (c->pos_mode && c->pos_mode != mode)
+            throw ArgError{"every batch between resets must agree on GKC_WITH_POS", GKC_EINVAL};
+        if (mode == 1 && c->pos_mode == 0) {        // no headers yet: a positions region for the current capacity
+            uint64_t th = 0;
+            for (uint32_t q = 0; q < c->prm.n_parts; q++) th += c->h_hdr_cap[q];
+            c->skmpos.ensure(th * sizeof(uint2) + 16);
+        }
+        c->pos_mode = mode;
         ChunkDesc cd;
         HIPX(hipEventRecord(c->ev[0], c->st));
         stage_batch(c, reads, offs, n, flags, cd);
@@ -731,6 +757,7 @@ int gkc_partition(gkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n
             done = partition_presized(c, cd);        // a failed attempt restores the cursors and the sketch
         if (!done) partition_plain(c, cd, flags);
         c->bases_done += cd.n_bases - cd.lead;
+        c->reads_done += (uint64_t)n;
         HIPX(hipEventRecord(c->ev[1], c->st));
         HIPX(hipEventSynchronize(c->ev[1]));
         float ms = 0;
@@ -911,6 +938,84 @@ int gkc_table_device(gkc_ctx* c, void** keys, uint32_t** counts, uint64_t* n_ent
         if (keys) *keys = c->tkeys.p;
         if (counts) *counts = c->tcnts.as<uint32_t>();
         if (n_entries) *n_entries = c->part_off[c->prm.n_parts];
+        return 0;
+    });
+}
+
+int64_t gkc_superkmer_positions(gkc_ctx* c, int32_t part, uint32_t* read, uint32_t* offset, uint32_t* n_kmers,
+                                uint64_t cap) {
+    return guarded(c, [&]() -> int64_t {
+        if (part < 0 || (uint32_t)part >= c->prm.n_parts) throw ArgError{"partition out of range", GKC_EINVAL};
+        if (c->pos_mode != 1) throw ArgError{"the bins hold no positions (partition with GKC_WITH_POS)", GKC_ESTATE};
+        const uint32_t n = c->h_hdr_cur[part];
+        if (!read && !offset && !n_kmers) return (int64_t)n;
+        if (n > cap) throw ArgError{"output buffers too small for the partition's superkmers", GKC_ECAP};
+        if (n == 0) return 0;
+        std::vector<SkmHdr> hd(n);
+        std::vector<uint2> ps(n);
+        HIPX(hipMemcpyAsync(hd.data(), c->hdrs.as<SkmHdr>() + c->h_hdr_base[part], n * sizeof(SkmHdr), hipMemcpyDeviceToHost, c->st));
+        HIPX(hipMemcpyAsync(ps.data(), c->skmpos.as<uint2>() + c->h_hdr_base[part], n * sizeof(uint2), hipMemcpyDeviceToHost, c->st));
+        HIPX(hipStreamSynchronize(c->st));
+        for (uint32_t i = 0; i < n; i++) {
+            if (read) read[i] = ps[i].x;
+            if (offset) offset[i] = ps[i].y;
+            if (n_kmers) n_kmers[i] = hd[i].n_kmers;
+        }
+        return (int64_t)n;
+    });
+}
+
+int gkc_kmer_positions(gkc_ctx* c, int32_t part, uint32_t sample_mod, uint64_t* keys, uint32_t* counts, uint64_t* pos_off,
+                       uint64_t* positions, uint64_t entry_cap, uint64_t pos_cap, uint64_t* n_entries, uint64_t* n_positions) {
+    return guarded(c, [&] {
+        if (part < 0 || (uint32_t)part >= c->prm.n_parts) throw ArgError{"partition out of range", GKC_EINVAL};
+        if (sample_mod == 0) throw ArgError{"sample_mod must be at least 1", GKC_EINVAL};
+        if (c->keyw != 8) throw ArgError{"k-mer positions are listed for k <= 32", GKC_EINVAL};
+        if (c->pos_mode != 1) throw ArgError{"the bins hold no positions (partition with GKC_WITH_POS)", GKC_ESTATE};
+        if (!c->counted) throw ArgError{"count the bins first (gkc_count)", GKC_ESTATE};
+        // the partition's table: sampled entries and their ranges of positions
+        const uint64_t a = c->part_start[part], ne = c->part_off[part + 1] - c->part_off[part];
+        std::vector<uint64_t> tk(ne), eo(ne, ~0ull);
+        std::vector<uint32_t> tc(ne);
+        if (ne) {
+            HIPX(hipMemcpyAsync(tk.data(), c->tkeys.as<uint64_t>() + a, ne * 8, hipMemcpyDeviceToHost, c->st));
+            HIPX(hipMemcpyAsync(tc.data(), c->tcnts.as<uint32_t>() + a, ne * 4, hipMemcpyDeviceToHost, c->st));
+            HIPX(hipStreamSynchronize(c->st));
+        }
+        uint64_t n_e = 0, n_p = 0;
+        for (uint64_t i = 0; i < ne; i++)
+            if (mix64(tk[i]) % sample_mod == 0) { eo[i] = n_p; n_p += tc[i]; n_e++; }
+        *n_entries = n_e; *n_positions = n_p;
+        if (n_e > entry_cap || n_p > pos_cap) throw ArgError{"output buffers too small (sizes returned)", GKC_ECAP};
+        if (n_e == 0) { if (pos_off) pos_off[0] = 0; return 0; }
+        DevBuf d_eo, d_cur, d_out;
+        d_eo.ensure(ne * 8); d_cur.ensure(ne * 4); d_out.ensure(n_p * 8 + 8);
+        HIPX(hipMemcpyAsync(d_eo.p, eo.data(), ne * 8, hipMemcpyHostToDevice, c->st));
+        zero(c, d_cur.p, ne);
+        const uint32_t nh = c->h_hdr_cur[part];
+        hipLaunchKernelGGL(k_pos_scatter, dim3(std::max<uint32_t>(1, std::min<uint32_t>((nh + 255) / 256, 8 * c->n_cu))),
+                           dim3(256), 0, c->st, (const SkmHdr*)(c->hdrs.as<SkmHdr>() + c->h_hdr_base[part]),
+                           (const uint2*)(c->skmpos.as<uint2>() + c->h_hdr_base[part]), nh,
+                           (const uint32_t*)(c->words.as<uint32_t>() + c->h_word_base[part]), c->prm.k, sample_mod,
+                           (const uint64_t*)(c->tkeys.as<uint64_t>() + a), (const uint32_t*)(c->tcnts.as<uint32_t>() + a),
+                           ne, (const uint64_t*)d_eo.as<uint64_t>(),
+                           d_cur.as<uint32_t>(), d_out.as<uint64_t>());
+        HIPX(hipGetLastError());
+        std::vector<uint64_t> out(n_p);
+        HIPX(hipMemcpyAsync(out.data(), d_out.p, n_p * 8, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipStreamSynchronize(c->st));
+        // each k-mer's occurrences, claimed in no set order on the device, in ascending order
+        uint64_t e = 0;
+        for (uint64_t i = 0; i < ne; i++) {
+            if (eo[i] == ~0ull) continue;
+            std::sort(out.begin() + eo[i], out.begin() + eo[i] + tc[i]);
+            if (keys) keys[e] = tk[i];
+            if (counts) counts[e] = tc[i];
+            if (pos_off) pos_off[e] = eo[i];
+            e++;
+        }
+        if (pos_off) pos_off[n_e] = n_p;
+        if (positions) std::copy(out.begin(), out.end(), positions);
         return 0;
     });
 }

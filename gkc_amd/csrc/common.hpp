@@ -20,6 +20,7 @@ struct ChunkDesc {
     uint32_t n_bases;         // offs[n] - offs[0]
     uint32_t n_tiles;         // ceil(n_bases / tile)
     uint32_t lead;            // positions [0, lead) precede read 0 (16-byte alignment of `reads`)
+    uint32_t read0;           // index of read 0 among all reads partitioned since the last reset
 };
 
 // Global-memory view of a pointer that the compiler cannot place (one read
@@ -56,12 +57,17 @@ __device__ __forceinline__ bool pmer_pass(uint64_t v, int p, int mode) {
     return (top != 0) & (top != 4) & (top != 8);
 }
 
+// murmur3's 64-bit finaliser: the hashed partition mode and the k-mer
+// sampling of position lists.
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x ^= x >> 33; x *= 0xff51afd7ed558ccdULL;
+    x ^= x >> 33; x *= 0xc4ceb9fe1a85ec53ULL;
+    x ^= x >> 33;
+    return x;
+}
+
 __device__ __forceinline__ uint32_t part_of(uint64_t mm, uint32_t P, int mode) {
-    if (mode == 1) {
-        mm ^= mm >> 33; mm *= 0xff51afd7ed558ccdULL;
-        mm ^= mm >> 33; mm *= 0xc4ceb9fe1a85ec53ULL;
-        mm ^= mm >> 33;
-    }
+    if (mode == 1) mm = mix64(mm);
     return (uint32_t)(mm % P);
 }
 

@@ -133,7 +133,10 @@ void GPUReset(int did) {
 
 void GenSuperkmerGPU(PinnedCSR& pinned_reads, int K_kmer, int P_minimizer, bool HPC, CUDAParams gpars,
                      CountTask task, int SKM_partitions, std::function<void(T_h_data)> process_func) {
-    (void)task; (void)SKM_partitions;
+    // SKMPartition and SKMPartWithPos hand on the same arrays (a superkmer's
+    // read and offset are its place in them); StoreMinimizerPos adds each
+    // window's minimizer position and strand
+    (void)SKM_partitions;
     gkc_ctx* c = thread_ctx(gpars.device_id, K_kmer, P_minimizer, HPC, 1, 0);
     const T_read_cnt per = std::max(1, gpars.NUM_BLOCKS_PER_GRID * gpars.NUM_THREADS_PER_BLOCK);
     for (T_read_cnt cur = 0; cur < pinned_reads.n_reads; cur += per) {
@@ -168,6 +171,13 @@ void GenSuperkmerGPU(PinnedCSR& pinned_reads, int K_kmer, int P_minimizer, bool 
             hd.read_len = new T_read_len[hd.reads_cnt];
             hd.hpc_orig_pos = new T_read_len[bytes + 1];
         }
+        if (task == StoreMinimizerPos) {  // before the HPC bases overwrite the reads
+            hd.mm_pos = new T_kvalue[bytes + 1];
+            hd.mm_strand = new char[bytes + 1];
+            if (gkc_window_minimizers(c, reads, reinterpret_cast<const uint64_t*>(offs), hd.reads_cnt, hd.minimizers,
+                                      hd.mm_pos, reinterpret_cast<int8_t*>(hd.mm_strand)) != 0)
+                die("GenSuperkmerGPU");
+        }
         if (gkc_superkmer_arrays(c, reads, reinterpret_cast<const uint64_t*>(offs), hd.reads_cnt, hd.minimizers,
                                  hd.superkmer_offs, reads, hd.read_len, hd.hpc_orig_pos) != 0)
             die("GenSuperkmerGPU");
@@ -176,6 +186,8 @@ void GenSuperkmerGPU(PinnedCSR& pinned_reads, int K_kmer, int P_minimizer, bool 
         delete[] hd.superkmer_offs;
         delete[] hd.read_len;
         delete[] hd.hpc_orig_pos;
+        delete[] hd.mm_pos;
+        delete[] hd.mm_strand;
     }
 }
 

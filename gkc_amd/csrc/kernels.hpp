@@ -23,6 +23,8 @@ struct BinSet {
     uint32_t* hdr_cur;          // [P]
     uint32_t* kmer_cur;         // [P] k-mers appended so far
     uint32_t* overflow;         // [1] set when any capacity is exceeded
+    uint2* pos;                 // beside each header: {read index since the last reset, in-read
+                                // offset of its first base}, or null (GKC_WITH_POS)
 };
 
 struct P1Params {
@@ -220,5 +222,11 @@ __global__ void k_tilescan(Plan* __restrict__ plan, SegArrays sa, uint32_t* __re
 template <typename K>
 __global__ void k_wavecopy(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys,
                            const K* __restrict__ alt, const uint2* __restrict__ ranges, HeavyList hl, TableOut to);
+
+// ---- positions (positions.hip) ----
+__global__ void k_pos_scatter(const SkmHdr* __restrict__ hdrs, const uint2* __restrict__ pos, uint32_t n_hdrs,
+                              const uint32_t* __restrict__ words, int k, uint32_t mod, const uint64_t* __restrict__ tkeys,
+                              const uint32_t* __restrict__ tcnts, uint64_t n_tab, const uint64_t* __restrict__ entry_off, uint32_t* __restrict__ cursor,
+                              uint64_t* __restrict__ out);
 
 }  // namespace gkc

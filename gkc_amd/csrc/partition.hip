@@ -146,6 +146,7 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
     __shared__ uint32_t s_ro[P1_RO];              // offsets (chunk positions) of reads r0 .. r0 + n_ro - 1
     __shared__ uint32_t s_ns, s_ovf;
     __shared__ uint32_t s_cr[2];                    // reads holding the current tile's window starts: first, last
+    __shared__ uint32_t s_slot_tile[P1_MT];         // the tile held in each slot (superkmer positions)
     // per partition: flush-local [47:32] headers, [31:16] k-mers, [15:0] words;
     // then the flush's global bases [63:32] words, [31:0] headers
     extern __shared__ unsigned long long s_loc[];
@@ -222,6 +223,7 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
         const uint32_t n_ro = r1 - r0 + 2;                                 // offsets r0 .. r1 + 1
         const bool ro_lds = n_ro <= P1_RO;
         uint32_t* const sc = s_code[slot];
+        if (tid == 0) s_slot_tile[slot] = tile;
 
         // 1. stage: packed codes of bases [t0, t0 + blen) (code 0 past blen), read offsets
         uint32_t badw = 0;                          // non-ACGT bytes of this thread's 16 (0 past blen)
@@ -566,7 +568,13 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
                         h.kmer_off = s_gk[part] + (uint32_t)((loc[q] >> 16) & 0xFFFFu);
                         h.n_kmers = n;
                         h.reserved = 0;
-                        bins.hdrs[bins.hdr_base[part] + (uint32_t)gb + (uint32_t)(loc[q] >> 32)] = h;
+                        const uint64_t hix = bins.hdr_base[part] + (uint32_t)gb + (uint32_t)(loc[q] >> 32);
+                        bins.hdrs[hix] = h;
+                        if (bins.pos) {     // the superkmer's read and in-read offset (search of the chunk's offsets)
+                            const uint32_t cp = s_slot_tile[rp >> 11] * P1_T + a;
+                            const uint32_t r = find_read(g_offs, cd.n_reads, cd.base + cp);
+                            bins.pos[hix] = make_uint2(cd.read0 + r, cp - (uint32_t)(g_offs[r] - cd.base));
+                        }
                         if (prm.sketch && k >= 4) {          // sampled superkmers' leading 4-mers
                             const uint32_t lead = tile_codes(csl, a, (uint32_t)min(k, 16));
                             if (sketch_sampled(lead))

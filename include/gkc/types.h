@@ -55,6 +55,12 @@ struct T_h_data {
     T_read_len *hpc_orig_pos;     /* original in-read index per HPC base (HPC only) */
     T_minimizer *minimizers;      /* per batch-local base: window minimizer */
     T_read_len *superkmer_offs;   /* per read: [0, i1, .., len-k+1, 0] at the read's batch-local start */
+    /* CountTask::StoreMinimizerPos only (the reference keeps these fields on
+     * the device, gkc_cuda.cu:159-160, and never fills them); null otherwise.
+     * Per batch-local base: the window minimizer's offset in the window (255
+     * none) and strand (0 forward, 1 reverse complement, -1 tie or none). */
+    T_kvalue *mm_pos;
+    char *mm_strand;
 };
 
 GKC_DROPIN_END

@@ -40,6 +40,8 @@ enum {
 #define GKC_LEARNED       16u   /* gkc_count with GKC_PRESIZED: capacities from the predictor set by
                                    gkc_set_predictor, computed on the device before the sort */
 #define GKC_EMIT_WINDOWS  32u   /* gkc_partition: also write per-window (minimizer, position, strand) */
+#define GKC_WITH_POS      64u   /* gkc_partition: keep each superkmer's read and offset in the bins
+                                   (every batch between resets alike) */
 
 /* Counting parameters (the reference's PAR fields K_kmer, P_minimizer, HPC,
  * SKM_partitions, kmer_min_freq, kmer_max_freq; utilities.hpp:104-123). */
@@ -165,6 +167,28 @@ int gkc_window_minimizers(gkc_ctx *ctx, const char *reads, const uint64_t *offs,
 /* Device pointers to the records written by the last gkc_partition with
  * GKC_EMIT_WINDOWS (valid until the next such call). */
 int gkc_windows_device(gkc_ctx *ctx, uint32_t **mm, uint8_t **pos, int8_t **strand, uint64_t *n);
+/* ---- position-carrying variant (the reference's CountTask::SKMPartWithPos,
+ * gkc_cuda.hpp:50, declared for "count a portion of k-mers and store their
+ * positions", README.md:15-18, never implemented) ----
+ * After gkc_partition(..., GKC_WITH_POS): partition `part`'s superkmers as
+ * (read, offset of the first base in the read, k-mers), reads numbered
+ * across batches from 0 at the last gkc_reset; offsets are HPC coordinates
+ * when hpc.  Returns the number of superkmers (GKC_ECAP when above cap; with
+ * the three outputs null, only the number). */
+int64_t gkc_superkmer_positions(gkc_ctx *ctx, int32_t part, uint32_t *read, uint32_t *offset, uint32_t *n_kmers,
+                                uint64_t cap);
+/* After gkc_count on GKC_WITH_POS bins (k <= 32): every occurrence of the
+ * partition's sampled table k-mers -- those whose canonical value v has
+ * murmur3-fmix64(v) % sample_mod == 0 (1: all) -- grouped by k-mer in key
+ * order: keys[e], counts[e], positions[pos_off[e] .. pos_off[e + 1]) holding
+ * read << 32 | (k-mer offset in the read) << 1 | strand (1 when the reverse
+ * complement is the canonical form), ascending within each k-mer.  Sets
+ * *n_entries and *n_positions; GKC_ECAP when a capacity is short (call with
+ * 0 capacities to size the buffers).  Superkmers are listed and k-mers found
+ * on the device; each k-mer's few positions are ordered on the host. */
+int gkc_kmer_positions(gkc_ctx *ctx, int32_t part, uint32_t sample_mod, uint64_t *keys, uint32_t *counts,
+                       uint64_t *pos_off, uint64_t *positions, uint64_t entry_cap, uint64_t pos_cap,
+                       uint64_t *n_entries, uint64_t *n_positions);
 /* Clear the bins and the table. */
 int gkc_reset(gkc_ctx *ctx);
 

@@ -164,6 +164,49 @@ def brute_force_table(reads: np.ndarray, offs: np.ndarray, k: int, hpc_on: bool 
     return table
 
 
+def mix64(v: int) -> int:
+    """murmur3's 64-bit finaliser (fmix64), the sampling hash of the
+    position-carrying listing."""
+    m = (1 << 64) - 1
+    v ^= v >> 33
+    v = (v * 0xFF51AFD7ED558CCD) & m
+    v ^= v >> 33
+    v = (v * 0xC4CEB9FE1A85EC53) & m
+    v ^= v >> 33
+    return v
+
+
+def brute_force_positions(reads: np.ndarray, offs: np.ndarray, k: int, hpc_on: bool = False,
+                          sample_mod: int = 1) -> dict:
+    """Canonical k-mer -> ascending list of read << 32 | offset << 1 | strand
+    for every occurrence, over the read strings as brute_force_table reads
+    them (offsets in HPC coordinates when hpc_on; strand 1 when the reverse
+    complement string is the smaller), keeping the k-mers with
+    mix64(value) % sample_mod == 0.  The reference declares this task
+    (CountTask::SKMPartWithPos, gkc_cuda.hpp:50) without implementing it, so
+    the listing is stated here from its description alone."""
+    out: dict[int, list[int]] = {}
+    for r in range(len(offs) - 1):
+        s = bytes(reads[int(offs[r] - offs[0]):int(offs[r + 1] - offs[0])])
+        if hpc_on and s:
+            s = bytes([s[0]] + [s[i] for i in range(1, len(s)) if s[i] != s[i - 1]])
+        for i in range(len(s) - k + 1):
+            fw = s[i:i + k].upper()
+            if fw.strip(b"ACGT"):
+                continue
+            rc = fw.translate(_COMP)[::-1]
+            c = min(fw, rc)
+            v = 0
+            for ch in c:
+                v = (v << 2) | _CODE[ch]
+            if mix64(v) % sample_mod:
+                continue
+            out.setdefault(v, []).append(r << 32 | i << 1 | int(rc < fw))
+    for v in out:
+        out[v].sort()
+    return out
+
+
 class ParseError(ValueError):
     pass
 

@@ -383,3 +383,62 @@ def test_partition_rule_per_window():
         for i in range(len(s) - k + 1):
             key = L.or_canonical64(_enc(s[i:i + k]), k)
             assert where[key] == int(mm[a + i]) % P
+
+
+# ---------------------------------------------------------------- k-mer positions
+
+
+def test_mix64_matches_partition_hash():
+    """The sampling hash is the one partition mode 1 applies in the C oracle
+    (mix64(mm) % P): equal residues for several moduli pin the 64-bit value."""
+    L = O.lib()
+    rng = np.random.default_rng(11)
+    for v in [0, 1, 2, 3, (1 << 62) - 1] + rng.integers(0, 1 << 62, size=40).tolist():
+        for P in (7, 64, 1021, 65535, (1 << 32) - 5):
+            assert L.or_partition(v, P, 1) == O.mix64(v) % P
+
+
+@pytest.mark.parametrize("k,hpc", [(21, False), (31, False), (25, True), (32, False)])
+def test_positions_brute_force_pins(k, hpc):
+    """Every listed occurrence spells its k-mer: the bases at (read, offset)
+    have the listed canonical value, the strand says which form was read;
+    the occurrences per k-mer are the brute-force counts; sampling keeps the
+    residue class exactly."""
+    rng = np.random.default_rng(k)
+    rs = W.generate(W.workload(1, scale=0.002))
+    reads = _with_other_bytes(rs, rng)
+    L = O.lib()
+    tab = O.brute_force_table(reads, rs.offs, k, hpc_on=hpc)
+    pos = O.brute_force_positions(reads, rs.offs, k, hpc_on=hpc)
+    assert {v: len(x) for v, x in pos.items()} == tab
+    if hpc:
+        src, lens, _ = O.hpc(reads, rs.offs)
+        seqs = [bytes(src[int(rs.offs[r]):int(rs.offs[r]) + int(lens[r])]) for r in range(rs.n)]
+    else:
+        seqs = [bytes(reads[int(rs.offs[r]):int(rs.offs[r + 1])]) for r in range(rs.n)]
+    for v, lst in list(pos.items())[::7]:
+        for e in lst:
+            r, i, st = e >> 32, (e & 0xFFFFFFFF) >> 1, e & 1
+            fw = _enc(seqs[r][i:i + k])
+            assert L.or_canonical64(fw, k) == v
+            assert (fw != v) == bool(st)
+    sub = O.brute_force_positions(reads, rs.offs, k, hpc_on=hpc, sample_mod=5)
+    assert sub == {v: x for v, x in pos.items() if O.mix64(v) % 5 == 0}
+    assert 0 < len(sub) < len(pos)
+
+
+def test_positions_reverse_complement_mirror():
+    """Reverse-complementing every read maps each occurrence (r, i, strand)
+    of a non-palindromic k-mer to (r, L - k - i, 1 - strand)."""
+    k = 21
+    rs = W.generate(W.workload(1, scale=0.002))
+    seqs = [bytes(rs.reads[int(rs.offs[r]):int(rs.offs[r + 1])]) for r in range(rs.n)]
+    r2, o2 = _csr([s.translate(COMP)[::-1] for s in seqs])
+    a = O.brute_force_positions(rs.reads, rs.offs, k)
+    b = O.brute_force_positions(r2, o2, k)
+    assert a.keys() == b.keys()
+    lens = np.diff(rs.offs).astype(np.int64)
+    for v, lst in a.items():
+        want = sorted(e >> 32 << 32 | (int(lens[e >> 32]) - k - ((e & 0xFFFFFFFF) >> 1)) << 1 | (1 - (e & 1))
+                      for e in lst)
+        assert b[v] == want
