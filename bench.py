@@ -255,7 +255,7 @@ def run_config(cid: int, scale: float, steps: int, warmup: int, forms: list[str]
         resp = autotune.tune_bin_passes(ctx, log=_log)
         cache = autotune.load_cache()
         cache[tkey] = {"sort_items": res["sort_items"], "sort_blocks_per_cu": res["sort_blocks_per_cu"],
-                       "ms_pass": res["ms_pass"], "bin_passes": resp["bin_passes"],
+                       "sort_waves": res["sort_waves"], "ms_pass": res["ms_pass"], "bin_passes": resp["bin_passes"],
                        "count_window": resp["count_window"], "ms_count": resp["ms_count"], "workload": w.name}
         autotune.save_cache(cache)
     tuning = autotune.apply(ctx, tkey)

@@ -1,7 +1,9 @@
 """Autotuner of the counting kernels.
 
-Picks the onesweep tile size (keys per thread) and the number of persistent
-workgroups per CU of the sort passes by timing every candidate on the data
+Picks the onesweep tile size (keys per thread), the waves per workgroup (4,
+or 8 for 64-bit keys: the same tile LDS with twice the waves to hide the
+loads behind) and the number of persistent workgroups per CU of the sort
+passes by timing every candidate on the data
 already in a counter's bins (the batched form, instrumented pass timing),
 then the number of onesweep passes before the binned count (2 or 3: fewer
 passes leave larger bins, which pays when the bins stay small enough for
@@ -20,8 +22,9 @@ import os
 from . import capi
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_cache.json")
-CANDIDATES_64 = [(i, b) for i in (8, 12, 16, 20, 24) for b in (0, 2)]
-CANDIDATES_128 = [(i, b) for i in (4, 8, 12) for b in (0, 2)]
+# (keys per thread, workgroups per CU (0: occupancy limit), waves per workgroup)
+CANDIDATES_64 = [(i, b, 4) for i in (8, 12, 16, 20, 24) for b in (0, 2)] + [(i, 0, 8) for i in (8, 10, 12)]
+CANDIDATES_128 = [(i, b, 4) for i in (4, 8, 12) for b in (0, 2)]
 
 
 def bucket_key(k: int, mean_read_len: float, keys_per_partition: float, n_parts: int | None = None) -> str:
@@ -71,35 +74,35 @@ def apply(counter: "capi.Counter", key: str) -> dict | None:
     t = nearest(key)
     if t:
         counter.set_tuning(t["sort_items"], t["sort_blocks_per_cu"], bin_passes=t.get("bin_passes", 0),
-                           count_window=t.get("count_window", 0))
+                           count_window=t.get("count_window", 0), sort_waves=t.get("sort_waves", 0))
     return t
 
 
 def tune_sort(counter: "capi.Counter", reps: int = 2, log=None) -> dict:
     """Time every candidate on the counter's current bins; returns the best
-    {sort_items, sort_blocks_per_cu, ms_pass} and the full sweep."""
+    {sort_items, sort_blocks_per_cu, sort_waves, ms_pass} and the full sweep."""
     cands = CANDIDATES_64 if counter.k <= 32 else CANDIDATES_128
     sweep = []
     prev = os.environ.get("GKC_NO_GRAPH")
     os.environ["GKC_NO_GRAPH"] = "1"
     try:
-        for items, bpc in cands:
-            counter.set_tuning(items, bpc)
+        for items, bpc, waves in cands:
+            counter.set_tuning(items, bpc, sort_waves=waves)
             best = None
             for _ in range(reps):
                 counter.count(capi.BATCHED)
                 ms = counter.stats()["ms_sort_pass_avg"]
                 best = ms if best is None else min(best, ms)
-            sweep.append({"sort_items": items, "sort_blocks_per_cu": bpc, "ms_pass": best})
+            sweep.append({"sort_items": items, "sort_blocks_per_cu": bpc, "sort_waves": waves, "ms_pass": best})
             if log:
-                log(f"tune items={items} blocks/CU={bpc or 'max'}: {best:.3f} ms/pass")
+                log(f"tune items={items} blocks/CU={bpc or 'max'} waves={waves}: {best:.3f} ms/pass")
     finally:
         if prev is None:
             os.environ.pop("GKC_NO_GRAPH", None)
         else:
             os.environ["GKC_NO_GRAPH"] = prev
     win = min(sweep, key=lambda r: r["ms_pass"])
-    counter.set_tuning(win["sort_items"], win["sort_blocks_per_cu"])
+    counter.set_tuning(win["sort_items"], win["sort_blocks_per_cu"], sort_waves=win["sort_waves"])
     return {**win, "sweep": sweep}
 
 
@@ -118,7 +121,7 @@ def tune_bin_passes(counter: "capi.Counter", reps: int = 3, log=None) -> dict:
     for npass in (3, 2):
         for cw in COUNT_WINDOWS:
             counter.set_tuning(t0["sort_items"], t0["sort_blocks_per_cu"], t0["single_list"], bin_passes=npass,
-                               count_window=cw)
+                               count_window=cw, sort_waves=t0["sort_waves"])
             best = None
             for _ in range(reps):
                 counter.count(capi.BATCHED)
@@ -129,5 +132,5 @@ def tune_bin_passes(counter: "capi.Counter", reps: int = 3, log=None) -> dict:
                 log(f"tune bin_passes={npass} count_window={cw}: {best:.3f} ms per count")
     win = min(sweep, key=lambda r: r["ms_count"])
     counter.set_tuning(t0["sort_items"], t0["sort_blocks_per_cu"], t0["single_list"], bin_passes=win["bin_passes"],
-                       count_window=win["count_window"])
+                       count_window=win["count_window"], sort_waves=t0["sort_waves"])
     return {**win, "sweep": sweep}

@@ -194,6 +194,7 @@ static int occupancy_blocks(const void* fn, int nt, int n_cu, size_t dyn_lds = 0
 static void zero(gkc_ctx* c, void* p, uint64_t n32);
 template <typename F> hipGraphExec_t capture(gkc_ctx* c, F&& enqueue);
 template <typename K> static int sort_items(gkc_ctx* c);
+template <typename K> static int sort_nt(gkc_ctx* c);
 
 // ---------------------------------------------------------------- bins
 
@@ -1044,6 +1045,8 @@ int gkc_set_tuning(gkc_ctx* c, const gkc_tuning* t) {
         if (!t) throw ArgError{"null tuning", GKC_EINVAL};
         if (t->count_mode < 0 || t->count_mode > 1) throw ArgError{"count_mode must be 0 or 1", GKC_EINVAL};
         if (t->bin_passes < 0 || t->bin_passes > 3) throw ArgError{"bin_passes must be 0..3", GKC_EINVAL};
+        if (t->sort_waves != 0 && t->sort_waves != 4 && t->sort_waves != 8)
+            throw ArgError{"sort_waves must be 0, 4 or 8", GKC_EINVAL};
         if (t->count_window != 0 && (t->count_window < 128 || t->count_window > 1024 || t->count_window % 64))
             throw ArgError{"count_window must be 0 or a multiple of 64 in 128..1024", GKC_EINVAL};
         c->tune = *t;
@@ -1055,6 +1058,7 @@ int gkc_get_tuning(gkc_ctx* c, gkc_tuning* t) {
     return guarded(c, [&] {
         gkc_tuning r = c->tune;
         r.sort_items = c->keyw == 8 ? sort_items<uint64_t>(c) : sort_items<u128k>(c);
+        r.sort_waves = (c->keyw == 8 ? sort_nt<uint64_t>(c) : sort_nt<u128k>(c)) / 64;
         if (t) *t = r;
         return 0;
     });
@@ -1141,34 +1145,46 @@ static void zero(gkc_ctx* c, void* p, uint64_t n32) {
 // Run extraction + sort for a plan whose per-segment arrays are set; the
 // sorted keys end in keysA or keysB (returned).  Host knows n_keys_total and
 // the sort tile count upper bound.
-// onesweep variants (keys per thread), selected by gkc_tuning.sort_items
+// onesweep variants (keys per thread x threads per workgroup), selected by
+// gkc_tuning.sort_items and gkc_tuning.sort_waves
 template <typename K>
 using OnesweepFn = void (*)(Plan*, SegArrays, WorkLists, const K*, K*, uint32_t*, int);
-template <typename K> static OnesweepFn<K> onesweep_variant(int items);
-template <> OnesweepFn<uint64_t> onesweep_variant<uint64_t>(int items) {
+template <typename K> static OnesweepFn<K> onesweep_variant(int items, int nt);
+template <> OnesweepFn<uint64_t> onesweep_variant<uint64_t>(int items, int nt) {
+    if (nt == 512) {
+        switch (items) {
+            case 8: return k_onesweep<uint64_t, 8, 512>;
+            case 12: return k_onesweep<uint64_t, 12, 512>;
+            default: return k_onesweep<uint64_t, 10, 512>;
+        }
+    }
     switch (items) {
-        case 8: return k_onesweep<uint64_t, 8>;
-        case 12: return k_onesweep<uint64_t, 12>;
-        case 20: return k_onesweep<uint64_t, 20>;
-        case 24: return k_onesweep<uint64_t, 24>;
-        default: return k_onesweep<uint64_t, 16>;
+        case 8: return k_onesweep<uint64_t, 8, 256>;
+        case 12: return k_onesweep<uint64_t, 12, 256>;
+        case 20: return k_onesweep<uint64_t, 20, 256>;
+        case 24: return k_onesweep<uint64_t, 24, 256>;
+        default: return k_onesweep<uint64_t, 16, 256>;
     }
 }
-template <> OnesweepFn<u128k> onesweep_variant<u128k>(int items) {
+template <> OnesweepFn<u128k> onesweep_variant<u128k>(int items, int) {
     switch (items) {
-        case 4: return k_onesweep<u128k, 4>;
-        case 12: return k_onesweep<u128k, 12>;
-        default: return k_onesweep<u128k, 8>;
+        case 4: return k_onesweep<u128k, 4, 256>;
+        case 12: return k_onesweep<u128k, 12, 256>;
+        default: return k_onesweep<u128k, 8, 256>;
     }
 }
+// threads per onesweep workgroup: 8 waves on request for 64-bit keys, else 4
+template <typename K> static int sort_nt(gkc_ctx* c) { return sizeof(K) == 8 && c->tune.sort_waves == 8 ? 512 : 256; }
 template <typename K> static int sort_items(gkc_ctx* c) {
     const int it = c->tune.sort_items;
+    if (sort_nt<K>(c) == 512) return (it == 8 || it == 10 || it == 12) ? it : 10;
     if (sizeof(K) == 8) return (it == 8 || it == 12 || it == 16 || it == 20 || it == 24) ? it : 16;
     return (it == 4 || it == 8 || it == 12) ? it : 8;
 }
+template <typename K> static int sort_tile_keys(gkc_ctx* c) { return sort_nt<K>(c) * sort_items<K>(c); }
 template <typename K> static int sort_grid(gkc_ctx* c) {
-    OnesweepFn<K> fn = onesweep_variant<K>(sort_items<K>(c));
-    int g = occupancy_blocks((const void*)fn, SORT_NT, c->n_cu);
+    OnesweepFn<K> fn = onesweep_variant<K>(sort_items<K>(c), sort_nt<K>(c));
+    int g = occupancy_blocks((const void*)fn, sort_nt<K>(c), c->n_cu);
     if (c->tune.sort_blocks_per_cu > 0) g = c->tune.sort_blocks_per_cu * c->n_cu;   // explicit (may exceed residency)
     return std::max(g, 1);
 }
@@ -1221,18 +1237,18 @@ static void set_lists(gkc_ctx* c, uint32_t n_segs, const uint32_t* keys, const u
 
 template <typename K>
 static void launch_onesweep(gkc_ctx* c, int grid, Plan* plan, SegArrays sa, const K* src, K* dst, int pass) {
-    OnesweepFn<K> fn = onesweep_variant<K>(sort_items<K>(c));
+    OnesweepFn<K> fn = onesweep_variant<K>(sort_items<K>(c), sort_nt<K>(c));
     uint32_t* lb = c->lookback.as<uint32_t>();
     WorkLists wl = c->wl;
     void* args[] = {&plan, &sa, &wl, &src, &dst, &lb, &pass};
-    HIPX(hipLaunchKernel((const void*)fn, dim3(grid), dim3(SORT_NT), args, 0, c->st));
+    HIPX(hipLaunchKernel((const void*)fn, dim3(grid), dim3(sort_nt<K>(c)), args, 0, c->st));
 }
 
 template <typename K>
 static K* sort_segments(gkc_ctx* c, SegArrays sa, const uint64_t* hdr_base, const uint64_t* word_base,
                         uint32_t n_segs, uint64_t n_keys, uint64_t n_hdrs, float* ms_ex, float* ms_sort, float* ms_pass_sum) {
     Plan* plan = c->d_plan.as<Plan>();
-    const int TILE = SORT_NT * sort_items<K>(c);
+    const int TILE = sort_tile_keys<K>(c);
     {
         const uint32_t nk = (uint32_t)n_keys, nh = (uint32_t)n_hdrs;
         set_lists(c, 1, &nk, &nh, 1, TILE);
@@ -1506,7 +1522,7 @@ template <typename K>
 static K* enqueue_sort_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs, uint64_t tiles_cap, bool timed) {
     const uint32_t P = c->prm.n_parts;
     Plan* plan = c->d_plan.as<Plan>();
-    const int TILE = SORT_NT * sort_items<K>(c);
+    const int TILE = sort_tile_keys<K>(c);
     hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(64), 0, c->st, plan, P, (uint32_t)bs.npass, bs.sh0);
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, c->st, plan, sa, TILE, bs.rl_tile);
     zero(c, sa.hist, (uint64_t)P * MAX_PASS * RADIX);
@@ -1657,7 +1673,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     c->stats.n_pass = bs.npass;
     c->stats.binned = bs.binned ? 1 : 0;
     ensure_count_buffers<K>(c, tot_keys, P);
-    const uint64_t tiles_cap = tot_keys / (SORT_NT * sort_items<K>(c)) + P + 1;
+    const uint64_t tiles_cap = tot_keys / (sort_tile_keys<K>(c)) + P + 1;
     const uint64_t rl_tiles_cap = tot_keys / bs.rl_tile + P + 1;
     SegArrays sa;
     sa.seg_keys = c->d_kmer_cur.as<uint32_t>(); sa.seg_hdrs = c->d_hdr_cur.as<uint32_t>();
@@ -1670,7 +1686,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     c->d_hres.ensure(rl_tiles_cap * sizeof(uint2));
     c->d_wkept.ensure(rl_tiles_cap * sizeof(uint32_t));
     set_lists(c, P, c->h_kmer_cur.data(), c->h_hdr_cur.data(), c->tune.single_list == 1 ? 1 : NLIST,
-              SORT_NT * sort_items<K>(c), bs.rl_tile);
+              sort_tile_keys<K>(c), bs.rl_tile);
 
     // table capacities: predicted on the device (learned), given by the caller
     // (presized) or from a count pass (count-then-allocate)
@@ -1699,6 +1715,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
                                     c->d_hlist.p, c->d_hres.p, c->d_wkept.p,
                                     (const void*)(uintptr_t)tiles_cap, (const void*)(uintptr_t)bs.npass,
                                     (const void*)(uintptr_t)sort_items<K>(c), (const void*)(uintptr_t)sort_grid<K>(c),
+                                    (const void*)(uintptr_t)sort_nt<K>(c),
                                     c->d_wl.p, (const void*)(uintptr_t)c->wl.n_lists,
                                     (const void*)(uintptr_t)(bs.binned ? 1 : 0), (const void*)(uintptr_t)(bs.wave ? 1 : 0),
                                     (const void*)(uintptr_t)bs.rl_tile};

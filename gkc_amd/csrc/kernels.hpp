@@ -151,7 +151,7 @@ __global__ void k_extract(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, c
                           const uint64_t* __restrict__ hdr_base, const uint32_t* __restrict__ words,
                           const uint64_t* __restrict__ word_base, K* __restrict__ keys, int k);
 __global__ void k_hist_scan(Plan* __restrict__ plan, uint32_t* __restrict__ hist);
-template <typename K, int ITEMS>
+template <typename K, int ITEMS, int NT>
 __global__ void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ in, K* __restrict__ out,
                            uint32_t* __restrict__ lookback, int pass);
 template <typename K>

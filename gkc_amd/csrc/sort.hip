@@ -197,7 +197,12 @@ __device__ __forceinline__ u128k canon_of(u128k x, int k) {
 // of them, prefix-sums their k-mer counts and strides its lanes over the
 // combined k-mers, so consecutive lanes write consecutive keys (a scatter
 // block's superkmers of one partition have consecutive k-mer offsets).
+// The wave first copies its superkmers' words into LDS (each lane its own
+// superkmer's few words), so the three or five words each k-mer spans are
+// LDS reads rather than gathers from L2; a wave whose superkmers hold more
+// than EX_SW words (long superkmers) reads them from global memory instead.
 constexpr int EX_U = 4;
+constexpr int EX_SW = 512;
 template <typename K>
 __global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const SkmHdr* __restrict__ hdrs,
                                                      const uint64_t* __restrict__ hdr_base,
@@ -208,6 +213,8 @@ __global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, Se
     __shared__ uint32_t s_pre[NWAVE][GKC_WAVE + 1];     // inclusive k-mer prefix of the wave's headers
     __shared__ uint32_t s_koff[NWAVE][GKC_WAVE], s_woff[NWAVE][GKC_WAVE];
     __shared__ uint32_t s_flag[NWAVE][EX_U * GKC_WAVE];  // 1 + header starting at each position of a block
+    __shared__ uint32_t s_words[NWAVE][EX_SW + 8];      // the wave's superkmer words (+ read-past padding)
+    __shared__ uint32_t s_wst[NWAVE][GKC_WAVE];         // each header's first word in s_words
     __shared__ uint32_t s_seg, s_idx, s_ok;
     const int npass = plan->n_pass;
     const int sh0 = plan->sh0;
@@ -249,6 +256,25 @@ __global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, Se
         __syncthreads();
         const uint32_t* wseg = words + word_base[seg];
         K* kseg = keys + sa.key_start[seg];
+        // stage the wave's words: header h's words [0, nws) (the bases of all
+        // its k-mers) at s_words[wv][s_wst[h]]
+        const uint32_t nws = n ? ((n + (uint32_t)k - 2) >> 4) + 1 : 0u;
+        uint32_t winc = nws;
+#pragma unroll
+        for (int off = 1; off < GKC_WAVE; off <<= 1) {
+            const uint32_t y = __shfl_up(winc, off);
+            if ((int)lane >= off) winc += y;
+        }
+        const bool staged = __shfl(winc, GKC_WAVE - 1) <= (uint32_t)EX_SW;       // wave-uniform
+        if (staged) {
+            const uint32_t w0 = winc - nws;
+            s_wst[wv][lane] = w0;
+            const uint32_t* src = wseg + hd.word_off;
+            for (uint32_t j = 0; j < nws; j++) s_words[wv][w0 + j] = src[j];
+            if (lane == GKC_WAVE - 1)
+                for (uint32_t j = 0; j < 8; j++) s_words[wv][winc + j] = 0;      // read-past padding
+            wave_lds_fence();
+        }
         // blocks of EX_U * 64 consecutive k-mers, k-mer qb + 64u + lane on
         // lane `lane`: each non-empty header flags its start in LDS; a k-mer's
         // header is the last flag at or before it (a ballot and a lane
@@ -274,8 +300,13 @@ __global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, Se
                 ho[u] = q - s_pre[wv][hj[u]];
             }
             K c[EX_U];
+            if (staged) {
 #pragma unroll
-            for (int u = 0; u < EX_U; u++) c[u] = canon_of(kmer_at<K>(wseg + s_woff[wv][hj[u]], ho[u], k), k);
+                for (int u = 0; u < EX_U; u++) c[u] = canon_of(kmer_at<K>(&s_words[wv][s_wst[wv][hj[u]]], ho[u], k), k);
+            } else {
+#pragma unroll
+                for (int u = 0; u < EX_U; u++) c[u] = canon_of(kmer_at<K>(wseg + s_woff[wv][hj[u]], ho[u], k), k);
+            }
 #pragma unroll
             for (int u = 0; u < EX_U; u++) {
                 if (qb + u * GKC_WAVE + lane < T) {
@@ -290,11 +321,16 @@ __global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, Se
         const bool more = s_ok;
         const uint32_t nseg = s_seg;
         if (more) hd = load_hdr(nseg, s_idx);
-        uint32_t* gh = sa.hist + (size_t)seg * MAX_PASS * RADIX;
-        for (int i = threadIdx.x; i < npass * RADIX; i += SORT_NT)
-            if (s_hist[i]) atomicAdd(&gh[i], s_hist[i]);
-        __syncthreads();                                 // flushed before the histograms are cleared
-        for (int i = threadIdx.x; i < npass * RADIX; i += SORT_NT) s_hist[i] = 0;
+        // the histograms go out when the segment changes (a workgroup's
+        // successive items are mostly of one segment: its list's segments are
+        // claimed in order), so most items add no global atomics
+        if (!more || nseg != seg) {
+            uint32_t* gh = sa.hist + (size_t)seg * MAX_PASS * RADIX;
+            for (int i = threadIdx.x; i < npass * RADIX; i += SORT_NT)
+                if (s_hist[i]) atomicAdd(&gh[i], s_hist[i]);
+            __syncthreads();                             // flushed before the histograms are cleared
+            for (int i = threadIdx.x; i < npass * RADIX; i += SORT_NT) s_hist[i] = 0;
+        }
         if (!more) break;
         seg = nseg;
     }
@@ -326,9 +362,12 @@ __global__ __launch_bounds__(256) void k_hist_scan(Plan* __restrict__ plan, uint
 }
 
 // register target of the onesweep pass: waves per SIMD (64-bit keys, no
-// prefetch; 128-bit keys, next tile prefetched in registers)
-template <typename K> constexpr int onesweep_waves() { return sizeof(K) == 16 ? 3 : 4; }
-// One onesweep pass over all segments.  Tile = SORT_NT * ITEMS keys of one
+// prefetch; 128-bit keys, next tile prefetched in registers).  A 512-thread
+// workgroup holds a tile of the same LDS size as a 256-thread one with twice
+// the waves to hide the loads' latency behind, so it is held to 80 VGPRs
+// (6 waves per SIMD: three such workgroups per CU, the LDS limit).
+template <typename K, int NT> constexpr int onesweep_waves() { return NT == 512 ? 6 : sizeof(K) == 16 ? 3 : 4; }
+// One onesweep pass over all segments.  Tile = NT * ITEMS keys of one
 // segment; tiles are claimed in order from the work lists, so every tile's
 // predecessors are claimed by running workgroups (forward progress of the
 // look-back).  Per tile (claimed, then its keys loaded at once: a tile
@@ -342,21 +381,24 @@ template <typename K> constexpr int onesweep_waves() { return sizeof(K) == 16 ? 
 //      counters live in LDS
 //   5. decoupled look-back per digit across the segment's earlier tiles
 //   6. reorder the tile in LDS by digit and write each digit's run out
-template <typename K, int ITEMS>
-__global__ __launch_bounds__(SORT_NT, onesweep_waves<K>()) void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ in,
+template <typename K, int ITEMS, int NT>
+__global__ __launch_bounds__(NT, (onesweep_waves<K, NT>())) void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ in,
                                                       K* __restrict__ out, uint32_t* __restrict__ lookback, int pass) {
-    constexpr int TILE = SORT_NT * ITEMS;
-    constexpr int NWAVE = SORT_NT / GKC_WAVE;
+    static_assert(NT == RADIX || NT == 2 * RADIX, "256 or 512 threads");
+    constexpr int TILE = NT * ITEMS;
+    constexpr int NWAVE = NT / GKC_WAVE;           // ranking waves
+    constexpr int DWAVE = RADIX / GKC_WAVE;        // waves holding one digit per thread
     __shared__ K s_keys[TILE];
     __shared__ uint32_t s_whist[NWAVE][RADIX];
     __shared__ uint32_t s_hist[RADIX];
     __shared__ uint32_t s_dstart[RADIX];
-    __shared__ uint32_t s_wsum[NWAVE];
+    __shared__ uint32_t s_wsum[DWAVE];
     __shared__ uint64_t s_gbase[RADIX];
     __shared__ uint32_t s_seg, s_idx, s_ok;
     const int sh = (int)plan->sh0 + pass * RADIX_BITS;
     const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE;
-    const uint32_t d = threadIdx.x;     // SORT_NT == RADIX: one digit per thread
+    const uint32_t d = threadIdx.x;     // threads below RADIX: one digit each
+    const bool dg = NT == RADIX || d < (uint32_t)RADIX;
     uint32_t* ctr = &plan->ctr[ctr_pass(pass)];
     uint32_t list = blockIdx.x % wl.n_lists;
     const uint32_t wbase = wv * GKC_WAVE * ITEMS;   // wave wv owns tile keys [wbase, wbase + 64*ITEMS)
@@ -390,8 +432,8 @@ __global__ __launch_bounds__(SORT_NT, onesweep_waves<K>()) void k_onesweep(Plan*
                 s_ok = claim<0>(wl, ctr, list, sg, ix);
                 s_seg = sg; s_idx = ix;
             }
-            s_hist[d] = 0;
-            for (int i = threadIdx.x; i < NWAVE * RADIX; i += SORT_NT) (&s_whist[0][0])[i] = 0;
+            if (dg) s_hist[d] = 0;
+            for (int i = threadIdx.x; i < NWAVE * RADIX; i += NT) (&s_whist[0][0])[i] = 0;
             __syncthreads();
             const bool have_next = s_ok;
             const uint32_t nxt_seg = s_seg, nxt_ti = s_idx;
@@ -406,9 +448,9 @@ __global__ __launch_bounds__(SORT_NT, onesweep_waves<K>()) void k_onesweep(Plan*
             }
             __syncthreads();
             // 2. aggregate out
-            const uint32_t total = s_hist[d];
+            const uint32_t total = dg ? s_hist[d] : 0u;
             uint32_t* lb = lookback + (size_t)tile * RADIX;
-            st_agent(&lb[d], (ti == 0 ? LB_INC : LB_AGG) | total);
+            if (dg) st_agent(&lb[d], (ti == 0 ? LB_INC : LB_AGG) | total);
             // 3. prefetch the next tile
             K nkey[ITEMS];
             uint32_t n_nseg = 0, n_tn = 0;
@@ -455,13 +497,13 @@ __global__ __launch_bounds__(SORT_NT, onesweep_waves<K>()) void k_onesweep(Plan*
                 const uint32_t y = __shfl_up(inc, off);
                 if ((int)lane >= off) inc += y;
             }
-            if (lane == GKC_WAVE - 1) s_wsum[wv] = inc;
+            if (dg && lane == GKC_WAVE - 1) s_wsum[wv] = inc;
             __syncthreads();                                         // ranking done, wave sums visible
             uint32_t wsum = 0;
     #pragma unroll
-            for (int w = 0; w < NWAVE; w++) wsum += (w < (int)wv) ? s_wsum[w] : 0u;
-            s_dstart[d] = wsum + inc - total;
-            {   // per digit: exclusive prefix over the waves' running counters
+            for (int w = 0; w < DWAVE; w++) wsum += (w < (int)wv) ? s_wsum[w] : 0u;
+            if (dg) {   // per digit: its tile-local start, exclusive prefix over the waves' running counters
+                s_dstart[d] = wsum + inc - total;
                 uint32_t acc = 0;
     #pragma unroll
                 for (int w = 0; w < NWAVE; w++) { const uint32_t c = s_whist[w][d]; s_whist[w][d] = acc; acc += c; }
@@ -469,7 +511,7 @@ __global__ __launch_bounds__(SORT_NT, onesweep_waves<K>()) void k_onesweep(Plan*
             __syncthreads();
             // 5. decoupled look-back, one digit per thread
             uint32_t excl = 0;
-            if (ti > 0) {
+            if (dg && ti > 0) {
                 int64_t j = (int64_t)tile - 1;
                 for (;;) {
                     uint32_t v = ld_agent(&lookback[(size_t)j * RADIX + d]);
@@ -481,7 +523,7 @@ __global__ __launch_bounds__(SORT_NT, onesweep_waves<K>()) void k_onesweep(Plan*
                 st_agent(&lb[d], LB_INC | (excl + total));
             }
             const uint32_t* hs = sa.hist + ((size_t)seg * MAX_PASS + pass) * RADIX;
-            s_gbase[d] = kbase + hs[d] + excl - s_dstart[d];
+            if (dg) s_gbase[d] = kbase + hs[d] + excl - s_dstart[d];
             // 6. reorder in LDS by digit, write each digit's run out
     #pragma unroll
             for (int i = 0; i < ITEMS; i++) {
@@ -489,7 +531,7 @@ __global__ __launch_bounds__(SORT_NT, onesweep_waves<K>()) void k_onesweep(Plan*
                 if (dd < RADIX) s_keys[s_dstart[dd] + s_whist[wv][dd] + (dig[i] >> 9)] = key[i];
             }
             __syncthreads();
-            for (uint32_t j = threadIdx.x; j < tn; j += SORT_NT) {
+            for (uint32_t j = threadIdx.x; j < tn; j += NT) {
                 const K x = s_keys[j];
                 out[s_gbase[KeyTraits<K>::digit(x, sh)] + j] = x;
             }
@@ -509,8 +551,8 @@ __global__ __launch_bounds__(SORT_NT, onesweep_waves<K>()) void k_onesweep(Plan*
                 s_ok = claim<0>(wl, ctr, list, sg, ix);
                 s_seg = sg; s_idx = ix;
             }
-            s_hist[d] = 0;
-            for (int i = threadIdx.x; i < NWAVE * RADIX; i += SORT_NT) (&s_whist[0][0])[i] = 0;
+            if (dg) s_hist[d] = 0;
+            for (int i = threadIdx.x; i < NWAVE * RADIX; i += NT) (&s_whist[0][0])[i] = 0;
             __syncthreads();
             if (!s_ok) break;
             seg = s_seg; ti = s_idx;
@@ -533,9 +575,9 @@ __global__ __launch_bounds__(SORT_NT, onesweep_waves<K>()) void k_onesweep(Plan*
             }
             __syncthreads();
             // 2. aggregate out
-            const uint32_t total = s_hist[d];
+            const uint32_t total = dg ? s_hist[d] : 0u;
             uint32_t* lb = lookback + (size_t)tile * RADIX;
-            st_agent(&lb[d], (ti == 0 ? LB_INC : LB_AGG) | total);
+            if (dg) st_agent(&lb[d], (ti == 0 ? LB_INC : LB_AGG) | total);
             // 4. stable rank within the wave
             const uint64_t lt = lanemask_lt();
     #pragma unroll
@@ -568,13 +610,13 @@ __global__ __launch_bounds__(SORT_NT, onesweep_waves<K>()) void k_onesweep(Plan*
                 const uint32_t y = __shfl_up(inc, off);
                 if ((int)lane >= off) inc += y;
             }
-            if (lane == GKC_WAVE - 1) s_wsum[wv] = inc;
+            if (dg && lane == GKC_WAVE - 1) s_wsum[wv] = inc;
             __syncthreads();                                         // ranking done, wave sums visible
             uint32_t wsum = 0;
     #pragma unroll
-            for (int w = 0; w < NWAVE; w++) wsum += (w < (int)wv) ? s_wsum[w] : 0u;
-            s_dstart[d] = wsum + inc - total;
-            {   // per digit: exclusive prefix over the waves' running counters
+            for (int w = 0; w < DWAVE; w++) wsum += (w < (int)wv) ? s_wsum[w] : 0u;
+            if (dg) {   // per digit: its tile-local start, exclusive prefix over the waves' running counters
+                s_dstart[d] = wsum + inc - total;
                 uint32_t acc = 0;
     #pragma unroll
                 for (int w = 0; w < NWAVE; w++) { const uint32_t c = s_whist[w][d]; s_whist[w][d] = acc; acc += c; }
@@ -582,7 +624,7 @@ __global__ __launch_bounds__(SORT_NT, onesweep_waves<K>()) void k_onesweep(Plan*
             __syncthreads();
             // 5. decoupled look-back, one digit per thread
             uint32_t excl = 0;
-            if (ti > 0) {
+            if (dg && ti > 0) {
                 int64_t j = (int64_t)tile - 1;
                 for (;;) {
                     uint32_t v = ld_agent(&lookback[(size_t)j * RADIX + d]);
@@ -594,7 +636,7 @@ __global__ __launch_bounds__(SORT_NT, onesweep_waves<K>()) void k_onesweep(Plan*
                 st_agent(&lb[d], LB_INC | (excl + total));
             }
             const uint32_t* hs = sa.hist + ((size_t)seg * MAX_PASS + pass) * RADIX;
-            s_gbase[d] = kbase + hs[d] + excl - s_dstart[d];
+            if (dg) s_gbase[d] = kbase + hs[d] + excl - s_dstart[d];
             // 6. reorder in LDS by digit, write each digit's run out
     #pragma unroll
             for (int i = 0; i < ITEMS; i++) {
@@ -602,7 +644,7 @@ __global__ __launch_bounds__(SORT_NT, onesweep_waves<K>()) void k_onesweep(Plan*
                 if (dd < RADIX) s_keys[s_dstart[dd] + s_whist[wv][dd] + (dig[i] >> 9)] = key[i];
             }
             __syncthreads();
-            for (uint32_t j = threadIdx.x; j < tn; j += SORT_NT) {
+            for (uint32_t j = threadIdx.x; j < tn; j += NT) {
                 const K x = s_keys[j];
                 out[s_gbase[KeyTraits<K>::digit(x, sh)] + j] = x;
             }
@@ -1829,9 +1871,10 @@ template __global__ void k_wavecopy<u128k>(Plan*, SegArrays, const u128k*, const
 
 template __global__ void k_extract<uint64_t>(Plan*, SegArrays, WorkLists, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, uint64_t*, int);
 template __global__ void k_extract<u128k>(Plan*, SegArrays, WorkLists, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, u128k*, int);
-#define GKC_OS(K, I) template __global__ void k_onesweep<K, I>(Plan*, SegArrays, WorkLists, const K*, K*, uint32_t*, int);
-GKC_OS(uint64_t, 8) GKC_OS(uint64_t, 12) GKC_OS(uint64_t, 16) GKC_OS(uint64_t, 20) GKC_OS(uint64_t, 24)
-GKC_OS(u128k, 4) GKC_OS(u128k, 8) GKC_OS(u128k, 12)
+#define GKC_OS(K, I, N) template __global__ void k_onesweep<K, I, N>(Plan*, SegArrays, WorkLists, const K*, K*, uint32_t*, int);
+GKC_OS(uint64_t, 8, 256) GKC_OS(uint64_t, 12, 256) GKC_OS(uint64_t, 16, 256) GKC_OS(uint64_t, 20, 256)
+GKC_OS(uint64_t, 24, 256) GKC_OS(uint64_t, 8, 512) GKC_OS(uint64_t, 10, 512) GKC_OS(uint64_t, 12, 512)
+GKC_OS(u128k, 4, 256) GKC_OS(u128k, 8, 256) GKC_OS(u128k, 12, 256)
 #undef GKC_OS
 template __global__ void k_runlength<uint64_t>(Plan*, SegArrays, WorkLists, const uint64_t*, TableOut, uint32_t*);
 template __global__ void k_runlength<u128k>(Plan*, SegArrays, WorkLists, const u128k*, TableOut, uint32_t*);

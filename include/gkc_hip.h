@@ -81,7 +81,8 @@ typedef struct gkc_stats {
 /* Kernel variants (picked by gkc_amd/autotune.py, keyed on k, read-length
  * distribution and partition size). */
 typedef struct gkc_tuning {
-    int32_t sort_items;          /* keys per thread of a onesweep tile: 8,12,16,20,24 (u64) / 4,8,12 (u128) */
+    int32_t sort_items;          /* keys per thread of a onesweep tile: 8,12,16,20,24 (u64, 4 waves),
+                                    8,10,12 (u64, 8 waves) / 4,8,12 (u128) */
     int32_t sort_blocks_per_cu;  /* persistent workgroups per CU in the sort passes (0: occupancy limit) */
     int32_t single_list;         /* 1: batched tiles in one global order instead of one work list per XCD */
     int32_t count_mode;          /* batched form: 0 binned (onesweep passes on the top key bits, then
@@ -89,6 +90,8 @@ typedef struct gkc_tuning {
     int32_t bin_passes;          /* binned form: 8-bit onesweep passes before the count, 1..3 (0: automatic) */
     int32_t count_window;        /* binned form: keys per count window (the bins starting in it), a multiple
                                     of 64 in 128..1024 (0: 256); the same table for every value */
+    int32_t sort_waves;          /* waves per onesweep workgroup, 4 or 8 (0: 4; 8 for 64-bit keys only, with
+                                    sort_items 8, 10 or 12) */
 } gkc_tuning;
 
 const char *gkc_last_error(void);

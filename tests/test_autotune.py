@@ -36,7 +36,7 @@ def test_nearest_prefers_exact_then_partition_size(tmp_path, monkeypatch):
 def test_cache_entries_are_valid_variants():
     for key, v in autotune.load_cache().items():
         cands = autotune.CANDIDATES_64 if key.startswith("u64") else autotune.CANDIDATES_128
-        assert (v["sort_items"], v["sort_blocks_per_cu"]) in cands, key
+        assert (v["sort_items"], v["sort_blocks_per_cu"], v.get("sort_waves", 4) or 4) in cands, key
         assert v.get("bin_passes", 0) in (0, 2, 3), key
         assert v.get("count_window", 0) in (0,) + autotune.COUNT_WINDOWS, key
 
@@ -49,9 +49,10 @@ def test_every_variant_bit_exact(gpu, cid, scale, k, p, P):
     cands = autotune.CANDIDATES_64 if k <= 32 else autotune.CANDIDATES_128
     with capi.Counter(k, p, P) as c:
         c.partition(rs.reads, rs.offs)
-        for items, bpc in cands:
-            print("[gpu-case] variant", items, bpc, flush=True)
-            c.set_tuning(items, bpc)
+        for items, bpc, waves in cands:
+            print("[gpu-case] variant", items, bpc, waves, flush=True)
+            c.set_tuning(items, bpc, sort_waves=waves)
+            assert c.get_tuning()["sort_waves"] == waves
             for flags in (0, capi.BATCHED):
                 c.count(flags)
                 assert_tables_equal(c.table(), ot, P, k)
