@@ -377,7 +377,7 @@ def run_config(cid: int, scale: float, steps: int, warmup: int, forms: list[str]
                                              "box shares out per GPU"}
         rep["bit_exact_sample"] = exact
     handle = {"ctx": ctx, "rs": rs, "N": N, "W_key": W_key, "pass_ms": pass_ms, "d_reads": d_reads,
-              "d_offs": d_offs, "h2d": h2d}
+              "d_offs": d_offs, "h2d": h2d, "k": w.k}
     return rep, handle
 
 
@@ -412,6 +412,24 @@ def copy_ceiling(torch, capi, nbytes: int) -> dict:
     ok = bool(torch.equal(a[-4096:], b[-4096:]))
     del a, b
     return {"bytes_each_way": nbytes, "ms": best, "GB_per_s": 2 * nbytes / best / 1e6, "output_equal": ok}
+
+
+def vendor_sort(torch, capi, n: int, begin_bit: int, end_bit: int) -> dict:
+    """rocPRIM's radix sort of n random 64-bit keys on the bits the onesweep
+    passes sort, timed on the device (best of three after a warm-up)."""
+    a = torch.randint(0, 2 ** 62, (n,), dtype=torch.int64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream().cuda_stream
+    capi.vendor_sort_u64(a.data_ptr(), b.data_ptr(), n, begin_bit, end_bit, st)
+    best = min(capi.vendor_sort_u64(a.data_ptr(), b.data_ptr(), n, begin_bit, end_bit, st) for _ in range(3))
+    d = (b[:1 << 20] >> begin_bit) & ((1 << (end_bit - begin_bit)) - 1)
+    ok = bool((d[1:] >= d[:-1]).all())
+    del a, b
+    passes = (end_bit - begin_bit) / 8
+    return {"library": "rocPRIM radix_sort_keys", "keys": n, "bits": [begin_bit, end_bit], "ms": best,
+            "ms_per_8bit_digit": best / passes, "GB_per_s_per_8bit_digit": 2 * 8 * n / (best / passes) / 1e6,
+            "sorted": ok}
 
 
 def main() -> int:
@@ -458,6 +476,13 @@ def main() -> int:
     ceil = copy_ceiling(torch, capi, hd["W_key"] * hd["N"])
     rep["roofline"]["copy_ceiling"] = ceil
     rep["roofline"]["frac_of_copy_ceiling"] = rep["roofline"]["achieved"] / ceil["GB_per_s"]
+    if hd["W_key"] == 8:    # the vendor library's sort of as many keys on the same 16 bits
+        kb = 2 * hd["k"]
+        vs = vendor_sort(torch, capi, hd["N"], max(kb - 16, 0), kb)
+        vs["onesweep_ms_per_pass"] = rep["roofline"]["avg_launch_ms"]
+        rep["roofline"]["vendor_sort"] = vs
+        _log(f"rocPRIM radix sort of {hd['N']} keys on bits {vs['bits']}: {vs['ms']:.2f} ms "
+             f"({vs['ms_per_8bit_digit']:.2f} ms per 8-bit digit; onesweep pass {vs['onesweep_ms_per_pass']:.2f} ms)")
 
     line = {
         "metric": METRIC,

@@ -24,7 +24,8 @@ INC = os.path.join(ROOT, "include")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 ARCH = os.environ.get("GKC_ARCH", "gfx950")
-HIP_SOURCES = ["partition.hip", "sort.hip", "mlp.hip", "ingest.hip", "calib.hip", "positions.hip", "capi.hip"]
+HIP_SOURCES = ["partition.hip", "sort.hip", "mlp.hip", "ingest.hip", "calib.hip", "positions.hip", "vendor_sort.hip",
+               "capi.hip"]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-result",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", INC, "-I", CSRC]
 

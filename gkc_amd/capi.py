@@ -102,6 +102,7 @@ SIGNATURES = {
     "gkc_parse_reads": (C.c_int64, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gkc_parse_reads_noalloc": (C.c_int64, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gkc_stream_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "gkc_vendor_sort_u64": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.c_void_p, _f32p]),
     "gkc_set_predictor": (C.c_int, [C.c_void_p, C.c_void_p, _f32p, _f32p, _f32p, _f32p, C.c_float]),
     "gkc_learned_caps": (C.c_int, [C.c_void_p, _u32p]),
     "gkc_superkmer_positions": (C.c_int64, [C.c_void_p, C.c_int32, _u32p, _u32p, _u32p, C.c_uint64]),
@@ -355,6 +356,15 @@ def stream_copy(dst_ptr: int, src_ptr: int, nbytes: int, stream: int = 0) -> Non
     """The library's streaming copy (device memory, 16-byte aligned sizes and
     addresses), asynchronous on `stream`: the practical HBM ceiling."""
     _check(lib().gkc_stream_copy(C.c_void_p(dst_ptr), C.c_void_p(src_ptr), nbytes, C.c_void_p(stream)))
+
+
+def vendor_sort_u64(in_ptr: int, out_ptr: int, n: int, begin_bit: int, end_bit: int, stream: int = 0) -> float:
+    """rocPRIM's radix sort of n device u64 keys on bits [begin_bit, end_bit)
+    (a baseline for the onesweep passes); returns its milliseconds."""
+    ms = np.zeros(1, np.float32)
+    _check(lib().gkc_vendor_sort_u64(C.c_void_p(in_ptr), C.c_void_p(out_ptr), n, begin_bit, end_bit,
+                                     C.c_void_p(stream), _p(ms, _f32p)))
+    return float(ms[0])
 
 
 # bytes of gkc_parse_reads_noalloc's fixed device scratch (GKC_PARSE_NOALLOC_SCRATCH_BYTES)

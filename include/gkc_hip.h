@@ -252,6 +252,12 @@ int gkc_device_select(int device);
  * reports its kernels against, and a kernel of known traffic for HBM byte
  * counters.  Asynchronous; GKC_EINVAL for misaligned or odd sizes. */
 int gkc_stream_copy(void *dst, const void *src, uint64_t bytes, void *stream);
+/* The vendor library's radix sort (rocPRIM radix_sort_keys) of n 64-bit keys
+ * on bits [begin_bit, end_bit), device buffers, timed with HIP events on
+ * `stream` into *ms (synchronous; allocates its temporary storage): a
+ * baseline for the onesweep passes, no reference counterpart. */
+int gkc_vendor_sort_u64(const void *keys_in, void *keys_out, uint64_t n, int begin_bit, int end_bit, void *stream,
+                        float *ms);
 
 /* Per-partition 4-mer sketch taken during gkc_partition's scatter: float
  * [P][256] holding log1p(count) of the leading 4-mers (first base most

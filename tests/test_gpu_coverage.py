@@ -251,3 +251,31 @@ def test_sketch_matches_cpu_recomputation(gpu, cid, scale, k, p, P, hpc, flags):
     want = _sketch_cpu(src, rs.offs, lens, k, p, P)
     assert want.sum() > 100
     np.testing.assert_array_equal(got, want)
+
+
+# ---------------------------------------------------------------- measurement helpers
+
+def test_stream_copy_and_vendor_sort(gpu):
+    """The two baselines the bench reports beside the onesweep pass: the
+    library's streaming copy copies, and the vendor sort sorts (on the bit
+    range asked for, and entirely)."""
+    import torch
+    _say("stream copy + vendor sort")
+    n = 1 << 20
+    a = torch.randint(0, 2 ** 62, (n,), dtype=torch.int64, device="cuda")
+    b = torch.zeros_like(a)
+    st = torch.cuda.current_stream().cuda_stream
+    capi.stream_copy(b.data_ptr(), a.data_ptr(), n * 8, st)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    with pytest.raises(capi.GkcError):
+        capi.stream_copy(b.data_ptr() + 4, a.data_ptr(), 64, st)     # 16-byte alignment required
+    host = a.cpu().numpy().astype(np.uint64)
+    ms = capi.vendor_sort_u64(a.data_ptr(), b.data_ptr(), n, 0, 64, st)
+    assert ms > 0
+    np.testing.assert_array_equal(b.cpu().numpy().astype(np.uint64), np.sort(host))
+    capi.vendor_sort_u64(a.data_ptr(), b.data_ptr(), n, 46, 62, st)
+    got = b.cpu().numpy().astype(np.uint64)
+    d = (got >> np.uint64(46)) & np.uint64(0xFFFF)
+    assert (np.diff(d.astype(np.int64)) >= 0).all()
+    np.testing.assert_array_equal(np.sort(got), np.sort(host))
