@@ -256,6 +256,7 @@ def run_config(cid: int, scale: float, steps: int, warmup: int, forms: list[str]
         cache = autotune.load_cache()
         cache[tkey] = {"sort_items": res["sort_items"], "sort_blocks_per_cu": res["sort_blocks_per_cu"],
                        "sort_waves": res["sort_waves"], "ms_pass": res["ms_pass"], "bin_passes": resp["bin_passes"],
+                       "digit_bits": resp["digit_bits"],
                        "count_window": resp["count_window"], "ms_count": resp["ms_count"], "workload": w.name}
         autotune.save_cache(cache)
     tuning = autotune.apply(ctx, tkey)

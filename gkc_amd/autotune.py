@@ -5,9 +5,10 @@ or 8 for 64-bit keys: the same tile LDS with twice the waves to hide the
 loads behind) and the number of persistent workgroups per CU of the sort
 passes by timing every candidate on the data
 already in a counter's bins (the batched form, instrumented pass timing),
-then the number of onesweep passes before the binned count (2 or 3: fewer
-passes leave larger bins, which pays when the bins stay small enough for
-the wave-level counts), and caches the winners keyed on
+then the digit width and number of onesweep passes before the binned count
+(8-, 7- or 6-bit digits, 2 or 3 passes: fewer or narrower passes leave
+larger bins, which pays when the bins stay small enough for the wave-level
+counts) and the count window, and caches the winners keyed on
 (key width, read-length bucket, partition-size bucket, partitions) in
 gkc_amd/tune_cache.json.  Every candidate produces the same table
 (bit-exactness across variants is checked in tests/test_autotune.py and
@@ -74,7 +75,8 @@ def apply(counter: "capi.Counter", key: str) -> dict | None:
     t = nearest(key)
     if t:
         counter.set_tuning(t["sort_items"], t["sort_blocks_per_cu"], bin_passes=t.get("bin_passes", 0),
-                           count_window=t.get("count_window", 0), sort_waves=t.get("sort_waves", 0))
+                           count_window=t.get("count_window", 0), sort_waves=t.get("sort_waves", 0),
+                           digit_bits=t.get("digit_bits", 0))
     return t
 
 
@@ -107,30 +109,34 @@ def tune_sort(counter: "capi.Counter", reps: int = 2, log=None) -> dict:
 
 
 COUNT_WINDOWS = (256, 384)
+DIGIT_BITS = (8, 7, 6)
 
 
 def tune_bin_passes(counter: "capi.Counter", reps: int = 3, log=None) -> dict:
     """Time whole batched counts (graph replay) of the counter's current bins
-    with 2 and 3 onesweep passes before the binned count and windows of 256
-    and 384 keys (larger windows spread each window's fixed work over more
-    keys, but hold more distinct keys); returns the best {bin_passes,
+    with digits of 8, 7 or 6 bits, 2 or 3 onesweep passes before the binned
+    count (bins of 12 to 24 key bits: narrower digits rank with fewer ballots
+    and smaller histograms but leave larger bins) and windows of 256 and 384
+    keys (larger windows spread each window's fixed work over more keys, but
+    hold more distinct keys); returns the best {digit_bits, bin_passes,
     count_window, ms_count} and the sweep.  The sort tuning already set is
-    kept."""
+    kept (6- and 7-bit passes use the nearest tile they are built for)."""
     t0 = counter.get_tuning()
     sweep = []
-    for npass in (3, 2):
-        for cw in COUNT_WINDOWS:
-            counter.set_tuning(t0["sort_items"], t0["sort_blocks_per_cu"], t0["single_list"], bin_passes=npass,
-                               count_window=cw, sort_waves=t0["sort_waves"])
-            best = None
-            for _ in range(reps):
-                counter.count(capi.BATCHED)
-                ms = counter.stats()["ms_count"]
-                best = ms if best is None else min(best, ms)
-            sweep.append({"bin_passes": npass, "count_window": cw, "ms_count": best})
-            if log:
-                log(f"tune bin_passes={npass} count_window={cw}: {best:.3f} ms per count")
+    for db in DIGIT_BITS:
+        for npass in (3, 2):
+            for cw in COUNT_WINDOWS:
+                counter.set_tuning(t0["sort_items"], t0["sort_blocks_per_cu"], t0["single_list"], bin_passes=npass,
+                                   count_window=cw, sort_waves=t0["sort_waves"], digit_bits=db)
+                best = None
+                for _ in range(reps):
+                    counter.count(capi.BATCHED)
+                    ms = counter.stats()["ms_count"]
+                    best = ms if best is None else min(best, ms)
+                sweep.append({"digit_bits": db, "bin_passes": npass, "count_window": cw, "ms_count": best})
+                if log:
+                    log(f"tune digit_bits={db} bin_passes={npass} count_window={cw}: {best:.3f} ms per count")
     win = min(sweep, key=lambda r: r["ms_count"])
     counter.set_tuning(t0["sort_items"], t0["sort_blocks_per_cu"], t0["single_list"], bin_passes=win["bin_passes"],
-                       count_window=win["count_window"], sort_waves=t0["sort_waves"])
+                       count_window=win["count_window"], sort_waves=t0["sort_waves"], digit_bits=win["digit_bits"])
     return {**win, "sweep": sweep}

@@ -42,7 +42,8 @@ class Params(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("sort_items", C.c_int32), ("sort_blocks_per_cu", C.c_int32), ("single_list", C.c_int32),
                 ("count_mode", C.c_int32),
-                ("bin_passes", C.c_int32), ("count_window", C.c_int32), ("sort_waves", C.c_int32)]
+                ("bin_passes", C.c_int32), ("count_window", C.c_int32), ("sort_waves", C.c_int32),
+                ("digit_bits", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -242,12 +243,15 @@ class Counter:
         _check(lib().gkc_set_table_caps(self.h, _p(caps, _u32p)))
 
     def set_tuning(self, sort_items: int = 0, sort_blocks_per_cu: int = 0, single_list: int = 0,
-                   count_mode: int = 0, bin_passes: int = 0, count_window: int = 0, sort_waves: int = 0) -> None:
+                   count_mode: int = 0, bin_passes: int = 0, count_window: int = 0, sort_waves: int = 0,
+                   digit_bits: int = 0) -> None:
         """count_mode (batched form): 0 binned counting, 1 full LSD sort + run-length;
         bin_passes: onesweep passes before the binned count (0 automatic);
         count_window: keys per window of the binned count (0: 256);
-        sort_waves: waves per onesweep workgroup, 4 or 8 (0: 4)."""
-        t = Tuning(sort_items, sort_blocks_per_cu, single_list, count_mode, bin_passes, count_window, sort_waves)
+        sort_waves: waves per onesweep workgroup, 4 or 8 (0: 4);
+        digit_bits: digit width of the binned form's passes, 6..8 (0: 8)."""
+        t = Tuning(sort_items, sort_blocks_per_cu, single_list, count_mode, bin_passes, count_window, sort_waves,
+                   digit_bits)
         _check(lib().gkc_set_tuning(self.h, C.byref(t)))
 
     def get_tuning(self) -> dict:
@@ -255,7 +259,7 @@ class Counter:
         _check(lib().gkc_get_tuning(self.h, C.byref(t)))
         return {"sort_items": t.sort_items, "sort_blocks_per_cu": t.sort_blocks_per_cu, "single_list": t.single_list,
                 "count_mode": t.count_mode, "bin_passes": t.bin_passes, "count_window": t.count_window,
-                "sort_waves": t.sort_waves}
+                "sort_waves": t.sort_waves, "digit_bits": t.digit_bits}
 
     def part_kmers(self) -> np.ndarray:
         out = np.zeros(self.n_parts, np.uint64)

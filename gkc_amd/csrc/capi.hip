@@ -193,8 +193,8 @@ static int occupancy_blocks(const void* fn, int nt, int n_cu, size_t dyn_lds = 0
 
 static void zero(gkc_ctx* c, void* p, uint64_t n32);
 template <typename F> hipGraphExec_t capture(gkc_ctx* c, F&& enqueue);
-template <typename K> static int sort_items(gkc_ctx* c);
-template <typename K> static int sort_nt(gkc_ctx* c);
+template <typename K> static int sort_items(gkc_ctx* c, int db);
+template <typename K> static int sort_nt(gkc_ctx* c, int db);
 
 // ---------------------------------------------------------------- bins
 
@@ -1047,6 +1047,8 @@ int gkc_set_tuning(gkc_ctx* c, const gkc_tuning* t) {
         if (t->bin_passes < 0 || t->bin_passes > 3) throw ArgError{"bin_passes must be 0..3", GKC_EINVAL};
         if (t->sort_waves != 0 && t->sort_waves != 4 && t->sort_waves != 8)
             throw ArgError{"sort_waves must be 0, 4 or 8", GKC_EINVAL};
+        if (t->digit_bits != 0 && (t->digit_bits < 6 || t->digit_bits > 8))
+            throw ArgError{"digit_bits must be 0 or 6..8", GKC_EINVAL};
         if (t->count_window != 0 && (t->count_window < 128 || t->count_window > 1024 || t->count_window % 64))
             throw ArgError{"count_window must be 0 or a multiple of 64 in 128..1024", GKC_EINVAL};
         c->tune = *t;
@@ -1057,8 +1059,9 @@ int gkc_set_tuning(gkc_ctx* c, const gkc_tuning* t) {
 int gkc_get_tuning(gkc_ctx* c, gkc_tuning* t) {
     return guarded(c, [&] {
         gkc_tuning r = c->tune;
-        r.sort_items = c->keyw == 8 ? sort_items<uint64_t>(c) : sort_items<u128k>(c);
-        r.sort_waves = (c->keyw == 8 ? sort_nt<uint64_t>(c) : sort_nt<u128k>(c)) / 64;
+        r.sort_items = c->keyw == 8 ? sort_items<uint64_t>(c, 8) : sort_items<u128k>(c, 8);
+        r.sort_waves = (c->keyw == 8 ? sort_nt<uint64_t>(c, 8) : sort_nt<u128k>(c, 8)) / 64;
+        r.digit_bits = c->tune.digit_bits ? c->tune.digit_bits : 8;
         if (t) *t = r;
         return 0;
     });
@@ -1145,46 +1148,55 @@ static void zero(gkc_ctx* c, void* p, uint64_t n32) {
 // Run extraction + sort for a plan whose per-segment arrays are set; the
 // sorted keys end in keysA or keysB (returned).  Host knows n_keys_total and
 // the sort tile count upper bound.
-// onesweep variants (keys per thread x threads per workgroup), selected by
-// gkc_tuning.sort_items and gkc_tuning.sort_waves
+// onesweep variants (keys per thread x threads per workgroup x digit bits),
+// selected by gkc_tuning.sort_items, sort_waves and digit_bits (the last in
+// the binned form only: the full LSD sort keeps 8-bit digits)
 template <typename K>
 using OnesweepFn = void (*)(Plan*, SegArrays, WorkLists, const K*, K*, uint32_t*, int);
-template <typename K> static OnesweepFn<K> onesweep_variant(int items, int nt);
-template <> OnesweepFn<uint64_t> onesweep_variant<uint64_t>(int items, int nt) {
+template <typename K> static OnesweepFn<K> onesweep_variant(int items, int nt, int db);
+template <> OnesweepFn<uint64_t> onesweep_variant<uint64_t>(int items, int nt, int db) {
+    if (db == 7) return items == 16 ? k_onesweep<uint64_t, 16, 256, 7> : k_onesweep<uint64_t, 20, 256, 7>;
+    if (db == 6) return items == 16 ? k_onesweep<uint64_t, 16, 256, 6> : k_onesweep<uint64_t, 20, 256, 6>;
     if (nt == 512) {
         switch (items) {
-            case 8: return k_onesweep<uint64_t, 8, 512>;
-            case 12: return k_onesweep<uint64_t, 12, 512>;
-            default: return k_onesweep<uint64_t, 10, 512>;
+            case 8: return k_onesweep<uint64_t, 8, 512, 8>;
+            case 12: return k_onesweep<uint64_t, 12, 512, 8>;
+            default: return k_onesweep<uint64_t, 10, 512, 8>;
         }
     }
     switch (items) {
-        case 8: return k_onesweep<uint64_t, 8, 256>;
-        case 12: return k_onesweep<uint64_t, 12, 256>;
-        case 20: return k_onesweep<uint64_t, 20, 256>;
-        case 24: return k_onesweep<uint64_t, 24, 256>;
-        default: return k_onesweep<uint64_t, 16, 256>;
+        case 8: return k_onesweep<uint64_t, 8, 256, 8>;
+        case 12: return k_onesweep<uint64_t, 12, 256, 8>;
+        case 20: return k_onesweep<uint64_t, 20, 256, 8>;
+        case 24: return k_onesweep<uint64_t, 24, 256, 8>;
+        default: return k_onesweep<uint64_t, 16, 256, 8>;
     }
 }
-template <> OnesweepFn<u128k> onesweep_variant<u128k>(int items, int) {
+template <> OnesweepFn<u128k> onesweep_variant<u128k>(int items, int, int db) {
+    if (db == 7) return items == 12 ? k_onesweep<u128k, 12, 256, 7> : k_onesweep<u128k, 8, 256, 7>;
+    if (db == 6) return items == 12 ? k_onesweep<u128k, 12, 256, 6> : k_onesweep<u128k, 8, 256, 6>;
     switch (items) {
-        case 4: return k_onesweep<u128k, 4, 256>;
-        case 12: return k_onesweep<u128k, 12, 256>;
-        default: return k_onesweep<u128k, 8, 256>;
+        case 4: return k_onesweep<u128k, 4, 256, 8>;
+        case 12: return k_onesweep<u128k, 12, 256, 8>;
+        default: return k_onesweep<u128k, 8, 256, 8>;
     }
 }
-// threads per onesweep workgroup: 8 waves on request for 64-bit keys, else 4
-template <typename K> static int sort_nt(gkc_ctx* c) { return sizeof(K) == 8 && c->tune.sort_waves == 8 ? 512 : 256; }
-template <typename K> static int sort_items(gkc_ctx* c) {
+// threads per onesweep workgroup: 8 waves on request for 64-bit keys and
+// 8-bit digits, else 4
+template <typename K> static int sort_nt(gkc_ctx* c, int db) {
+    return sizeof(K) == 8 && db == 8 && c->tune.sort_waves == 8 ? 512 : 256;
+}
+template <typename K> static int sort_items(gkc_ctx* c, int db) {
     const int it = c->tune.sort_items;
-    if (sort_nt<K>(c) == 512) return (it == 8 || it == 10 || it == 12) ? it : 10;
+    if (db != 8) return sizeof(K) == 8 ? (it == 16 ? 16 : 20) : (it == 12 ? 12 : 8);
+    if (sort_nt<K>(c, db) == 512) return (it == 8 || it == 10 || it == 12) ? it : 10;
     if (sizeof(K) == 8) return (it == 8 || it == 12 || it == 16 || it == 20 || it == 24) ? it : 16;
     return (it == 4 || it == 8 || it == 12) ? it : 8;
 }
-template <typename K> static int sort_tile_keys(gkc_ctx* c) { return sort_nt<K>(c) * sort_items<K>(c); }
-template <typename K> static int sort_grid(gkc_ctx* c) {
-    OnesweepFn<K> fn = onesweep_variant<K>(sort_items<K>(c), sort_nt<K>(c));
-    int g = occupancy_blocks((const void*)fn, sort_nt<K>(c), c->n_cu);
+template <typename K> static int sort_tile_keys(gkc_ctx* c, int db) { return sort_nt<K>(c, db) * sort_items<K>(c, db); }
+template <typename K> static int sort_grid(gkc_ctx* c, int db) {
+    OnesweepFn<K> fn = onesweep_variant<K>(sort_items<K>(c, db), sort_nt<K>(c, db), db);
+    int g = occupancy_blocks((const void*)fn, sort_nt<K>(c, db), c->n_cu);
     if (c->tune.sort_blocks_per_cu > 0) g = c->tune.sort_blocks_per_cu * c->n_cu;   // explicit (may exceed residency)
     return std::max(g, 1);
 }
@@ -1236,24 +1248,24 @@ static void set_lists(gkc_ctx* c, uint32_t n_segs, const uint32_t* keys, const u
 }
 
 template <typename K>
-static void launch_onesweep(gkc_ctx* c, int grid, Plan* plan, SegArrays sa, const K* src, K* dst, int pass) {
-    OnesweepFn<K> fn = onesweep_variant<K>(sort_items<K>(c), sort_nt<K>(c));
+static void launch_onesweep(gkc_ctx* c, int grid, Plan* plan, SegArrays sa, const K* src, K* dst, int pass, int db) {
+    OnesweepFn<K> fn = onesweep_variant<K>(sort_items<K>(c, db), sort_nt<K>(c, db), db);
     uint32_t* lb = c->lookback.as<uint32_t>();
     WorkLists wl = c->wl;
     void* args[] = {&plan, &sa, &wl, &src, &dst, &lb, &pass};
-    HIPX(hipLaunchKernel((const void*)fn, dim3(grid), dim3(sort_nt<K>(c)), args, 0, c->st));
+    HIPX(hipLaunchKernel((const void*)fn, dim3(grid), dim3(sort_nt<K>(c, db)), args, 0, c->st));
 }
 
 template <typename K>
 static K* sort_segments(gkc_ctx* c, SegArrays sa, const uint64_t* hdr_base, const uint64_t* word_base,
                         uint32_t n_segs, uint64_t n_keys, uint64_t n_hdrs, float* ms_ex, float* ms_sort, float* ms_pass_sum) {
     Plan* plan = c->d_plan.as<Plan>();
-    const int TILE = sort_tile_keys<K>(c);
+    const int TILE = sort_tile_keys<K>(c, 8);
     {
         const uint32_t nk = (uint32_t)n_keys, nh = (uint32_t)n_hdrs;
         set_lists(c, 1, &nk, &nh, 1, TILE);
     }
-    hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(64), 0, c->st, plan, n_segs, (uint32_t)c->npass, 0u);
+    hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(64), 0, c->st, plan, n_segs, (uint32_t)c->npass, 0u, (uint32_t)RADIX_BITS);
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, c->st, plan, sa, TILE, RL_TILE);
     HIPX(hipGetLastError());
     zero(c, sa.hist, (uint64_t)n_segs * MAX_PASS * RADIX);
@@ -1268,13 +1280,13 @@ static K* sort_segments(gkc_ctx* c, SegArrays sa, const uint64_t* hdr_base, cons
     HIPX(hipGetLastError());
     HIPX(hipEventRecord(c->ev[3], c->st));
     const uint64_t tiles = n_keys / TILE + n_segs;
-    int g_s = (int)std::min<uint64_t>(tiles, sort_grid<K>(c));
+    int g_s = (int)std::min<uint64_t>(tiles, sort_grid<K>(c, 8));
     K* src = c->keysA.as<K>();
     K* dst = c->keysB.as<K>();
     for (int ps = 0; ps < c->npass; ps++) {
         zero(c, c->lookback.p, tiles * RADIX);
         HIPX(hipEventRecord(c->evp[2 * ps], c->st));
-        launch_onesweep<K>(c, std::max(g_s, 1), plan, sa, src, dst, ps);
+        launch_onesweep<K>(c, std::max(g_s, 1), plan, sa, src, dst, ps, 8);
         HIPX(hipGetLastError());
         HIPX(hipEventRecord(c->evp[2 * ps + 1], c->st));
         std::swap(src, dst);
@@ -1505,14 +1517,16 @@ struct BatchShape {
     uint32_t sh0;
     int rl_tile;
     int bin_bits;
+    int dbits;      // digit width of the passes
 };
 template <typename K>
 static BatchShape batch_shape(const gkc_ctx* c) {
     const int kb = 2 * c->prm.k;
-    if (c->tune.count_mode == 1 || c->prm.k < 8) return BatchShape{false, false, c->npass, 0u, RL_TILE, 0};
-    const int np = std::min(c->tune.bin_passes > 0 ? c->tune.bin_passes : 3, (kb + 7) / 8);
+    if (c->tune.count_mode == 1 || c->prm.k < 8) return BatchShape{false, false, c->npass, 0u, RL_TILE, 0, RADIX_BITS};
+    const int db = c->tune.digit_bits > 0 ? c->tune.digit_bits : RADIX_BITS;
+    const int np = std::min(c->tune.bin_passes > 0 ? c->tune.bin_passes : 3, (kb + db - 1) / db);
     const int win = c->tune.count_window > 0 ? c->tune.count_window : WC_TILE;
-    return BatchShape{true, true, np, (uint32_t)std::max(0, kb - 8 * np), win, 8 * np};
+    return BatchShape{true, true, np, (uint32_t)std::max(0, kb - db * np), win, db * np, db};
 }
 
 // Enqueue extraction + the onesweep passes over every partition at once.
@@ -1522,8 +1536,8 @@ template <typename K>
 static K* enqueue_sort_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs, uint64_t tiles_cap, bool timed) {
     const uint32_t P = c->prm.n_parts;
     Plan* plan = c->d_plan.as<Plan>();
-    const int TILE = sort_tile_keys<K>(c);
-    hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(64), 0, c->st, plan, P, (uint32_t)bs.npass, bs.sh0);
+    const int TILE = sort_tile_keys<K>(c, bs.dbits);
+    hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(64), 0, c->st, plan, P, (uint32_t)bs.npass, bs.sh0, (uint32_t)bs.dbits);
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, c->st, plan, sa, TILE, bs.rl_tile);
     zero(c, sa.hist, (uint64_t)P * MAX_PASS * RADIX);
     if (timed) HIPX(hipEventRecord(c->ev[2], c->st));
@@ -1534,13 +1548,13 @@ static K* enqueue_sort_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs, uint6
                        c->keysA.as<K>(), c->prm.k);
     hipLaunchKernelGGL(k_hist_scan, dim3((P * MAX_PASS + 3) / 4), dim3(256), 0, c->st, plan, sa.hist);
     if (timed) HIPX(hipEventRecord(c->ev[3], c->st));
-    const int g_s = sort_grid<K>(c);
+    const int g_s = sort_grid<K>(c, bs.dbits);
     K* src = c->keysA.as<K>();
     K* dst = c->keysB.as<K>();
     for (int ps = 0; ps < bs.npass; ps++) {
         zero(c, c->lookback.p, tiles_cap * RADIX);
         if (timed) HIPX(hipEventRecord(c->evp[2 * ps], c->st));
-        launch_onesweep<K>(c, g_s, plan, sa, src, dst, ps);
+        launch_onesweep<K>(c, g_s, plan, sa, src, dst, ps, bs.dbits);
         if (timed) HIPX(hipEventRecord(c->evp[2 * ps + 1], c->st));
         std::swap(src, dst);
     }
@@ -1673,7 +1687,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     c->stats.n_pass = bs.npass;
     c->stats.binned = bs.binned ? 1 : 0;
     ensure_count_buffers<K>(c, tot_keys, P);
-    const uint64_t tiles_cap = tot_keys / (sort_tile_keys<K>(c)) + P + 1;
+    const uint64_t tiles_cap = tot_keys / (sort_tile_keys<K>(c, bs.dbits)) + P + 1;
     const uint64_t rl_tiles_cap = tot_keys / bs.rl_tile + P + 1;
     SegArrays sa;
     sa.seg_keys = c->d_kmer_cur.as<uint32_t>(); sa.seg_hdrs = c->d_hdr_cur.as<uint32_t>();
@@ -1686,7 +1700,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     c->d_hres.ensure(rl_tiles_cap * sizeof(uint2));
     c->d_wkept.ensure(rl_tiles_cap * sizeof(uint32_t));
     set_lists(c, P, c->h_kmer_cur.data(), c->h_hdr_cur.data(), c->tune.single_list == 1 ? 1 : NLIST,
-              sort_tile_keys<K>(c), bs.rl_tile);
+              sort_tile_keys<K>(c, bs.dbits), bs.rl_tile);
 
     // table capacities: predicted on the device (learned), given by the caller
     // (presized) or from a count pass (count-then-allocate)
@@ -1694,7 +1708,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     if ((flags & GKC_LEARNED) && !c->mlp) throw ArgError{"GKC_LEARNED without gkc_set_predictor", GKC_ESTATE};
     const bool presized = learned || ((flags & GKC_PRESIZED) && c->presize_caps.size() == P);
     (void)occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu, (size_t)bs.npass * RADIX * sizeof(uint32_t));
-    (void)sort_grid<K>(c);
+    (void)sort_grid<K>(c, bs.dbits);
     (void)occupancy_blocks((const void*)k_runlength<K>, SORT_NT, c->n_cu);
     (void)occupancy_blocks((const void*)k_binranges<K>, 256, c->n_cu);
     (void)occupancy_blocks((const void*)k_bincount<K>, BC_NT, c->n_cu);
@@ -1714,8 +1728,8 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     std::vector<const void*> key = {c->keysA.p, c->keysB.p, c->lookback.p, c->words.p, c->hdrs.p, c->d_ranges.p,
                                     c->d_hlist.p, c->d_hres.p, c->d_wkept.p,
                                     (const void*)(uintptr_t)tiles_cap, (const void*)(uintptr_t)bs.npass,
-                                    (const void*)(uintptr_t)sort_items<K>(c), (const void*)(uintptr_t)sort_grid<K>(c),
-                                    (const void*)(uintptr_t)sort_nt<K>(c),
+                                    (const void*)(uintptr_t)sort_items<K>(c, bs.dbits), (const void*)(uintptr_t)sort_grid<K>(c, bs.dbits),
+                                    (const void*)(uintptr_t)sort_nt<K>(c, bs.dbits), (const void*)(uintptr_t)bs.dbits,
                                     c->d_wl.p, (const void*)(uintptr_t)c->wl.n_lists,
                                     (const void*)(uintptr_t)(bs.binned ? 1 : 0), (const void*)(uintptr_t)(bs.wave ? 1 : 0),
                                     (const void*)(uintptr_t)bs.rl_tile};

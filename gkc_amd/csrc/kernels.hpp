@@ -70,6 +70,7 @@ struct Plan {
     uint32_t tot_sort_tiles;
     uint32_t tot_rl_tiles;
     uint32_t sh0;               // bit position of pass 0's digit (0: full LSD sort)
+    uint32_t dbits;             // digit width of the passes (8; 6 or 7 in the binned form)
     uint64_t tot_keys;
     uint32_t ctr[NCTR];         // work counters, NLIST per stage: see ctr_extract / ctr_pass / ctr_rl
 };
@@ -117,6 +118,8 @@ template <typename K> struct KeyTraits;
 template <> struct KeyTraits<uint64_t> {
     static constexpr int ITEMS = 16;               // keys per thread in a sort tile
     __device__ static inline uint32_t digit(uint64_t x, int sh) { return (uint32_t)(x >> sh) & (RADIX - 1); }
+    // bits [sh, sh + nb), nb <= 8
+    __device__ static inline uint32_t digit_n(uint64_t x, int sh, int nb) { return (uint32_t)(x >> sh) & ((1u << nb) - 1u); }
 };
 template <> struct KeyTraits<u128k> {
     static constexpr int ITEMS = 8;
@@ -125,6 +128,12 @@ template <> struct KeyTraits<u128k> {
         uint64_t v = x.lo >> sh;
         if (sh > 64 - RADIX_BITS) v |= x.hi << (64 - sh);
         return (uint32_t)v & (RADIX - 1);
+    }
+    __device__ static inline uint32_t digit_n(const u128k& x, int sh, int nb) {   // bits [sh, sh+nb), nb <= 8
+        if (sh >= 64) return (uint32_t)(x.hi >> (sh - 64)) & ((1u << nb) - 1u);
+        uint64_t v = x.lo >> sh;
+        if (sh > 64 - nb) v |= x.hi << (64 - sh);
+        return (uint32_t)v & ((1u << nb) - 1u);
     }
 };
 template <typename K> constexpr int sort_tile() { return SORT_NT * KeyTraits<K>::ITEMS; }
@@ -144,14 +153,14 @@ __global__ void k_plan(Plan* plan, SegArrays sa, int sort_tile_keys, int rl_tile
 __global__ void k_zero32(uint32_t* p, uint64_t n);
 __global__ void k_set_u64(uint64_t* p, uint64_t v);
 __global__ void k_set_u32(uint32_t* p, uint32_t v);
-__global__ void k_plan_init(Plan* plan, uint32_t n_segs, uint32_t n_pass, uint32_t sh0);
+__global__ void k_plan_init(Plan* plan, uint32_t n_segs, uint32_t n_pass, uint32_t sh0, uint32_t dbits);
 __global__ void k_set_chunk(ChunkDesc* dst, ChunkDesc v);
 template <typename K>
 __global__ void k_extract(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const SkmHdr* __restrict__ hdrs,
                           const uint64_t* __restrict__ hdr_base, const uint32_t* __restrict__ words,
                           const uint64_t* __restrict__ word_base, K* __restrict__ keys, int k);
 __global__ void k_hist_scan(Plan* __restrict__ plan, uint32_t* __restrict__ hist);
-template <typename K, int ITEMS, int NT>
+template <typename K, int ITEMS, int NT, int DB>
 __global__ void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ in, K* __restrict__ out,
                            uint32_t* __restrict__ lookback, int pass);
 template <typename K>

@@ -134,10 +134,10 @@ __global__ __launch_bounds__(1024) void k_plan(Plan* plan, SegArrays sa, int sor
 // launch, so no host buffer has to outlive an asynchronous copy).
 __global__ void k_set_u64(uint64_t* p, uint64_t v) { *p = v; }
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
-__global__ void k_plan_init(Plan* plan, uint32_t n_segs, uint32_t n_pass, uint32_t sh0) {
+__global__ void k_plan_init(Plan* plan, uint32_t n_segs, uint32_t n_pass, uint32_t sh0, uint32_t dbits) {
     const int t = threadIdx.x;
     if (t == 0) { plan->n_segs = n_segs; plan->n_pass = n_pass; plan->tot_items = 0; plan->tot_sort_tiles = 0;
-                  plan->tot_rl_tiles = 0; plan->sh0 = sh0; plan->tot_keys = 0; }
+                  plan->tot_rl_tiles = 0; plan->sh0 = sh0; plan->dbits = dbits; plan->tot_keys = 0; }
     for (int i = t; i < NCTR; i += blockDim.x) plan->ctr[i] = 0;
 }
 __global__ void k_set_chunk(ChunkDesc* dst, ChunkDesc v) { *dst = v; }
@@ -150,7 +150,8 @@ __global__ void k_zero32(uint32_t* p, uint64_t n) {
 
 // Extract the canonical k-mers of each superkmer into the segment's key
 // range and accumulate the digit histograms of all passes (pass p's digit is
-// key bits [sh0 + 8p, sh0 + 8p + 8)).
+// key bits [sh0 + d p, sh0 + d p + d), d = plan->dbits; each pass's histogram
+// keeps RADIX slots, the first 2^d used).
 // Bases are 2-bit, 16 per u32 word, MSB first.
 template <typename K>
 __device__ __forceinline__ K kmer_at(const uint32_t* wds, uint32_t o, int k);
@@ -218,6 +219,7 @@ __global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, Se
     __shared__ uint32_t s_seg, s_idx, s_ok;
     const int npass = plan->n_pass;
     const int sh0 = plan->sh0;
+    const int db = plan->dbits;
     const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE;
     uint32_t list = blockIdx.x % wl.n_lists;
     // the next item is claimed, and its headers loaded, while this one is
@@ -312,7 +314,7 @@ __global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, Se
                 if (qb + u * GKC_WAVE + lane < T) {
                     kseg[s_koff[wv][hj[u]] + ho[u]] = c[u];
                     for (int ps = 0; ps < npass; ps++)
-                        atomicAdd(&s_hist[ps * RADIX + KeyTraits<K>::digit(c[u], sh0 + ps * RADIX_BITS)], 1u);
+                        atomicAdd(&s_hist[ps * RADIX + KeyTraits<K>::digit_n(c[u], sh0 + ps * db, db)], 1u);
                 }
             }
         }
@@ -367,7 +369,8 @@ __global__ __launch_bounds__(256) void k_hist_scan(Plan* __restrict__ plan, uint
 // the waves to hide the loads' latency behind, so it is held to 80 VGPRs
 // (6 waves per SIMD: three such workgroups per CU, the LDS limit).
 template <typename K, int NT> constexpr int onesweep_waves() { return NT == 512 ? 6 : sizeof(K) == 16 ? 3 : 4; }
-// One onesweep pass over all segments.  Tile = NT * ITEMS keys of one
+// One onesweep pass over all segments, DB-bit digits (RD = 2^DB <= NT
+// digits, one per thread of the first RD).  Tile = NT * ITEMS keys of one
 // segment; tiles are claimed in order from the work lists, so every tile's
 // predecessors are claimed by running workgroups (forward progress of the
 // look-back).  Per tile (claimed, then its keys loaded at once: a tile
@@ -377,28 +380,29 @@ template <typename K, int NT> constexpr int onesweep_waves() { return NT == 512 
 //   1. digits and the tile's digit histogram (LDS atomics, no returns)
 //   2. publish the aggregate at once, so successors can look back early
 //   4. stable rank within each wave in (item, lane) order: lanes with equal
-//      digits are matched with RADIX_BITS ballots, the running per-wave digit
+//      digits are matched with DB ballots, the running per-wave digit
 //      counters live in LDS
 //   5. decoupled look-back per digit across the segment's earlier tiles
 //   6. reorder the tile in LDS by digit and write each digit's run out
-template <typename K, int ITEMS, int NT>
+template <typename K, int ITEMS, int NT, int DB>
 __global__ __launch_bounds__(NT, (onesweep_waves<K, NT>())) void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ in,
                                                       K* __restrict__ out, uint32_t* __restrict__ lookback, int pass) {
-    static_assert(NT == RADIX || NT == 2 * RADIX, "256 or 512 threads");
+    constexpr int RD = 1 << DB;                    // digits of this pass
+    static_assert(DB >= 6 && DB <= RADIX_BITS && NT % RD == 0, "one digit per thread of whole waves");
     constexpr int TILE = NT * ITEMS;
     constexpr int NWAVE = NT / GKC_WAVE;           // ranking waves
-    constexpr int DWAVE = RADIX / GKC_WAVE;        // waves holding one digit per thread
+    constexpr int DWAVE = RD / GKC_WAVE;           // waves holding one digit per thread
     __shared__ K s_keys[TILE];
-    __shared__ uint32_t s_whist[NWAVE][RADIX];
-    __shared__ uint32_t s_hist[RADIX];
-    __shared__ uint32_t s_dstart[RADIX];
+    __shared__ uint32_t s_whist[NWAVE][RD];
+    __shared__ uint32_t s_hist[RD];
+    __shared__ uint32_t s_dstart[RD];
     __shared__ uint32_t s_wsum[DWAVE];
-    __shared__ uint64_t s_gbase[RADIX];
+    __shared__ uint64_t s_gbase[RD];
     __shared__ uint32_t s_seg, s_idx, s_ok;
-    const int sh = (int)plan->sh0 + pass * RADIX_BITS;
+    const int sh = (int)plan->sh0 + pass * DB;
     const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE;
-    const uint32_t d = threadIdx.x;     // threads below RADIX: one digit each
-    const bool dg = NT == RADIX || d < (uint32_t)RADIX;
+    const uint32_t d = threadIdx.x;     // threads below RD: one digit each
+    const bool dg = NT == RD || d < (uint32_t)RD;
     uint32_t* ctr = &plan->ctr[ctr_pass(pass)];
     uint32_t list = blockIdx.x % wl.n_lists;
     const uint32_t wbase = wv * GKC_WAVE * ITEMS;   // wave wv owns tile keys [wbase, wbase + 64*ITEMS)
@@ -433,7 +437,7 @@ __global__ __launch_bounds__(NT, (onesweep_waves<K, NT>())) void k_onesweep(Plan
                 s_seg = sg; s_idx = ix;
             }
             if (dg) s_hist[d] = 0;
-            for (int i = threadIdx.x; i < NWAVE * RADIX; i += NT) (&s_whist[0][0])[i] = 0;
+            for (int i = threadIdx.x; i < NWAVE * RD; i += NT) (&s_whist[0][0])[i] = 0;
             __syncthreads();
             const bool have_next = s_ok;
             const uint32_t nxt_seg = s_seg, nxt_ti = s_idx;
@@ -443,8 +447,8 @@ __global__ __launch_bounds__(NT, (onesweep_waves<K, NT>())) void k_onesweep(Plan
     #pragma unroll
             for (int i = 0; i < ITEMS; i++) {
                 const uint32_t j = wbase + i * GKC_WAVE + lane;
-                dig[i] = (j < tn) ? KeyTraits<K>::digit(key[i], sh) : RADIX;
-                if (dig[i] < RADIX) atomicAdd(&s_hist[dig[i]], 1u);
+                dig[i] = (j < tn) ? KeyTraits<K>::digit_n(key[i], sh, DB) : (uint32_t)RD;
+                if (dig[i] < (uint32_t)RD) atomicAdd(&s_hist[dig[i]], 1u);
             }
             __syncthreads();
             // 2. aggregate out
@@ -472,16 +476,16 @@ __global__ __launch_bounds__(NT, (onesweep_waves<K, NT>())) void k_onesweep(Plan
                 const uint32_t dd = dig[i];
                 // peers (lanes with this lane's digit): the complement of the OR
                 // over the digit bits of ballot(bit) ^ own bit, in 32-bit halves
-                const uint64_t v = __ballot(dd < RADIX);
+                const uint64_t v = __ballot(dd < (uint32_t)RD);
                 uint32_t mlo = ~(uint32_t)v, mhi = ~(uint32_t)(v >> 32);
     #pragma unroll
-                for (int b = 0; b < RADIX_BITS; b++) {
+                for (int b = 0; b < DB; b++) {
                     const uint32_t bm = (uint32_t)((int32_t)(dd << (31 - b)) >> 31);     // 0 or ~0
                     const uint64_t bal = __ballot(bm != 0u);
                     mlo |= (uint32_t)bal ^ bm;
                     mhi |= (uint32_t)(bal >> 32) ^ bm;
                 }
-                if (dd < RADIX) {
+                if (dd < (uint32_t)RD) {
                     const uint64_t peers = ~(((uint64_t)mhi << 32) | mlo);
                     const uint32_t before = s_whist[wv][dd];
                     const uint64_t lower = peers & lt;
@@ -528,12 +532,12 @@ __global__ __launch_bounds__(NT, (onesweep_waves<K, NT>())) void k_onesweep(Plan
     #pragma unroll
             for (int i = 0; i < ITEMS; i++) {
                 const uint32_t dd = dig[i] & 511u;
-                if (dd < RADIX) s_keys[s_dstart[dd] + s_whist[wv][dd] + (dig[i] >> 9)] = key[i];
+                if (dd < (uint32_t)RD) s_keys[s_dstart[dd] + s_whist[wv][dd] + (dig[i] >> 9)] = key[i];
             }
             __syncthreads();
             for (uint32_t j = threadIdx.x; j < tn; j += NT) {
                 const K x = s_keys[j];
-                out[s_gbase[KeyTraits<K>::digit(x, sh)] + j] = x;
+                out[s_gbase[KeyTraits<K>::digit_n(x, sh, DB)] + j] = x;
             }
             if (!have_next) break;
             seg = nxt_seg; ti = nxt_ti; nseg = n_nseg; kbase = n_kbase; tn = n_tn;
@@ -552,7 +556,7 @@ __global__ __launch_bounds__(NT, (onesweep_waves<K, NT>())) void k_onesweep(Plan
                 s_seg = sg; s_idx = ix;
             }
             if (dg) s_hist[d] = 0;
-            for (int i = threadIdx.x; i < NWAVE * RADIX; i += NT) (&s_whist[0][0])[i] = 0;
+            for (int i = threadIdx.x; i < NWAVE * RD; i += NT) (&s_whist[0][0])[i] = 0;
             __syncthreads();
             if (!s_ok) break;
             seg = s_seg; ti = s_idx;
@@ -570,8 +574,8 @@ __global__ __launch_bounds__(NT, (onesweep_waves<K, NT>())) void k_onesweep(Plan
     #pragma unroll
             for (int i = 0; i < ITEMS; i++) {
                 const uint32_t j = wbase + i * GKC_WAVE + lane;
-                dig[i] = (j < tn) ? KeyTraits<K>::digit(key[i], sh) : RADIX;
-                if (dig[i] < RADIX) atomicAdd(&s_hist[dig[i]], 1u);
+                dig[i] = (j < tn) ? KeyTraits<K>::digit_n(key[i], sh, DB) : (uint32_t)RD;
+                if (dig[i] < (uint32_t)RD) atomicAdd(&s_hist[dig[i]], 1u);
             }
             __syncthreads();
             // 2. aggregate out
@@ -585,16 +589,16 @@ __global__ __launch_bounds__(NT, (onesweep_waves<K, NT>())) void k_onesweep(Plan
                 const uint32_t dd = dig[i];
                 // peers (lanes with this lane's digit): the complement of the OR
                 // over the digit bits of ballot(bit) ^ own bit, in 32-bit halves
-                const uint64_t v = __ballot(dd < RADIX);
+                const uint64_t v = __ballot(dd < (uint32_t)RD);
                 uint32_t mlo = ~(uint32_t)v, mhi = ~(uint32_t)(v >> 32);
     #pragma unroll
-                for (int b = 0; b < RADIX_BITS; b++) {
+                for (int b = 0; b < DB; b++) {
                     const uint32_t bm = (uint32_t)((int32_t)(dd << (31 - b)) >> 31);     // 0 or ~0
                     const uint64_t bal = __ballot(bm != 0u);
                     mlo |= (uint32_t)bal ^ bm;
                     mhi |= (uint32_t)(bal >> 32) ^ bm;
                 }
-                if (dd < RADIX) {
+                if (dd < (uint32_t)RD) {
                     const uint64_t peers = ~(((uint64_t)mhi << 32) | mlo);
                     const uint32_t before = s_whist[wv][dd];
                     const uint64_t lower = peers & lt;
@@ -641,12 +645,12 @@ __global__ __launch_bounds__(NT, (onesweep_waves<K, NT>())) void k_onesweep(Plan
     #pragma unroll
             for (int i = 0; i < ITEMS; i++) {
                 const uint32_t dd = dig[i] & 511u;
-                if (dd < RADIX) s_keys[s_dstart[dd] + s_whist[wv][dd] + (dig[i] >> 9)] = key[i];
+                if (dd < (uint32_t)RD) s_keys[s_dstart[dd] + s_whist[wv][dd] + (dig[i] >> 9)] = key[i];
             }
             __syncthreads();
             for (uint32_t j = threadIdx.x; j < tn; j += NT) {
                 const K x = s_keys[j];
-                out[s_gbase[KeyTraits<K>::digit(x, sh)] + j] = x;
+                out[s_gbase[KeyTraits<K>::digit_n(x, sh, DB)] + j] = x;
             }
         }
     }
@@ -1871,10 +1875,12 @@ template __global__ void k_wavecopy<u128k>(Plan*, SegArrays, const u128k*, const
 
 template __global__ void k_extract<uint64_t>(Plan*, SegArrays, WorkLists, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, uint64_t*, int);
 template __global__ void k_extract<u128k>(Plan*, SegArrays, WorkLists, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, u128k*, int);
-#define GKC_OS(K, I, N) template __global__ void k_onesweep<K, I, N>(Plan*, SegArrays, WorkLists, const K*, K*, uint32_t*, int);
-GKC_OS(uint64_t, 8, 256) GKC_OS(uint64_t, 12, 256) GKC_OS(uint64_t, 16, 256) GKC_OS(uint64_t, 20, 256)
-GKC_OS(uint64_t, 24, 256) GKC_OS(uint64_t, 8, 512) GKC_OS(uint64_t, 10, 512) GKC_OS(uint64_t, 12, 512)
-GKC_OS(u128k, 4, 256) GKC_OS(u128k, 8, 256) GKC_OS(u128k, 12, 256)
+#define GKC_OS(K, I, N, D) template __global__ void k_onesweep<K, I, N, D>(Plan*, SegArrays, WorkLists, const K*, K*, uint32_t*, int);
+GKC_OS(uint64_t, 8, 256, 8) GKC_OS(uint64_t, 12, 256, 8) GKC_OS(uint64_t, 16, 256, 8) GKC_OS(uint64_t, 20, 256, 8)
+GKC_OS(uint64_t, 24, 256, 8) GKC_OS(uint64_t, 8, 512, 8) GKC_OS(uint64_t, 10, 512, 8) GKC_OS(uint64_t, 12, 512, 8)
+GKC_OS(u128k, 4, 256, 8) GKC_OS(u128k, 8, 256, 8) GKC_OS(u128k, 12, 256, 8)
+GKC_OS(uint64_t, 16, 256, 7) GKC_OS(uint64_t, 20, 256, 7) GKC_OS(uint64_t, 16, 256, 6) GKC_OS(uint64_t, 20, 256, 6)
+GKC_OS(u128k, 8, 256, 7) GKC_OS(u128k, 12, 256, 7) GKC_OS(u128k, 8, 256, 6) GKC_OS(u128k, 12, 256, 6)
 #undef GKC_OS
 template __global__ void k_runlength<uint64_t>(Plan*, SegArrays, WorkLists, const uint64_t*, TableOut, uint32_t*);
 template __global__ void k_runlength<u128k>(Plan*, SegArrays, WorkLists, const u128k*, TableOut, uint32_t*);

@@ -92,6 +92,9 @@ typedef struct gkc_tuning {
                                     of 64 in 128..1024 (0: 256); the same table for every value */
     int32_t sort_waves;          /* waves per onesweep workgroup, 4 or 8 (0: 4; 8 for 64-bit keys only, with
                                     sort_items 8, 10 or 12) */
+    int32_t digit_bits;          /* binned form: digit width of the onesweep passes, 6..8 (0: 8), so bins are
+                                    bin_passes x digit_bits key bits; 6 and 7 with sort_items 16 or 20 (u64),
+                                    8 or 12 (u128) and 4 waves.  The full LSD sort keeps 8 */
 } gkc_tuning;
 
 const char *gkc_last_error(void);

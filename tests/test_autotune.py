@@ -39,6 +39,7 @@ def test_cache_entries_are_valid_variants():
         assert (v["sort_items"], v["sort_blocks_per_cu"], v.get("sort_waves", 4) or 4) in cands, key
         assert v.get("bin_passes", 0) in (0, 2, 3), key
         assert v.get("count_window", 0) in (0,) + autotune.COUNT_WINDOWS, key
+        assert v.get("digit_bits", 0) in (0,) + autotune.DIGIT_BITS, key
 
 
 @pytest.mark.gpu

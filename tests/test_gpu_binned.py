@@ -24,10 +24,11 @@ def _say(*a):
 
 
 def _count(reads, offs, k, p, P, mode, hpc=False, min_count=1, max_count=0, batches=2, caps=None, bin_passes=0,
-           count_window=0):
+           count_window=0, digit_bits=0, sort_items=0):
     n = len(offs) - 1
     with capi.Counter(k, p, P, hpc=hpc, min_count=min_count, max_count=max_count) as c:
-        c.set_tuning(count_mode=mode, bin_passes=bin_passes, count_window=count_window)
+        c.set_tuning(sort_items, count_mode=mode, bin_passes=bin_passes, count_window=count_window,
+                     digit_bits=digit_bits)
         bounds = np.linspace(0, n, batches + 1).astype(int)
         for a, b in zip(bounds[:-1], bounds[1:]):
             if b > a:
@@ -85,6 +86,22 @@ def test_bin_passes(gpu, k, p):
         assert_tables_equal(gt, ot, 4, k)
         heavy.append(st["heavy_ranges"])
     assert heavy[0] >= heavy[1] >= heavy[2]
+
+
+@pytest.mark.parametrize("k,p,P,db,npss,items", [(31, 11, 4, 7, 2, 20), (31, 11, 4, 6, 3, 16), (31, 11, 64, 6, 2, 20),
+                                                (21, 7, 8, 7, 3, 16), (9, 5, 5, 6, 3, 20), (33, 17, 4, 7, 3, 8),
+                                                (45, 15, 8, 6, 2, 12), (63, 19, 16, 7, 3, 12)])
+def test_digit_widths(gpu, k, p, P, db, npss, items):
+    """Onesweep passes of 6- or 7-bit digits (bins of npss x db key bits, a
+    digit straddling bit 64 of 128-bit keys for k = 33) give the oracle's
+    table."""
+    cid, scale = (4, 0.002) if k > 50 else (1, 0.03)
+    rs = W.generate(W.workload(cid, scale=scale))
+    ot = O.count(rs.reads, rs.offs, k, p, P)
+    _say("digit bits", k, p, P, db, npss, items)
+    gt, st = _count(rs.reads, rs.offs, k, p, P, mode=0, bin_passes=npss, digit_bits=db, sort_items=items)
+    assert st["binned"] == 1 and st["n_pass"] == min(npss, (2 * k + db - 1) // db)
+    assert_tables_equal(gt, ot, P, k)
 
 
 @pytest.mark.parametrize("minc,maxc", [(2, 0), (1, 3), (3, 40)])
