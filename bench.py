@@ -21,6 +21,10 @@ Forms timed on the headline workload:
   h2d_batched_learned  the reads start in pinned host memory: each chunk's
                      upload runs on a copy stream while the previous chunk is
                      partitioned (PCIe included; never `value`)
+  h2d_pipelined_batched_learned  jobs back to back from pinned host memory:
+                     as above, and the next job's uploads run during this
+                     job's count (the steady state of a stream of jobs; PCIe
+                     included; never `value`)
 `value` is the batched + learned k-mers/s with the reads resident in HBM.
 The other four workloads are measured too (plain, batched and batched +
 learned, fewer steps) under `configs`.  One JSON line is printed.
@@ -208,7 +212,7 @@ def run_config(cid: int, scale: float, steps: int, warmup: int, forms: list[str]
     # pinned host copy of the reads and double-buffered device staging for the
     # PCIe-inclusive form (set up outside the timed region)
     h2d = None
-    if "h2d_batched_learned" in forms:
+    if "h2d_batched_learned" in forms or "h2d_pipelined_batched_learned" in forms:
         h_reads = torch.empty(rs.bases, dtype=torch.uint8, pin_memory=True)
         h_reads.numpy()[:] = rs.reads
         h_offs = torch.empty(rs.n + 1, dtype=torch.int64, pin_memory=True)
@@ -219,6 +223,11 @@ def run_config(cid: int, scale: float, steps: int, warmup: int, forms: list[str]
         doff = [torch.empty(max_r + 1, dtype=torch.int64, device=dev) for _ in range(2)]
         s_cp = torch.cuda.Stream()
         h2d = (h_reads, h_offs, dbuf, doff, s_cp)
+    pipe = None
+    if "h2d_pipelined_batched_learned" in forms:   # two device copies of the whole job's reads
+        pipe = {"buf": [torch.empty(rs.bases + 64, dtype=torch.uint8, device=dev) for _ in range(2)],
+                "off": [torch.empty(rs.n + 1, dtype=torch.int64, device=dev) for _ in range(2)],
+                "par": 0, "next": None}
 
     def step_h2d():
         h_reads, h_offs, dbuf, doff, s_cp = h2d
@@ -242,11 +251,42 @@ def run_config(cid: int, scale: float, steps: int, warmup: int, forms: list[str]
             ev = nxt
         count(capi.BATCHED, True)
 
+    def upload_job(par):
+        """Every chunk of the job to device copy `par` on the copy stream;
+        one event per chunk."""
+        h_reads, h_offs, _, _, s_cp = h2d
+        evs = []
+        with torch.cuda.stream(s_cp):
+            for a, b in chunks:
+                oa, ob = int(rs.offs[a]), int(rs.offs[b])
+                pipe["buf"][par][oa:ob].copy_(h_reads[oa:ob], non_blocking=True)
+                pipe["off"][par][a:b + 1].copy_(h_offs[a:b + 1], non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(s_cp)
+                evs.append(ev)
+        return evs
+
+    def step_h2d_pipelined():
+        """Jobs back to back, reads in pinned host memory: each chunk's
+        partitioning waits for its upload, and the next job's uploads run
+        during this job's count (steady state of a stream of jobs)."""
+        par = pipe["par"]
+        evs = pipe["next"] or upload_job(par)
+        ctx.reset()
+        bp, opp = pipe["buf"][par].data_ptr(), pipe["off"][par].data_ptr()
+        for (a, b), ev in zip(chunks, evs):
+            ev.synchronize()
+            ctx.partition(bp + int(rs.offs[a]), opp + 8 * a, b - a, flags=capi.DEVICE_INPUT | capi.BATCHED)
+        pipe["par"] = 1 - par
+        pipe["next"] = upload_job(1 - par)
+        count(capi.BATCHED, True)
+
     runners = {
         "plain": lambda: step_resident(0, 0),
         "batched": lambda: step_resident(capi.BATCHED, capi.BATCHED),
         "batched_learned": lambda: step_resident(capi.BATCHED, capi.BATCHED, True),
         "h2d_batched_learned": step_h2d,
+        "h2d_pipelined_batched_learned": step_h2d_pipelined,
     }
     tkey = autotune.bucket_key(w.k, rs.bases / max(rs.n, 1), N / w.n_parts, w.n_parts)
     if tune:
@@ -283,7 +323,7 @@ def run_config(cid: int, scale: float, steps: int, warmup: int, forms: list[str]
                          "n_entries": st["n_entries"]}
         _log(f"{w.name} {name}: {ms:.2f} ms/step, {N / ms / 1e6:.2f} G k-mers/s, count {st['ms_count']:.2f} ms, "
              f"syncs {st['host_syncs']}, fallbacks {st['fallbacks']}")
-    if h2d is not None:     # the uploads alone, for the PCIe rate
+    if h2d is not None and "h2d_batched_learned" in results:     # the uploads alone, for the PCIe rate
         h_reads, h_offs, dbuf, doff, s_cp = h2d
         torch.cuda.synchronize()
         t = time.perf_counter()
@@ -442,7 +482,7 @@ def main() -> int:
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-reads", type=int, default=1_500_000, help="reads in the CPU baseline sample")
-    ap.add_argument("--forms", default="plain,batched,batched_learned,h2d_batched_learned")
+    ap.add_argument("--forms", default="plain,batched,batched_learned,h2d_batched_learned,h2d_pipelined_batched_learned")
     ap.add_argument("--other-configs", default="1,3,4,5",
                     help="workloads also measured (fewer steps), comma-separated; empty for none")
     ap.add_argument("--other-cpu-reads", type=int, default=200_000, help="reads in their CPU baseline samples")
