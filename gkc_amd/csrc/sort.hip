@@ -202,7 +202,7 @@ __device__ __forceinline__ u128k canon_of(u128k x, int k) {
 // superkmer's few words), so the three or five words each k-mer spans are
 // LDS reads rather than gathers from L2; a wave whose superkmers hold more
 // than EX_SW words (long superkmers) reads them from global memory instead.
-constexpr int EX_U = 4;
+constexpr int EX_U = 2;
 constexpr int EX_SW = 512;
 template <typename K>
 __global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const SkmHdr* __restrict__ hdrs,
