@@ -151,10 +151,11 @@ class Counter:
         if not h:
             raise GkcError(lib().gkc_last_error().decode())
         self.h = h
+        self._destroy = lib().gkc_destroy      # held here: module globals may be gone when __del__ runs at exit
 
     def close(self) -> None:
         if getattr(self, "h", None):
-            lib().gkc_destroy(self.h)
+            self._destroy(self.h)
             self.h = None
 
     def __del__(self):

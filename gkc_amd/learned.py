@@ -47,6 +47,7 @@ class MLP:
         self.h = L.gkc_mlp_create(device, seed)
         if not self.h:
             raise capi.GkcError(L.gkc_mlp_last_error().decode())
+        self._destroy = L.gkc_mlp_destroy      # held here: module globals may be gone when __del__ runs at exit
         self.n_params = int(L.gkc_mlp_n_params())
 
     def _chk(self, rc):
@@ -55,7 +56,7 @@ class MLP:
 
     def close(self):
         if getattr(self, "h", None):
-            capi.lib().gkc_mlp_destroy(self.h)
+            self._destroy(self.h)
             self.h = None
 
     def __del__(self):
