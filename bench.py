@@ -414,8 +414,8 @@ def run_config(cid: int, scale: float, steps: int, warmup: int, forms: list[str]
         rep["cpu_baseline"] = {"value": n_sub / cpu_s, "unit": "k-mers/s", "cores": threads, "kind": "port",
                                "sample": f"first {nr} reads of the same workload ({n_sub} k-mers), "
                                          f"oracle phase1 {ot.times[0]:.2f}s + phase2 {ot.times[1]:.2f}s",
-                               "cores_note": "the host cores this process may run on, at most the 16 a GPU "
-                                             "box shares out per GPU"}
+                               "cores_note": "the host cores this process may run on, at most the 16 a "
+                                             "one-GPU node shares out per GPU"}
         rep["bit_exact_sample"] = exact
     handle = {"ctx": ctx, "rs": rs, "N": N, "W_key": W_key, "pass_ms": pass_ms, "d_reads": d_reads,
               "d_offs": d_offs, "h2d": h2d, "k": w.k}
@@ -424,7 +424,7 @@ def run_config(cid: int, scale: float, steps: int, warmup: int, forms: list[str]
 
 def cpu_cores() -> int:
     """Host cores for the CPU baseline: the cores this process may run on,
-    capped at 16, the host share a GPU box gives one GPU."""
+    capped at 16, the host cores a one-GPU node shares out per GPU."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:

@@ -12,8 +12,8 @@ driver must fail to link (its SKMStore and T_h_data have other layouts; the
 drop-in's types live in an inline namespace, so the symbols do not match)
 rather than link and corrupt memory.
 
-Skipped when the reference tree or g++ is absent (the GPU box has no
-reference tree).  Nothing is copied into the repository.
+Skipped when the reference tree or g++ is absent (machines that only run
+the built library have no reference tree).  Nothing is copied into the repository.
 """
 from __future__ import annotations
 
