@@ -257,3 +257,26 @@ def test_wave_table_full_after_small(gpu, npss, minc):
     gt, st = _count(reads, offs, 31, 11, 2, mode=0, min_count=minc, bin_passes=npss)
     assert st["binned"] == 1 and st["n_pass"] == npss
     assert_tables_equal(gt, ot, 2, 31)
+
+
+def test_tuning_changes_between_counts(gpu):
+    """One context counted with 8-, 6- and again 8-bit digits, 4 and 8 waves
+    per onesweep workgroup and a changed window: each count (its captured
+    graph rebuilt when the shape changes) gives the oracle's table; values
+    outside the allowed sets are refused."""
+    rs = W.generate(W.workload(1, scale=0.02))
+    k, p, P = 31, 11, 16
+    ot = O.count(rs.reads, rs.offs, k, p, P)
+    with capi.Counter(k, p, P) as c:
+        c.partition(rs.reads, rs.offs, flags=capi.BATCHED)
+        for items, waves, db, npss, cw in [(20, 4, 8, 2, 384), (16, 4, 6, 3, 256), (10, 8, 8, 2, 256), (20, 4, 8, 2, 384)]:
+            _say("retune", items, waves, db, npss, cw)
+            c.set_tuning(items, bin_passes=npss, count_window=cw, sort_waves=waves, digit_bits=db)
+            t = c.get_tuning()
+            assert (t["sort_items"], t["sort_waves"], t["digit_bits"]) == (items, waves, db)
+            for _ in range(2):
+                c.count(capi.BATCHED)
+                assert_tables_equal(c.table(), ot, P, k)
+        for bad in ({"digit_bits": 5}, {"digit_bits": 9}, {"sort_waves": 6}, {"bin_passes": 4}, {"count_window": 100}):
+            with pytest.raises(capi.GkcError):
+                c.set_tuning(**bad)
