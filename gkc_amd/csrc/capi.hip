@@ -98,7 +98,9 @@ struct gkc_ctx {
     std::vector<uint64_t> h_word_base, h_hdr_base;
     std::vector<uint32_t> h_word_cap, h_hdr_cap, h_word_cur, h_hdr_cur, h_kmer_cur;
     DevBuf d_word_base, d_hdr_base, d_word_cap, d_hdr_cap, d_word_cur, d_hdr_cur, d_kmer_cur, d_overflow;
-    DevBuf d_tmp_hdr, d_tmp_word, d_tmp_kmer;    // per-batch counts of the count pass
+    DevBuf d_hk_cur;                             // [P] headers | k-mers << 32 (the kernels' cursors; split
+                                                 // into d_hdr_cur / d_kmer_cur after each scatter)
+    DevBuf d_tmp_hk, d_tmp_word;                 // per-batch counts of the count pass
     DevBuf d_sketch;
 
     // batch staging
@@ -206,12 +208,13 @@ static void bins_init(gkc_ctx* c) {
     c->d_word_base.ensure(P * 8); c->d_hdr_base.ensure(P * 8);
     c->d_word_cap.ensure(P * 4); c->d_hdr_cap.ensure(P * 4);
     c->d_word_cur.ensure(P * 4); c->d_hdr_cur.ensure(P * 4); c->d_kmer_cur.ensure(P * 4);
-    c->d_tmp_hdr.ensure(P * 4); c->d_tmp_word.ensure(P * 4); c->d_tmp_kmer.ensure(P * 4);
+    c->d_hk_cur.ensure(P * 8); c->d_tmp_hk.ensure(P * 8); c->d_tmp_word.ensure(P * 4);
     c->d_overflow.ensure(16);
     c->d_sketch.ensure((size_t)P * 256 * 4);
     HIPX(hipMemsetAsync(c->d_word_cur.p, 0, P * 4, c->st));
     HIPX(hipMemsetAsync(c->d_hdr_cur.p, 0, P * 4, c->st));
     HIPX(hipMemsetAsync(c->d_kmer_cur.p, 0, P * 4, c->st));
+    HIPX(hipMemsetAsync(c->d_hk_cur.p, 0, P * 8, c->st));
     HIPX(hipMemsetAsync(c->d_word_cap.p, 0, P * 4, c->st));
     HIPX(hipMemsetAsync(c->d_hdr_cap.p, 0, P * 4, c->st));
     HIPX(hipMemsetAsync(c->d_sketch.p, 0, (size_t)P * 256 * 4, c->st));
@@ -273,9 +276,9 @@ static BinSet binset(gkc_ctx* c, bool counting) {
     b.word_base = c->d_word_base.as<uint64_t>(); b.hdr_base = c->d_hdr_base.as<uint64_t>();
     b.word_cap = c->d_word_cap.as<uint32_t>(); b.hdr_cap = c->d_hdr_cap.as<uint32_t>();
     if (counting) {
-        b.word_cur = c->d_tmp_word.as<uint32_t>(); b.hdr_cur = c->d_tmp_hdr.as<uint32_t>(); b.kmer_cur = c->d_tmp_kmer.as<uint32_t>();
+        b.word_cur = c->d_tmp_word.as<uint32_t>(); b.hk_cur = c->d_tmp_hk.as<unsigned long long>();
     } else {
-        b.word_cur = c->d_word_cur.as<uint32_t>(); b.hdr_cur = c->d_hdr_cur.as<uint32_t>(); b.kmer_cur = c->d_kmer_cur.as<uint32_t>();
+        b.word_cur = c->d_word_cur.as<uint32_t>(); b.hk_cur = c->d_hk_cur.as<unsigned long long>();
     }
     b.overflow = c->d_overflow.as<uint32_t>();
     b.pos = c->pos_mode == 1 ? c->skmpos.as<uint2>() : nullptr;
@@ -485,7 +488,7 @@ int gkc_reset(gkc_ctx* c) {
         std::fill(c->h_word_cur.begin(), c->h_word_cur.end(), 0u);
         std::fill(c->h_hdr_cur.begin(), c->h_hdr_cur.end(), 0u);
         std::fill(c->h_kmer_cur.begin(), c->h_kmer_cur.end(), 0u);
-        zero(c, c->d_word_cur.p, P); zero(c, c->d_hdr_cur.p, P); zero(c, c->d_kmer_cur.p, P);
+        zero(c, c->d_word_cur.p, P); zero(c, c->d_hdr_cur.p, P); zero(c, c->d_kmer_cur.p, P); zero(c, c->d_hk_cur.p, 2 * P);
         zero(c, c->d_sketch.p, (uint64_t)P * 256); zero(c, c->d_overflow.p, 4);
         c->bases_done = 0;
         c->reads_done = 0;
@@ -523,9 +526,8 @@ int gkc_superkmer_arrays(gkc_ctx* c, const char* reads, const uint64_t* offs, in
         P1Params prm = p1params(c, 0);
         prm.mm_out = c->d_mm.as<uint32_t>();
         BinSet bs = binset(c, true);
-        HIPX(hipMemsetAsync(c->d_tmp_hdr.p, 0, c->prm.n_parts * 4, c->st));
+        HIPX(hipMemsetAsync(c->d_tmp_hk.p, 0, c->prm.n_parts * 8, c->st));
         HIPX(hipMemsetAsync(c->d_tmp_word.p, 0, c->prm.n_parts * 4, c->st));
-        HIPX(hipMemsetAsync(c->d_tmp_kmer.p, 0, c->prm.n_parts * 4, c->st));
         launch_superkmers(c, cd, prm, bs);
         // superkmer lists (reuse d_hpc_pos-sized scratch: i32 per base)
         DevBuf d_skm;
@@ -560,7 +562,7 @@ int gkc_window_minimizers(gkc_ctx* c, const char* reads, const uint64_t* offs, i
         prm.win_mm = c->d_win_mm.as<uint32_t>(); prm.win_pos = c->d_win_pos.as<uint8_t>();
         prm.win_strand = c->d_win_str.as<int8_t>();
         const uint32_t P = c->prm.n_parts;
-        zero(c, c->d_tmp_hdr.p, P); zero(c, c->d_tmp_word.p, P); zero(c, c->d_tmp_kmer.p, P);
+        zero(c, c->d_tmp_hk.p, 2 * P); zero(c, c->d_tmp_word.p, P);
         launch_superkmers(c, cd, prm, binset(c, true));
         if (mm) HIPX(hipMemcpyAsync(mm, c->d_win_mm.p, B * 4, hipMemcpyDeviceToHost, c->st));
         if (pos) HIPX(hipMemcpyAsync(pos, c->d_win_pos.p, B, hipMemcpyDeviceToHost, c->st));
@@ -583,16 +585,34 @@ int gkc_windows_device(gkc_ctx* c, uint32_t** mm, uint8_t** pos, int8_t** strand
 }
 
 // Plain phase 1 of one batch: count pass, host sync, exact reservation, scatter.
+// The kernels' combined cursors into the per-partition header and k-mer
+// counts the counting launch reads.
+__global__ void k_split_hk(const unsigned long long* __restrict__ hk, uint32_t* __restrict__ hdr,
+                           uint32_t* __restrict__ kmer, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const unsigned long long v = hk[i];
+        hdr[i] = (uint32_t)v;
+        kmer[i] = (uint32_t)(v >> 32);
+    }
+}
+static void split_cursors(gkc_ctx* c) {
+    const uint32_t P = c->prm.n_parts;
+    hipLaunchKernelGGL(k_split_hk, dim3((P + 255) / 256), dim3(256), 0, c->st, c->d_hk_cur.as<unsigned long long>(),
+                       c->d_hdr_cur.as<uint32_t>(), c->d_kmer_cur.as<uint32_t>(), P);
+    HIPX(hipGetLastError());
+}
+
 static void partition_plain(gkc_ctx* c, const ChunkDesc& cd, uint32_t flags) {
     const uint32_t P = c->prm.n_parts;
-    zero(c, c->d_tmp_hdr.p, P); zero(c, c->d_tmp_word.p, P); zero(c, c->d_tmp_kmer.p, P);
+    zero(c, c->d_tmp_hk.p, 2 * P); zero(c, c->d_tmp_word.p, P);
     P1Params pc = p1params(c, 0);
     launch_superkmers(c, cd, pc, binset(c, true));
     std::vector<uint32_t> nh(P), nw(P), nk(P);
-    HIPX(hipMemcpyAsync(nh.data(), c->d_tmp_hdr.p, P * 4, hipMemcpyDeviceToHost, c->st));
+    std::vector<uint64_t> hk(P);
+    HIPX(hipMemcpyAsync(hk.data(), c->d_tmp_hk.p, P * 8, hipMemcpyDeviceToHost, c->st));
     HIPX(hipMemcpyAsync(nw.data(), c->d_tmp_word.p, P * 4, hipMemcpyDeviceToHost, c->st));
-    HIPX(hipMemcpyAsync(nk.data(), c->d_tmp_kmer.p, P * 4, hipMemcpyDeviceToHost, c->st));
     HIPX(hipStreamSynchronize(c->st));
+    for (uint32_t q = 0; q < P; q++) { nh[q] = (uint32_t)hk[q]; nk[q] = (uint32_t)(hk[q] >> 32); }
     for (uint32_t q = 0; q < P; q++)
         if ((uint64_t)c->h_kmer_cur[q] + nk[q] >= (1ull << 30)) throw ArgError{"partition exceeds 2^30 k-mers; use more partitions", GKC_EINVAL};
     bins_reserve(c, nw, nh);
@@ -607,6 +627,7 @@ static void partition_plain(gkc_ctx* c, const ChunkDesc& cd, uint32_t flags) {
         c->win_n = B; c->win_lead = cd.lead;
     }
     launch_superkmers(c, cd, ps, binset(c, false));
+    split_cursors(c);
     uint32_t ovf = 0;
     HIPX(hipMemcpyAsync(&ovf, c->d_overflow.p, 4, hipMemcpyDeviceToHost, c->st));
     HIPX(hipStreamSynchronize(c->st));
@@ -614,12 +635,6 @@ static void partition_plain(gkc_ctx* c, const ChunkDesc& cd, uint32_t flags) {
     for (uint32_t q = 0; q < P; q++) { c->h_hdr_cur[q] += nh[q]; c->h_word_cur[q] += nw[q]; c->h_kmer_cur[q] += nk[q]; }
 }
 
-__global__ void k_copy_u32(uint32_t* __restrict__ dst, const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
-                           const uint32_t* __restrict__ cc, uint32_t n) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        dst[i] = a[i]; dst[n + i] = b[i]; dst[2 * n + i] = cc[i];
-    }
-}
 // Dense table layout of a presized binned count, from the exact kept
 // entries per segment (counted by then): starts = exclusive scan, and each
 // segment's capacity = what is left of the allocation after its start, so
@@ -650,12 +665,6 @@ __global__ void k_copy_words(uint32_t* __restrict__ dst, const uint32_t* __restr
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
 }
-__global__ void k_restore_u32(const uint32_t* __restrict__ src, uint32_t* __restrict__ a, uint32_t* __restrict__ b,
-                              uint32_t* __restrict__ cc, uint32_t n) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        a[i] = src[i]; b[i] = src[n + i]; cc[i] = src[2 * n + i];
-    }
-}
 
 // Batched phase 1 of one batch: bins presized from the per-partition shares of
 // the batches already partitioned (for the first batch, from a count pass
@@ -673,12 +682,14 @@ static bool partition_presized(gkc_ctx* c, const ChunkDesc& cd) {
         sc.n_bases = std::min<uint32_t>(cd.n_bases, sc.n_tiles * (uint32_t)P1_T);
         c->d_chunk_s.ensure(sizeof(ChunkDesc));
         hipLaunchKernelGGL(k_set_chunk, dim3(1), dim3(1), 0, c->st, c->d_chunk_s.as<ChunkDesc>(), sc);
-        zero(c, c->d_tmp_hdr.p, P); zero(c, c->d_tmp_word.p, P); zero(c, c->d_tmp_kmer.p, P);
+        zero(c, c->d_tmp_hk.p, 2 * P); zero(c, c->d_tmp_word.p, P);
         launch_superkmers(c, sc, p1params(c, 0), binset(c, true), false, c->d_chunk_s.as<ChunkDesc>());
         std::vector<uint32_t> sw(P), sh(P);
+        std::vector<uint64_t> shk(P);
         HIPX(hipMemcpyAsync(sw.data(), c->d_tmp_word.p, P * 4, hipMemcpyDeviceToHost, c->st));
-        HIPX(hipMemcpyAsync(sh.data(), c->d_tmp_hdr.p, P * 4, hipMemcpyDeviceToHost, c->st));
+        HIPX(hipMemcpyAsync(shk.data(), c->d_tmp_hk.p, P * 8, hipMemcpyDeviceToHost, c->st));
         HIPX(hipStreamSynchronize(c->st));
+        for (uint32_t q = 0; q < P; q++) sh[q] = (uint32_t)shk[q];
         const double f = (double)cd.n_bases / (double)sc.n_bases;
         for (uint32_t q = 0; q < P; q++) {
             nw[q] = (uint32_t)std::min<double>(std::ceil(sw[q] * f * 1.25) + 1024, 4.0e9);
@@ -696,14 +707,17 @@ static bool partition_presized(gkc_ctx* c, const ChunkDesc& cd) {
     std::vector<const void*> key = {c->words.p, c->hdrs.p, c->d_chunk.p, c->d_backup.p, c->skmpos.p,
                                     (const void*)(uintptr_t)c->pos_mode};
     auto enqueue = [&] {
-        hipLaunchKernelGGL(k_copy_u32, dim3((P + 255) / 256), dim3(256), 0, c->st, c->d_backup.as<uint32_t>(),
-                           c->d_word_cur.as<uint32_t>(), c->d_hdr_cur.as<uint32_t>(), c->d_kmer_cur.as<uint32_t>(), P);
+        hipLaunchKernelGGL(k_copy_words, dim3(4), dim3(256), 0, c->st, c->d_backup.as<uint32_t>(),
+                           (const uint32_t*)c->d_word_cur.as<uint32_t>(), (uint64_t)P);
+        hipLaunchKernelGGL(k_copy_words, dim3(8), dim3(256), 0, c->st, c->d_backup.as<uint32_t>() + P,
+                           (const uint32_t*)c->d_hk_cur.as<uint32_t>(), (uint64_t)2 * P);
         hipLaunchKernelGGL(k_copy_words, dim3(256), dim3(256), 0, c->st, c->d_backup.as<uint32_t>() + 3 * P,
                            (const uint32_t*)c->d_sketch.as<uint32_t>(), (uint64_t)P * 256);
         zero(c, c->d_overflow.p, 1);
         P1Params ps = p1params(c, 1);
         ps.sketch = c->d_sketch.as<uint32_t>();
         launch_superkmers(c, cd, ps, binset(c, false), true);
+        split_cursors(c);
     };
     // occupancy is queried (and cached) outside the capture
     (void)occupancy_blocks(c->prm.p > 16 ? (const void*)k_superkmers<uint64_t, false> : (const void*)k_superkmers<uint32_t, false>,
@@ -724,8 +738,11 @@ static bool partition_presized(gkc_ctx* c, const ChunkDesc& cd) {
     bool too_big = false;
     for (uint32_t q = 0; q < P; q++) too_big |= cur[2 * P + q] >= (1u << 30);
     if (ovf || too_big) {
-        hipLaunchKernelGGL(k_restore_u32, dim3((P + 255) / 256), dim3(256), 0, c->st, c->d_backup.as<uint32_t>(),
-                           c->d_word_cur.as<uint32_t>(), c->d_hdr_cur.as<uint32_t>(), c->d_kmer_cur.as<uint32_t>(), P);
+        hipLaunchKernelGGL(k_copy_words, dim3(4), dim3(256), 0, c->st, c->d_word_cur.as<uint32_t>(),
+                           (const uint32_t*)c->d_backup.as<uint32_t>(), (uint64_t)P);
+        hipLaunchKernelGGL(k_copy_words, dim3(8), dim3(256), 0, c->st, c->d_hk_cur.as<uint32_t>(),
+                           (const uint32_t*)(c->d_backup.as<uint32_t>() + P), (uint64_t)2 * P);
+        split_cursors(c);
         hipLaunchKernelGGL(k_copy_words, dim3(256), dim3(256), 0, c->st, c->d_sketch.as<uint32_t>(),
                            (const uint32_t*)(c->d_backup.as<uint32_t>() + 3 * P), (uint64_t)P * 256);
         zero(c, c->d_overflow.p, 1);

@@ -20,8 +20,7 @@ struct BinSet {
     const uint32_t* word_cap;   // [P]
     const uint32_t* hdr_cap;    // [P]
     uint32_t* word_cur;         // [P] cursors (persist across chunks)
-    uint32_t* hdr_cur;          // [P]
-    uint32_t* kmer_cur;         // [P] k-mers appended so far
+    unsigned long long* hk_cur; // [P] headers (low 32 bits) and k-mers (high 32 bits) appended so far
     uint32_t* overflow;         // [1] set when any capacity is exceeded
     uint2* pos;                 // beside each header: {read index since the last reset, in-read
                                 // offset of its first base}, or null (GKC_WITH_POS)

@@ -536,9 +536,11 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
                 const unsigned long long v = s_loc[q];
                 const uint32_t h = (uint32_t)(v >> 32), kk = (uint32_t)(v >> 16) & 0xFFFFu, ww = (uint32_t)v & 0xFFFFu;
                 if (h) {
-                    const uint32_t gh = atomicAdd(&bins.hdr_cur[q], h);
+                    // headers and k-mers in one atomic: k-mer offsets then rise with
+                    // the header index inside a partition, across flushes too
+                    const unsigned long long ghk = atomicAdd(&bins.hk_cur[q], ((unsigned long long)kk << 32) | h);
+                    const uint32_t gh = (uint32_t)ghk, gk = (uint32_t)(ghk >> 32);
                     const uint32_t gw = atomicAdd(&bins.word_cur[q], ww);
-                    const uint32_t gk = atomicAdd(&bins.kmer_cur[q], kk);
                     if (prm.scatter && (gh + h > bins.hdr_cap[q] || gw + ww > bins.word_cap[q])) {
                         s_ovf = 1;
                         atomicOr(bins.overflow, 1u);
