@@ -2,8 +2,9 @@
 
 Picks the onesweep tile size (keys per thread), the waves per workgroup (4,
 or 8 for 64-bit keys: the same tile LDS with twice the waves to hide the
-loads behind) and the number of persistent workgroups per CU of the sort
-passes by timing every candidate on the data
+loads behind, or tiles of 8192 / 9216 keys, two workgroups per CU, whose
+longer digit runs write more whole lines) and the number of persistent
+workgroups per CU of the sort passes by timing every candidate on the data
 already in a counter's bins (the batched form, instrumented pass timing),
 then the digit width and number of onesweep passes before the binned count
 (8-, 7- or 6-bit digits, 2 or 3 passes: fewer or narrower passes leave
@@ -24,7 +25,7 @@ from . import capi
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_cache.json")
 # (keys per thread, workgroups per CU (0: occupancy limit), waves per workgroup)
-CANDIDATES_64 = [(i, b, 4) for i in (8, 12, 16, 20, 24) for b in (0, 2)] + [(i, 0, 8) for i in (8, 10, 12)]
+CANDIDATES_64 = [(i, b, 4) for i in (8, 12, 16, 20, 24) for b in (0, 2)] + [(i, 0, 8) for i in (8, 10, 12, 16, 18)]
 CANDIDATES_128 = [(i, b, 4) for i in (4, 8, 12) for b in (0, 2)]
 
 

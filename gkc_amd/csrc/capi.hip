@@ -123,6 +123,7 @@ struct gkc_ctx {
 
     // counting
     DevBuf keysA, keysB, lookback, d_plan, d_segk, d_segh, d_keystart, d_itemstart, d_tilestart, d_rlstart, d_hist;
+    DevBuf d_tdesc;                            // sort tile descriptors in work-list order (k_tile_desc)
     DevBuf d_tstart, d_tcap, d_nent, d_ndist, d_toverflow;
     DevBuf d_tcap_total;              // total table capacity of a presized binned count (u64)
     uint64_t h_tcap_total = 0;
@@ -1141,6 +1142,7 @@ static void ensure_count_buffers(gkc_ctx* c, uint64_t max_keys, uint32_t n_segs)
     c->keysB.ensure((max_keys + 1) * kw);
     const uint64_t tiles = max_keys / (SORT_NT * (kw == 8 ? 8 : 4)) + n_segs + 1;   // smallest tile variant
     c->lookback.ensure(tiles * RADIX * 4);
+    c->d_tdesc.ensure(tiles * sizeof(TileDesc));
     c->d_plan.ensure(sizeof(Plan));
     c->d_keystart.ensure((n_segs + 1) * 8);
     c->d_itemstart.ensure((n_segs + 1) * 4);
@@ -1169,7 +1171,7 @@ static void zero(gkc_ctx* c, void* p, uint64_t n32) {
 // selected by gkc_tuning.sort_items, sort_waves and digit_bits (the last in
 // the binned form only: the full LSD sort keeps 8-bit digits)
 template <typename K>
-using OnesweepFn = void (*)(Plan*, SegArrays, WorkLists, const K*, K*, uint32_t*, int);
+using OnesweepFn = void (*)(Plan*, SegArrays, WorkLists, const TileDesc*, const K*, K*, uint32_t*, int);
 template <typename K> static OnesweepFn<K> onesweep_variant(int items, int nt, int db);
 template <> OnesweepFn<uint64_t> onesweep_variant<uint64_t>(int items, int nt, int db) {
     if (db == 7) return items == 16 ? k_onesweep<uint64_t, 16, 256, 7> : k_onesweep<uint64_t, 20, 256, 7>;
@@ -1178,6 +1180,8 @@ template <> OnesweepFn<uint64_t> onesweep_variant<uint64_t>(int items, int nt, i
         switch (items) {
             case 8: return k_onesweep<uint64_t, 8, 512, 8>;
             case 12: return k_onesweep<uint64_t, 12, 512, 8>;
+            case 16: return k_onesweep<uint64_t, 16, 512, 8>;
+            case 18: return k_onesweep<uint64_t, 18, 512, 8>;
             default: return k_onesweep<uint64_t, 10, 512, 8>;
         }
     }
@@ -1206,7 +1210,7 @@ template <typename K> static int sort_nt(gkc_ctx* c, int db) {
 template <typename K> static int sort_items(gkc_ctx* c, int db) {
     const int it = c->tune.sort_items;
     if (db != 8) return sizeof(K) == 8 ? (it == 16 ? 16 : 20) : (it == 12 ? 12 : 8);
-    if (sort_nt<K>(c, db) == 512) return (it == 8 || it == 10 || it == 12) ? it : 10;
+    if (sort_nt<K>(c, db) == 512) return (it == 8 || it == 10 || it == 12 || it == 16 || it == 18) ? it : 10;
     if (sizeof(K) == 8) return (it == 8 || it == 12 || it == 16 || it == 20 || it == 24) ? it : 16;
     return (it == 4 || it == 8 || it == 12) ? it : 8;
 }
@@ -1240,7 +1244,7 @@ static void set_lists(gkc_ctx* c, uint32_t n_segs, const uint32_t* keys, const u
         load[best] += st[s] + 1;
     }
     std::vector<uint32_t>& w = c->h_wl;
-    w.assign(33 + 4 * (size_t)n, 0);
+    w.assign(33 + 4 * (size_t)n + 9, 0);
     uint32_t pos = 0;
     for (uint32_t L = 0; L < n_lists; L++) {
         w[24 + L] = pos;
@@ -1257,6 +1261,8 @@ static void set_lists(gkc_ctx* c, uint32_t n_segs, const uint32_t* keys, const u
         w[L] = (uint32_t)ts; w[8 + L] = (uint32_t)tr; w[16 + L] = (uint32_t)ti;
     }
     for (uint32_t L = n_lists; L <= 8; L++) w[24 + L] = pos;
+    uint32_t tb = 0;                     // each list's first sort tile in the lists' concatenation
+    for (uint32_t L = 0; L <= 8; L++) { w[33 + 4 * (size_t)n + L] = tb; if (L < 8) tb += w[L]; }
     c->d_wl.ensure(w.size() * 4);
     HIPX(hipMemcpyAsync(c->d_wl.p, w.data(), w.size() * 4, hipMemcpyHostToDevice, c->st));
     c->wl.w = c->d_wl.as<uint32_t>();
@@ -1269,7 +1275,8 @@ static void launch_onesweep(gkc_ctx* c, int grid, Plan* plan, SegArrays sa, cons
     OnesweepFn<K> fn = onesweep_variant<K>(sort_items<K>(c, db), sort_nt<K>(c, db), db);
     uint32_t* lb = c->lookback.as<uint32_t>();
     WorkLists wl = c->wl;
-    void* args[] = {&plan, &sa, &wl, &src, &dst, &lb, &pass};
+    const TileDesc* td = c->d_tdesc.as<TileDesc>();
+    void* args[] = {&plan, &sa, &wl, &td, &src, &dst, &lb, &pass};
     HIPX(hipLaunchKernel((const void*)fn, dim3(grid), dim3(sort_nt<K>(c, db)), args, 0, c->st));
 }
 
@@ -1284,6 +1291,7 @@ static K* sort_segments(gkc_ctx* c, SegArrays sa, const uint64_t* hdr_base, cons
     }
     hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(64), 0, c->st, plan, n_segs, (uint32_t)c->npass, 0u, (uint32_t)RADIX_BITS);
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, c->st, plan, sa, TILE, RL_TILE);
+    hipLaunchKernelGGL(k_tile_desc, dim3(n_segs), dim3(256), 0, c->st, sa, c->wl, TILE, c->d_tdesc.as<TileDesc>());
     HIPX(hipGetLastError());
     zero(c, sa.hist, (uint64_t)n_segs * MAX_PASS * RADIX);
     HIPX(hipEventRecord(c->ev[2], c->st));
@@ -1556,6 +1564,7 @@ static K* enqueue_sort_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs, uint6
     const int TILE = sort_tile_keys<K>(c, bs.dbits);
     hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(64), 0, c->st, plan, P, (uint32_t)bs.npass, bs.sh0, (uint32_t)bs.dbits);
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, c->st, plan, sa, TILE, bs.rl_tile);
+    hipLaunchKernelGGL(k_tile_desc, dim3(P), dim3(256), 0, c->st, sa, c->wl, TILE, c->d_tdesc.as<TileDesc>());
     zero(c, sa.hist, (uint64_t)P * MAX_PASS * RADIX);
     if (timed) HIPX(hipEventRecord(c->ev[2], c->st));
     const size_t ex_lds = (size_t)bs.npass * RADIX * sizeof(uint32_t);     // digit histograms
@@ -1742,7 +1751,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
         HIPX(hipMemcpyAsync(c->h_lsum, c->d_tcap_total.p, 8, hipMemcpyDeviceToHost, c->st));
         HIPX(hipEventRecord(c->ev_lsum, c->st));
     }
-    std::vector<const void*> key = {c->keysA.p, c->keysB.p, c->lookback.p, c->words.p, c->hdrs.p, c->d_ranges.p,
+    std::vector<const void*> key = {c->keysA.p, c->keysB.p, c->lookback.p, c->d_tdesc.p, c->words.p, c->hdrs.p, c->d_ranges.p,
                                     c->d_hlist.p, c->d_hres.p, c->d_wkept.p,
                                     (const void*)(uintptr_t)tiles_cap, (const void*)(uintptr_t)bs.npass,
                                     (const void*)(uintptr_t)sort_items<K>(c, bs.dbits), (const void*)(uintptr_t)sort_grid<K>(c, bs.dbits),

@@ -86,9 +86,19 @@ struct Plan {
 //   [16, 24)      extraction items per list  [24, 33)  first order index of each list
 //   [33, 33+n)    segment order              then per order index: sort tile offset in
 //   its list [n], run-length tile offset [n], extraction item offset [n]
+//   [33+4n, 33+4n+9)  first sort tile of each list in the lists' concatenation
 struct WorkLists {
     const uint32_t* w;
     uint32_t n_lists, n_segs;
+};
+
+// A sort tile of the lists' concatenation (k_tile_desc): its segment's first
+// key, its index in the segment, segment << 16 | keys in the tile.  Tile g's
+// predecessor in its segment is tile g - 1, so g is also its look-back slot.
+struct TileDesc {
+    uint64_t kbase;
+    uint32_t ti;
+    uint32_t seg_tn;
 };
 
 struct SegArrays {
@@ -159,9 +169,10 @@ __global__ void k_extract(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, c
                           const uint64_t* __restrict__ hdr_base, const uint32_t* __restrict__ words,
                           const uint64_t* __restrict__ word_base, K* __restrict__ keys, int k);
 __global__ void k_hist_scan(Plan* __restrict__ plan, uint32_t* __restrict__ hist);
+__global__ void k_tile_desc(SegArrays sa, WorkLists wl, int tile_keys, TileDesc* __restrict__ desc);
 template <typename K, int ITEMS, int NT, int DB>
-__global__ void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ in, K* __restrict__ out,
-                           uint32_t* __restrict__ lookback, int pass);
+__global__ void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const TileDesc* __restrict__ tdesc,
+                           const K* __restrict__ in, K* __restrict__ out, uint32_t* __restrict__ lookback, int pass);
 template <typename K>
 __global__ void k_runlength(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ keys, TableOut to,
                             uint32_t* __restrict__ lookback);

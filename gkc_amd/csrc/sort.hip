@@ -11,6 +11,8 @@
 //   run-length count, clamp to 0xFFFFFFFF, [min,max]  kmer_counting.cu:148-191
 // A segment is one partition: every kernel here works on all segments of a
 // plan at once (the plain path passes a plan with a single segment).
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace gkc {
@@ -85,6 +87,42 @@ __device__ __forceinline__ void unit_at(const WorkLists& wl, uint32_t L, uint32_
     }
     seg = wl.w[33 + lo];
     idx = t - off[lo];
+}
+
+// Claim the next sort tile (run by one thread): its index g in the lists'
+// concatenation (the list's first tile + the list counter's value), which
+// indexes its descriptor and its look-back slot.  The counter's atomic is
+// issued together with the loads of the list's size and base.  Returns false
+// when every list is exhausted.
+__device__ __forceinline__ bool claim_tile(const WorkLists& wl, uint32_t* ctr, uint32_t& list, uint32_t& g) {
+    const uint32_t* base = wl.w + 33 + 4 * wl.n_segs;
+    for (uint32_t tries = 0; tries < wl.n_lists; tries++) {
+        const uint32_t L = list;
+        const uint32_t t = atomicAdd(&ctr[L], 1u);
+        const uint32_t tot = wl.w[L], b = base[L];
+        if (t < tot) { g = b + t; return true; }
+        list = (L + 1) % wl.n_lists;
+    }
+    return false;
+}
+
+// Sort tile descriptors in work-list order (one workgroup per list-order
+// segment): tile g = its list's first tile + the segment's tile offset in the
+// list + its index in the segment.  Tiles hold tile_keys keys, the last one
+// of a segment the rest.
+__global__ __launch_bounds__(256) void k_tile_desc(SegArrays sa, WorkLists wl, int tile_keys, TileDesc* __restrict__ desc) {
+    const uint32_t n = wl.n_segs;
+    for (uint32_t pos = blockIdx.x; pos < n; pos += gridDim.x) {
+        uint32_t L = 0;
+        while (L < 8 && wl.w[24 + L + 1] <= pos) L++;          // the list holding order index pos
+        const uint32_t seg = wl.w[33 + pos];
+        const uint32_t g0 = wl.w[33 + 4 * n + L] + wl.w[33 + n + pos];
+        const uint32_t nk = sa.seg_keys[seg];
+        const uint32_t nt = (uint32_t)(((uint64_t)nk + tile_keys - 1) / tile_keys);
+        const uint64_t kb = sa.key_start[seg];
+        for (uint32_t ti = threadIdx.x; ti < nt; ti += blockDim.x)
+            desc[g0 + ti] = TileDesc{kb, ti, (seg << 16) | min((uint32_t)tile_keys, nk - ti * (uint32_t)tile_keys)};
+    }
 }
 
 // Build the segment layout from per-segment key and header counts (one block).
@@ -365,18 +403,40 @@ __global__ __launch_bounds__(256) void k_hist_scan(Plan* __restrict__ plan, uint
 
 // register target of the onesweep pass: waves per SIMD (64-bit keys, no
 // prefetch; 128-bit keys, next tile prefetched in registers).  A 512-thread
-// workgroup holds a tile of the same LDS size as a 256-thread one with twice
-// the waves to hide the loads' latency behind, so it is held to 80 VGPRs
-// (6 waves per SIMD: three such workgroups per CU, the LDS limit).
-template <typename K, int NT> constexpr int onesweep_waves() { return NT == 512 ? 6 : sizeof(K) == 16 ? 3 : 4; }
+// workgroup with a tile of up to 6144 keys (the LDS of a 256-thread one, three
+// workgroups per CU) is held to 80 VGPRs (6 waves per SIMD); with 8192 or
+// more keys (runs of 32 or more keys per digit on the way out, two
+// workgroups per CU) to 128 VGPRs (4 waves per SIMD).
+template <typename K, int NT, int ITEMS> constexpr int onesweep_waves() {
+    return NT == 512 ? (NT * ITEMS > 6144 ? 4 : 6) : sizeof(K) == 16 ? 3 : 4;
+}
+
+// Lanes of the wave whose item has this lane's digit dd (digits >= RD mark
+// no item and match nobody): the valid lanes that agree with dd on each of
+// its DB bits, one ballot per bit.  Returned as 32-bit halves.
+template <int DB>
+__device__ __forceinline__ void wave_peers(uint32_t dd, uint32_t RD, uint32_t& plo, uint32_t& phi) {
+    const uint64_t v = __ballot(dd < RD);
+    uint32_t alo = (uint32_t)v, ahi = (uint32_t)(v >> 32);
+#pragma unroll
+    for (int b = 0; b < DB; b++) {
+        int32_t bm = __builtin_amdgcn_sbfe((int32_t)dd, b, 1);     // 0 or ~0: this lane's bit b
+        asm("" : "+v"(bm));      // the ballot tests bm itself: one compare per bit
+        const uint64_t bal = __ballot(bm != 0);
+        alo &= ~((uint32_t)bal ^ (uint32_t)bm);
+        ahi &= ~((uint32_t)(bal >> 32) ^ (uint32_t)bm);
+    }
+    plo = alo; phi = ahi;
+}
+
 // One onesweep pass over all segments, DB-bit digits (RD = 2^DB <= NT
 // digits, one per thread of the first RD).  Tile = NT * ITEMS keys of one
-// segment; tiles are claimed in order from the work lists, so every tile's
-// predecessors are claimed by running workgroups (forward progress of the
-// look-back).  Per tile (claimed, then its keys loaded at once: a tile
-// claimed ahead of time would publish its aggregate later and hold up its
-// successors' look-back, which costs more than the load latency the other
-// workgroups on the CU hide):
+// segment; tiles are claimed in order from the work lists (one atomic, then
+// the tile's descriptor), so every tile's predecessors are claimed by running
+// workgroups (forward progress of the look-back).  Per tile (claimed, then
+// its keys loaded at once: a tile claimed ahead of time would publish its
+// aggregate later and hold up its successors' look-back, which costs more
+// than the load latency the other workgroups on the CU hide):
 //   1. digits and the tile's digit histogram (LDS atomics, no returns)
 //   2. publish the aggregate at once, so successors can look back early
 //   4. stable rank within each wave in (item, lane) order: lanes with equal
@@ -384,21 +444,29 @@ template <typename K, int NT> constexpr int onesweep_waves() { return NT == 512 
 //      counters live in LDS
 //   5. decoupled look-back per digit across the segment's earlier tiles
 //   6. reorder the tile in LDS by digit and write each digit's run out
+// The per-wave counters become each wave's tile positions per digit (digit
+// start + earlier waves' counts), and a digit's output base is kept as a
+// 32-bit segment offset less its tile start, so a key at tile position j
+// goes to segment index base + j.
 template <typename K, int ITEMS, int NT, int DB>
-__global__ __launch_bounds__(NT, (onesweep_waves<K, NT>())) void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const K* __restrict__ in,
+__global__ __launch_bounds__(NT, (onesweep_waves<K, NT, ITEMS>())) void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl,
+                                                      const TileDesc* __restrict__ tdesc, const K* __restrict__ in,
                                                       K* __restrict__ out, uint32_t* __restrict__ lookback, int pass) {
     constexpr int RD = 1 << DB;                    // digits of this pass
     static_assert(DB >= 6 && DB <= RADIX_BITS && NT % RD == 0, "one digit per thread of whole waves");
     constexpr int TILE = NT * ITEMS;
+    static_assert(TILE < 65536, "keys of a tile fit the descriptor's 16 bits");
     constexpr int NWAVE = NT / GKC_WAVE;           // ranking waves
     constexpr int DWAVE = RD / GKC_WAVE;           // waves holding one digit per thread
+    // per-wave digit counters, then tile positions (< TILE): 16 bits in
+    // 512-thread workgroups, whose tiles take all but a few KB of their LDS
+    using whist_t = typename std::conditional<(NT >= 512), uint16_t, uint32_t>::type;
     __shared__ K s_keys[TILE];
-    __shared__ uint32_t s_whist[NWAVE][RD];
+    __shared__ whist_t s_whist[NWAVE][RD];
     __shared__ uint32_t s_hist[RD];
-    __shared__ uint32_t s_dstart[RD];
     __shared__ uint32_t s_wsum[DWAVE];
-    __shared__ uint64_t s_gbase[RD];
-    __shared__ uint32_t s_seg, s_idx, s_ok;
+    __shared__ uint32_t s_gbase[RD];
+    __shared__ uint32_t s_g, s_ok;
     const int sh = (int)plan->sh0 + pass * DB;
     const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE;
     const uint32_t d = threadIdx.x;     // threads below RD: one digit each
@@ -407,251 +475,158 @@ __global__ __launch_bounds__(NT, (onesweep_waves<K, NT>())) void k_onesweep(Plan
     uint32_t list = blockIdx.x % wl.n_lists;
     const uint32_t wbase = wv * GKC_WAVE * ITEMS;   // wave wv owns tile keys [wbase, wbase + 64*ITEMS)
 
+    // steps 1-6 for the tile `tile` whose keys are in key[]; `mid` runs once
+    // the tile's aggregate is out (the 128-bit prefetch of the next tile)
+    auto sort_tile = [&](const K (&key)[ITEMS], uint32_t tile, uint32_t seg, uint32_t ti, uint32_t tn, uint64_t kbase,
+                         auto&& mid) {
+        // the segment's digit starts, loaded while the ranking runs
+        const uint32_t hsd = dg ? sa.hist[((size_t)seg * MAX_PASS + pass) * RADIX + d] : 0u;
+        // 1. digits + tile histogram
+        uint32_t dig[ITEMS];          // digit in bits [8:0] (RD = no key), rank in [31:9]
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            const uint32_t j = wbase + i * GKC_WAVE + lane;
+            dig[i] = (j < tn) ? KeyTraits<K>::digit_n(key[i], sh, DB) : (uint32_t)RD;
+            if (dig[i] < (uint32_t)RD) atomicAdd(&s_hist[dig[i]], 1u);
+        }
+        __syncthreads();
+        // 2. aggregate out
+        const uint32_t total = dg ? s_hist[d] : 0u;
+        uint32_t* lb = lookback + (size_t)tile * RADIX;
+        if (dg) st_agent(&lb[d], (ti == 0 ? LB_INC : LB_AGG) | total);
+        mid();
+        // 4. stable rank within the wave
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            const uint32_t dd = dig[i];
+            uint32_t plo, phi;
+            wave_peers<DB>(dd, (uint32_t)RD, plo, phi);
+            if (dd < (uint32_t)RD) {
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
+                const uint32_t before = s_whist[wv][dd];
+                dig[i] = dd | ((before + below) << 9);
+                if (below == 0) s_whist[wv][dd] = (whist_t)(before + (uint32_t)__popc(plo) + (uint32_t)__popc(phi));
+            }
+        }
+        // tile-local digit starts: exclusive scan over the digits, one
+        // wave-level shuffle scan per 64 digits plus the wave sums
+        uint32_t inc = total;
+#pragma unroll
+        for (int off = 1; off < GKC_WAVE; off <<= 1) {
+            const uint32_t y = __shfl_up(inc, off);
+            if ((int)lane >= off) inc += y;
+        }
+        if (dg && lane == GKC_WAVE - 1) s_wsum[wv] = inc;
+        __syncthreads();                                         // ranking done, wave sums visible
+        uint32_t wsum = 0;
+#pragma unroll
+        for (int w = 0; w < DWAVE; w++) wsum += (w < (int)wv) ? s_wsum[w] : 0u;
+        uint32_t dstart = 0;
+        if (dg) {   // per digit and wave: its first tile position
+            dstart = wsum + inc - total;
+            uint32_t acc = dstart;
+#pragma unroll
+            for (int w = 0; w < NWAVE; w++) { const uint32_t c = s_whist[w][d]; s_whist[w][d] = (whist_t)acc; acc += c; }
+        }
+        __syncthreads();
+        // 5. decoupled look-back, one digit per thread
+        uint32_t excl = 0;
+        if (dg && ti > 0) {
+            int64_t j = (int64_t)tile - 1;
+            for (;;) {
+                uint32_t v = ld_agent(&lookback[(size_t)j * RADIX + d]);
+                while ((v & ~LB_VAL) == 0) { __builtin_amdgcn_s_sleep(1); v = ld_agent(&lookback[(size_t)j * RADIX + d]); }
+                excl += v & LB_VAL;
+                if (v & LB_INC) break;
+                j--;
+            }
+            st_agent(&lb[d], LB_INC | (excl + total));
+        }
+        if (dg) s_gbase[d] = hsd + excl - dstart;    // mod 2^32; base + j is the key's segment index
+        // 6. reorder in LDS by digit, write each digit's run out
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            const uint32_t dd = dig[i] & 511u;
+            if (dd < (uint32_t)RD) s_keys[s_whist[wv][dd] + (dig[i] >> 9)] = key[i];
+        }
+        __syncthreads();
+        K* const sout = out + kbase;
+        for (uint32_t j = threadIdx.x; j < tn; j += NT) {
+            const K x = s_keys[j];
+            sout[(uint32_t)(s_gbase[KeyTraits<K>::digit_n(x, sh, DB)] + j)] = x;
+        }
+    };
+    auto claim_next = [&]() {
+        if (threadIdx.x == 0) {
+            uint32_t g = 0;
+            s_ok = claim_tile(wl, ctr, list, g) ? 1u : 0u;
+            s_g = g;
+        }
+        if (dg) s_hist[d] = 0;
+        for (int i = threadIdx.x; i < NWAVE * RD; i += NT) (&s_whist[0][0])[i] = 0;
+    };
+
     if constexpr (sizeof(K) == 16) {
         // 128-bit keys: the next tile's keys are prefetched into registers
         // during this one (their tiles are large enough that it pays)
-        // prologue: first tile
         if (threadIdx.x == 0) {
-            uint32_t sg = 0, ix = 0;
-            s_ok = claim<0>(wl, ctr, list, sg, ix);
-            s_seg = sg; s_idx = ix;
+            uint32_t g = 0;
+            s_ok = claim_tile(wl, ctr, list, g) ? 1u : 0u;
+            s_g = g;
         }
         __syncthreads();
         if (!s_ok) return;
-        uint32_t seg = s_seg, ti = s_idx;
-        uint32_t nseg = sa.seg_keys[seg];
-        uint64_t kbase = sa.key_start[seg];
-        uint32_t tn = min((uint32_t)TILE, nseg - ti * TILE);
+        uint32_t tile = __builtin_amdgcn_readfirstlane(s_g);
+        TileDesc td = tdesc[tile];
         K key[ITEMS];
-    #pragma unroll
+#pragma unroll
         for (int i = 0; i < ITEMS; i++) {
             const uint32_t j = wbase + i * GKC_WAVE + lane;
-            if (j < tn) key[i] = in[kbase + (uint64_t)ti * TILE + j];
+            if (j < (td.seg_tn & 0xFFFFu)) key[i] = in[td.kbase + (uint64_t)td.ti * TILE + j];
         }
         for (;;) {
-            // claim the next tile while this one is being processed
             __syncthreads();
-            if (threadIdx.x == 0) {
-                uint32_t sg = 0, ix = 0;
-                s_ok = claim<0>(wl, ctr, list, sg, ix);
-                s_seg = sg; s_idx = ix;
-            }
-            if (dg) s_hist[d] = 0;
-            for (int i = threadIdx.x; i < NWAVE * RD; i += NT) (&s_whist[0][0])[i] = 0;
+            claim_next();
             __syncthreads();
             const bool have_next = s_ok;
-            const uint32_t nxt_seg = s_seg, nxt_ti = s_idx;
-            const uint32_t tile = sa.sort_tile_start[seg] + ti;      // look-back slot
-            // 1. digits + tile histogram
-            uint32_t dig[ITEMS];          // digit in bits [8:0] (RADIX = invalid), rank in [31:9]
-    #pragma unroll
-            for (int i = 0; i < ITEMS; i++) {
-                const uint32_t j = wbase + i * GKC_WAVE + lane;
-                dig[i] = (j < tn) ? KeyTraits<K>::digit_n(key[i], sh, DB) : (uint32_t)RD;
-                if (dig[i] < (uint32_t)RD) atomicAdd(&s_hist[dig[i]], 1u);
-            }
-            __syncthreads();
-            // 2. aggregate out
-            const uint32_t total = dg ? s_hist[d] : 0u;
-            uint32_t* lb = lookback + (size_t)tile * RADIX;
-            if (dg) st_agent(&lb[d], (ti == 0 ? LB_INC : LB_AGG) | total);
-            // 3. prefetch the next tile
+            const uint32_t nxt = __builtin_amdgcn_readfirstlane(s_g);
             K nkey[ITEMS];
-            uint32_t n_nseg = 0, n_tn = 0;
-            uint64_t n_kbase = 0;
-            if (have_next) {
-                n_nseg = sa.seg_keys[nxt_seg];
-                n_kbase = sa.key_start[nxt_seg];
-                n_tn = min((uint32_t)TILE, n_nseg - nxt_ti * TILE);
-    #pragma unroll
-                for (int i = 0; i < ITEMS; i++) {
-                    const uint32_t j = wbase + i * GKC_WAVE + lane;
-                    if (j < n_tn) nkey[i] = in[n_kbase + (uint64_t)nxt_ti * TILE + j];
+            TileDesc ntd{};
+            sort_tile(key, tile, td.seg_tn >> 16, td.ti, td.seg_tn & 0xFFFFu, td.kbase, [&] {
+                if (have_next) {     // 3. the next tile's keys load during this one
+                    ntd = tdesc[nxt];
+#pragma unroll
+                    for (int i = 0; i < ITEMS; i++) {
+                        const uint32_t j = wbase + i * GKC_WAVE + lane;
+                        if (j < (ntd.seg_tn & 0xFFFFu)) nkey[i] = in[ntd.kbase + (uint64_t)ntd.ti * TILE + j];
+                    }
                 }
-            }
-            // 4. stable rank within the wave
-            const uint64_t lt = lanemask_lt();
-    #pragma unroll
-            for (int i = 0; i < ITEMS; i++) {
-                const uint32_t dd = dig[i];
-                // peers (lanes with this lane's digit): the complement of the OR
-                // over the digit bits of ballot(bit) ^ own bit, in 32-bit halves
-                const uint64_t v = __ballot(dd < (uint32_t)RD);
-                uint32_t mlo = ~(uint32_t)v, mhi = ~(uint32_t)(v >> 32);
-    #pragma unroll
-                for (int b = 0; b < DB; b++) {
-                    const uint32_t bm = (uint32_t)((int32_t)(dd << (31 - b)) >> 31);     // 0 or ~0
-                    const uint64_t bal = __ballot(bm != 0u);
-                    mlo |= (uint32_t)bal ^ bm;
-                    mhi |= (uint32_t)(bal >> 32) ^ bm;
-                }
-                if (dd < (uint32_t)RD) {
-                    const uint64_t peers = ~(((uint64_t)mhi << 32) | mlo);
-                    const uint32_t before = s_whist[wv][dd];
-                    const uint64_t lower = peers & lt;
-                    dig[i] = dd | ((before + (uint32_t)__popcll(lower)) << 9);
-                    if (lower == 0) s_whist[wv][dd] = before + (uint32_t)__popcll(peers);
-                }
-            }
-            // tile-local digit starts: exclusive scan over the 256 digits, one
-            // wave-level shuffle scan per 64 digits plus the 4 wave sums
-            uint32_t inc = total;
-    #pragma unroll
-            for (int off = 1; off < GKC_WAVE; off <<= 1) {
-                const uint32_t y = __shfl_up(inc, off);
-                if ((int)lane >= off) inc += y;
-            }
-            if (dg && lane == GKC_WAVE - 1) s_wsum[wv] = inc;
-            __syncthreads();                                         // ranking done, wave sums visible
-            uint32_t wsum = 0;
-    #pragma unroll
-            for (int w = 0; w < DWAVE; w++) wsum += (w < (int)wv) ? s_wsum[w] : 0u;
-            if (dg) {   // per digit: its tile-local start, exclusive prefix over the waves' running counters
-                s_dstart[d] = wsum + inc - total;
-                uint32_t acc = 0;
-    #pragma unroll
-                for (int w = 0; w < NWAVE; w++) { const uint32_t c = s_whist[w][d]; s_whist[w][d] = acc; acc += c; }
-            }
-            __syncthreads();
-            // 5. decoupled look-back, one digit per thread
-            uint32_t excl = 0;
-            if (dg && ti > 0) {
-                int64_t j = (int64_t)tile - 1;
-                for (;;) {
-                    uint32_t v = ld_agent(&lookback[(size_t)j * RADIX + d]);
-                    while ((v & ~LB_VAL) == 0) { __builtin_amdgcn_s_sleep(1); v = ld_agent(&lookback[(size_t)j * RADIX + d]); }
-                    excl += v & LB_VAL;
-                    if (v & LB_INC) break;
-                    j--;
-                }
-                st_agent(&lb[d], LB_INC | (excl + total));
-            }
-            const uint32_t* hs = sa.hist + ((size_t)seg * MAX_PASS + pass) * RADIX;
-            if (dg) s_gbase[d] = kbase + hs[d] + excl - s_dstart[d];
-            // 6. reorder in LDS by digit, write each digit's run out
-    #pragma unroll
-            for (int i = 0; i < ITEMS; i++) {
-                const uint32_t dd = dig[i] & 511u;
-                if (dd < (uint32_t)RD) s_keys[s_dstart[dd] + s_whist[wv][dd] + (dig[i] >> 9)] = key[i];
-            }
-            __syncthreads();
-            for (uint32_t j = threadIdx.x; j < tn; j += NT) {
-                const K x = s_keys[j];
-                out[s_gbase[KeyTraits<K>::digit_n(x, sh, DB)] + j] = x;
-            }
+            });
             if (!have_next) break;
-            seg = nxt_seg; ti = nxt_ti; nseg = n_nseg; kbase = n_kbase; tn = n_tn;
-    #pragma unroll
+            tile = nxt; td = ntd;
+#pragma unroll
             for (int i = 0; i < ITEMS; i++) key[i] = nkey[i];
         }
     } else {
         K key[ITEMS];
-        uint32_t seg = 0, ti = 0, nseg = 0, tn = 0;
-        uint64_t kbase = 0;
         for (;;) {
             __syncthreads();
-            if (threadIdx.x == 0) {
-                uint32_t sg = 0, ix = 0;
-                s_ok = claim<0>(wl, ctr, list, sg, ix);
-                s_seg = sg; s_idx = ix;
-            }
-            if (dg) s_hist[d] = 0;
-            for (int i = threadIdx.x; i < NWAVE * RD; i += NT) (&s_whist[0][0])[i] = 0;
+            claim_next();
             __syncthreads();
             if (!s_ok) break;
-            seg = s_seg; ti = s_idx;
-            nseg = sa.seg_keys[seg];
-            kbase = sa.key_start[seg];
-            tn = min((uint32_t)TILE, nseg - ti * TILE);
-    #pragma unroll
-            for (int i = 0; i < ITEMS; i++) {
-                const uint32_t j = wbase + i * GKC_WAVE + lane;
-                if (j < tn) key[i] = in[kbase + (uint64_t)ti * TILE + j];
+            const uint32_t tile = __builtin_amdgcn_readfirstlane(s_g);     // look-back slot
+            const TileDesc td = tdesc[tile];
+            const uint32_t tn = td.seg_tn & 0xFFFFu;
+            const K* tin = in + td.kbase + (uint64_t)td.ti * TILE + wbase + lane;
+            if (tn == (uint32_t)TILE) {
+#pragma unroll
+                for (int i = 0; i < ITEMS; i++) key[i] = tin[i * GKC_WAVE];
+            } else {
+#pragma unroll
+                for (int i = 0; i < ITEMS; i++)
+                    if (wbase + i * GKC_WAVE + lane < tn) key[i] = tin[i * GKC_WAVE];
             }
-            const uint32_t tile = sa.sort_tile_start[seg] + ti;      // look-back slot
-            // 1. digits + tile histogram
-            uint32_t dig[ITEMS];          // digit in bits [8:0] (RADIX = invalid), rank in [31:9]
-    #pragma unroll
-            for (int i = 0; i < ITEMS; i++) {
-                const uint32_t j = wbase + i * GKC_WAVE + lane;
-                dig[i] = (j < tn) ? KeyTraits<K>::digit_n(key[i], sh, DB) : (uint32_t)RD;
-                if (dig[i] < (uint32_t)RD) atomicAdd(&s_hist[dig[i]], 1u);
-            }
-            __syncthreads();
-            // 2. aggregate out
-            const uint32_t total = dg ? s_hist[d] : 0u;
-            uint32_t* lb = lookback + (size_t)tile * RADIX;
-            if (dg) st_agent(&lb[d], (ti == 0 ? LB_INC : LB_AGG) | total);
-            // 4. stable rank within the wave
-            const uint64_t lt = lanemask_lt();
-    #pragma unroll
-            for (int i = 0; i < ITEMS; i++) {
-                const uint32_t dd = dig[i];
-                // peers (lanes with this lane's digit): the complement of the OR
-                // over the digit bits of ballot(bit) ^ own bit, in 32-bit halves
-                const uint64_t v = __ballot(dd < (uint32_t)RD);
-                uint32_t mlo = ~(uint32_t)v, mhi = ~(uint32_t)(v >> 32);
-    #pragma unroll
-                for (int b = 0; b < DB; b++) {
-                    const uint32_t bm = (uint32_t)((int32_t)(dd << (31 - b)) >> 31);     // 0 or ~0
-                    const uint64_t bal = __ballot(bm != 0u);
-                    mlo |= (uint32_t)bal ^ bm;
-                    mhi |= (uint32_t)(bal >> 32) ^ bm;
-                }
-                if (dd < (uint32_t)RD) {
-                    const uint64_t peers = ~(((uint64_t)mhi << 32) | mlo);
-                    const uint32_t before = s_whist[wv][dd];
-                    const uint64_t lower = peers & lt;
-                    dig[i] = dd | ((before + (uint32_t)__popcll(lower)) << 9);
-                    if (lower == 0) s_whist[wv][dd] = before + (uint32_t)__popcll(peers);
-                }
-            }
-            // tile-local digit starts: exclusive scan over the 256 digits, one
-            // wave-level shuffle scan per 64 digits plus the 4 wave sums
-            uint32_t inc = total;
-    #pragma unroll
-            for (int off = 1; off < GKC_WAVE; off <<= 1) {
-                const uint32_t y = __shfl_up(inc, off);
-                if ((int)lane >= off) inc += y;
-            }
-            if (dg && lane == GKC_WAVE - 1) s_wsum[wv] = inc;
-            __syncthreads();                                         // ranking done, wave sums visible
-            uint32_t wsum = 0;
-    #pragma unroll
-            for (int w = 0; w < DWAVE; w++) wsum += (w < (int)wv) ? s_wsum[w] : 0u;
-            if (dg) {   // per digit: its tile-local start, exclusive prefix over the waves' running counters
-                s_dstart[d] = wsum + inc - total;
-                uint32_t acc = 0;
-    #pragma unroll
-                for (int w = 0; w < NWAVE; w++) { const uint32_t c = s_whist[w][d]; s_whist[w][d] = acc; acc += c; }
-            }
-            __syncthreads();
-            // 5. decoupled look-back, one digit per thread
-            uint32_t excl = 0;
-            if (dg && ti > 0) {
-                int64_t j = (int64_t)tile - 1;
-                for (;;) {
-                    uint32_t v = ld_agent(&lookback[(size_t)j * RADIX + d]);
-                    while ((v & ~LB_VAL) == 0) { __builtin_amdgcn_s_sleep(1); v = ld_agent(&lookback[(size_t)j * RADIX + d]); }
-                    excl += v & LB_VAL;
-                    if (v & LB_INC) break;
-                    j--;
-                }
-                st_agent(&lb[d], LB_INC | (excl + total));
-            }
-            const uint32_t* hs = sa.hist + ((size_t)seg * MAX_PASS + pass) * RADIX;
-            if (dg) s_gbase[d] = kbase + hs[d] + excl - s_dstart[d];
-            // 6. reorder in LDS by digit, write each digit's run out
-    #pragma unroll
-            for (int i = 0; i < ITEMS; i++) {
-                const uint32_t dd = dig[i] & 511u;
-                if (dd < (uint32_t)RD) s_keys[s_dstart[dd] + s_whist[wv][dd] + (dig[i] >> 9)] = key[i];
-            }
-            __syncthreads();
-            for (uint32_t j = threadIdx.x; j < tn; j += NT) {
-                const K x = s_keys[j];
-                out[s_gbase[KeyTraits<K>::digit_n(x, sh, DB)] + j] = x;
-            }
+            sort_tile(key, tile, td.seg_tn >> 16, td.ti, tn, td.kbase, [] {});
         }
     }
 }
@@ -1875,9 +1850,10 @@ template __global__ void k_wavecopy<u128k>(Plan*, SegArrays, const u128k*, const
 
 template __global__ void k_extract<uint64_t>(Plan*, SegArrays, WorkLists, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, uint64_t*, int);
 template __global__ void k_extract<u128k>(Plan*, SegArrays, WorkLists, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, u128k*, int);
-#define GKC_OS(K, I, N, D) template __global__ void k_onesweep<K, I, N, D>(Plan*, SegArrays, WorkLists, const K*, K*, uint32_t*, int);
+#define GKC_OS(K, I, N, D) template __global__ void k_onesweep<K, I, N, D>(Plan*, SegArrays, WorkLists, const TileDesc*, const K*, K*, uint32_t*, int);
 GKC_OS(uint64_t, 8, 256, 8) GKC_OS(uint64_t, 12, 256, 8) GKC_OS(uint64_t, 16, 256, 8) GKC_OS(uint64_t, 20, 256, 8)
 GKC_OS(uint64_t, 24, 256, 8) GKC_OS(uint64_t, 8, 512, 8) GKC_OS(uint64_t, 10, 512, 8) GKC_OS(uint64_t, 12, 512, 8)
+GKC_OS(uint64_t, 16, 512, 8) GKC_OS(uint64_t, 18, 512, 8)
 GKC_OS(u128k, 4, 256, 8) GKC_OS(u128k, 8, 256, 8) GKC_OS(u128k, 12, 256, 8)
 GKC_OS(uint64_t, 16, 256, 7) GKC_OS(uint64_t, 20, 256, 7) GKC_OS(uint64_t, 16, 256, 6) GKC_OS(uint64_t, 20, 256, 6)
 GKC_OS(u128k, 8, 256, 7) GKC_OS(u128k, 12, 256, 7) GKC_OS(u128k, 8, 256, 6) GKC_OS(u128k, 12, 256, 6)
