@@ -418,7 +418,7 @@ def run_config(cid: int, scale: float, steps: int, warmup: int, forms: list[str]
                                              "one-GPU node shares out per GPU"}
         rep["bit_exact_sample"] = exact
     handle = {"ctx": ctx, "rs": rs, "N": N, "W_key": W_key, "pass_ms": pass_ms, "d_reads": d_reads,
-              "d_offs": d_offs, "h2d": h2d, "k": w.k}
+              "d_offs": d_offs, "h2d": h2d, "k": w.k, "P": w.n_parts}
     return rep, handle
 
 
@@ -453,6 +453,38 @@ def copy_ceiling(torch, capi, nbytes: int) -> dict:
     ok = bool(torch.equal(a[-4096:], b[-4096:]))
     del a, b
     return {"bytes_each_way": nbytes, "ms": best, "GB_per_s": 2 * nbytes / best / 1e6, "output_equal": ok}
+
+
+def scatter_ceiling(torch, capi, n_keys: int, n_parts: int, runs_list=(256, 128)) -> dict:
+    """The library's scatter copy (one onesweep pass's write pattern: 5120-key
+    tiles, each written as `runs` runs into its segment's buckets, segments the
+    size of an average partition on 8 per-die lists, 3 workgroups per CU, no
+    ranking, histogram or look-back) over about as many keys as the pass: the
+    ceiling of that access pattern.  Best of five after a warm-up."""
+    tile = 5120
+    seg = max(tile, (n_keys // n_parts) // tile * tile)
+    n = seg * max(1, n_keys // seg)
+    a = torch.ones(n, dtype=torch.int64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream().cuda_stream
+    props = torch.cuda.get_device_properties(0)
+    wg = 3 * props.multi_processor_count
+    wg -= wg % 8
+    out = {"keys": n, "seg_keys": seg, "tile_keys": tile, "workgroups": wg}
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    for runs in runs_list:
+        capi.scatter_copy(b.data_ptr(), a.data_ptr(), n, seg, runs, wg, st)
+        best = 1e9
+        for _ in range(5):
+            ev[0].record()
+            capi.scatter_copy(b.data_ptr(), a.data_ptr(), n, seg, runs, wg, st)
+            ev[1].record()
+            ev[1].synchronize()
+            best = min(best, ev[0].elapsed_time(ev[1]))
+        out[f"runs_{runs}"] = {"keys_per_run": tile // runs, "ms": best, "GB_per_s": 16 * n / best / 1e6}
+    del a, b
+    return out
 
 
 def vendor_sort(torch, capi, n: int, begin_bit: int, end_bit: int) -> dict:
@@ -517,6 +549,13 @@ def main() -> int:
     ceil = copy_ceiling(torch, capi, hd["W_key"] * hd["N"])
     rep["roofline"]["copy_ceiling"] = ceil
     rep["roofline"]["frac_of_copy_ceiling"] = rep["roofline"]["achieved"] / ceil["GB_per_s"]
+    if hd["W_key"] == 8:    # the ceiling of the pass's write pattern (runs of a 5120-key tile's 256 digits)
+        sc = scatter_ceiling(torch, capi, hd["N"], hd["P"])
+        rep["roofline"]["scatter_ceiling"] = sc
+        rep["roofline"]["frac_of_scatter_ceiling"] = rep["roofline"]["achieved"] / sc["runs_256"]["GB_per_s"]
+        _log(f"scatter copy with the pass's write pattern ({sc['seg_keys']}-key segments, 5120-key tiles): "
+             + ", ".join(f"{v['keys_per_run']}-key runs {v['GB_per_s']:.0f} GB/s" for k_, v in sc.items()
+                         if k_.startswith("runs_")))
     if hd["W_key"] == 8:    # the vendor library's sort of as many keys on the same 16 bits
         kb = 2 * hd["k"]
         vs = vendor_sort(torch, capi, hd["N"], max(kb - 16, 0), kb)

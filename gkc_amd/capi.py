@@ -103,6 +103,7 @@ SIGNATURES = {
     "gkc_parse_reads": (C.c_int64, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gkc_parse_reads_noalloc": (C.c_int64, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gkc_stream_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "gkc_scatter_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p]),
     "gkc_vendor_sort_u64": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.c_void_p, _f32p]),
     "gkc_set_predictor": (C.c_int, [C.c_void_p, C.c_void_p, _f32p, _f32p, _f32p, _f32p, C.c_float]),
     "gkc_learned_caps": (C.c_int, [C.c_void_p, _u32p]),
@@ -361,6 +362,14 @@ def stream_copy(dst_ptr: int, src_ptr: int, nbytes: int, stream: int = 0) -> Non
     """The library's streaming copy (device memory, 16-byte aligned sizes and
     addresses), asynchronous on `stream`: the practical HBM ceiling."""
     _check(lib().gkc_stream_copy(C.c_void_p(dst_ptr), C.c_void_p(src_ptr), nbytes, C.c_void_p(stream)))
+
+
+def scatter_copy(dst_ptr: int, src_ptr: int, n_keys: int, seg_keys: int, runs: int, workgroups: int,
+                 stream: int = 0) -> None:
+    """Copy u64 keys with one onesweep pass's write pattern and none of its
+    work (gkc_scatter_copy), asynchronous on `stream`: that pattern's ceiling."""
+    _check(lib().gkc_scatter_copy(C.c_void_p(dst_ptr), C.c_void_p(src_ptr), n_keys, seg_keys, runs, workgroups,
+                                  C.c_void_p(stream)))
 
 
 def vendor_sort_u64(in_ptr: int, out_ptr: int, n: int, begin_bit: int, end_bit: int, stream: int = 0) -> float:

@@ -255,6 +255,15 @@ int gkc_device_select(int device);
  * reports its kernels against, and a kernel of known traffic for HBM byte
  * counters.  Asynchronous; GKC_EINVAL for misaligned or odd sizes. */
 int gkc_stream_copy(void *dst, const void *src, uint64_t bytes, void *stream);
+/* Copy n_keys 64-bit keys with the write pattern of one onesweep pass and
+ * none of its work: segments of seg_keys keys (a multiple of 5120; n_keys a
+ * multiple of seg_keys), tiles of 5120 keys read in order, each tile written
+ * as `runs` runs (runs divides 5120) into the segment's `runs` bucket regions,
+ * segments spread over 8 lists, `workgroups` (a multiple of 8) workgroups.
+ * The ceiling of that access pattern, for the benchmark; asynchronous on
+ * `stream`; GKC_EINVAL for shapes outside these rules. */
+int gkc_scatter_copy(void *dst, const void *src, uint64_t n_keys, uint64_t seg_keys, uint32_t runs,
+                     uint32_t workgroups, void *stream);
 /* The vendor library's radix sort (rocPRIM radix_sort_keys) of n 64-bit keys
  * on bits [begin_bit, end_bit), device buffers, timed with HIP events on
  * `stream` into *ms (synchronous; allocates its temporary storage): a
