@@ -287,8 +287,10 @@ static BinSet binset(gkc_ctx* c, bool counting) {
 }
 
 // Stage a batch on the device; returns the number of bases. Fills d_chunk.
+// With HPC, hpc_pos also keeps each run's in-read origin (d_hpc_pos: only the
+// reference-layout arrays of gkc_superkmer_arrays hand it out).
 static uint64_t stage_batch(gkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n, uint32_t flags,
-                            ChunkDesc& cd) {
+                            ChunkDesc& cd, bool hpc_pos = false) {
     uint64_t o0, on;
     const uint8_t* dreads;
     const uint64_t* doffs;
@@ -324,10 +326,11 @@ static uint64_t stage_batch(gkc_ctx* c, const char* reads, const uint64_t* offs,
         // start at the same position, so the tiles index them like the input
         c->d_hpc_reads.ensure(B + lead + 64);
         c->d_lens.ensure((n + 1) * 4);
-        c->d_hpc_pos.ensure((B + 1) * 4);
+        if (hpc_pos) c->d_hpc_pos.ensure((B + 1) * 4);
         int blocks = std::min<int64_t>((n + 3) / 4, 65535);
         hipLaunchKernelGGL(k_hpc, dim3(blocks), dim3(256), 0, c->st, dreads + lead, doffs, (uint32_t)n,
-                           c->d_hpc_reads.as<uint8_t>() + lead, c->d_lens.as<int32_t>(), c->d_hpc_pos.as<int32_t>());
+                           c->d_hpc_reads.as<uint8_t>() + lead, c->d_lens.as<int32_t>(),
+                           hpc_pos ? c->d_hpc_pos.as<int32_t>() : nullptr);
         HIPX(hipGetLastError());
         cd.reads = c->d_hpc_reads.as<uint8_t>();
         cd.lens = c->d_lens.as<int32_t>();
@@ -521,7 +524,7 @@ int gkc_superkmer_arrays(gkc_ctx* c, const char* reads, const uint64_t* offs, in
                                      "lists cannot skip windows)", GKC_EINVAL};
         }
         ChunkDesc cd;
-        const uint64_t B = stage_batch(c, reads, offs, n, 0, cd);
+        const uint64_t B = stage_batch(c, reads, offs, n, 0, cd, true);
         c->d_mm.ensure((B + 1) * 4);
         HIPX(hipMemsetAsync(c->d_mm.p, 0, (B + 1) * 4, c->st));
         P1Params prm = p1params(c, 0);

@@ -612,32 +612,85 @@ __global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __rest
 
 // Homopolymer compression (gkc_cuda.cu:178-190), one wave per read: keep
 // byte j when it differs from byte j-1; out[beg + h] = byte of run h,
-// orig_pos[beg + h] = in-read index of the run's first byte, lens[r] = runs.
-__global__ __launch_bounds__(256) void k_hpc(const uint8_t* __restrict__ reads, const uint64_t* __restrict__ offs,
-                                             uint32_t n_reads, uint8_t* __restrict__ out,
-                                             int32_t* __restrict__ lens, int32_t* __restrict__ orig_pos) {
+// lens[r] = runs, and (when orig_pos is not null) orig_pos[beg + h] = the
+// in-read index of run h's first byte.  Each round a wave takes 1024 bytes,
+// 16 per lane, read as the one or two aligned 16-byte blocks holding them
+// (each holds a byte of the read, so no load leaves its buffer); a lane
+// finds its kept bytes by comparing each byte with the one before (the
+// previous lane's last byte, or the previous round's), places them in the
+// wave's LDS buffer at their rank, and the wave stores the round's output
+// from there, consecutive bytes across the lanes.
+constexpr int HPC_NT = 256;
+__device__ __forceinline__ uint32_t sel4(uint32_t q, uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    return q == 0 ? a : q == 1 ? b : q == 2 ? c : d;
+}
+__global__ __launch_bounds__(HPC_NT) void k_hpc(const uint8_t* __restrict__ reads, const uint64_t* __restrict__ offs,
+                                                uint32_t n_reads, uint8_t* __restrict__ out,
+                                                int32_t* __restrict__ lens, int32_t* __restrict__ orig_pos) {
+    __shared__ uint8_t s_out[HPC_NT / GKC_WAVE][16 * GKC_WAVE];
+    uint8_t* so = s_out[threadIdx.x / GKC_WAVE];
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / GKC_WAVE;
     const uint32_t lane = __lane_id();
     const uint32_t nwaves = gridDim.x * blockDim.x / GKC_WAVE;
     const uint64_t base = offs[0];
     for (uint32_t r = wave; r < n_reads; r += nwaves) {
         const uint64_t beg = offs[r] - base, L = offs[r + 1] - offs[r];
-        uint32_t cnt = 0;
-        for (uint64_t j0 = 0; j0 < L; j0 += GKC_WAVE) {
-            const uint64_t j = j0 + lane;
-            bool keep = false;
-            uint8_t c = 0;
-            if (j < L) {
-                c = reads[beg + j];
-                keep = (j == 0) || (c != reads[beg + j - 1]);
+        const uint8_t* rd = reads + beg;
+        uint32_t cnt = 0;                       // runs so far
+        uint32_t carry = 0x100u;                // the previous round's last byte (none: equal to no byte)
+        for (uint64_t j0 = 0; j0 < L; j0 += 16 * GKC_WAVE) {
+            const uint64_t j = j0 + 16u * lane;
+            const uint32_t nb = j < L ? (uint32_t)(L - j < 16 ? L - j : 16) : 0u;    // this lane's bytes
+            uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;    // bytes j .. j+15, byte j lowest
+            if (nb) {
+                const uintptr_t a0 = reinterpret_cast<uintptr_t>(rd + j);
+                const uintptr_t al = a0 & ~(uintptr_t)15;
+                const uint4 b0 = *reinterpret_cast<const uint4*>(al);
+                uint4 b1 = make_uint4(0u, 0u, 0u, 0u);
+                if (((a0 + nb - 1) & ~(uintptr_t)15) != al) b1 = *reinterpret_cast<const uint4*>(al + 16);
+                const uint32_t sh = (uint32_t)(a0 & 15), q = sh >> 2, sb = sh & 3;
+                // dwords q .. q+4 of the 32 bytes b0:b1, then shifted by sb bytes
+                const uint32_t d0 = sel4(q, b0.x, b0.y, b0.z, b0.w), d1 = sel4(q, b0.y, b0.z, b0.w, b1.x);
+                const uint32_t d2 = sel4(q, b0.z, b0.w, b1.x, b1.y), d3 = sel4(q, b0.w, b1.x, b1.y, b1.z);
+                const uint32_t d4 = sel4(q, b1.x, b1.y, b1.z, b1.w);
+                w0 = __builtin_amdgcn_alignbyte(d1, d0, sb);
+                w1 = __builtin_amdgcn_alignbyte(d2, d1, sb);
+                w2 = __builtin_amdgcn_alignbyte(d3, d2, sb);
+                w3 = __builtin_amdgcn_alignbyte(d4, d3, sb);
             }
-            const uint64_t m = __ballot(keep);
-            if (keep) {
-                const uint32_t h = cnt + __popcll(m & lanemask_lt());
-                out[beg + h] = c;
-                orig_pos[beg + h] = (int32_t)j;
+            // the byte before each of the 16: the previous lane's last, or the carry
+            const uint32_t up = __shfl_up(w3 >> 24, 1);
+            const uint32_t pb = lane == 0 ? carry : up;
+            const uint32_t p0 = pb > 0xFFu ? (w0 ^ 1u) & 0xFFu : pb;         // no byte before: keep byte 0
+            const uint32_t y0 = (w0 << 8) | p0, y1 = __builtin_amdgcn_alignbyte(w1, w0, 3),
+                           y2 = __builtin_amdgcn_alignbyte(w2, w1, 3), y3 = __builtin_amdgcn_alignbyte(w3, w2, 3);
+            auto nz4 = [](uint32_t d) {        // bit i: byte i of d nonzero
+                const uint32_t t = (((d & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | d) & 0x80808080u;
+                return ((t >> 7) * 0x01020408u) >> 24;
+            };
+            uint32_t keep = nz4(w0 ^ y0) | (nz4(w1 ^ y1) << 4) | (nz4(w2 ^ y2) << 8) | (nz4(w3 ^ y3) << 12);
+            keep &= nb >= 16 ? 0xFFFFu : ((1u << nb) - 1u);
+            const uint32_t c = (uint32_t)__popc(keep);
+            uint32_t inc = c;
+#pragma unroll
+            for (int off = 1; off < GKC_WAVE; off <<= 1) {
+                const uint32_t y = __shfl_up(inc, off);
+                if ((int)lane >= off) inc += y;
             }
-            cnt += __popcll(m);
+            const uint32_t tot = __shfl(inc, GKC_WAVE - 1);
+            uint32_t o = inc - c;
+            for (uint32_t m = keep; m; m &= m - 1) {
+                const uint32_t t = (uint32_t)__builtin_ctz(m);
+                const uint32_t wd = sel4(t >> 2, w0, w1, w2, w3);
+                so[o] = (uint8_t)(wd >> (8 * (t & 3)));
+                if (orig_pos) orig_pos[beg + cnt + o] = (int32_t)(j + t);
+                o++;
+            }
+            wave_lds_fence();       // every lane's bytes placed
+            for (uint32_t t = lane; t < tot; t += GKC_WAVE) out[beg + cnt + t] = so[t];
+            wave_lds_fence();       // read before the next round's placements
+            cnt += tot;
+            carry = __shfl(w3 >> 24, GKC_WAVE - 1);
         }
         if (lane == 0) lens[r] = (int32_t)cnt;
     }

@@ -1,6 +1,8 @@
 """GPU parity of the HIP counting path against the CPU oracle (bit-exact)."""
 from __future__ import annotations
 
+import types
+
 import numpy as np
 import pytest
 
@@ -139,6 +141,43 @@ def test_superkmer_arrays_hpc(gpu):
         L = int(hl[r])
         np.testing.assert_array_equal(out["reads"][s:s + L], hr[s:s + L])
         np.testing.assert_array_equal(out["hpc_orig_pos"][s:s + L], hp[s:s + L])
+
+
+def _hpc_edge_reads():
+    """A/C/G/T reads whose homopolymer runs cross the HPC pass's 16-byte lane
+    and 1024-byte round boundaries, of every length class around them (0, 1,
+    15-17, 1023-1025 bytes, a few thousand), one run of a single base, and
+    random reads in between so that read starts take every alignment."""
+    rng = np.random.default_rng(11)
+    seqs = [b"", b"A", b"AAAA", b"ACGT" * 4, b"C" * 15, b"G" * 16 + b"T", b"T" * 3000]
+    for L in (15, 16, 17, 31, 33, 1023, 1024, 1025, 2047, 2049, 3001):
+        runs = []
+        while sum(len(r) for r in runs) < L:
+            runs.append(bytes([b"ACGT"[int(rng.integers(4))]]) * int(rng.integers(1, 40)))
+        seqs.append(b"".join(runs)[:L])
+    for _ in range(200):
+        seqs.append(bytes(rng.choice(list(b"ACGT"), size=int(rng.integers(0, 300))).tolist()))
+    reads = np.frombuffer(b"".join(seqs), np.uint8).copy()
+    offs = np.concatenate([[0], np.cumsum([len(x) for x in seqs])]).astype(np.uint64)
+    return reads, offs
+
+
+def test_hpc_edge_reads(gpu):
+    reads, offs = _hpc_edge_reads()
+    hr, hl, hp = O.hpc(reads, offs)
+    _say("hpc edge reads", len(offs) - 1)
+    with capi.Counter(25, 15, 16, hpc=True) as c:
+        out = c.superkmer_arrays(reads, offs)
+    np.testing.assert_array_equal(out["read_len"], hl)
+    for r in range(len(offs) - 1):
+        s, L = int(offs[r]), int(hl[r])
+        np.testing.assert_array_equal(out["reads"][s:s + L], hr[s:s + L])
+        np.testing.assert_array_equal(out["hpc_orig_pos"][s:s + L], hp[s:s + L])
+    rs = types.SimpleNamespace(reads=reads, offs=offs, n=len(offs) - 1)
+    ot = O.count(reads, offs, 13, 5, 16, hpc=True)
+    for flags in (0, capi.BATCHED):
+        gt, _ = _run_gpu(rs, 13, 5, 16, hpc=True, flags=flags, batches=2)
+        assert_tables_equal(gt, ot, 16, 13)
 
 
 def test_count_packed_single_partition(gpu):
