@@ -49,10 +49,11 @@ def _log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(workload: str, items: int):
+def measured_traffic(workload: str, items: int, waves: int = 4):
     """HBM bytes per onesweep launch from the committed rocprofv3 counter runs
     (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel's readings) of the same
-    workload and kernel variant, or (None, None)."""
+    workload and kernel variant (keys per thread, waves per workgroup), or
+    (None, None)."""
     import glob
     import re
     here = os.path.dirname(os.path.abspath(__file__))
@@ -60,7 +61,7 @@ def measured_traffic(workload: str, items: int):
         m = re.search(r"<!-- traffic-json: (\{.*\}) -->", open(f).read())
         if m:
             d = json.loads(m.group(1))
-            if d.get("workload") == workload and d.get("items") == items:
+            if d.get("workload") == workload and d.get("items") == items and d.get("waves", 4) == waves:
                 return int(d["traffic_bytes"]), os.path.relpath(f, here)
     return None, None
 
@@ -366,7 +367,8 @@ def run_config(cid: int, scale: float, steps: int, warmup: int, forms: list[str]
     A = (r + (s_w + h_b) + (s_w + h_b + W_key) + st["n_pass"] * 2 * W_key + (W_key + (W_key + 4) * delta)
          + (2 * (W_key + 4) * delta if st["binned"] else 0.0))
     roof_kmers = HBM_PEAK_GBS * 1e9 / A
-    traffic, traffic_src = measured_traffic(w.name, ctx.get_tuning()["sort_items"])
+    tn = ctx.get_tuning()
+    traffic, traffic_src = measured_traffic(w.name, tn["sort_items"], tn["sort_waves"] or 4)
     best = results.get("batched_learned", results.get("plain", {}))
 
     rep = {
