@@ -109,7 +109,7 @@ def tune_sort(counter: "capi.Counter", reps: int = 2, log=None) -> dict:
     return {**win, "sweep": sweep}
 
 
-COUNT_WINDOWS = (256, 384)
+COUNT_WINDOWS = (128, 256, 384)
 DIGIT_BITS = (8, 7, 6)
 
 
@@ -117,9 +117,9 @@ def tune_bin_passes(counter: "capi.Counter", reps: int = 3, log=None) -> dict:
     """Time whole batched counts (graph replay) of the counter's current bins
     with digits of 8, 7 or 6 bits, 2 or 3 onesweep passes before the binned
     count (bins of 12 to 24 key bits: narrower digits rank with fewer ballots
-    and smaller histograms but leave larger bins) and windows of 256 and 384
-    keys (larger windows spread each window's fixed work over more keys, but
-    hold more distinct keys); returns the best {digit_bits, bin_passes,
+    and smaller histograms but leave larger bins) and windows of 128, 256 and
+    384 keys (larger windows spread each window's fixed work over more keys,
+    but hold more distinct keys, each ranked against all of them); returns the best {digit_bits, bin_passes,
     count_window, ms_count} and the sweep.  The sort tuning already set is
     kept (6- and 7-bit passes use the nearest tile they are built for)."""
     t0 = counter.get_tuning()

@@ -131,7 +131,7 @@ __device__ __forceinline__ uint32_t part_mm(MM mm, uint32_t P, int mode) {
 // and the chosen start and strand are recovered per window.
 // Dynamic LDS: P x (8 + 4) bytes (per-partition counters).
 template <typename MM, bool WIN>
-__global__ __launch_bounds__(P1_NT, 4) void k_superkmers(const ChunkDesc* __restrict__ cdp, P1Params prm, BinSet bins) {
+__global__ __launch_bounds__(P1_NT, (sizeof(MM) == 8 ? 3 : 4)) void k_superkmers(const ChunkDesc* __restrict__ cdp, P1Params prm, BinSet bins) {
     constexpr int CSH = WIN ? 14 : 0;
     constexpr int NQ = P1_RCAP / P1_NT;             // records per thread in a flush
     __shared__ int8_t s_str[WIN ? P1_T + MAXK : 1];
