@@ -124,6 +124,7 @@ struct gkc_ctx {
     // counting
     DevBuf keysA, keysB, lookback, d_plan, d_segk, d_segh, d_keystart, d_itemstart, d_tilestart, d_rlstart, d_hist;
     DevBuf d_tdesc;                            // sort tile descriptors in work-list order (k_tile_desc)
+    DevBuf d_iseg;                             // segment of each extraction item (k_tile_desc)
     DevBuf d_tstart, d_tcap, d_nent, d_ndist, d_toverflow;
     DevBuf d_tcap_total;              // total table capacity of a presized binned count (u64)
     uint64_t h_tcap_total = 0;
@@ -243,7 +244,7 @@ static void bins_reserve(gkc_ctx* c, const std::vector<uint32_t>& need_w, const 
         tw += ww; th += hh;
     }
     DevBuf nw, nh;
-    nw.ensure((tw + 8) * 4);      // +8 words: extraction reads up to 4 words past a k-mer
+    nw.ensure((tw + 8) * 4);      // +8 words: extraction reads up to 7 words past a superkmer's last one
     nh.ensure(th * sizeof(SkmHdr) + 16);
     for (uint32_t q = 0; q < P; q++) {
         if (c->h_word_cur[q]) HIPX(hipMemcpyAsync(nw.as<uint32_t>() + nwb[q], c->words.as<uint32_t>() + c->h_word_base[q],
@@ -1294,14 +1295,16 @@ static K* sort_segments(gkc_ctx* c, SegArrays sa, const uint64_t* hdr_base, cons
     }
     hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(64), 0, c->st, plan, n_segs, (uint32_t)c->npass, 0u, (uint32_t)RADIX_BITS);
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, c->st, plan, sa, TILE, RL_TILE);
-    hipLaunchKernelGGL(k_tile_desc, dim3(n_segs), dim3(256), 0, c->st, sa, c->wl, TILE, c->d_tdesc.as<TileDesc>());
+    c->d_iseg.ensure((n_hdrs / EX_HDRS + n_segs + 1) * 4);
+    hipLaunchKernelGGL(k_tile_desc, dim3(n_segs), dim3(256), 0, c->st, sa, c->wl, TILE, c->d_tdesc.as<TileDesc>(),
+                       c->d_iseg.as<uint32_t>());
     HIPX(hipGetLastError());
     zero(c, sa.hist, (uint64_t)n_segs * MAX_PASS * RADIX);
     HIPX(hipEventRecord(c->ev[2], c->st));
     const uint64_t items = n_hdrs / EX_HDRS + n_segs;
     const size_t ex_lds = (size_t)c->npass * RADIX * sizeof(uint32_t);     // digit histograms
     int g_ex = (int)std::min<uint64_t>(items, occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu, ex_lds));
-    hipLaunchKernelGGL(k_extract<K>, dim3(std::max(g_ex, 1)), dim3(SORT_NT), ex_lds, c->st, plan, sa, c->wl, c->hdrs.as<SkmHdr>(), hdr_base,
+    hipLaunchKernelGGL(k_extract<K>, dim3(std::max(g_ex, 1)), dim3(SORT_NT), ex_lds, c->st, plan, sa, c->d_iseg.as<uint32_t>(), c->hdrs.as<SkmHdr>(), hdr_base,
                        c->words.as<uint32_t>(), word_base, c->keysA.as<K>(), c->prm.k);
     HIPX(hipGetLastError());
     hipLaunchKernelGGL(k_hist_scan, dim3(std::max<uint32_t>(1, (n_segs * MAX_PASS + 3) / 4)), dim3(256), 0, c->st, plan, sa.hist);
@@ -1567,12 +1570,13 @@ static K* enqueue_sort_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs, uint6
     const int TILE = sort_tile_keys<K>(c, bs.dbits);
     hipLaunchKernelGGL(k_plan_init, dim3(1), dim3(64), 0, c->st, plan, P, (uint32_t)bs.npass, bs.sh0, (uint32_t)bs.dbits);
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, c->st, plan, sa, TILE, bs.rl_tile);
-    hipLaunchKernelGGL(k_tile_desc, dim3(P), dim3(256), 0, c->st, sa, c->wl, TILE, c->d_tdesc.as<TileDesc>());
+    hipLaunchKernelGGL(k_tile_desc, dim3(P), dim3(256), 0, c->st, sa, c->wl, TILE, c->d_tdesc.as<TileDesc>(),
+                       c->d_iseg.as<uint32_t>());
     zero(c, sa.hist, (uint64_t)P * MAX_PASS * RADIX);
     if (timed) HIPX(hipEventRecord(c->ev[2], c->st));
     const size_t ex_lds = (size_t)bs.npass * RADIX * sizeof(uint32_t);     // digit histograms
     const int g_ex = occupancy_blocks((const void*)k_extract<K>, SORT_NT, c->n_cu, ex_lds);
-    hipLaunchKernelGGL(k_extract<K>, dim3(g_ex), dim3(SORT_NT), ex_lds, c->st, plan, sa, c->wl, c->hdrs.as<SkmHdr>(),
+    hipLaunchKernelGGL(k_extract<K>, dim3(g_ex), dim3(SORT_NT), ex_lds, c->st, plan, sa, c->d_iseg.as<uint32_t>(), c->hdrs.as<SkmHdr>(),
                        c->d_hdr_base.as<uint64_t>(), c->words.as<uint32_t>(), c->d_word_base.as<uint64_t>(),
                        c->keysA.as<K>(), c->prm.k);
     hipLaunchKernelGGL(k_hist_scan, dim3((P * MAX_PASS + 3) / 4), dim3(256), 0, c->st, plan, sa.hist);
@@ -1716,6 +1720,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     c->stats.n_pass = bs.npass;
     c->stats.binned = bs.binned ? 1 : 0;
     ensure_count_buffers<K>(c, tot_keys, P);
+    c->d_iseg.ensure((tot_hdrs / EX_HDRS + P + 1) * 4);
     const uint64_t tiles_cap = tot_keys / (sort_tile_keys<K>(c, bs.dbits)) + P + 1;
     const uint64_t rl_tiles_cap = tot_keys / bs.rl_tile + P + 1;
     SegArrays sa;
@@ -1754,7 +1759,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
         HIPX(hipMemcpyAsync(c->h_lsum, c->d_tcap_total.p, 8, hipMemcpyDeviceToHost, c->st));
         HIPX(hipEventRecord(c->ev_lsum, c->st));
     }
-    std::vector<const void*> key = {c->keysA.p, c->keysB.p, c->lookback.p, c->d_tdesc.p, c->words.p, c->hdrs.p, c->d_ranges.p,
+    std::vector<const void*> key = {c->keysA.p, c->keysB.p, c->lookback.p, c->d_tdesc.p, c->d_iseg.p, c->words.p, c->hdrs.p, c->d_ranges.p,
                                     c->d_hlist.p, c->d_hres.p, c->d_wkept.p,
                                     (const void*)(uintptr_t)tiles_cap, (const void*)(uintptr_t)bs.npass,
                                     (const void*)(uintptr_t)sort_items<K>(c, bs.dbits), (const void*)(uintptr_t)sort_grid<K>(c, bs.dbits),

@@ -165,11 +165,11 @@ __global__ void k_set_u32(uint32_t* p, uint32_t v);
 __global__ void k_plan_init(Plan* plan, uint32_t n_segs, uint32_t n_pass, uint32_t sh0, uint32_t dbits);
 __global__ void k_set_chunk(ChunkDesc* dst, ChunkDesc v);
 template <typename K>
-__global__ void k_extract(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const SkmHdr* __restrict__ hdrs,
+__global__ void k_extract(Plan* __restrict__ plan, SegArrays sa, const uint32_t* __restrict__ iseg, const SkmHdr* __restrict__ hdrs,
                           const uint64_t* __restrict__ hdr_base, const uint32_t* __restrict__ words,
                           const uint64_t* __restrict__ word_base, K* __restrict__ keys, int k);
 __global__ void k_hist_scan(Plan* __restrict__ plan, uint32_t* __restrict__ hist);
-__global__ void k_tile_desc(SegArrays sa, WorkLists wl, int tile_keys, TileDesc* __restrict__ desc);
+__global__ void k_tile_desc(SegArrays sa, WorkLists wl, int tile_keys, TileDesc* __restrict__ desc, uint32_t* __restrict__ iseg);
 template <typename K, int ITEMS, int NT, int DB>
 __global__ void k_onesweep(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const TileDesc* __restrict__ tdesc,
                            const K* __restrict__ in, K* __restrict__ out, uint32_t* __restrict__ lookback, int pass);

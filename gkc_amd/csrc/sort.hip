@@ -109,8 +109,10 @@ __device__ __forceinline__ bool claim_tile(const WorkLists& wl, uint32_t* ctr, u
 // Sort tile descriptors in work-list order (one workgroup per list-order
 // segment): tile g = its list's first tile + the segment's tile offset in the
 // list + its index in the segment.  Tiles hold tile_keys keys, the last one
-// of a segment the rest.
-__global__ __launch_bounds__(256) void k_tile_desc(SegArrays sa, WorkLists wl, int tile_keys, TileDesc* __restrict__ desc) {
+// of a segment the rest.  Also the segment of each extraction item (segment
+// order, item_start from k_plan).
+__global__ __launch_bounds__(256) void k_tile_desc(SegArrays sa, WorkLists wl, int tile_keys, TileDesc* __restrict__ desc,
+                                                   uint32_t* __restrict__ iseg) {
     const uint32_t n = wl.n_segs;
     for (uint32_t pos = blockIdx.x; pos < n; pos += gridDim.x) {
         uint32_t L = 0;
@@ -122,6 +124,8 @@ __global__ __launch_bounds__(256) void k_tile_desc(SegArrays sa, WorkLists wl, i
         const uint64_t kb = sa.key_start[seg];
         for (uint32_t ti = threadIdx.x; ti < nt; ti += blockDim.x)
             desc[g0 + ti] = TileDesc{kb, ti, (seg << 16) | min((uint32_t)tile_keys, nk - ti * (uint32_t)tile_keys)};
+        const uint32_t i0 = sa.item_start[seg], i1 = sa.item_start[seg + 1];
+        for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) iseg[i] = seg;
     }
 }
 
@@ -191,188 +195,332 @@ __global__ void k_zero32(uint32_t* p, uint64_t n) {
 // key bits [sh0 + d p, sh0 + d p + d), d = plan->dbits; each pass's histogram
 // keeps RADIX slots, the first 2^d used).
 // Bases are 2-bit, 16 per u32 word, MSB first.
-template <typename K>
-__device__ __forceinline__ K kmer_at(const uint32_t* wds, uint32_t o, int k);
+//
+// A lane takes a chunk of R consecutive k-mers of one superkmer.  The
+// chunk's window of words is read once (4 words for 64-bit keys, 6 for
+// 128-bit ones: every word a chunk may reach, whether or not this one reaches
+// it; the word buffers end with 8 words of padding); each forward k-mer is a
+// constant shift of the window, the first reverse complement is computed in
+// full and each following one is the previous one shifted down one base with
+// the complement of the base entering the forward k-mer put in at the top.
+// k-mers per chunk.  128-bit keys take one: their extraction is bound by
+// the key writes (16 bytes a k-mer), which the LDS staging of multi-k-mer
+// chunks' keys only slows (more LDS per workgroup, fewer waves).
+template <typename K> constexpr int ex_r() { return sizeof(K) == 8 ? 4 : 1; }
+// blocks of 64 chunks per wave iteration
+template <typename K> constexpr int ex_u() { return 2; }
+constexpr int EX_SW = 512;   // staged words per wave
+constexpr int EX_SWP = 8;    // zero words after the staged ones (window reads past the last superkmer)
+constexpr uint32_t EX_B = 8; // items per claimed batch
 
-// Every word a k-mer may reach is loaded, whether or not this one reaches it
-// (the word buffers end with 8 words of padding), and the shifts are
-// branch-free (exact for sh = 0 too), so no load waits on another and the
-// loads of several k-mers are in flight together.
-template <>
-__device__ __forceinline__ uint64_t kmer_at<uint64_t>(const uint32_t* wds, uint32_t o, int k) {
-    const uint32_t wi = o >> 4, sh = (o & 15) * 2;        // sh <= 30
-    const uint32_t w0 = wds[wi], w1 = wds[wi + 1], w2 = wds[wi + 2];
-    const uint64_t a = ((uint64_t)w0 << 32) | w1;
-    const uint64_t x = (a << sh) | ((uint64_t)w2 >> (32 - sh));
-    return x >> (64 - 2 * k);
-}
-template <>
-__device__ __forceinline__ u128k kmer_at<u128k>(const uint32_t* wds, uint32_t o, int k) {
-    const uint32_t wi = o >> 4, sh = (o & 15) * 2;        // sh <= 30
-    const uint32_t w0 = wds[wi], w1 = wds[wi + 1], w2 = wds[wi + 2], w3 = wds[wi + 3], w4 = wds[wi + 4];
-    uint64_t h = ((uint64_t)w0 << 32) | w1;
-    uint64_t l = ((uint64_t)w2 << 32) | w3;
-    {
-        const uint64_t n = (uint64_t)w4 << 32;
-        const uint32_t rs = 63 - sh;                       // (v >> 1) >> (63 - sh) == v >> (64 - sh), 0 for sh = 0
-        h = (h << sh) | ((l >> 1) >> rs);
-        l = (l << sh) | ((n >> 1) >> rs);
+template <typename K> struct ExChunk;
+
+template <> struct ExChunk<uint64_t> {
+    // window: bases [o0, o0 + 64 - (o0 & 15)) as hi:lo, base o0 at the top of
+    // hi; k-mer r takes bases [o0 + r, o0 + r + k), within the window for
+    // R + k <= 50
+    template <int R, typename W>
+    __device__ static __forceinline__ void kmers(const W* wds, uint32_t o0, int k, uint64_t (&c)[R]) {
+        const uint32_t wi = o0 >> 4, sh = (o0 & 15) * 2;   // sh <= 30
+        const uint32_t w0 = wds[wi], w1 = wds[wi + 1], w2 = wds[wi + 2], w3 = wds[wi + 3];
+        const uint64_t a = ((uint64_t)w0 << 32) | w1, b = ((uint64_t)w2 << 32) | w3;
+        const uint64_t hi = (a << sh) | ((b >> 1) >> (63 - sh));    // (v >> 1) >> (63 - sh) == v >> (64 - sh), 0 for sh = 0
+        const uint64_t lo = b << sh;
+        const int ks = 64 - 2 * k;
+        uint64_t f = hi >> ks;
+        uint64_t rc = revcomp64(f, k);
+        c[0] = rc < f ? rc : f;
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+            f = ((hi << (2 * r)) | (lo >> (64 - 2 * r))) >> ks;
+            rc = (rc >> 2) | ((uint64_t)(~(uint32_t)f & 3u) << (2 * k - 2));
+            c[r] = rc < f ? rc : f;
+        }
     }
-    // top 2k bits of h:l -> value
-    const int s = 128 - 2 * k;
-    u128k r;
-    if (s == 0) { r.hi = h; r.lo = l; }
-    else if (s < 64) { r.lo = (l >> s) | (h << (64 - s)); r.hi = h >> s; }
-    else { r.lo = h >> (s - 64); r.hi = 0; }
-    return r;
-}
-__device__ __forceinline__ uint64_t canon_of(uint64_t x, int k) { return canonical64(x, k); }
-__device__ __forceinline__ u128k canon_of(u128k x, int k) {
-    u128k r = revcomp128(x, k);
-    return less128(r, x) ? r : x;
-}
+};
 
-// Work item = EX_HDRS consecutive headers of one segment; each wave takes 64
-// of them, prefix-sums their k-mer counts and strides its lanes over the
-// combined k-mers, so consecutive lanes write consecutive keys (a scatter
-// block's superkmers of one partition have consecutive k-mer offsets).
-// The wave first copies its superkmers' words into LDS (each lane its own
-// superkmer's few words), so the three or five words each k-mer spans are
-// LDS reads rather than gathers from L2; a wave whose superkmers hold more
-// than EX_SW words (long superkmers) reads them from global memory instead.
-constexpr int EX_U = 2;
-constexpr int EX_SW = 512;
+template <> struct ExChunk<u128k> {
+    // the smaller of the two, word by word (a select of whole structs is
+    // lowered to an indexed private array)
+    __device__ static __forceinline__ void pick(u128k& c, const u128k& rc, const u128k& f) {
+        const bool lt = less128(rc, f);
+        c.lo = lt ? rc.lo : f.lo;
+        c.hi = lt ? rc.hi : f.hi;
+    }
+    // top 2k bits of t_hi:t_lo (s = 128 - 2k, wave-uniform)
+    __device__ static __forceinline__ u128k top(uint64_t th, uint64_t tl, int s) {
+        u128k x;
+        if (s == 0) { x.hi = th; x.lo = tl; }
+        else if (s < 64) { x.lo = (tl >> s) | (th << (64 - s)); x.hi = th >> s; }
+        else { x.lo = th >> (s - 64); x.hi = 0; }
+        return x;
+    }
+    // window: bases [o0, o0 + 96 - (o0 & 15)) as x0:x1:x2, base o0 at the top
+    // of x0; within the window for R + k <= 82
+    template <int R, typename W>
+    __device__ static __forceinline__ void kmers(const W* wds, uint32_t o0, int k, u128k (&c)[R]) {
+        const uint32_t wi = o0 >> 4, sh = (o0 & 15) * 2, rs = 63 - sh;
+        uint64_t x0 = ((uint64_t)wds[wi] << 32) | wds[wi + 1];
+        uint64_t x1 = ((uint64_t)wds[wi + 2] << 32) | wds[wi + 3];
+        uint64_t x2 = ((uint64_t)wds[wi + 4] << 32) | wds[wi + 5];
+        x0 = (x0 << sh) | ((x1 >> 1) >> rs);
+        x1 = (x1 << sh) | ((x2 >> 1) >> rs);
+        x2 = x2 << sh;
+        const int s = 128 - 2 * k;
+        const int ins = 2 * k - 2;                       // bit of the reverse complement's top base
+        u128k f = top(x0, x1, s);
+        u128k rc = revcomp128(f, k);
+        pick(c[0], rc, f);
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+            f = top((x0 << (2 * r)) | (x1 >> (64 - 2 * r)), (x1 << (2 * r)) | (x2 >> (64 - 2 * r)), s);
+            const uint64_t cb = ~f.lo & 3u;
+            rc.lo = (rc.lo >> 2) | (rc.hi << 62);
+            rc.hi >>= 2;
+            if (ins >= 64) rc.hi |= cb << (ins - 64);
+            else rc.lo |= cb << ins;
+            pick(c[r], rc, f);
+        }
+    }
+};
+
+// Work item = EX_HDRS consecutive headers of one segment.  Workgroups claim
+// batches of EX_B consecutive items (segment order) from one counter, and
+// find a batch's segment in the per-item segment table (k_tile_desc); a
+// batch's next claim is taken while it is extracted.  Each wave takes 64
+// headers of an item, prefix-sums their chunk counts and strides its lanes
+// over the combined chunks.  The wave first copies its superkmers' words into
+// LDS (each lane its own superkmer's few words, all its loads issued before
+// the first wait), so the chunk windows are LDS reads rather than gathers
+// from L2; a wave whose superkmers hold more than EX_SW words (long
+// superkmers) reads them from global memory instead.  The next item's headers
+// are loaded while this one is extracted, and the waves of a workgroup meet
+// only at batch ends and where the segment changes (the histogram flush): all
+// other LDS is per wave.
 template <typename K>
-__global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, const SkmHdr* __restrict__ hdrs,
+__global__ __launch_bounds__(SORT_NT) void k_extract(Plan* __restrict__ plan, SegArrays sa, const uint32_t* __restrict__ iseg, const SkmHdr* __restrict__ hdrs,
                                                      const uint64_t* __restrict__ hdr_base,
                                                      const uint32_t* __restrict__ words,
                                                      const uint64_t* __restrict__ word_base, K* __restrict__ keys, int k) {
     constexpr int NWAVE = SORT_NT / GKC_WAVE;
+    constexpr int EX_U = ex_u<K>();
+    constexpr int EX_R = ex_r<K>();
     extern __shared__ uint32_t s_hist[];                // [n_pass * RADIX] (dynamic LDS)
-    __shared__ uint32_t s_pre[NWAVE][GKC_WAVE + 1];     // inclusive k-mer prefix of the wave's headers
-    __shared__ uint32_t s_koff[NWAVE][GKC_WAVE], s_woff[NWAVE][GKC_WAVE];
-    __shared__ uint32_t s_flag[NWAVE][EX_U * GKC_WAVE];  // 1 + header starting at each position of a block
-    __shared__ uint32_t s_words[NWAVE][EX_SW + 8];      // the wave's superkmer words (+ read-past padding)
-    __shared__ uint32_t s_wst[NWAVE][GKC_WAVE];         // each header's first word in s_words
-    __shared__ uint32_t s_seg, s_idx, s_ok;
+    __shared__ uint4 s_hi[NWAVE][GKC_WAVE];             // per header: k-mer offset, word base, first chunk, k-mers
+    __shared__ uint32_t s_flag[NWAVE][EX_U * GKC_WAVE]; // 1 + header whose first chunk is at each position of a block
+    __shared__ uint32_t s_words[NWAVE][EX_SW + EX_SWP]; // the wave's superkmer words (+ read-past padding)
+    __shared__ K s_keys[NWAVE][EX_R * GKC_WAVE];        // a block's keys in chunk order
+    __shared__ uint2 s_cd[NWAVE][GKC_WAVE];             // a block's chunks: first key (segment offset), k-mers
     const int npass = plan->n_pass;
     const int sh0 = plan->sh0;
     const int db = plan->dbits;
     const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE;
-    uint32_t list = blockIdx.x % wl.n_lists;
-    // the next item is claimed, and its headers loaded, while this one is
-    // extracted and its histograms are flushed
-    auto load_hdr = [&](uint32_t sg, uint32_t ix) {
-        const uint32_t h = ix * EX_HDRS + threadIdx.x;
-        SkmHdr hd{0u, 0u, 0u, 0u};
-        if (h < sa.seg_hdrs[sg]) hd = hdrs[hdr_base[sg] + h];
-        return hd;
+    // (the item and segment values are read as wave-uniform, so the control
+    // flow around the workgroup barriers below is uniform)
+    auto u32 = [](uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane(v); };
+    auto u64 = [&](uint64_t v) { return ((uint64_t)u32((uint32_t)(v >> 32)) << 32) | u32((uint32_t)v); };
+    // a segment's values, loaded once per segment: header count, first and
+    // end item, first header, word and key
+    struct SegV { uint32_t nh, start, end; uint64_t hb, wb, kb; };
+    auto seg_vals = [&](uint32_t s) {
+        return SegV{u32(sa.seg_hdrs[s]), u32(sa.item_start[s]), u32(sa.item_start[s + 1]), u64(hdr_base[s]), u64(word_base[s]),
+                    u64(sa.key_start[s])};
     };
+    // an item's headers; past the segment's last header a valid load (its
+    // first), marked empty
+    auto load_hdr = [&](const SegV& v, uint32_t ix, bool& valid) {
+        const uint32_t h = ix * EX_HDRS + threadIdx.x;
+        valid = h < v.nh;
+        return hdrs[v.hb + (valid ? h : 0u)];
+    };
+    const uint32_t T = plan->tot_items;
+    uint32_t* ctr = &plan->ctr[ctr_extract()];
+    __shared__ uint32_t s_claim[2];                      // next batch: first item (>= T: none), its segment
     if (threadIdx.x == 0) {
-        uint32_t sg = 0, ix = 0;
-        s_ok = claim<2>(wl, &plan->ctr[ctr_extract()], list, sg, ix);
-        s_seg = sg; s_idx = ix;
+        const uint32_t g0 = atomicAdd(ctr, 1u) * EX_B;
+        s_claim[0] = g0;
+        s_claim[1] = iseg[min(g0, T - 1)];
     }
     for (int i = threadIdx.x; i < npass * RADIX; i += SORT_NT) s_hist[i] = 0;
-    __syncthreads();
-    if (!s_ok) return;
-    uint32_t seg = s_seg;
-    SkmHdr hd = load_hdr(seg, s_idx);
-    for (;;) {
-        uint32_t c_ok = 0, c_sg = 0, c_ix = 0;
-        if (threadIdx.x == 0) c_ok = claim<2>(wl, &plan->ctr[ctr_extract()], list, c_sg, c_ix) ? 1u : 0u;
-        const uint32_t n = hd.n_kmers;
-        s_koff[wv][lane] = hd.kmer_off;
-        s_woff[wv][lane] = hd.word_off;
-        uint32_t inc = n;
-#pragma unroll
-        for (int off = 1; off < GKC_WAVE; off <<= 1) {
-            const uint32_t y = __shfl_up(inc, off);
-            if ((int)lane >= off) inc += y;
-        }
-        s_pre[wv][lane + 1] = inc;
-        if (lane == 0) s_pre[wv][0] = 0;
-        const uint32_t T = __shfl(inc, GKC_WAVE - 1);
-        __syncthreads();
-        const uint32_t* wseg = words + word_base[seg];
-        K* kseg = keys + sa.key_start[seg];
-        // stage the wave's words: header h's words [0, nws) (the bases of all
-        // its k-mers) at s_words[wv][s_wst[h]]
-        const uint32_t nws = n ? ((n + (uint32_t)k - 2) >> 4) + 1 : 0u;
-        uint32_t winc = nws;
-#pragma unroll
-        for (int off = 1; off < GKC_WAVE; off <<= 1) {
-            const uint32_t y = __shfl_up(winc, off);
-            if ((int)lane >= off) winc += y;
-        }
-        const bool staged = __shfl(winc, GKC_WAVE - 1) <= (uint32_t)EX_SW;       // wave-uniform
-        if (staged) {
-            const uint32_t w0 = winc - nws;
-            s_wst[wv][lane] = w0;
-            const uint32_t* src = wseg + hd.word_off;
-            for (uint32_t j = 0; j < nws; j++) s_words[wv][w0 + j] = src[j];
-            if (lane == GKC_WAVE - 1)
-                for (uint32_t j = 0; j < 8; j++) s_words[wv][winc + j] = 0;      // read-past padding
-            wave_lds_fence();
-        }
-        // blocks of EX_U * 64 consecutive k-mers, k-mer qb + 64u + lane on
-        // lane `lane`: each non-empty header flags its start in LDS; a k-mer's
-        // header is the last flag at or before it (a ballot and a lane
-        // shuffle), or the header carried from the previous positions
-        const uint32_t pre = inc - n;            // this lane's header's first k-mer
-        uint32_t cur = 0;                        // header of the position before the current 64
-        for (uint32_t qb = 0; qb < T; qb += EX_U * GKC_WAVE) {
-#pragma unroll
-            for (int u = 0; u < EX_U; u++) s_flag[wv][u * GKC_WAVE + lane] = 0;
-            wave_lds_fence();    // other lanes' clears
-            if (n && pre >= qb && pre < qb + EX_U * GKC_WAVE) s_flag[wv][pre - qb] = lane + 1;
-            wave_lds_fence();    // other lanes' flags
-            uint32_t hj[EX_U], ho[EX_U];
-#pragma unroll
-            for (int u = 0; u < EX_U; u++) {
-                const uint32_t f = s_flag[wv][u * GKC_WAVE + lane];
-                const uint64_t M = __ballot(f != 0);
-                const uint64_t le = M & (lanemask_lt() | (1ull << lane));
-                const uint32_t fs = __shfl(f, le ? 63 - __builtin_clzll(le) : 0);
-                hj[u] = le ? fs - 1 : cur;
-                if (M) cur = __shfl(f, 63 - __builtin_clzll(M)) - 1;
-                const uint32_t q = min(qb + u * GKC_WAVE + lane, T - 1);   // past T: a valid load, not stored
-                ho[u] = q - s_pre[wv][hj[u]];
+    __syncthreads();                                     // histograms cleared, first claim out
+    uint32_t g = u32(s_claim[0]);
+    uint32_t seg = u32(s_claim[1]);
+    __syncthreads();                                     // the claim read before the batch end rewrites it
+    if (g >= T) return;
+    SegV cs = seg_vals(seg);
+    uint32_t idx = g - cs.start;
+    bool hv;
+    SkmHdr hd = load_hdr(cs, idx, hv);
+    for (;;) {                                           // batches
+        const uint32_t gend = min(T, g + EX_B);
+        uint32_t nb = 0, nbseg = 0;                      // the next batch's claim (thread 0)
+        if (threadIdx.x == 0) nb = atomicAdd(ctr, 1u);
+        bool asked = false;
+        for (;;) {                                       // items [g, gend)
+            // the next item of the batch (the current one again after its last, not used)
+            const bool more = g + 1 < gend;
+            uint32_t nseg = seg, nidx = idx + (more ? 1u : 0u);
+            SegV ns = cs;
+            if (more && g + 1 == cs.end) {
+                nseg = seg + 1;
+                while (u32(sa.item_start[nseg + 1]) <= g + 1) nseg++;
+                nidx = 0;
+                ns = seg_vals(nseg);
             }
-            K c[EX_U];
+            const uint32_t n = hv ? hd.n_kmers : 0u;
+            const uint32_t* wseg = words + cs.wb;
+            K* kseg = keys + cs.kb;
+            // stage the wave's words: header h's words [0, nws) (the bases of all
+            // its k-mers) at s_words[wv][wst]
+            const uint32_t nws = n ? ((n + (uint32_t)k - 2) >> 4) + 1 : 0u;
+            uint32_t winc = nws;
+            const uint32_t nch = (n + EX_R - 1) / EX_R;
+            uint32_t cinc = nch;
+#pragma unroll
+            for (int off = 1; off < GKC_WAVE; off <<= 1) {
+                const uint32_t y = __shfl_up(winc, off), z = __shfl_up(cinc, off);
+                if ((int)lane >= off) { winc += y; cinc += z; }
+            }
+            const bool staged = __shfl(winc, GKC_WAVE - 1) <= (uint32_t)EX_SW;       // wave-uniform
+            const uint32_t Ct = __shfl(cinc, GKC_WAVE - 1);                            // the wave's chunks
+            const uint32_t wst = winc - nws, cpre = cinc - nch;
             if (staged) {
+                const uint32_t* src = wseg + hd.word_off;
+                for (uint32_t j0 = 0; j0 < nws; j0 += 8) {
+                    uint32_t v[8];
 #pragma unroll
-                for (int u = 0; u < EX_U; u++) c[u] = canon_of(kmer_at<K>(&s_words[wv][s_wst[wv][hj[u]]], ho[u], k), k);
-            } else {
+                    for (int t = 0; t < 8; t++) v[t] = src[j0 + t];      // past the superkmer: its segment's next words or the padding
 #pragma unroll
-                for (int u = 0; u < EX_U; u++) c[u] = canon_of(kmer_at<K>(wseg + s_woff[wv][hj[u]], ho[u], k), k);
+                    for (int t = 0; t < 8; t++)
+                        if (j0 + t < nws) s_words[wv][wst + j0 + t] = v[t];
+                }
+                if (lane == GKC_WAVE - 1)
+#pragma unroll
+                    for (int t = 0; t < EX_SWP; t++) s_words[wv][winc + t] = 0;
             }
+            s_hi[wv][lane] = make_uint4(hd.kmer_off, staged ? wst : hd.word_off, cpre, n);
+            wave_lds_fence();
+            // the claim has returned by now (the staging waited on it): look up
+            // its segment, read at the batch end
+            if (threadIdx.x == 0 && !asked) {
+                asked = true;
+                nbseg = iseg[min(nb * EX_B, T - 1)];
+            }
+            bool hnv;
+            const SkmHdr hn = load_hdr(ns, nidx, hnv);
+            // blocks of EX_U * 64 chunks, chunk qb + 64u + lane on lane `lane`:
+            // each header with chunks flags its first one in LDS; a chunk's header
+            // is the last flag at or before it (a ballot and a lane shuffle), or
+            // the header carried from the previous positions
+            uint32_t cur = 0;
+            for (uint32_t qb = 0; qb < Ct; qb += EX_U * GKC_WAVE) {
 #pragma unroll
-            for (int u = 0; u < EX_U; u++) {
-                if (qb + u * GKC_WAVE + lane < T) {
-                    kseg[s_koff[wv][hj[u]] + ho[u]] = c[u];
-                    for (int ps = 0; ps < npass; ps++)
-                        atomicAdd(&s_hist[ps * RADIX + KeyTraits<K>::digit_n(c[u], sh0 + ps * db, db)], 1u);
+                for (int u = 0; u < EX_U; u++) s_flag[wv][u * GKC_WAVE + lane] = 0;
+                wave_lds_fence();    // other lanes' clears
+                if (nch && cpre >= qb && cpre < qb + EX_U * GKC_WAVE) s_flag[wv][cpre - qb] = lane + 1;
+                wave_lds_fence();    // other lanes' flags
+                uint32_t hj[EX_U];
+#pragma unroll
+                for (int u = 0; u < EX_U; u++) {
+                    const uint32_t f = s_flag[wv][u * GKC_WAVE + lane];
+                    const uint64_t M = __ballot(f != 0);
+                    const uint64_t le = M & (lanemask_lt() | (1ull << lane));
+                    const uint32_t fs = __shfl(f, le ? 63 - __builtin_clzll(le) : 0);
+                    hj[u] = le ? fs - 1 : cur;
+                    if (M) cur = __shfl(f, 63 - __builtin_clzll(M)) - 1;
+                }
+                K c[EX_U][EX_R];
+                uint2 cd[EX_U];                               // chunk's first key (segment offset), k-mers
+#pragma unroll
+                for (int u = 0; u < EX_U; u++) {
+                    const uint32_t q = qb + u * GKC_WAVE + lane;
+                    const uint4 h = s_hi[wv][hj[u]];
+                    const bool ok = q < Ct;                   // past Ct: the header's first chunk, read but not stored
+                    const uint32_t o0 = ok ? (q - h.z) * EX_R : 0u;
+                    cd[u] = make_uint2(h.x + o0, ok ? min((uint32_t)EX_R, h.w - o0) : 0u);
+                    if (staged) ExChunk<K>::template kmers<EX_R>(&s_words[wv][h.y], o0, k, c[u]);
+                    else ExChunk<K>::template kmers<EX_R>(wseg + h.y, o0, k, c[u]);
+                }
+                // stores through LDS: lane l of store s takes key l % EX_R of
+                // chunk (64 s + l) / EX_R, so each store covers a run of
+                // consecutive keys (chunk-strided lane stores would write each
+                // line in EX_R partial pieces); one-k-mer chunks store directly
+                if constexpr (EX_R == 1) {
+#pragma unroll
+                    for (int u = 0; u < EX_U; u++) {
+                        if (cd[u].y) {
+                            kseg[cd[u].x] = c[u][0];
+                            for (int ps = 0; ps < npass; ps++)
+                                atomicAdd(&s_hist[ps * RADIX + KeyTraits<K>::digit_n(c[u][0], sh0 + ps * db, db)], 1u);
+                        }
+                    }
+                    continue;
+                }
+#pragma unroll
+                for (int u = 0; u < EX_U; u++) {
+                    s_cd[wv][lane] = cd[u];
+#pragma unroll
+                    for (int r = 0; r < EX_R; r++) s_keys[wv][lane * EX_R + r] = c[u][r];
+                    wave_lds_fence();
+#pragma unroll
+                    for (int st = 0; st < EX_R; st++) {
+                        const uint32_t slot = st * GKC_WAVE + lane, r = lane % EX_R;
+                        const uint2 d = s_cd[wv][slot / EX_R];
+                        if (r < d.y) {
+                            const K key = s_keys[wv][slot];
+                            kseg[d.x + r] = key;
+                            for (int ps = 0; ps < npass; ps++)
+                                atomicAdd(&s_hist[ps * RADIX + KeyTraits<K>::digit_n(key, sh0 + ps * db, db)], 1u);
+                        }
+                    }
+                    wave_lds_fence();                         // read before the next block's writes
                 }
             }
+            wave_lds_fence();                                 // this item's LDS reads before the next item's writes
+            if (!more) break;
+            // the histograms go out when the segment changes
+            if (nseg != seg) {
+                __syncthreads();                             // every wave's k-mers of the segment counted
+                uint32_t* gh = sa.hist + (size_t)seg * MAX_PASS * RADIX;
+                for (int i = threadIdx.x; i < npass * RADIX; i += SORT_NT)
+                    if (s_hist[i]) atomicAdd(&gh[i], s_hist[i]);
+                __syncthreads();                             // flushed before the histograms are cleared
+                for (int i = threadIdx.x; i < npass * RADIX; i += SORT_NT) s_hist[i] = 0;
+                __syncthreads();                             // cleared before the next segment's counts
+            }
+            g++;
+            seg = nseg;
+            idx = nidx;
+            cs = ns;
+            hd = hn;
+            hv = hnv;
         }
-        if (threadIdx.x == 0) { s_ok = c_ok; s_seg = c_sg; s_idx = c_ix; }
-        __syncthreads();                                 // the item's k-mers are out; the next item is known
-        const bool more = s_ok;
-        const uint32_t nseg = s_seg;
-        if (more) hd = load_hdr(nseg, s_idx);
-        // the histograms go out when the segment changes (a workgroup's
-        // successive items are mostly of one segment: its list's segments are
-        // claimed in order), so most items add no global atomics
-        if (!more || nseg != seg) {
+        // batch done: the next one, and the histograms out if its segment differs
+        if (threadIdx.x == 0) {
+            s_claim[0] = nb * EX_B;
+            s_claim[1] = nbseg;
+        }
+        __syncthreads();                                 // every wave's k-mers of the batch counted; the claim out
+        const uint32_t ng = u32(s_claim[0]), ngseg = u32(s_claim[1]);
+        const bool last = ng >= T;
+        if (last || ngseg != seg) {
             uint32_t* gh = sa.hist + (size_t)seg * MAX_PASS * RADIX;
             for (int i = threadIdx.x; i < npass * RADIX; i += SORT_NT)
                 if (s_hist[i]) atomicAdd(&gh[i], s_hist[i]);
-            __syncthreads();                             // flushed before the histograms are cleared
-            for (int i = threadIdx.x; i < npass * RADIX; i += SORT_NT) s_hist[i] = 0;
+            if (!last) {
+                __syncthreads();                         // flushed before the histograms are cleared
+                for (int i = threadIdx.x; i < npass * RADIX; i += SORT_NT) s_hist[i] = 0;
+            }
         }
-        if (!more) break;
-        seg = nseg;
+        if (last) break;
+        __syncthreads();                                 // the claim read (and the histograms cleared) before the next batch
+        if (ngseg != seg) {
+            seg = ngseg;
+            cs = seg_vals(seg);
+        }
+        g = ng;
+        idx = g - cs.start;
+        hd = load_hdr(cs, idx, hv);
     }
 }
 
@@ -1848,8 +1996,8 @@ template __global__ void k_wavecount<u128k>(Plan*, SegArrays, WorkLists, u128k*,
 template __global__ void k_wavecopy<uint64_t>(Plan*, SegArrays, const uint64_t*, const uint64_t*, const uint2*, HeavyList, TableOut);
 template __global__ void k_wavecopy<u128k>(Plan*, SegArrays, const u128k*, const u128k*, const uint2*, HeavyList, TableOut);
 
-template __global__ void k_extract<uint64_t>(Plan*, SegArrays, WorkLists, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, uint64_t*, int);
-template __global__ void k_extract<u128k>(Plan*, SegArrays, WorkLists, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, u128k*, int);
+template __global__ void k_extract<uint64_t>(Plan*, SegArrays, const uint32_t*, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, uint64_t*, int);
+template __global__ void k_extract<u128k>(Plan*, SegArrays, const uint32_t*, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, u128k*, int);
 #define GKC_OS(K, I, N, D) template __global__ void k_onesweep<K, I, N, D>(Plan*, SegArrays, WorkLists, const TileDesc*, const K*, K*, uint32_t*, int);
 GKC_OS(uint64_t, 8, 256, 8) GKC_OS(uint64_t, 12, 256, 8) GKC_OS(uint64_t, 16, 256, 8) GKC_OS(uint64_t, 20, 256, 8)
 GKC_OS(uint64_t, 24, 256, 8) GKC_OS(uint64_t, 8, 512, 8) GKC_OS(uint64_t, 10, 512, 8) GKC_OS(uint64_t, 12, 512, 8)
