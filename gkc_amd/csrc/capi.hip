@@ -129,6 +129,7 @@ struct gkc_ctx {
     DevBuf d_tcap_total;              // total table capacity of a presized binned count (u64)
     uint64_t h_tcap_total = 0;
     DevBuf d_ranges;                           // binned-count tile ranges {a, n}
+    DevBuf d_wdesc;                            // binned-count windows {segment, n, first key low, high}
     DevBuf d_tiler0;                           // read holding the first position of each phase-1 tile
     DevBuf d_chunk_s;                          // descriptor of a chunk's sample (first-batch presizing)
     DevBuf d_hlist, d_hres;                    // heavy tiles of a wave-level count and their {kept, distinct}
@@ -1638,13 +1639,13 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs
                 zero(c, &plan->ctr[ctr_rl()], NLIST);
                 const int g_r = occupancy_blocks((const void*)k_binranges<K>, 256, c->n_cu);
                 hipLaunchKernelGGL(k_binranges<K>, dim3(g_r), dim3(256), 0, c->st, plan, sa, (const K*)sorted, ranges,
-                                   bs.rl_tile, (int)bs.sh0, bs.bin_bits, (uint32_t)WC_CAP, hl);
+                                   c->d_wdesc.as<uint4>(), bs.rl_tile, (int)bs.sh0, bs.bin_bits, (uint32_t)WC_CAP, hl);
                 const int g_h = occupancy_blocks((const void*)k_bincount<K>, BC_NT, c->n_cu);
                 hipLaunchKernelGGL(k_bincount<K>, dim3(g_h), dim3(BC_NT), 0, c->st, plan, sa, sorted, alt,
                                    (const uint2*)ranges, to, hl, 0);
                 const int g_w = occupancy_blocks((const void*)k_wavecount<K>, WC_NT, c->n_cu);
-                hipLaunchKernelGGL(k_wavecount<K>, dim3(g_w), dim3(WC_NT), 0, c->st, plan, sa, c->wl, sorted, alt,
-                                   (const uint2*)ranges, hl, to);
+                hipLaunchKernelGGL(k_wavecount<K>, dim3(g_w), dim3(WC_NT), 0, c->st, plan, sa, sorted, alt,
+                                   (const uint4*)c->d_wdesc.as<uint4>(), hl, to);
                 if constexpr (sizeof(K) == 8) {     // the medium windows k_wavecount handed on
                     const int g_m0 = occupancy_blocks((const void*)k_midcount<0>, GKC_WAVE, c->n_cu);
                     hipLaunchKernelGGL(k_midcount<0>, dim3(g_m0), dim3(GKC_WAVE), 0, c->st, plan, sa, sorted, alt,
@@ -1730,6 +1731,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     sa.hist = c->d_hist.as<uint32_t>();
     K* sorted = (bs.npass % 2 == 0) ? c->keysA.as<K>() : c->keysB.as<K>();
     c->d_ranges.ensure(rl_tiles_cap * sizeof(uint2));
+    if (bs.wave) c->d_wdesc.ensure(rl_tiles_cap * sizeof(uint4));
     c->d_hlist.ensure(4 * rl_tiles_cap * sizeof(uint32_t));
     c->d_hres.ensure(rl_tiles_cap * sizeof(uint2));
     c->d_wkept.ensure(rl_tiles_cap * sizeof(uint32_t));
@@ -1760,6 +1762,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
         HIPX(hipEventRecord(c->ev_lsum, c->st));
     }
     std::vector<const void*> key = {c->keysA.p, c->keysB.p, c->lookback.p, c->d_tdesc.p, c->d_iseg.p, c->words.p, c->hdrs.p, c->d_ranges.p,
+                                    c->d_wdesc.p,
                                     c->d_hlist.p, c->d_hres.p, c->d_wkept.p,
                                     (const void*)(uintptr_t)tiles_cap, (const void*)(uintptr_t)bs.npass,
                                     (const void*)(uintptr_t)sort_items<K>(c, bs.dbits), (const void*)(uintptr_t)sort_grid<K>(c, bs.dbits),

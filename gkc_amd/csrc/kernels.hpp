@@ -227,7 +227,7 @@ constexpr uint32_t MID_MAX = mid_max(1);
 
 template <typename K>
 __global__ void k_binranges(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys, uint2* __restrict__ ranges,
-                            int tile_keys, int bsh, int bin_bits, uint32_t heavy_cap, HeavyList hl);
+                            uint4* __restrict__ wdesc, int tile_keys, int bsh, int bin_bits, uint32_t heavy_cap, HeavyList hl);
 template <typename K>
 __global__ void k_bincount(Plan* __restrict__ plan, SegArrays sa, K* __restrict__ keys, K* __restrict__ alt,
                            const uint2* __restrict__ ranges, TableOut to, HeavyList hl, int late);
@@ -235,8 +235,8 @@ template <int L>
 __global__ void k_midcount(Plan* __restrict__ plan, SegArrays sa, uint64_t* __restrict__ keys, uint64_t* __restrict__ alt,
                            const uint2* __restrict__ ranges, TableOut to, HeavyList hl);
 template <typename K>
-__global__ void k_wavecount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl, K* __restrict__ keys,
-                            K* __restrict__ alt, const uint2* __restrict__ ranges, HeavyList hl, TableOut to);
+__global__ void k_wavecount(Plan* __restrict__ plan, SegArrays sa, K* __restrict__ keys,
+                            K* __restrict__ alt, const uint4* __restrict__ wdesc, HeavyList hl, TableOut to);
 __global__ void k_tilescan(Plan* __restrict__ plan, SegArrays sa, uint32_t* __restrict__ kept);
 template <typename K>
 __global__ void k_wavecopy(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys,

@@ -981,15 +981,16 @@ __device__ __forceinline__ uint32_t bin_run_end(const K* kp, uint32_t j, uint32_
 }
 
 // Window ranges: ranges[t] = {a, n} (n = 0 when no bin starts in the
-// window); bins are key bits [bsh, bsh + bin_bits).  One lane per window:
+// window) and, when wdesc is given, wdesc[t] = {segment, n, first key of the
+// range (absolute index) low, high}; bins are key bits [bsh, bsh + bin_bits).  One lane per window:
 // a = the first bin start in [lo, hi), the range ends at the first bin start
 // at or after hi.  Windows whose range holds more than heavy_cap keys are
 // appended to k_bincount's list (when one is given), except 64-bit ranges of
 // up to WC_TRY_MAX keys, which k_wavecount tries first.
 template <typename K>
 __global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys,
-                                                   uint2* __restrict__ ranges, int tile_keys, int bsh, int bin_bits,
-                                                   uint32_t heavy_cap, HeavyList hl) {
+                                                   uint2* __restrict__ ranges, uint4* __restrict__ wdesc, int tile_keys, int bsh,
+                                                   int bin_bits, uint32_t heavy_cap, HeavyList hl) {
     const uint32_t TILE = (uint32_t)tile_keys;
     const uint32_t total = plan->tot_rl_tiles, S = plan->n_segs;
     const uint32_t nthreads = gridDim.x * blockDim.x;
@@ -1021,6 +1022,10 @@ __global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegA
             n = b - a;
         }
         ranges[t] = make_uint2(a, n);
+        if (wdesc) {
+            const uint64_t abs = sa.key_start[seg] + a;
+            wdesc[t] = make_uint4(seg, n, (uint32_t)abs, (uint32_t)(abs >> 32));
+        }
         if (hl.tiles && n > heavy_cap && !(hl.mid[0] && n <= WC_TRY_MAX)) hl.tiles[atomicAdd(hl.count, 1u)] = t;
     }
 }
@@ -1722,56 +1727,78 @@ template <> struct WaveTable<u128k> {
 // waves per SIMD the LDS tables allow (64-bit keys: 6 KB per wave; 128-bit: 12 KB)
 template <typename K> constexpr int wavecount_waves() { return sizeof(K) == 16 ? 3 : 6; }
 template <typename K>
-__global__ __launch_bounds__(WC_NT, wavecount_waves<K>()) void k_wavecount(Plan* __restrict__ plan, SegArrays sa, WorkLists wl,
+__global__ __launch_bounds__(WC_NT, wavecount_waves<K>()) void k_wavecount(Plan* __restrict__ plan, SegArrays sa,
                                                      K* __restrict__ keys, K* __restrict__ alt,
-                                                     const uint2* __restrict__ ranges, HeavyList hl, TableOut to) {
+                                                     const uint4* __restrict__ wdesc, HeavyList hl, TableOut to) {
     constexpr int NW = WC_NT / GKC_WAVE;
     constexpr int IT = WC_CAP / GKC_WAVE;
     static_assert((WC_CAP & (WC_CAP - 1)) == 0, "table size");
     __shared__ __align__(16) WaveTable<K> s_tab[NW];
     const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE;
     WaveTable<K>& T = s_tab[wv];
-    uint32_t list = (blockIdx.x * NW + wv) % wl.n_lists;
     uint32_t acc_seg = 0xFFFFFFFFu, acc_d = 0, acc_m = 0;     // per-segment totals, flushed on change
-    uint32_t t = 0, t1 = 0;                                   // claimed batch [t, t1) of list `list`
-    uint32_t oi = 0;                                          // order index of t's segment in the list
-    const uint32_t* uoff = wl.w + 33 + wl.n_segs + 1 * wl.n_segs;   // window offsets by order index
     constexpr bool MIDS = sizeof(K) == 8;                     // 64-bit keys: medium windows tried here
     uint32_t n_mid = 0;                                       // such windows taken (lane 0's count is used)
     uint32_t no_small = 0;                                    // windows left to count in the full table at once
-    for (;;) {
-        uint32_t seg = 0, ti = 0, ok = 1;
-        if (t >= t1) {
-            uint32_t a0 = 0, a1 = 0;
-            if (lane == 0) ok = claim_batch<1>(wl, &plan->ctr[ctr_rl()], list, WC_BATCH, a0, a1) ? 1u : 0u;
-            ok = __shfl(ok, 0); list = __shfl(list, 0); t = __shfl(a0, 0); t1 = __shfl(a1, 0);
-            oi = 0xFFFFFFFFu;
+    // windows in batches of WC_BATCH consecutive ones (segment order), the
+    // following batch claimed when one starts; a window's descriptor is loaded
+    // while the previous window is counted, and so are its keys for 64-bit
+    // keys (128-bit ones would need more registers than their 3 waves a SIMD
+    // leave)
+    constexpr bool AHEAD = sizeof(K) == 8;
+    const uint32_t TW = plan->tot_rl_tiles;
+    uint32_t* ctr = &plan->ctr[ctr_rl()];
+    uint32_t bt = 0, bt1 = 0, nb = 0;
+    if (lane == 0) nb = atomicAdd(ctr, 1u);
+    auto next_window = [&]() -> uint32_t {
+        if (bt >= bt1) {
+            const uint32_t b = __shfl(nb, 0);
+            bt = min(b, TW / WC_BATCH + 1) * WC_BATCH;
+            bt1 = min(TW, bt + WC_BATCH);
+            if (bt >= TW) { bt1 = bt; return TW; }
+            if (lane == 0) nb = atomicAdd(ctr, 1u);
         }
-        if (ok) {
-            if (oi == 0xFFFFFFFFu) {            // first window of the batch: locate by binary search
-                uint32_t lo = wl.w[24 + list], hi = wl.w[24 + list + 1];
-                while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (uoff[mid] <= t) lo = mid; else hi = mid; }
-                oi = lo;
-            } else {                            // later windows: the same or a following segment of the list
-                while (oi + 1 < wl.w[24 + list + 1] && uoff[oi + 1] <= t) oi++;
+        return bt++;
+    };
+    // keys [0, min(n, WC_CAP)) of a window into x: the whole window when the
+    // table takes it, a medium window's first chunk
+    auto load_keys = [&](const uint4& w, K (&x)[IT]) {
+        const uint32_t n = w.y;
+        const K* kp = keys + (((uint64_t)w.w << 32) | w.z);
+        if (n && (n <= (uint32_t)WC_CAP || (MIDS && hl.mid[0] && n <= WC_TRY_MAX))) {
+#pragma unroll
+            for (int i = 0; i < IT; i++) {
+                const uint32_t j = i * GKC_WAVE + lane;
+                if (i * GKC_WAVE < (int)n && j < n) x[i] = kp[j];
             }
-            seg = wl.w[33 + oi];
-            ti = t - uoff[oi];
-            t++;
         }
-        if (seg != acc_seg || !ok) {
-            if (lane == 0 && acc_seg != 0xFFFFFFFFu) {
-                if (acc_d) atomicAdd(&to.n_distinct[acc_seg], acc_d);
-                if (acc_m) atomicAdd(&to.n_entries[acc_seg], acc_m);
-            }
+    };
+    auto flush_acc = [&]() {
+        if (lane == 0 && acc_seg != 0xFFFFFFFFu) {
+            if (acc_d) atomicAdd(&to.n_distinct[acc_seg], acc_d);
+            if (acc_m) atomicAdd(&to.n_entries[acc_seg], acc_m);
+        }
+    };
+    uint32_t t = next_window();
+    uint4 wd = t < TW ? wdesc[t] : make_uint4(0u, 0u, 0u, 0u);
+    K x[IT];
+    if constexpr (AHEAD) load_keys(wd, x);
+    while (t < TW) {
+        uint32_t tn = TW;
+        uint4 wn = make_uint4(0u, 0u, 0u, 0u);
+        if constexpr (AHEAD) {
+            tn = next_window();
+            if (tn < TW) wn = wdesc[tn];
+        } else {
+            load_keys(wd, x);
+        }
+        const uint32_t seg = wd.x, tile = t, n = wd.y;
+        if (seg != acc_seg) {
+            flush_acc();
             acc_seg = seg; acc_d = 0; acc_m = 0;
         }
-        if (!ok) break;
-        const uint32_t tile = sa.rl_tile_start[seg] + ti;
-        const uint2 rg = ranges[tile];
-        const uint32_t n = rg.y;
-        K* kp = keys + sa.key_start[seg] + rg.x;
-        K* sk = alt + sa.key_start[seg] + rg.x;
+        K* kp = keys + (((uint64_t)wd.w << 32) | wd.z);
+        K* sk = alt + (((uint64_t)wd.w << 32) | wd.z);
         uint32_t* sc = reinterpret_cast<uint32_t*>(kp);
         uint32_t d = 0, m = 0;
         uint32_t slots = WC_CAP;                 // table slots in use
@@ -1781,13 +1808,11 @@ __global__ __launch_bounds__(WC_NT, wavecount_waves<K>()) void k_wavecount(Plan*
             const uint2 hr = hl.res[tile];
             m = hr.x; d = hr.y;
         } else if (n) {
-            // hash-count the range in the wave's table
-            K x[IT];
+            // hash-count the range (its keys in x, loaded ahead) in the wave's table
             const bool small = !mid && no_small == 0;
 #pragma unroll
             for (int i = 0; i < IT; i++) {
                 const uint32_t j = i * GKC_WAVE + lane;
-                if (i * GKC_WAVE < (int)n && j < n) x[i] = kp[j];
                 if (!small || i * GKC_WAVE < (int)WC_SMALL) T.clear(j);
             }
             wave_lds_fence();    // other lanes' slots cleared
@@ -1851,6 +1876,7 @@ __global__ __launch_bounds__(WC_NT, wavecount_waves<K>()) void k_wavecount(Plan*
                 }
             }
         }
+        if constexpr (AHEAD) load_keys(wn, x);   // the next window's keys, in flight while this one is ranked
         if (n && !handed && !(n > (uint32_t)WC_CAP && !mid)) {
             // compact the kept distinct keys to the front of the table
 #pragma unroll
@@ -1884,7 +1910,14 @@ __global__ __launch_bounds__(WC_NT, wavecount_waves<K>()) void k_wavecount(Plan*
         }
         if (lane == 0 && !handed) hl.kept[tile] = m;
         acc_d += d; acc_m += m;
+        if constexpr (!AHEAD) {
+            tn = next_window();
+            if (tn < TW) wn = wdesc[tn];
+        }
+        t = tn;
+        wd = wn;
     }
+    flush_acc();
     if (MIDS && lane == 0 && n_mid && hl.mid_seen) atomicAdd(hl.mid_seen, n_mid);
 }
 
@@ -1987,12 +2020,12 @@ __global__ __launch_bounds__(256) void k_wavecopy(Plan* __restrict__ plan, SegAr
     }
 }
 
-template __global__ void k_binranges<uint64_t>(Plan*, SegArrays, const uint64_t*, uint2*, int, int, int, uint32_t, HeavyList);
-template __global__ void k_binranges<u128k>(Plan*, SegArrays, const u128k*, uint2*, int, int, int, uint32_t, HeavyList);
+template __global__ void k_binranges<uint64_t>(Plan*, SegArrays, const uint64_t*, uint2*, uint4*, int, int, int, uint32_t, HeavyList);
+template __global__ void k_binranges<u128k>(Plan*, SegArrays, const u128k*, uint2*, uint4*, int, int, int, uint32_t, HeavyList);
 template __global__ void k_bincount<uint64_t>(Plan*, SegArrays, uint64_t*, uint64_t*, const uint2*, TableOut, HeavyList, int);
 template __global__ void k_bincount<u128k>(Plan*, SegArrays, u128k*, u128k*, const uint2*, TableOut, HeavyList, int);
-template __global__ void k_wavecount<uint64_t>(Plan*, SegArrays, WorkLists, uint64_t*, uint64_t*, const uint2*, HeavyList, TableOut);
-template __global__ void k_wavecount<u128k>(Plan*, SegArrays, WorkLists, u128k*, u128k*, const uint2*, HeavyList, TableOut);
+template __global__ void k_wavecount<uint64_t>(Plan*, SegArrays, uint64_t*, uint64_t*, const uint4*, HeavyList, TableOut);
+template __global__ void k_wavecount<u128k>(Plan*, SegArrays, u128k*, u128k*, const uint4*, HeavyList, TableOut);
 template __global__ void k_wavecopy<uint64_t>(Plan*, SegArrays, const uint64_t*, const uint64_t*, const uint2*, HeavyList, TableOut);
 template __global__ void k_wavecopy<u128k>(Plan*, SegArrays, const u128k*, const u128k*, const uint2*, HeavyList, TableOut);
 
