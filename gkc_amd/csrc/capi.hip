@@ -128,7 +128,6 @@ struct gkc_ctx {
     DevBuf d_tstart, d_tcap, d_nent, d_ndist, d_toverflow;
     DevBuf d_tcap_total;              // total table capacity of a presized binned count (u64)
     uint64_t h_tcap_total = 0;
-    DevBuf d_ranges;                           // binned-count tile ranges {a, n}
     DevBuf d_wdesc;                            // binned-count windows {segment, n, first key low, high}
     DevBuf d_tiler0;                           // read holding the first position of each phase-1 tile
     DevBuf d_chunk_s;                          // descriptor of a chunk's sample (first-batch presizing)
@@ -1619,7 +1618,6 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs
     to.overflow = c->d_toverflow.as<uint32_t>();
     to.min_cnt = c->prm.min_count; to.max_cnt = c->prm.max_count; to.emit = emit;
     K* alt = sorted == c->keysA.as<K>() ? c->keysB.as<K>() : c->keysA.as<K>();
-    uint2* ranges = c->d_ranges.as<uint2>();
     // d_toverflow: [0] table overflow, [1] [2] k_bincount's list count and claim counter, [3] [4]
     // and [5] [6] the two k_midcount lists' (64-bit keys; the second and third thirds of d_hlist
     // hold those lists), [7] medium windows k_wavecount took, [8] [9] the second k_bincount
@@ -1638,23 +1636,23 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs
                 zero(c, ctrs + 1, 9);
                 zero(c, &plan->ctr[ctr_rl()], NLIST);
                 const int g_r = occupancy_blocks((const void*)k_binranges<K>, 256, c->n_cu);
-                hipLaunchKernelGGL(k_binranges<K>, dim3(g_r), dim3(256), 0, c->st, plan, sa, (const K*)sorted, ranges,
+                hipLaunchKernelGGL(k_binranges<K>, dim3(g_r), dim3(256), 0, c->st, plan, sa, (const K*)sorted,
                                    c->d_wdesc.as<uint4>(), bs.rl_tile, (int)bs.sh0, bs.bin_bits, (uint32_t)WC_CAP, hl);
                 const int g_h = occupancy_blocks((const void*)k_bincount<K>, BC_NT, c->n_cu);
                 hipLaunchKernelGGL(k_bincount<K>, dim3(g_h), dim3(BC_NT), 0, c->st, plan, sa, sorted, alt,
-                                   (const uint2*)ranges, to, hl, 0);
+                                   (const uint4*)c->d_wdesc.as<uint4>(), to, hl, 0);
                 const int g_w = occupancy_blocks((const void*)k_wavecount<K>, WC_NT, c->n_cu);
                 hipLaunchKernelGGL(k_wavecount<K>, dim3(g_w), dim3(WC_NT), 0, c->st, plan, sa, sorted, alt,
                                    (const uint4*)c->d_wdesc.as<uint4>(), hl, to);
                 if constexpr (sizeof(K) == 8) {     // the medium windows k_wavecount handed on
                     const int g_m0 = occupancy_blocks((const void*)k_midcount<0>, GKC_WAVE, c->n_cu);
                     hipLaunchKernelGGL(k_midcount<0>, dim3(g_m0), dim3(GKC_WAVE), 0, c->st, plan, sa, sorted, alt,
-                                       (const uint2*)ranges, to, hl);
+                                       (const uint4*)c->d_wdesc.as<uint4>(), to, hl);
                     const int g_m1 = occupancy_blocks((const void*)k_midcount<1>, GKC_WAVE, c->n_cu);
                     hipLaunchKernelGGL(k_midcount<1>, dim3(g_m1), dim3(GKC_WAVE), 0, c->st, plan, sa, sorted, alt,
-                                       (const uint2*)ranges, to, hl);
+                                       (const uint4*)c->d_wdesc.as<uint4>(), to, hl);
                     hipLaunchKernelGGL(k_bincount<K>, dim3(g_h), dim3(BC_NT), 0, c->st, plan, sa, sorted, alt,
-                                       (const uint2*)ranges, to, hl, 1);
+                                       (const uint4*)c->d_wdesc.as<uint4>(), to, hl, 1);
                 }
                 hipLaunchKernelGGL(k_tilescan, dim3(std::min<uint32_t>(P, 4 * c->n_cu)), dim3(1024), 0, c->st, plan, sa,
                                    hl.kept);
@@ -1667,7 +1665,7 @@ static void enqueue_runlength_all(gkc_ctx* c, SegArrays sa, const BatchShape& bs
                 zero(c, c->d_toverflow.p, 1);
                 const int g_c = occupancy_blocks((const void*)k_wavecopy<K>, 256, c->n_cu);
                 hipLaunchKernelGGL(k_wavecopy<K>, dim3(g_c), dim3(256), 0, c->st, plan, sa, (const K*)sorted,
-                                   (const K*)alt, (const uint2*)ranges, hl, to);
+                                   (const K*)alt, (const uint4*)c->d_wdesc.as<uint4>(), hl, to);
             }
         }
         HIPX(hipGetLastError());
@@ -1730,7 +1728,6 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
     sa.sort_tile_start = c->d_tilestart.as<uint32_t>(); sa.rl_tile_start = c->d_rlstart.as<uint32_t>();
     sa.hist = c->d_hist.as<uint32_t>();
     K* sorted = (bs.npass % 2 == 0) ? c->keysA.as<K>() : c->keysB.as<K>();
-    c->d_ranges.ensure(rl_tiles_cap * sizeof(uint2));
     if (bs.wave) c->d_wdesc.ensure(rl_tiles_cap * sizeof(uint4));
     c->d_hlist.ensure(4 * rl_tiles_cap * sizeof(uint32_t));
     c->d_hres.ensure(rl_tiles_cap * sizeof(uint2));
@@ -1761,7 +1758,7 @@ static void count_batched(gkc_ctx* c, uint32_t flags) {
         HIPX(hipMemcpyAsync(c->h_lsum, c->d_tcap_total.p, 8, hipMemcpyDeviceToHost, c->st));
         HIPX(hipEventRecord(c->ev_lsum, c->st));
     }
-    std::vector<const void*> key = {c->keysA.p, c->keysB.p, c->lookback.p, c->d_tdesc.p, c->d_iseg.p, c->words.p, c->hdrs.p, c->d_ranges.p,
+    std::vector<const void*> key = {c->keysA.p, c->keysB.p, c->lookback.p, c->d_tdesc.p, c->d_iseg.p, c->words.p, c->hdrs.p,
                                     c->d_wdesc.p,
                                     c->d_hlist.p, c->d_hres.p, c->d_wkept.p,
                                     (const void*)(uintptr_t)tiles_cap, (const void*)(uintptr_t)bs.npass,

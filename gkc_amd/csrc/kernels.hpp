@@ -226,21 +226,21 @@ __host__ __device__ constexpr uint32_t mid_max(int list) { return mid_slots(list
 constexpr uint32_t MID_MAX = mid_max(1);
 
 template <typename K>
-__global__ void k_binranges(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys, uint2* __restrict__ ranges,
-                            uint4* __restrict__ wdesc, int tile_keys, int bsh, int bin_bits, uint32_t heavy_cap, HeavyList hl);
+__global__ void k_binranges(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys, uint4* __restrict__ wdesc,
+                            int tile_keys, int bsh, int bin_bits, uint32_t heavy_cap, HeavyList hl);
 template <typename K>
 __global__ void k_bincount(Plan* __restrict__ plan, SegArrays sa, K* __restrict__ keys, K* __restrict__ alt,
-                           const uint2* __restrict__ ranges, TableOut to, HeavyList hl, int late);
+                           const uint4* __restrict__ wdesc, TableOut to, HeavyList hl, int late);
 template <int L>
 __global__ void k_midcount(Plan* __restrict__ plan, SegArrays sa, uint64_t* __restrict__ keys, uint64_t* __restrict__ alt,
-                           const uint2* __restrict__ ranges, TableOut to, HeavyList hl);
+                           const uint4* __restrict__ wdesc, TableOut to, HeavyList hl);
 template <typename K>
 __global__ void k_wavecount(Plan* __restrict__ plan, SegArrays sa, K* __restrict__ keys,
                             K* __restrict__ alt, const uint4* __restrict__ wdesc, HeavyList hl, TableOut to);
 __global__ void k_tilescan(Plan* __restrict__ plan, SegArrays sa, uint32_t* __restrict__ kept);
 template <typename K>
 __global__ void k_wavecopy(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys,
-                           const K* __restrict__ alt, const uint2* __restrict__ ranges, HeavyList hl, TableOut to);
+                           const K* __restrict__ alt, const uint4* __restrict__ wdesc, HeavyList hl, TableOut to);
 
 // ---- positions (positions.hip) ----
 __global__ void k_pos_scatter(const SkmHdr* __restrict__ hdrs, const uint2* __restrict__ pos, uint32_t n_hdrs,

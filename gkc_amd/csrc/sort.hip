@@ -980,17 +980,17 @@ __device__ __forceinline__ uint32_t bin_run_end(const K* kp, uint32_t j, uint32_
     return hi;
 }
 
-// Window ranges: ranges[t] = {a, n} (n = 0 when no bin starts in the
-// window) and, when wdesc is given, wdesc[t] = {segment, n, first key of the
-// range (absolute index) low, high}; bins are key bits [bsh, bsh + bin_bits).  One lane per window:
+// Window ranges: wdesc[t] = {segment, n, first key of the range (absolute
+// index) low, high}, the range starting at the window's first bin start, n = 0
+// when no bin starts in the window; bins are key bits [bsh, bsh + bin_bits).  One lane per window:
 // a = the first bin start in [lo, hi), the range ends at the first bin start
 // at or after hi.  Windows whose range holds more than heavy_cap keys are
 // appended to k_bincount's list (when one is given), except 64-bit ranges of
 // up to WC_TRY_MAX keys, which k_wavecount tries first.
 template <typename K>
 __global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys,
-                                                   uint2* __restrict__ ranges, uint4* __restrict__ wdesc, int tile_keys, int bsh,
-                                                   int bin_bits, uint32_t heavy_cap, HeavyList hl) {
+                                                   uint4* __restrict__ wdesc, int tile_keys, int bsh, int bin_bits,
+                                                   uint32_t heavy_cap, HeavyList hl) {
     const uint32_t TILE = (uint32_t)tile_keys;
     const uint32_t total = plan->tot_rl_tiles, S = plan->n_segs;
     const uint32_t nthreads = gridDim.x * blockDim.x;
@@ -1021,11 +1021,8 @@ __global__ __launch_bounds__(256) void k_binranges(Plan* __restrict__ plan, SegA
                                        : bin_run_end(kp, hi, nseg, bits_of(kp[hi - 1], bsh, bin_bits), bsh, bin_bits);
             n = b - a;
         }
-        ranges[t] = make_uint2(a, n);
-        if (wdesc) {
-            const uint64_t abs = sa.key_start[seg] + a;
-            wdesc[t] = make_uint4(seg, n, (uint32_t)abs, (uint32_t)(abs >> 32));
-        }
+        const uint64_t abs = sa.key_start[seg] + a;
+        wdesc[t] = make_uint4(seg, n, (uint32_t)abs, (uint32_t)(abs >> 32));
         if (hl.tiles && n > heavy_cap && !(hl.mid[0] && n <= WC_TRY_MAX)) hl.tiles[atomicAdd(hl.count, 1u)] = t;
     }
 }
@@ -1088,7 +1085,7 @@ __device__ __forceinline__ uint32_t digit_scan(uint32_t v, uint32_t tid, uint32_
 
 template <typename K>
 __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, SegArrays sa, K* __restrict__ keys,
-                                                    K* __restrict__ alt, const uint2* __restrict__ ranges,
+                                                    K* __restrict__ alt, const uint4* __restrict__ wdesc,
                                                     TableOut to, HeavyList hl, int late) {
     constexpr int ITEMS = BinTraits<K>::ITEMS;
     constexpr uint32_t CAP = bc_cap<K>();
@@ -1106,7 +1103,8 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
     __shared__ uint32_t s_ck[NCH];               // kept entries of each chunk, then their exclusive prefix
     __shared__ uint32_t s_red[NWAVE];
     __shared__ uint32_t s_orv[NWAVE][4];
-    __shared__ uint32_t s_seg, s_idx, s_ok, s_nd, s_out, s_nh, s_occ, s_hfail;
+    __shared__ uint4 s_wd;                       // the window: segment, keys, first key (absolute) low, high
+    __shared__ uint32_t s_tile, s_ok, s_nd, s_out, s_nh, s_occ, s_hfail;
     const uint32_t lane = __lane_id(), wv = threadIdx.x / GKC_WAVE, tid = threadIdx.x;
     const uint32_t wbase = wv * GKC_WAVE * ITEMS;     // wave wv holds chunk keys [wbase, wbase + 64*ITEMS)
     constexpr uint64_t EMPTY = ~0ull;  // never canonical: its reverse complement (0) is smaller
@@ -1114,15 +1112,15 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
     constexpr uint32_t HASH_LIMIT = CAP / 4 * 3;   // distinct keys a hashed range may hold
     for (;;) {
         if (tid == 0) {
-            uint32_t sg = 0, ix = 0;
+            uint32_t t = 0;
+            uint4 wd = make_uint4(0u, 0u, 0u, 0u);
             const uint32_t q = atomicAdd(late ? hl.late_next : hl.next, 1u);
             const bool ok = q < *(late ? hl.late_count : hl.count);
             if (ok) {
-                const uint32_t t = (late ? hl.late : hl.tiles)[q];
-                sg = find_seg(sa.rl_tile_start, plan->n_segs, t);
-                ix = t - sa.rl_tile_start[sg];
+                t = (late ? hl.late : hl.tiles)[q];
+                wd = wdesc[t];
             }
-            s_ok = ok; s_seg = sg; s_idx = ix; s_nd = 0; s_out = 0; s_nh = 0; s_occ = 0; s_hfail = 0;
+            s_ok = ok; s_tile = t; s_wd = wd; s_nd = 0; s_out = 0; s_nh = 0; s_occ = 0; s_hfail = 0;
         }
         // 64-bit keys: equal keys are counted in an LDS hash table (while the
         // range is read, CAP keys at a time), so only the distinct keys are
@@ -1132,18 +1130,18 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
             for (uint32_t i = tid; i < CAP; i += BC_NT) { s_keys[i] = EMPTY; s_cnt[i] = 0; }
         __syncthreads();
         if (!s_ok) break;
-        const uint32_t seg = s_seg, ti = s_idx;
+        const uint4 wd = s_wd;
+        const uint32_t seg = wd.x, tile = s_tile;
+        const uint64_t abs = ((uint64_t)wd.w << 32) | wd.z;
         // per-tile copy of the lane's base index, opaque to the compiler: the
         // per-item addresses are recomputed in the loop instead of being
         // hoisted out of it (which would exceed the 128 VGPRs of 16 waves/CU)
         uint32_t jb = wbase + lane;
         asm volatile("" : "+v"(jb));
-        const uint32_t tile = sa.rl_tile_start[seg] + ti;
-        const uint2 rg = ranges[tile];
-        const uint32_t n = rg.y;
+        const uint32_t n = wd.y;
         const bool heavy = n > CAP;                             // larger than LDS: chunks through global memory
-        K* src = keys + sa.key_start[seg] + rg.x;
-        K* dst = alt + sa.key_start[seg] + rg.x;
+        K* src = keys + abs;
+        K* dst = alt + abs;
         // 1. bits that vary inside the range (and, for 64-bit keys, the hash count)
         K key[ITEMS];
         uint64_t olo = 0, ohi = 0;
@@ -1278,8 +1276,8 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
                         }
                     }
                     __syncthreads();
-                    K* tk = alt + sa.key_start[seg] + rg.x;
-                    uint32_t* tc = reinterpret_cast<uint32_t*>(keys + sa.key_start[seg] + rg.x);
+                    K* tk = alt + abs;
+                    uint32_t* tc = reinterpret_cast<uint32_t*>(keys + abs);
 #pragma unroll
                     for (int i = 0; i < ITEMS; i++) {
                         if (bp[i] == 0xFFFFFFFFu) continue;
@@ -1534,8 +1532,8 @@ __global__ __launch_bounds__(BC_NT) void k_bincount(Plan* __restrict__ plan, Seg
             __syncthreads();
             // 4. stage the kept (k-mer, count) entries
             if (ctot) {
-                K* tk = alt + sa.key_start[seg] + rg.x;
-                uint32_t* tc = reinterpret_cast<uint32_t*>(keys + sa.key_start[seg] + rg.x);
+                K* tk = alt + abs;
+                uint32_t* tc = reinterpret_cast<uint32_t*>(keys + abs);
                 const uint32_t base = s_out;
 #pragma unroll
                 for (int i = 0; i < ITEMS; i++) {
@@ -1593,7 +1591,7 @@ __device__ __forceinline__ uint32_t count_below(const u128k* a, uint32_t m, cons
 // to the segment's totals.
 template <int L>
 __global__ __launch_bounds__(GKC_WAVE) void k_midcount(Plan* __restrict__ plan, SegArrays sa, uint64_t* __restrict__ keys,
-                                                       uint64_t* __restrict__ alt, const uint2* __restrict__ ranges,
+                                                       uint64_t* __restrict__ alt, const uint4* __restrict__ wdesc,
                                                        TableOut to, HeavyList hl) {
     constexpr uint32_t MID_SLOTS = mid_slots(L);
     __shared__ __align__(16) uint64_t s_key[MID_SLOTS];
@@ -1610,10 +1608,11 @@ __global__ __launch_bounds__(GKC_WAVE) void k_midcount(Plan* __restrict__ plan, 
         q = __shfl(q, 0);
         if (q >= nmid) break;
         const uint32_t tile = hl.mid[L][q];
-        const uint2 rg = ranges[tile];
-        const uint32_t n = rg.y;                 // n <= mid_max(L)
-        const uint32_t seg = find_seg(sa.rl_tile_start, plan->n_segs, tile);
-        uint64_t* kp = keys + sa.key_start[seg] + rg.x;
+        const uint4 wd = wdesc[tile];
+        const uint32_t n = wd.y;                 // n <= mid_max(L)
+        const uint32_t seg = wd.x;
+        const uint64_t abs = ((uint64_t)wd.w << 32) | wd.z;
+        uint64_t* kp = keys + abs;
         for (uint32_t i = lane; i < MID_SLOTS; i += GKC_WAVE) { s_key[i] = EMPTY; s_cnt[i] = 0; }
         wave_lds_fence();    // other lanes' slots cleared
         for (uint32_t j = lane; j < n; j += GKC_WAVE) {
@@ -1643,7 +1642,7 @@ __global__ __launch_bounds__(GKC_WAVE) void k_midcount(Plan* __restrict__ plan, 
             m += (uint32_t)__popcll(kb);
         }
         wave_lds_fence();    // compacted entries of every lane
-        uint64_t* tk = alt + sa.key_start[seg] + rg.x;
+        uint64_t* tk = alt + abs;
         uint32_t* tc = reinterpret_cast<uint32_t*>(kp);
         for (uint32_t e0 = 0; e0 < m; e0 += GKC_WAVE) {
             const uint32_t e = e0 + lane;
@@ -1660,8 +1659,8 @@ __global__ __launch_bounds__(GKC_WAVE) void k_midcount(Plan* __restrict__ plan, 
         wave_lds_fence();    // this window's reads before the next clear
     }
 }
-template __global__ void k_midcount<0>(Plan*, SegArrays, uint64_t*, uint64_t*, const uint2*, TableOut, HeavyList);
-template __global__ void k_midcount<1>(Plan*, SegArrays, uint64_t*, uint64_t*, const uint2*, TableOut, HeavyList);
+template __global__ void k_midcount<0>(Plan*, SegArrays, uint64_t*, uint64_t*, const uint4*, TableOut, HeavyList);
+template __global__ void k_midcount<1>(Plan*, SegArrays, uint64_t*, uint64_t*, const uint4*, TableOut, HeavyList);
 
 // One wave's LDS hash table of a window (k_wavecount).  64-bit keys: the key
 // word is the slot's state (EMPTY = all ones, never a canonical k-mer: its
@@ -1954,7 +1953,7 @@ __global__ __launch_bounds__(1024) void k_tilescan(Plan* __restrict__ plan, SegA
 // window): the binned count's emission; repeatable with other capacities.
 template <typename K>
 __global__ __launch_bounds__(256) void k_wavecopy(Plan* __restrict__ plan, SegArrays sa, const K* __restrict__ keys,
-                                                  const K* __restrict__ alt, const uint2* __restrict__ ranges,
+                                                  const K* __restrict__ alt, const uint4* __restrict__ wdesc,
                                                   HeavyList hl, TableOut to) {
     // a wave takes 64 consecutive windows: lane l reads window t0 + l's
     // offsets and its segment's bases (no chain of dependent loads per
@@ -1970,11 +1969,12 @@ __global__ __launch_bounds__(256) void k_wavecopy(Plan* __restrict__ plan, SegAr
         uint32_t off = 0, m = 0, a = 0, cap = 0;
         uint64_t kst = 0, tst = 0;
         if (t < total) {
-            const uint32_t seg = find_seg(sa.rl_tile_start, S, t);
+            const uint4 wd = wdesc[t];
+            const uint32_t seg = wd.x;
             const uint32_t last = sa.rl_tile_start[seg + 1] - 1;
             off = hl.kept[t];
             m = (t < last ? hl.kept[t + 1] : to.n_entries[seg]) - off;
-            if (m) { a = ranges[t].x; kst = sa.key_start[seg]; tst = to.table_start[seg]; cap = to.table_cap[seg]; }
+            if (m) { kst = ((uint64_t)wd.w << 32) | wd.z; tst = to.table_start[seg]; cap = to.table_cap[seg]; }
         }
         const uint64_t over = __ballot(m && off + m > cap);
         if (over && lane == (uint32_t)__builtin_ctzll(over)) atomicOr(to.overflow, 1u);   // one flag per wave
@@ -2020,14 +2020,14 @@ __global__ __launch_bounds__(256) void k_wavecopy(Plan* __restrict__ plan, SegAr
     }
 }
 
-template __global__ void k_binranges<uint64_t>(Plan*, SegArrays, const uint64_t*, uint2*, uint4*, int, int, int, uint32_t, HeavyList);
-template __global__ void k_binranges<u128k>(Plan*, SegArrays, const u128k*, uint2*, uint4*, int, int, int, uint32_t, HeavyList);
-template __global__ void k_bincount<uint64_t>(Plan*, SegArrays, uint64_t*, uint64_t*, const uint2*, TableOut, HeavyList, int);
-template __global__ void k_bincount<u128k>(Plan*, SegArrays, u128k*, u128k*, const uint2*, TableOut, HeavyList, int);
+template __global__ void k_binranges<uint64_t>(Plan*, SegArrays, const uint64_t*, uint4*, int, int, int, uint32_t, HeavyList);
+template __global__ void k_binranges<u128k>(Plan*, SegArrays, const u128k*, uint4*, int, int, int, uint32_t, HeavyList);
+template __global__ void k_bincount<uint64_t>(Plan*, SegArrays, uint64_t*, uint64_t*, const uint4*, TableOut, HeavyList, int);
+template __global__ void k_bincount<u128k>(Plan*, SegArrays, u128k*, u128k*, const uint4*, TableOut, HeavyList, int);
 template __global__ void k_wavecount<uint64_t>(Plan*, SegArrays, uint64_t*, uint64_t*, const uint4*, HeavyList, TableOut);
 template __global__ void k_wavecount<u128k>(Plan*, SegArrays, u128k*, u128k*, const uint4*, HeavyList, TableOut);
-template __global__ void k_wavecopy<uint64_t>(Plan*, SegArrays, const uint64_t*, const uint64_t*, const uint2*, HeavyList, TableOut);
-template __global__ void k_wavecopy<u128k>(Plan*, SegArrays, const u128k*, const u128k*, const uint2*, HeavyList, TableOut);
+template __global__ void k_wavecopy<uint64_t>(Plan*, SegArrays, const uint64_t*, const uint64_t*, const uint4*, HeavyList, TableOut);
+template __global__ void k_wavecopy<u128k>(Plan*, SegArrays, const u128k*, const u128k*, const uint4*, HeavyList, TableOut);
 
 template __global__ void k_extract<uint64_t>(Plan*, SegArrays, const uint32_t*, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, uint64_t*, int);
 template __global__ void k_extract<u128k>(Plan*, SegArrays, const uint32_t*, const SkmHdr*, const uint64_t*, const uint32_t*, const uint64_t*, u128k*, int);
