@@ -153,6 +153,10 @@ struct gkc_ctx {
     DevBuf d_pnorm, d_px, d_py, d_lcaps;
     uint64_t* h_lsum = nullptr;
     hipEvent_t ev_lsum = nullptr;
+    // the batched partition's cursors and overflow flag packed on the device,
+    // read back by one copy into pinned memory per chunk
+    DevBuf d_curpack;
+    uint32_t* h_curpack = nullptr;
     bool counted = false;
     gkc_stats stats{};
 };
@@ -476,6 +480,7 @@ void gkc_destroy(gkc_ctx* c) {
     for (auto& e : c->evp) if (e) (void)hipEventDestroy(e);
     if (c->ev_lsum) (void)hipEventDestroy(c->ev_lsum);
     if (c->h_lsum) (void)hipHostFree(c->h_lsum);
+    if (c->h_curpack) (void)hipHostFree(c->h_curpack);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
 }
@@ -600,6 +605,14 @@ __global__ void k_split_hk(const unsigned long long* __restrict__ hk, uint32_t* 
         kmer[i] = (uint32_t)(v >> 32);
     }
 }
+// The word, header and k-mer cursors and the overflow flag in one array
+// [3P + 1], so one copy brings them to the host.
+__global__ void k_pack_cursors(const uint32_t* __restrict__ wcur, const uint32_t* __restrict__ hcur,
+                               const uint32_t* __restrict__ kcur, const uint32_t* __restrict__ ovf,
+                               uint32_t* __restrict__ out, uint32_t P) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * P + 1; i += gridDim.x * blockDim.x)
+        out[i] = i < P ? wcur[i] : i < 2 * P ? hcur[i - P] : i < 3 * P ? kcur[i - 2 * P] : *ovf;
+}
 static void split_cursors(gkc_ctx* c) {
     const uint32_t P = c->prm.n_parts;
     hipLaunchKernelGGL(k_split_hk, dim3((P + 255) / 256), dim3(256), 0, c->st, c->d_hk_cur.as<unsigned long long>(),
@@ -709,8 +722,10 @@ static bool partition_presized(gkc_ctx* c, const ChunkDesc& cd) {
     }
     bins_reserve(c, nw, nh);
     c->d_backup.ensure((size_t)(3 + 256) * P * 4);     // cursors, then the sketch
+    c->d_curpack.ensure((3 * (size_t)P + 1) * 4);
+    if (!c->h_curpack) HIPX(hipHostMalloc((void**)&c->h_curpack, (3 * (size_t)P + 1) * 4, hipHostMallocDefault));
     std::vector<const void*> key = {c->words.p, c->hdrs.p, c->d_chunk.p, c->d_backup.p, c->skmpos.p,
-                                    (const void*)(uintptr_t)c->pos_mode};
+                                    (const void*)(uintptr_t)c->pos_mode, c->d_curpack.p};
     auto enqueue = [&] {
         hipLaunchKernelGGL(k_copy_words, dim3(4), dim3(256), 0, c->st, c->d_backup.as<uint32_t>(),
                            (const uint32_t*)c->d_word_cur.as<uint32_t>(), (uint64_t)P);
@@ -723,6 +738,10 @@ static bool partition_presized(gkc_ctx* c, const ChunkDesc& cd) {
         ps.sketch = c->d_sketch.as<uint32_t>();
         launch_superkmers(c, cd, ps, binset(c, false), true);
         split_cursors(c);
+        hipLaunchKernelGGL(k_pack_cursors, dim3((3 * P + 1 + 255) / 256), dim3(256), 0, c->st,
+                           (const uint32_t*)c->d_word_cur.as<uint32_t>(), (const uint32_t*)c->d_hdr_cur.as<uint32_t>(),
+                           (const uint32_t*)c->d_kmer_cur.as<uint32_t>(), (const uint32_t*)c->d_overflow.as<uint32_t>(),
+                           c->d_curpack.as<uint32_t>(), P);
     };
     // occupancy is queried (and cached) outside the capture
     (void)occupancy_blocks(c->prm.p > 16 ? (const void*)k_superkmers<uint64_t, false> : (const void*)k_superkmers<uint32_t, false>,
@@ -733,13 +752,10 @@ static bool partition_presized(gkc_ctx* c, const ChunkDesc& cd) {
         c->g_p1_key = key;
     }
     HIPX(hipGraphLaunch(c->g_p1, c->st));
-    std::vector<uint32_t> cur(3 * (size_t)P);
-    uint32_t ovf = 0;
-    HIPX(hipMemcpyAsync(cur.data(), c->d_word_cur.p, P * 4, hipMemcpyDeviceToHost, c->st));
-    HIPX(hipMemcpyAsync(cur.data() + P, c->d_hdr_cur.p, P * 4, hipMemcpyDeviceToHost, c->st));
-    HIPX(hipMemcpyAsync(cur.data() + 2 * P, c->d_kmer_cur.p, P * 4, hipMemcpyDeviceToHost, c->st));
-    HIPX(hipMemcpyAsync(&ovf, c->d_overflow.p, 4, hipMemcpyDeviceToHost, c->st));
+    HIPX(hipMemcpyAsync(c->h_curpack, c->d_curpack.p, (3 * (size_t)P + 1) * 4, hipMemcpyDeviceToHost, c->st));
     HIPX(hipStreamSynchronize(c->st));
+    const uint32_t* cur = c->h_curpack;
+    const uint32_t ovf = cur[3 * P];
     bool too_big = false;
     for (uint32_t q = 0; q < P; q++) too_big |= cur[2 * P + q] >= (1u << 30);
     if (ovf || too_big) {
