@@ -157,6 +157,10 @@ struct gkc_ctx {
     // read back by one copy into pinned memory per chunk
     DevBuf d_curpack;
     uint32_t* h_curpack = nullptr;
+    // a device-input batch's first and end read offsets, gathered on the
+    // device and read back by one copy into pinned memory
+    DevBuf d_oend;
+    uint64_t* h_oend = nullptr;
     bool counted = false;
     gkc_stats stats{};
 };
@@ -294,6 +298,11 @@ static BinSet binset(gkc_ctx* c, bool counting) {
 // Stage a batch on the device; returns the number of bases. Fills d_chunk.
 // With HPC, hpc_pos also keeps each run's in-read origin (d_hpc_pos: only the
 // reference-layout arrays of gkc_superkmer_arrays hand it out).
+__global__ void k_gather_oend(const uint64_t* __restrict__ offs, uint64_t n, uint64_t* __restrict__ out) {
+    out[0] = offs[0];
+    out[1] = offs[n];
+}
+
 static uint64_t stage_batch(gkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n, uint32_t flags,
                             ChunkDesc& cd, bool hpc_pos = false) {
     uint64_t o0, on;
@@ -303,9 +312,14 @@ static uint64_t stage_batch(gkc_ctx* c, const char* reads, const uint64_t* offs,
     if (flags & GKC_DEVICE_INPUT) {
         // on the context's own (non-blocking) stream: a null-stream copy would
         // also wait for the caller's work on other streams (its next upload)
-        HIPX(hipMemcpyAsync(&o0, offs, 8, hipMemcpyDeviceToHost, c->st));
-        HIPX(hipMemcpyAsync(&on, offs + n, 8, hipMemcpyDeviceToHost, c->st));
+        c->d_oend.ensure(16);
+        if (!c->h_oend) HIPX(hipHostMalloc((void**)&c->h_oend, 16, hipHostMallocDefault));
+        hipLaunchKernelGGL(k_gather_oend, dim3(1), dim3(1), 0, c->st, offs, (uint64_t)n, c->d_oend.as<uint64_t>());
+        HIPX(hipGetLastError());
+        HIPX(hipMemcpyAsync(c->h_oend, c->d_oend.p, 16, hipMemcpyDeviceToHost, c->st));
         HIPX(hipStreamSynchronize(c->st));
+        o0 = c->h_oend[0];
+        on = c->h_oend[1];
         // align the base pointer down to 16 bytes; positions [0, lead) belong to no read
         lead = (uint32_t)(reinterpret_cast<uintptr_t>(reads) & 15);
         dreads = reinterpret_cast<const uint8_t*>(reads) - lead;
@@ -481,6 +495,7 @@ void gkc_destroy(gkc_ctx* c) {
     if (c->ev_lsum) (void)hipEventDestroy(c->ev_lsum);
     if (c->h_lsum) (void)hipHostFree(c->h_lsum);
     if (c->h_curpack) (void)hipHostFree(c->h_curpack);
+    if (c->h_oend) (void)hipHostFree(c->h_oend);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
 }
