@@ -27,8 +27,8 @@ __device__ __forceinline__ uint32_t find_seg(const uint32_t* start, uint32_t n_s
     return lo;
 }
 
-// Claim the next unit of work of kind KIND (0 sort tile, 1 run-length tile,
-// 2 extraction item) from the work lists; run by one thread.  `list` is the
+// Claim the next unit of work of kind KIND (1: run-length tile of the plain
+// count) from the work lists; run by one thread.  `list` is the
 // workgroup's current list (advanced when a list is empty).  Returns false when
 // every list is exhausted.
 template <int KIND>
@@ -54,39 +54,6 @@ __device__ __forceinline__ bool claim(const WorkLists& wl, uint32_t* ctr, uint32
         list = (L + 1) % wl.n_lists;
     }
     return false;
-}
-
-// Claim a batch of up to G consecutive units of kind KIND (one atomic per
-// batch, so a counter is not hit once per small unit); run by one lane.
-// Returns the batch's list-order units [t0, t1) of list `list`, or false
-// when every list is exhausted.
-template <int KIND>
-__device__ __forceinline__ bool claim_batch(const WorkLists& wl, uint32_t* ctr, uint32_t& list, uint32_t G,
-                                            uint32_t& t0, uint32_t& t1) {
-    for (uint32_t tries = 0; tries < wl.n_lists; tries++) {
-        const uint32_t L = list;
-        const uint32_t tot = wl.w[KIND * 8 + L];
-        if (tot) {
-            const uint32_t b = atomicAdd(&ctr[L], 1u);
-            if ((uint64_t)b * G < tot) { t0 = b * G; t1 = min(t0 + G, tot); return true; }
-        }
-        list = (L + 1) % wl.n_lists;
-    }
-    return false;
-}
-
-// Segment and in-segment index of unit t (list order) of list L.
-template <int KIND>
-__device__ __forceinline__ void unit_at(const WorkLists& wl, uint32_t L, uint32_t t, uint32_t& seg, uint32_t& idx) {
-    const uint32_t n = wl.n_segs;
-    const uint32_t* off = wl.w + 33 + n + KIND * n;
-    uint32_t lo = wl.w[24 + L], hi = wl.w[24 + L + 1];
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (off[mid] <= t) lo = mid; else hi = mid;
-    }
-    seg = wl.w[33 + lo];
-    idx = t - off[lo];
 }
 
 // Claim the next sort tile (run by one thread): its index g in the lists'
